@@ -1,0 +1,194 @@
+"""Benchmark of the MERMAKE image hot path on one MI355X.
+
+Workload (one step = one FOV): the full hyb FOV, 3 colour channels of
+40 x 2800 x 2800 uint16 with a float32 flat field, one 40x64x64 Gaussian PSF,
+tile_size 500, overlap 89 (36 padded 678^2 tiles per stack), beta 1e-4,
+blur_radius 30, threshold 3600, delta 1, delta_fit 3, zpad = sz - 1.  Each
+channel goes through deconvolution -> blur subtraction -> local maxima ->
+delta_fit -> keep-filter, batched over its 36 tiles.  Synthetic seeded data
+(puncta density 1/20^3, Poisson background 500, quadratic illumination).
+
+Prints one JSON line: FOV voxels/s, the roofline of the dominant kernel
+(HIP-event timed on the stream the kernel runs on), the CPU baseline (the
+float32 NumPy/SciPy restatement on a bounded tile sample, extrapolated), and
+the plain one-tile-at-a-time GPU form for comparison.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK = 8.0e12
+CFG = dict(sz=40, sx=2800, sy=2800, channels=3, tile_size=500, overlap=89, beta=1e-4, blur_radius=30,
+           threshold=3600, delta=1, delta_fit=3, sigmaZ=1, sigmaXY=1.5, psf_shape=(40, 64, 64), psf_sigma=(1.5, 1.2),
+           density=1.0 / 20 ** 3)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def m1_bytes_per_stack(geom, sz, T, Zp, flat=True):
+    """Algorithmic-bytes model M1 per channel stack (one HBM round trip per stage)."""
+    H = T // 2 + 1
+    tot = 0
+    for g in geom:
+        tot += 2 * sz * g["xd"] * g["yd"] + (4 * g["xd"] * g["yd"] if flat else 0) + 8 * Zp * T * H \
+            + 2 * 8 * Zp * T * H + 2 * 4 * sz * g["xe"] * g["ye"]
+    return tot
+
+
+def pass_bytes(geom, sz, T, Zp, single_psf=True, flat=True):
+    """Compulsory HBM bytes of each pass of mm_deconv_tiles for one batch (the kernels' algorithmic traffic)."""
+    H = T // 2 + 1
+    n = len(geom)
+    spec = 8 * sz * T * H
+    inp = sum(2 * sz * g["xd"] * g["yd"] + (4 * g["xd"] * g["yd"] if flat else 0) for g in geom)
+    W = 8 * Zp * T * H * (1 if single_psf else n)
+    return {"r2c": inp + n * spec, "x_fwd": 2 * n * spec, "z_wiener": 2 * n * spec + W, "x_inv": 2 * n * spec,
+            "c2r": n * spec + n * 4 * sz * T * T}
+
+
+def cpu_baseline(stack_np, flat_np, psf, cfg, ntiles_per_stack):
+    """Oracle (float32 mode) on a tile sample: corner, two edges, interior of one channel."""
+    from oracle import deconv as odec
+    sample = {(0, 0), (0, 1000), (2500, 1000), (1000, 1000)}
+    zpad = cfg["sz"] - 1
+    W = odec.wiener(psf, cfg["sz"], cfg["tile_size"], zpad, cfg["overlap"], cfg["beta"], mode="fp32")
+    kw = dict(blur_radius=cfg["blur_radius"], threshold=cfg["threshold"], delta=cfg["delta"],
+              delta_fit=cfg["delta_fit"], sigmaZ=cfg["sigmaZ"], sigmaXY=cfg["sigmaXY"], mode="fp32")
+    odec.hyb_spots(stack_np, W, flat_np, cfg["tile_size"], cfg["overlap"], zpad, tile_filter={(1000, 1000)}, **kw)
+    t0 = time.perf_counter()
+    odec.hyb_spots(stack_np, W, flat_np, cfg["tile_size"], cfg["overlap"], zpad, tile_filter=sample, **kw)
+    per_tile = (time.perf_counter() - t0) / len(sample)
+    fov_vox = cfg["channels"] * cfg["sz"] * cfg["sx"] * cfg["sy"]
+    fov_s = per_tile * ntiles_per_stack * cfg["channels"]
+    cores = len(os.sched_getaffinity(0))
+    return {"value": fov_vox / fov_s, "unit": "voxels/s", "cores": cores, "kind": "port",
+            "sample": f"oracle float32 (NumPy/SciPy, scipy.fft workers={cores}) hyb chain on 4 tiles "
+                      f"(corner, 2 edges, interior) of channel 0, {per_tile:.2f} s/tile, extrapolated to "
+                      f"{ntiles_per_stack} tiles x {cfg['channels']} channels"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain per-tile GPU form")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
+              "Tiles and FOVs are independent; see DESIGN.md for how the path would shard.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from mermake_amd import _lib as L
+    from mermake_amd.deconvolver import tile_geometry
+    from mermake_amd.pipeline import HybPipeline
+    from mermake_amd.synth import gaussian_psf, stack_torch
+
+    dev = torch.device("cuda", 0)
+    cfg = CFG
+    shape = (cfg["sz"], cfg["sx"], cfg["sy"])
+    log("generating synthetic FOV", shape, "x", cfg["channels"])
+    stacks, flat = [], None
+    for c in range(cfg["channels"]):
+        s, f = stack_torch(1000 + c, shape, cfg["density"], dev, flat=True)
+        stacks.append(s)
+        flat = f
+    psf = gaussian_psf(cfg["psf_shape"], *cfg["psf_sigma"])
+    torch.cuda.synchronize()
+    log("building Wiener spectrum")
+    pipe = HybPipeline.from_settings(psf, shape, tile_size=cfg["tile_size"], overlap=cfg["overlap"], beta=cfg["beta"],
+                                     blur_radius=cfg["blur_radius"], threshold=cfg["threshold"], delta=cfg["delta"],
+                                     delta_fit=cfg["delta_fit"], sigmaZ=cfg["sigmaZ"], sigmaXY=cfg["sigmaXY"])
+    d = pipe.d
+    T, Zp = d._plan.T, d._plan.Zp
+    geom = tile_geometry(shape, cfg["tile_size"], cfg["overlap"], 1)
+
+    def fov():
+        return [pipe(s, flat) for s in stacks]
+
+    for _ in range(args.warmup):
+        out = fov()
+    torch.cuda.synchronize()
+    log("timing", args.steps, "FOVs")
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        a = time.perf_counter()
+        out = fov()
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - a)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    nspots = [int(o.shape[0]) for o in out]
+    fov_vox = cfg["channels"] * cfg["sz"] * cfg["sx"] * cfg["sy"]
+    value = fov_vox * args.steps / el
+    ms = el / args.steps * 1e3
+
+    # per-pass timing with HIP events on the pipeline's stream (one profiled channel)
+    L.call("mm_deconv_set_profiling", d._plan.h, 1)
+    st = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    pipe(stacks[0], flat)
+    passes = (ctypes.c_float * 5)()
+    L.call("mm_deconv_pass_times", d._plan.h, passes)
+    L.call("mm_deconv_set_profiling", d._plan.h, 0)
+    names = ["r2c", "x_fwd", "z_wiener", "x_inv", "c2r"]
+    pt = dict(zip(names, [float(v) for v in passes]))
+    pb = pass_bytes(geom, cfg["sz"], T, Zp)
+    ev[0].record(st)
+    pipe(stacks[0], flat)
+    ev[1].record(st)
+    torch.cuda.synchronize()
+    stack_ms = ev[0].elapsed_time(ev[1])
+    dom = max(pt, key=pt.get)
+    achieved = pb[dom] / (pt[dom] * 1e-3)
+    m1 = m1_bytes_per_stack(geom, cfg["sz"], T, Zp)
+    m1_frac = (m1 * cfg["channels"] / (ms * 1e-3)) / HBM_PEAK
+
+    plain = None
+    if not args.no_plain:
+        pp = HybPipeline(d, plain=True, blur_radius=cfg["blur_radius"], threshold=cfg["threshold"],
+                         delta=cfg["delta"], delta_fit=cfg["delta_fit"], sigmaZ=cfg["sigmaZ"], sigmaXY=cfg["sigmaXY"])
+        [pp(s, flat) for s in stacks]
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        [pp(s, flat) for s in stacks]
+        torch.cuda.synchronize()
+        pms = (time.perf_counter() - a) * 1e3
+        plain = {"ms_per_fov": pms, "fovs_per_s": 1e3 / pms, "batched_speedup": pms / ms}
+
+    cpu = None
+    if not args.no_cpu:
+        log("CPU baseline (oracle, bounded sample)")
+        cpu = cpu_baseline(stacks[0].cpu().numpy(), flat.cpu().numpy(), psf, cfg, len(geom))
+
+    lat_sorted = sorted(lat)
+    rec = {
+        "metric": "FOV voxels/s (deconv->maxima->fit)", "value": value, "unit": "voxels/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded puncta + Poisson background, flat field)",
+        "config": {"workload": "full hyb FOV: 3 channels x 40x2800x2800 uint16, tile 500 overlap 89 (36 tiles of "
+                               "118x678x678 per channel), 1 PSF 40x64x64, beta 1e-4, blur 30, thr 3600, delta 1, "
+                               "delta_fit 3", "fov_voxels": fov_vox, "spots_per_channel": nspots},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None, "kernel": dom,
+                     "kernel_ms": pt[dom], "kernel_algorithmic_bytes": pb[dom]},
+        "m1_roofline_frac": m1_frac,
+        "pass_ms": pt, "stack_ms": stack_ms,
+        "latency_ms": {"p50": 1e3 * lat_sorted[len(lat) // 2], "p99": 1e3 * lat_sorted[-1]},
+        "plain_gpu": plain,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

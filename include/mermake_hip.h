@@ -1,0 +1,178 @@
+/*
+ * C ABI of libmermake_hip.so: the MI355X (gfx950) implementation of MERMAKE's
+ * image hot path (deconvolution -> blur subtraction -> local maxima -> fit,
+ * plus the FFT correlation used by the drift).
+ *
+ * Conventions
+ *  - Every pointer argument named *_dev, img, res, raw, out..., ws, W is a
+ *    device pointer owned by the caller (the library never frees it).
+ *  - `stream` is a hipStream_t (0 = the null stream); all work is enqueued on
+ *    it and the calls return without synchronising.
+ *  - Every function returns 0 on success, a hipError_t value for a HIP failure,
+ *    or an MM_E* code (>= 1000); mm_last_error() describes the last failure of
+ *    the calling thread.  No C++ exception crosses this boundary.
+ *  - Volumes are C-ordered (z, x, y) with unit stride along y; strides are in
+ *    elements.
+ */
+#ifndef MERMAKE_HIP_H
+#define MERMAKE_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MM_EINVAL 1000
+#define MM_ECAP 1001
+#define MM_ENOMEM 1002
+
+int mm_version(void);
+const char* mm_last_error(void);
+
+/* ------------------------------------------------------------------ blur
+ * Replaces the CUDA kernel box_1d (mermake/blur.cu:1-52) launched by
+ * blur.box_1d (mermake/blur.py:157-188): mean over the window
+ * [i-delta, i+delta] clipped to the axis, summed sequentially from the low to
+ * the high index in float32, then sum/num.  `in` and `out` must not overlap.
+ */
+int mm_box_1d(const float* in, float* out, int size_z, int size_x, int size_y, int delta, int axis, void* stream);
+/* out = res - box_1d(buf, axis): the subtraction of deconvolver.py:160-163
+ * fused into the second blur pass.  `out` may alias `res`, not `buf`. */
+int mm_box_1d_sub(const float* res, const float* buf, float* out, int size_z, int size_x, int size_y, int delta,
+                  int axis, void* stream);
+
+/* ------------------------------------------------------------------ fill
+ * Replaces the CUDA kernel `mirror` (mermake/fill.py:6-52) launched by
+ * fill.reflect (mermake/fill.py:54-111), in place on `arr` (float32, up to 8
+ * dims, element strides).  mode 0 = 'out', 1 = 'in'.
+ */
+int mm_reflect(float* arr, int ndim, const long long* shape, const long long* strides, int axis, int i, int mode,
+               void* stream);
+
+/* ------------------------------------------------------------- deconvolution
+ * One Deconvolver geometry (mermake/deconvolver.py:47-96): stack depth sz,
+ * tile_size, overlap, zpad.  Padded tile T = tile_size + 2*overlap,
+ * Zp = sz + 2*zpad, half spectrum H = T/2 + 1.
+ */
+typedef struct mm_deconv mm_deconv;
+
+/* Tile descriptor, one per overlapped tile (deconvolver.py:186-206 window,
+ * :118-148 placement/reflection).  Layout is part of the ABI (104 bytes). */
+typedef struct mm_tile_desc {
+  const void* img;   /* stack base (uint16 or float32), element (0,0,0) */
+  const float* flat; /* 2D flat field base, or NULL */
+  long long img_sz;  /* stack stride between z planes */
+  long long img_sx;  /* stack stride between x rows */
+  long long flat_sx; /* flat stride between x rows */
+  int X0, Y0;        /* window origin max(x-ov,0), max(y-ov,0) */
+  int xd, yd;        /* window extents */
+  int xs, ys;        /* data start in the padded tile: ov if the tile origin is 0 else 0 */
+  int xfirst, yfirst;/* tile origin is 0 ('in' reflection) */
+  int xe, ye;        /* extents of the yielded views, clipped to T */
+  int psf;           /* Wiener spectrum index */
+  int ox, oy;        /* tile origin (x, y) */
+  int img_f32;       /* 1: float32 stack, 0: uint16 */
+  int pad0, pad1;
+} mm_tile_desc;
+
+/* replaces Deconvolver.__init__'s buffers and spectra set-up (deconvolver.py:48-96) */
+int mm_deconv_create(int sz, int tile_size, int overlap, int zpad, mm_deconv** out);
+int mm_deconv_destroy(mm_deconv* d);
+int mm_deconv_dims(const mm_deconv* d, int* T, int* Zp, int* H);
+size_t mm_deconv_wiener_ws_bytes(const mm_deconv* d);
+/* W = conj(H)/(H conj(H) + beta |L|^2) / (Zp*T*T) as a (Zp, T, H) complex64
+ * half spectrum, from one centred PSF (sz, tile_size, tile_size) float32
+ * (deconvolver.py:60-74; the PSF FFT runs in float64). */
+int mm_deconv_build_wiener(mm_deconv* d, const float* psf_centered, double beta, void* W_out, void* ws,
+                           size_t ws_bytes, void* stream);
+/* bytes of the per-tile half-spectrum workspace (sz*T*H complex64) */
+size_t mm_deconv_spec_bytes(const mm_deconv* d);
+/* Replaces the FFT part of Deconvolver.tile_wise (deconvolver.py:116-157) for
+ * a batch of tiles: res[t] (sz, T, T) float32 = ifftn(fftn(pad_t) * W)[zpad:-zpad].real.
+ * tiles_dev: device array of mm_tile_desc.  W: n_psf spectra of (Zp,T,H).
+ * tiles_per_block: tiles sharing one load of W (use ntiles when all tiles use
+ * the same PSF, 1 otherwise).  spec_ws: ntiles * mm_deconv_spec_bytes. */
+int mm_deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const void* W, int n_psf,
+                    int tiles_per_block, void* spec_ws, float* res, void* stream);
+/* Timing of mm_deconv_tiles with HIP events recorded on its stream around
+ * its five passes (no counterpart in the reference; used by the benchmark):
+ * ms5 = r2c, x forward, z Wiener, x inverse, c2r of the last profiled call. */
+int mm_deconv_set_profiling(mm_deconv* d, int on);
+int mm_deconv_pass_times(mm_deconv* d, float* ms5);
+/* The `raw` views of tile_wise (deconvolver.py:165-170), (sz, T, T) per tile,
+ * zero outside [0,xe) x [0,ye). */
+int mm_deconv_tile_raw(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, float* raw, void* stream);
+/* Deconvolver.apply's stitch (deconvolver.py:179-183) into out (sz, SX, SY). */
+int mm_deconv_stitch(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const float* res, float* out,
+                     long long o_sz, long long o_sx, int SX, int SY, void* stream);
+
+/* DAPI normalisation (mermake/__main__.py:221-222): a /= ||a||_2/sqrt(n) in place;
+ * *rms_out (device float) receives the divisor.  ws: mm_rms_ws_bytes(). */
+size_t mm_rms_ws_bytes(void);
+int mm_rms_normalize(float* a, long long n, void* ws, float* rms_out, void* stream);
+
+/* In-place 3D C2C FFT of a contiguous complex64 (n0,n1,n2) volume over the axes
+ * in axes_mask (bit i = axis i); inverse scales by 1/n per axis.  Replaces the
+ * cupy.fft.fftn/ifftn calls of utils.fftconvolve (mermake/utils.py:10-36). */
+int mm_fft3d_c2c(void* data, int n0, int n1, int n2, int axes_mask, int inverse, void* stream);
+
+/* ------------------------------------------------------------------ maxima
+ * Replaces local_maxima_count + local_maxima (mermake/maxima.cu:1-145) as
+ * driven by find_local_maxima (mermake/maxima.py:30-77): voxels with
+ * sign*v > threshold and sign*v >= every neighbour of the delta sphere
+ * (single reflection at the borders), written as int32 (z, x, y) triplets in
+ * raster order; *count (device) receives the number found (entries beyond
+ * cap are dropped).  ws: mm_maxima_ws_bytes(D*H*W, 1) bytes.
+ */
+size_t mm_maxima_ws_bytes(long long nvox_max, int nvol);
+int mm_local_maxima(const float* img, int D, int H, int W, long long sZ, long long sX, float sign, float threshold,
+                    int delta, int* coords, int cap, int* count, void* ws, size_t ws_bytes, void* stream);
+/* Replaces delta_fit_cross_corr (mermake/maxima.cu:148-341): out8 rows
+ * [zc, xc, yc, background, a, habs, hn, h] for the *count candidates.
+ * raw: float32 (raw_f32=1) or uint16 volume with strides rZ, rX. */
+int mm_delta_fit(const float* img, int D, int H, int W, long long sZ, long long sX, float sign, const void* raw,
+                 int raw_f32, long long rZ, long long rX, const int* coords, const int* count, int cap, int delta_fit,
+                 float sigmaZ, float sigmaXY, float* out8, void* stream);
+
+/* Batched forms over the tile views res[t][:, :xe, :ye] of mm_deconv_tiles:
+ * the hyb loop of mermake/__main__.py:254-265.  coords: ntiles*cap*3,
+ * counts: ntiles, out8: ntiles*cap*8, ws: mm_maxima_ws_bytes(sz*T*T, ntiles). */
+int mm_tiles_maxima(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const float* res, float threshold,
+                    int delta, int* coords, int cap, int* counts, void* ws, size_t ws_bytes, void* stream);
+int mm_tiles_fit(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const float* res, const int* coords,
+                 const int* counts, int cap, int delta_fit, float sigmaZ, float sigmaXY, float* out8, void* stream);
+/* keep-filter on the fitted x,y (ov <= . < view extent - ov), offset to stack
+ * coordinates, stacked in tile order into Xh (out_cap rows); *total (device)
+ * = rows kept.  kept_ws: ntiles ints. (mermake/__main__.py:256-265) */
+int mm_tiles_keep(const mm_tile_desc* tiles_dev, int ntiles, int overlap, const float* fit, const int* counts,
+                  int cap, int* kept_ws, float* Xh, int out_cap, int* total, void* stream);
+
+/* ------------------------------------------------------------------ drift
+ * Building blocks of Aligner (mermake/align.py:37-118) and utils.fftconvolve
+ * (mermake/utils.py:10-36).  Points: n rows of >= 3 float32 (row stride ld).
+ */
+/* Xi (n*3 int32) = round_half_even(X[:, :3] / resc); bounds = min xyz, max xyz */
+int mm_points_round_bounds(const float* X, int n, int ld, float resc, int* Xi, int* bounds, void* stream);
+/* over the points inside [lo, hi] (host int[3] each): out4 = max xyz, count */
+int mm_points_kept_max(const int* Xi, int n, const int* lo, const int* hi, int* out4, void* stream);
+/* writes 1+0i at every kept point of a zeroed complex64 (L0,L1,L2) grid; flip
+ * mirrors each axis within the image size R (im[::-1,::-1,::-1]) */
+int mm_points_raster(const int* Xi, int n, const int* lo, const int* hi, void* buf, int L0, int L1, int L2, int flip,
+                     int R0, int R1, int R2, void* stream);
+/* a *= b elementwise, complex64 */
+int mm_cmul(void* a, const void* b, long long n, void* stream);
+/* first raster index (over [0,F0)x[0,F1)x[0,F2)) of the maximum of rint(Re(buf)) */
+size_t mm_argmax_ws_bytes(void);
+int mm_argmax_rounded(const void* buf, int L0, int L1, int L2, int F0, int F1, int F2, long long* out_i, float* out_v,
+                      void* ws, void* stream);
+/* one refinement round of Aligner.get_Xtzxy (align.py:66-76): nearest reference
+ * point of X - t (float64), keep distances < th, t = mean of the kept
+ * differences when any; *npts = kept count.  ds_ws: n doubles, idx_ws: n ints. */
+int mm_nn_round(const float* X, int n, int ldx, const float* R, int m, int ldr, double* t, double th, double* ds_ws,
+                int* idx_ws, int* npts, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

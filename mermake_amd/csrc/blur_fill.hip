@@ -1,0 +1,179 @@
+// Box blur (reference mermake/blur.cu:1-52) and mirror reflection
+// (reference mermake/fill.py:6-52) for gfx950.
+//
+// The blur keeps the reference arithmetic exactly: each output sums its
+// clipped window sequentially from the lowest to the highest index in float32
+// and divides once by the element count, so results are bit-identical to the
+// reference kernel for the same input.
+#include "common.h"
+#include "mermake_hip.h"
+
+template <bool SUB>
+__global__ __launch_bounds__(256) void k_box_1d(const float* __restrict__ in, const float* __restrict__ res,
+                                                float* out, int sz, int sx, int sy, int delta, int axis) {
+  const long long total = (long long)sz * sx * sy;
+  const long long plane = (long long)sx * sy;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int z = (int)(idx / plane);
+    const int rem = (int)(idx - (long long)z * plane);
+    const int x = rem / sy, y = rem - (rem / sy) * sy;
+    float sum = 0.0f;
+    int num = 0;
+    if (axis == 0) {
+      const int s = max(0, z - delta), e = min(sz - 1, z + delta);
+      for (int i = s; i <= e; ++i) { sum += in[(long long)i * plane + (long long)x * sy + y]; ++num; }
+    } else if (axis == 1) {
+      const int s = max(0, x - delta), e = min(sx - 1, x + delta);
+      const float* p = in + (long long)z * plane + y;
+      for (int i = s; i <= e; ++i) { sum += p[(long long)i * sy]; ++num; }
+    } else {
+      const int s = max(0, y - delta), e = min(sy - 1, y + delta);
+      const float* p = in + (long long)z * plane + (long long)x * sy;
+      for (int i = s; i <= e; ++i) { sum += p[i]; ++num; }
+    }
+    const float m = num > 0 ? sum / (float)num : 0.0f;
+    if (SUB) out[idx] = res[idx] - m;
+    else out[idx] = m;
+  }
+}
+
+static int box_launch(const float* in, const float* res, float* out, int sz, int sx, int sy, int delta, int axis,
+                      bool sub, void* stream) {
+  if (!in || !out || sz < 0 || sx < 0 || sy < 0 || delta < 0 || axis < 0 || axis > 2) {
+    mm_set_error("mm_box_1d: bad argument");
+    return MM_EINVAL;
+  }
+  const long long total = (long long)sz * sx * sy;
+  if (total == 0) return MM_OK;
+  long long blocks = (total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (sub)
+    hipLaunchKernelGGL(k_box_1d<true>, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, in, res, out, sz,
+                       sx, sy, delta, axis);
+  else
+    hipLaunchKernelGGL(k_box_1d<false>, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, in, nullptr, out,
+                       sz, sx, sy, delta, axis);
+  MM_LAUNCH_CHECK();
+  return MM_OK;
+}
+
+extern "C" int mm_box_1d(const float* in, float* out, int size_z, int size_x, int size_y, int delta, int axis,
+                         void* stream) {
+  return box_launch(in, nullptr, out, size_z, size_x, size_y, delta, axis, false, stream);
+}
+
+extern "C" int mm_box_1d_sub(const float* res, const float* buf, float* out, int size_z, int size_x, int size_y,
+                             int delta, int axis, void* stream) {
+  if (!res) { mm_set_error("mm_box_1d_sub: null res"); return MM_EINVAL; }
+  return box_launch(buf, res, out, size_z, size_x, size_y, delta, axis, true, stream);
+}
+
+// ------------------------------------------------------------------ reflect
+struct NdDesc {
+  int ndim;
+  long long shape[8];
+  long long stride[8];
+};
+
+// fill.py:21-40: mode 0 ('out') writes j > i from 2i-j >= 0, mode 1 ('in')
+// writes j < i from 2i-j < n.  Each mode reads one side of i and writes the
+// other, so the in-place update has no race.
+__global__ __launch_bounds__(256) void k_reflect(float* a, NdDesc d, int axis, int i, int mode, long long total) {
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * blockDim.x) {
+    long long tmp = idx, off = 0, j = 0;
+    for (int k = d.ndim - 1; k >= 0; --k) {
+      const long long c = tmp % d.shape[k];
+      tmp /= d.shape[k];
+      off += c * d.stride[k];
+      if (k == axis) j = c;
+    }
+    const long long n = d.shape[axis];
+    const long long src = 2LL * i - j;
+    if (mode == 0) {
+      if (j <= i || src < 0) continue;
+    } else {
+      if (j >= i || src >= n) continue;
+    }
+    a[off] = a[off + (src - j) * d.stride[axis]];
+  }
+}
+
+extern "C" int mm_reflect(float* arr, int ndim, const long long* shape, const long long* strides, int axis, int i,
+                          int mode, void* stream) {
+  if (!arr || ndim < 1 || ndim > 8 || axis < 0 || axis >= ndim || (mode != 0 && mode != 1)) {
+    mm_set_error("mm_reflect: bad argument");
+    return MM_EINVAL;
+  }
+  NdDesc d;
+  d.ndim = ndim;
+  long long total = 1;
+  for (int k = 0; k < 8; ++k) {
+    d.shape[k] = k < ndim ? shape[k] : 1;
+    d.stride[k] = k < ndim ? strides[k] : 0;
+  }
+  for (int k = 0; k < ndim; ++k) total *= shape[k];
+  if (i < 0 || i >= shape[axis]) { mm_set_error("mm_reflect: index out of range"); return MM_EINVAL; }
+  if (total == 0) return MM_OK;
+  long long blocks = (total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_reflect, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, arr, d, axis, i, mode,
+                     total);
+  MM_LAUNCH_CHECK();
+  return MM_OK;
+}
+
+// ------------------------------------------------------------ RMS normalise
+// DAPI path of mermake/__main__.py:221-222: buffer /= ||buffer||_2 / sqrt(N),
+// with the norm accumulated in float64 and the divisor rounded to float32.
+__global__ __launch_bounds__(256) void k_sumsq(const float* __restrict__ a, long long n, double* __restrict__ part) {
+  double s = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const double v = a[i];
+    s += v * v;
+  }
+  __shared__ double sh[256];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(256) void k_rms_final(const double* __restrict__ part, int np, long long n,
+                                                   float* __restrict__ rms) {
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (int k = threadIdx.x; k < np; k += blockDim.x) s += part[k];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *rms = (float)(sqrt(sh[0]) / sqrt((double)n));
+}
+
+__global__ __launch_bounds__(256) void k_div_by(float* __restrict__ a, long long n, const float* __restrict__ d) {
+  const float q = *d;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    a[i] = a[i] / q;
+}
+
+extern "C" size_t mm_rms_ws_bytes(void) { return 1024 * sizeof(double) + 256; }
+
+extern "C" int mm_rms_normalize(float* a, long long n, void* ws, float* rms_out, void* stream) {
+  if (!a || !ws || !rms_out || n < 1) { mm_set_error("mm_rms_normalize: bad argument"); return MM_EINVAL; }
+  hipStream_t st = (hipStream_t)stream;
+  double* part = (double*)ws;
+  hipLaunchKernelGGL(k_sumsq, dim3(1024), dim3(256), 0, st, a, n, part);
+  MM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_rms_final, dim3(1), dim3(256), 0, st, part, 1024, n, rms_out);
+  MM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_div_by, dim3(4096), dim3(256), 0, st, a, n, rms_out);
+  MM_LAUNCH_CHECK();
+  return MM_OK;
+}

@@ -1,0 +1,78 @@
+// Device-side descriptors shared by the deconvolution, maxima and fit kernels.
+#pragma once
+#include "fft_engine.h"
+
+// One overlapped tile of a z-stack (reference: deconvolver.py:186-206 for the
+// window, :118-148 for the placement and reflections).  Filled on the host.
+struct mm_tile_dev {
+  const void* img;          // stack base (uint16 or float32), element (0,0,0)
+  const float* flat;        // flat-field base (2D, element (0,0)) or nullptr
+  long long img_sz;         // stack stride between z planes (elements)
+  long long img_sx;         // stack stride between x rows (elements); y stride is 1
+  long long flat_sx;        // flat stride between x rows (elements)
+  int X0, Y0;               // window origin in the stack
+  int xd, yd;               // data extents of the window
+  int xs, ys;               // where the data starts in the padded tile (overlap or 0)
+  int xfirst, yfirst;       // tile origin is 0 along x / y ('in' reflection)
+  int xe, ye;               // extents of the yielded view (clipped to T)
+  int psf;                  // index of the Wiener spectrum
+  int ox, oy;               // tile origin in the stack (for spot offsets)
+  int img_f32;              // 1: float32 stack, 0: uint16 stack
+  int pad0, pad1;
+};
+
+// Closed form of one axis' reflection (deconvolver.py:131-144): padded index j
+// -> data index, or -1 for a position that stays zero.
+__device__ __forceinline__ int pad_map(int j, int s, int d, int first, int T) {
+  const int rel = j - s;
+  if (rel >= 0 && rel < d) return rel;
+  if (first) {                      // reflect(ov, 'in'): j < s takes 2s - j if < s + d
+    if (j < s) {
+      const int r2 = s - j;
+      return r2 < d ? r2 : -1;
+    }
+    return -1;
+  }
+  if (d < T && j > d - 1) {         // reflect(d-1, 'out'): j > d-1 takes 2(d-1) - j if >= 0
+    const int src = 2 * (d - 1) - j;
+    return src >= 0 ? src : -1;
+  }
+  return -1;
+}
+
+// Value of the flat-divided padded tile at data position (z, r, c) of the window.
+__device__ __forceinline__ float tile_src(const mm_tile_dev& t, int z, int r, int c) {
+  const long long o = (long long)z * t.img_sz + (long long)(t.X0 + r) * t.img_sx + (t.Y0 + c);
+  float v = t.img_f32 ? ((const float*)t.img)[o] : (float)((const uint16_t*)t.img)[o];
+  if (t.flat) v = v / t.flat[(long long)(t.X0 + r) * t.flat_sx + (t.Y0 + c)];
+  return v;
+}
+
+// `raw` view value at (z, x, y) of tile_pad[zpad:-zpad, :xend, :yend] after the
+// flat re-multiply of deconvolver.py:166-168: (v/f)*f on the data mask,
+// flat-divided reflected values in the pads, 0 where nothing was reflected.
+__device__ __forceinline__ float tile_raw(const mm_tile_dev& t, int T, int z, int x, int y) {
+  const int r = pad_map(x, t.xs, t.xd, t.xfirst, T);
+  const int c = pad_map(y, t.ys, t.yd, t.yfirst, T);
+  if (r < 0 || c < 0) return 0.0f;
+  const long long o = (long long)z * t.img_sz + (long long)(t.X0 + r) * t.img_sx + (t.Y0 + c);
+  float v = t.img_f32 ? ((const float*)t.img)[o] : (float)((const uint16_t*)t.img)[o];
+  if (t.flat) {
+    const float f = t.flat[(long long)(t.X0 + r) * t.flat_sx + (t.Y0 + c)];
+    v = v / f;
+    const bool inmask = (x - t.xs) >= 0 && (x - t.xs) < t.xd && (y - t.ys) >= 0 && (y - t.ys) < t.yd;
+    if (inmask) v = v * f;
+  }
+  return v;
+}
+
+// Plan of one Deconvolver geometry (host object behind the opaque mm_deconv*).
+struct mm_deconv {
+  int sz, ts, ov, zpad, T, Zp, H;
+  int* zmap_dev;
+  FftPlanDev<float2> pT, pZ;
+  FftPlanDev<double2> pT64, pZ64;
+  int gT_r2c, gT_x, gZ, gT64_r2c, gT64_x, gZ64;
+  int prof;                 // record events around the five passes of mm_deconv_tiles
+  hipEvent_t ev[6];
+};

@@ -1,0 +1,554 @@
+// 3D local maxima with deterministic raster-order compaction, delta_fit
+// refinement, and the hyb keep-filter (gfx950).
+//
+// Reference: mermake/maxima.cu:1-145 (maxima test), :148-341 (fit),
+// mermake/maxima.py:17-98 (driver), mermake/__main__.py:254-265 (keep-filter).
+//
+// Maxima are found in three passes that never use atomics, so the candidate
+// list is in raster order and identical from run to run:
+//   k_max_mark     one bit per voxel (64-voxel ballot words) + per-block counts
+//   k_max_scan     exclusive scan of the block counts of each volume
+//   k_max_compact  popcount ranks -> (z, x, y) at block offset + rank
+// The fit runs one wavefront per candidate: the lanes sample the delta_fit
+// sphere, and the statistics are wave reductions (two-pass mean/variance).
+#include "deconv.h"
+#include "mermake_hip.h"
+
+#include <mutex>
+#include <vector>
+
+struct VolDesc {
+  const float* p;
+  int D, H, W;
+  long long sZ, sX;
+};
+
+struct OneVol {
+  VolDesc d;
+  __device__ __forceinline__ VolDesc operator()(int) const { return d; }
+};
+
+struct TileVols {  // view res[:, :xe, :ye] of every tile (deconvolver.py:170)
+  const mm_tile_dev* tiles;
+  const float* res;
+  int sz, T;
+  __device__ __forceinline__ VolDesc operator()(int v) const {
+    VolDesc d;
+    d.p = res + (long long)v * sz * T * T;
+    d.D = sz;
+    d.H = tiles[v].xe;
+    d.W = tiles[v].ye;
+    d.sZ = (long long)T * T;
+    d.sX = T;
+    return d;
+  }
+};
+
+#define MX_BLOCK_VOX 1024
+#define MX_WORDS 16
+
+__device__ __forceinline__ int refl1(int n, int N) {  // maxima.cu:44-52
+  if (n < 0) return -n;
+  if (n >= N) return 2 * N - n - 2;
+  return n;
+}
+
+__device__ __forceinline__ bool is_max(const VolDesc& V, long long idx, float sign, float thr, int delta) {
+  const long long hw = (long long)V.H * V.W;
+  const int z = (int)(idx / hw);
+  const int rem = (int)(idx - (long long)z * hw);
+  const int x = rem / V.W, y = rem - (rem / V.W) * V.W;
+  const float c = sign * V.p[z * V.sZ + x * V.sX + y];
+  if (c <= thr) return false;  // maxima.cu:19 (a NaN threshold rejects nothing)
+  const int d2 = delta * delta;
+  for (int dz = -delta; dz <= delta; ++dz) {
+    const int nz = refl1(z + dz, V.D);
+    const bool zok = nz >= 0 && nz < V.D;
+    for (int dx = -delta; dx <= delta; ++dx) {
+      const int nx = refl1(x + dx, V.H);
+      const bool xok = nx >= 0 && nx < V.H;
+      for (int dy = -delta; dy <= delta; ++dy) {
+        if (dz == 0 && dx == 0 && dy == 0) continue;
+        if (dz * dz + dx * dx + dy * dy > d2) continue;
+        const int ny = refl1(y + dy, V.W);
+        // neighbours still outside after the single reflection (axes shorter
+        // than delta+1) are skipped instead of read out of bounds
+        if (!zok || !xok || ny < 0 || ny >= V.W) continue;
+        if (c < sign * V.p[nz * V.sZ + nx * V.sX + ny]) return false;
+      }
+    }
+  }
+  return true;
+}
+
+template <class VS>
+__global__ __launch_bounds__(256) void k_max_mark(VS vs, float sign, float thr, int delta,
+                                                  unsigned long long* __restrict__ masks, int* __restrict__ bcnt,
+                                                  int nblk) {
+  const int v = blockIdx.y;
+  const VolDesc V = vs(v);
+  const long long nvox = (long long)V.D * V.H * V.W;
+  const long long start = (long long)blockIdx.x * MX_BLOCK_VOX;
+  unsigned long long* mw = masks + ((long long)v * nblk + blockIdx.x) * MX_WORDS;
+  __shared__ int wcnt[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int cnt = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const long long idx = start + i * 256 + threadIdx.x;
+    bool m = false;
+    if (idx < nvox) m = is_max(V, idx, sign, thr, delta);
+    const unsigned long long b = __ballot(m);
+    if (lane == 0) mw[i * 4 + wave] = b;
+    cnt += __popcll(b);
+  }
+  if (lane == 0) wcnt[wave] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) bcnt[(long long)v * nblk + blockIdx.x] = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+}
+
+// exclusive scan of the block counts of volume blockIdx.x; total -> counts[v]
+__global__ __launch_bounds__(1024) void k_max_scan(int* __restrict__ bcnt, int nblk, int* __restrict__ counts) {
+  const int v = blockIdx.x;
+  int* c = bcnt + (long long)v * nblk;
+  __shared__ int wsum[16];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  int carry = 0;
+  for (int base = 0; base < nblk; base += blockDim.x) {
+    const int i = base + threadIdx.x;
+    const int x = i < nblk ? c[i] : 0;
+    int s = x;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int y = __shfl_up(s, off, 64);
+      if (lane >= off) s += y;
+    }
+    if (lane == 63) wsum[wave] = s;
+    __syncthreads();
+    int wpre = 0, tot = 0;
+    for (int w = 0; w < nw; ++w) {
+      if (w < wave) wpre += wsum[w];
+      tot += wsum[w];
+    }
+    if (i < nblk) c[i] = carry + wpre + s - x;
+    carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) counts[v] = carry;
+}
+
+template <class VS>
+__global__ __launch_bounds__(256) void k_max_compact(VS vs, const unsigned long long* __restrict__ masks,
+                                                     const int* __restrict__ boff, int nblk, int* __restrict__ coords,
+                                                     int cap) {
+  const int v = blockIdx.y;
+  const VolDesc V = vs(v);
+  __shared__ unsigned long long w[MX_WORDS];
+  __shared__ int pre[MX_WORDS];
+  const unsigned long long* mw = masks + ((long long)v * nblk + blockIdx.x) * MX_WORDS;
+  if (threadIdx.x < MX_WORDS) w[threadIdx.x] = mw[threadIdx.x];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int s = 0;
+    for (int k = 0; k < MX_WORDS; ++k) { pre[k] = s; s += __popcll(w[k]); }
+  }
+  __syncthreads();
+  const int base = boff[(long long)v * nblk + blockIdx.x];
+  const long long start = (long long)blockIdx.x * MX_BLOCK_VOX;
+  const long long hw = (long long)V.H * V.W;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int k = wave; k < MX_WORDS; k += 4) {
+    const unsigned long long m = w[k];
+    if ((m >> lane) & 1ull) {
+      const int pos = base + pre[k] + __popcll(m & ((1ull << lane) - 1ull));
+      if (pos < cap) {
+        const long long idx = start + 64LL * k + lane;
+        const int z = (int)(idx / hw);
+        const int rem = (int)(idx - (long long)z * hw);
+        int* o = coords + ((long long)v * cap + pos) * 3;
+        o[0] = z;
+        o[1] = rem / V.W;
+        o[2] = rem - (rem / V.W) * V.W;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------- fit
+struct RawPlain {
+  const void* p;
+  int f32;
+  long long sZ, sX;
+  struct Ctx {};
+  __device__ __forceinline__ Ctx ctx(int) const { return Ctx{}; }
+  __device__ __forceinline__ float at(const Ctx&, int z, int x, int y) const {
+    const long long o = z * sZ + x * sX + y;
+    return f32 ? ((const float*)p)[o] : (float)((const uint16_t*)p)[o];
+  }
+};
+
+struct RawTile {  // tile_pad[zpad:-zpad, :xend, :yend] after the flat re-multiply
+  const mm_tile_dev* tiles;
+  int T;
+  typedef mm_tile_dev Ctx;
+  __device__ __forceinline__ Ctx ctx(int v) const { return tiles[v]; }
+  __device__ __forceinline__ float at(const Ctx& t, int z, int x, int y) const { return tile_raw(t, T, z, x, y); }
+};
+
+__device__ __forceinline__ float wsum(float v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ int wsumi(int v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ float wmin(float v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+#define MX_MAX_KERNEL_POINTS 600
+
+// Sample k of the sphere for a candidate: reflected coordinates and validity
+// (maxima.cu:190-206).  The 600-sample cap of maxima.cu:194 counts valid
+// samples in loop order; `run` is the number of valid samples before this
+// 64-sample chunk.
+struct Sample {
+  bool ok;
+  int dz, dx, dy;
+  float v, r, g;
+};
+
+template <class RS>
+__device__ __forceinline__ Sample get_sample(const VolDesc& V, float sign, const RS& raw,
+                                             const typename RS::Ctx& rc, const int* offs, int P, int k, int z0,
+                                             int x0, int y0, float isz, float isxy, int& run) {
+  Sample s;
+  s.ok = false;
+  s.dz = s.dx = s.dy = 0;
+  s.v = s.r = s.g = 0.0f;
+  int zz = 0, xx = 0, yy = 0;
+  bool valid = false;
+  if (k < P) {
+    const int o = offs[k];
+    s.dz = (o >> 16) - 128;
+    s.dx = ((o >> 8) & 255) - 128;
+    s.dy = (o & 255) - 128;
+    zz = refl1(z0 + s.dz, V.D);
+    xx = refl1(x0 + s.dx, V.H);
+    yy = refl1(y0 + s.dy, V.W);
+    valid = zz >= 0 && zz < V.D && xx >= 0 && xx < V.H && yy >= 0 && yy < V.W;
+  }
+  const unsigned long long b = __ballot(valid);
+  const int lane = threadIdx.x & 63;
+  const int before = run + __popcll(b & ((1ull << lane) - 1ull));
+  run += __popcll(b);
+  if (valid && before < MX_MAX_KERNEL_POINTS) {
+    s.ok = true;
+    s.v = sign * V.p[zz * V.sZ + xx * V.sX + yy];
+    s.r = raw.at(rc, zz, xx, yy);
+    s.g = expf(-((float)(s.dz * s.dz) * isz + (float)(s.dx * s.dx) * isxy + (float)(s.dy * s.dy) * isxy));
+  }
+  return s;
+}
+
+template <class VS, class RS>
+__global__ __launch_bounds__(256) void k_fit(VS vs, float sign, RS raw, const int* __restrict__ coords,
+                                             const int* __restrict__ counts, int cap, const int* __restrict__ offs,
+                                             int P, float sigmaZ, float sigmaXY, float* __restrict__ out) {
+  const int v = blockIdx.y;
+  const VolDesc V = vs(v);
+  const typename RS::Ctx rc = raw.ctx(v);
+  const int n = min(counts[v], cap);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // reference expression dz*dz / (2*sigmaZ*sigmaZ) as a multiply by the reciprocal
+  const float isz = 1.0f / (2.0f * sigmaZ * sigmaZ);
+  const float isxy = 1.0f / (2.0f * sigmaXY * sigmaXY);
+  const int nch = (P + 63) / 64;
+  for (int i = blockIdx.x * 4 + wave; i < n; i += gridDim.x * 4) {
+    const int* c = coords + ((long long)v * cap + i) * 3;
+    const int z0 = c[0], x0 = c[1], y0 = c[2];
+    // pass 1: count, sum G, min
+    int cnt = 0, run = 0;
+    float sg = 0.f, mn = 3.0e38f;
+    for (int ch = 0; ch < nch; ++ch) {
+      const Sample s = get_sample(V, sign, raw, rc, offs, P, ch * 64 + lane, z0, x0, y0, isz, isxy, run);
+      if (s.ok) { cnt += 1; sg += s.g; mn = fminf(mn, s.v); }
+    }
+    cnt = wsumi(cnt);
+    float* o = out + ((long long)v * cap + i) * 8;
+    if (cnt == 0) {  // maxima.cu:233 leaves the zero-initialised row
+      if (lane < 8) o[lane] = 0.0f;
+      continue;
+    }
+    sg = wsum(sg);
+    const float bg = wmin(mn);
+    const float fn = (float)cnt;
+    const float mg = sg / fn;
+    // pass 2: var G, sums of s and raw, centroid moments
+    float vg = 0.f, ss = 0.f, sr = 0.f, mz = 0.f, mx = 0.f, my = 0.f;
+    run = 0;
+    for (int ch = 0; ch < nch; ++ch) {
+      const Sample s = get_sample(V, sign, raw, rc, offs, P, ch * 64 + lane, z0, x0, y0, isz, isxy, run);
+      if (s.ok) {
+        const float dg = s.g - mg;
+        vg += dg * dg;
+        const float w = s.v - bg;
+        ss += w;
+        sr += s.r;
+        mz += (float)s.dz * w;
+        mx += (float)s.dx * w;
+        my += (float)s.dy * w;
+      }
+    }
+    vg = wsum(vg); ss = wsum(ss); sr = wsum(sr);
+    mz = wsum(mz); mx = wsum(mx); my = wsum(my);
+    const float ms = ss / fn, mr = sr / fn;
+    // pass 3: variances of s and raw
+    float vs2 = 0.f, vr2 = 0.f;
+    run = 0;
+    for (int ch = 0; ch < nch; ++ch) {
+      const Sample s = get_sample(V, sign, raw, rc, offs, P, ch * 64 + lane, z0, x0, y0, isz, isxy, run);
+      if (s.ok) {
+        const float a = (s.v - bg) - ms, b = s.r - mr;
+        vs2 += a * a;
+        vr2 += b * b;
+      }
+    }
+    vs2 = wsum(vs2); vr2 = wsum(vr2);
+    const float stdg = sqrtf(vg / fn);  // no guard: 0/0 -> NaN as in maxima.cu:249-254
+    const float sds = sqrtf(vs2 / fn), sdr = sqrtf(vr2 / fn);
+    const float ds = sds > 0.f ? sds : 1.0f, dr = sdr > 0.f ? sdr : 1.0f;
+    // pass 4: correlations
+    float hn = 0.f, aa = 0.f;
+    run = 0;
+    for (int ch = 0; ch < nch; ++ch) {
+      const Sample s = get_sample(V, sign, raw, rc, offs, P, ch * 64 + lane, z0, x0, y0, isz, isxy, run);
+      if (s.ok) {
+        const float ng = (s.g - mg) / stdg;
+        hn += ((s.v - bg) - ms) / ds * ng;
+        aa += (s.r - mr) / dr * ng;
+      }
+    }
+    hn = wsum(hn); aa = wsum(aa);
+    if (lane == 0) {
+      float zc = (float)z0, xc = (float)x0, yc = (float)y0;
+      if (ss > 0.f) {  // centroid over the unreflected coordinates (maxima.cu:308-325)
+        zc = (float)z0 + mz / ss;
+        xc = (float)x0 + mx / ss;
+        yc = (float)y0 + my / ss;
+      }
+      const long long ci = z0 * V.sZ + x0 * V.sX + y0;
+      o[0] = zc;
+      o[1] = xc;
+      o[2] = yc;
+      o[3] = bg;
+      o[4] = aa / fn;
+      o[5] = raw.at(rc, z0, x0, y0);
+      o[6] = hn / fn;
+      o[7] = sign * V.p[ci];
+    }
+  }
+}
+
+// ------------------------------------------------------------- keep filter
+// __main__.py:256-261: keep rows with ov <= fitted x,y < view extent - ov,
+// then shift to stack coordinates (float32 adds), stacked in tile order.
+__device__ __forceinline__ bool keep_row(const float* r, float lo, float hx, float hy) {
+  return r[1] >= lo && r[2] >= lo && r[1] < hx && r[2] < hy;
+}
+
+__global__ __launch_bounds__(256) void k_keep_count(const float* __restrict__ fit, const int* __restrict__ counts,
+                                                    int cap, const mm_tile_dev* __restrict__ tiles, int ov,
+                                                    int* __restrict__ kept) {
+  const int v = blockIdx.x;
+  const int n = min(counts[v], cap);
+  const float lo = (float)ov, hx = (float)(tiles[v].xe - ov), hy = (float)(tiles[v].ye - ov);
+  int c = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) c += keep_row(fit + ((long long)v * cap + i) * 8, lo, hx, hy);
+  __shared__ int red[256];
+  red[threadIdx.x] = c;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) kept[v] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_keep_write(const float* __restrict__ fit, const int* __restrict__ counts,
+                                                    int cap, const mm_tile_dev* __restrict__ tiles, int ov,
+                                                    const int* __restrict__ kept, int ntiles, float* __restrict__ Xh,
+                                                    int out_cap, int* __restrict__ total) {
+  const int v = blockIdx.x;
+  __shared__ int base_s;
+  __shared__ int wc[4];
+  if (threadIdx.x == 0) {
+    int b = 0;
+    for (int u = 0; u < v; ++u) b += kept[u];
+    base_s = b;
+    if (v == ntiles - 1) *total = b + kept[v];
+  }
+  __syncthreads();
+  int base = base_s;
+  const int n = min(counts[v], cap);
+  const mm_tile_dev td = tiles[v];
+  const float lo = (float)ov, hx = (float)(td.xe - ov), hy = (float)(td.ye - ov);
+  const float offx = (float)(td.ox - ov), offy = (float)(td.oy - ov);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int i0 = 0; i0 < n; i0 += blockDim.x) {
+    const int i = i0 + threadIdx.x;
+    const float* r = fit + ((long long)v * cap + i) * 8;
+    const bool k = i < n && keep_row(r, lo, hx, hy);
+    const unsigned long long b = __ballot(k);
+    if (lane == 0) wc[wave] = __popcll(b);
+    __syncthreads();
+    int pre = 0, tot = 0;
+    for (int w = 0; w < 4; ++w) { if (w < wave) pre += wc[w]; tot += wc[w]; }
+    if (k) {
+      const int pos = base + pre + __popcll(b & ((1ull << lane) - 1ull));
+      if (pos < out_cap) {
+        float* o = Xh + (long long)pos * 8;
+        o[0] = r[0];
+        o[1] = r[1] + offx;
+        o[2] = r[2] + offy;
+        for (int q = 3; q < 8; ++q) o[q] = r[q];
+      }
+    }
+    base += tot;
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ host
+static inline int mx_nblk(long long nvox) { return (int)((nvox + MX_BLOCK_VOX - 1) / MX_BLOCK_VOX); }
+
+extern "C" size_t mm_maxima_ws_bytes(long long nvox_max, int nvol) {
+  const long long nb = mx_nblk(nvox_max);
+  return (size_t)nvol * nb * (MX_WORDS * 8 + 4) + 256;
+}
+
+// sphere offsets of delta_fit in dz -> dx -> dy order, packed (dz+128)<<16 | (dx+128)<<8 | (dy+128)
+static int sphere_table(int dfit, int** dev_out, int* P_out) {
+  static std::mutex mu;
+  static int* cache[32] = {nullptr};
+  static int cacheP[32] = {0};
+  static int cache_dev[32] = {0};
+  if (dfit < 0 || dfit > 31) { mm_set_error("delta_fit %d outside [0, 31]", dfit); return MM_EINVAL; }
+  int dev = 0;
+  MM_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(mu);
+  if (!cache[dfit] || cache_dev[dfit] != dev) {
+    std::vector<int> h;
+    for (int dz = -dfit; dz <= dfit; ++dz)
+      for (int dx = -dfit; dx <= dfit; ++dx)
+        for (int dy = -dfit; dy <= dfit; ++dy)
+          if (dz * dz + dx * dx + dy * dy <= dfit * dfit)
+            h.push_back(((dz + 128) << 16) | ((dx + 128) << 8) | (dy + 128));
+    int* d = nullptr;
+    MM_CHECK(hipMalloc(&d, sizeof(int) * h.size()));
+    MM_CHECK(hipMemcpy(d, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice));
+    cache[dfit] = d;
+    cacheP[dfit] = (int)h.size();
+    cache_dev[dfit] = dev;
+  }
+  *dev_out = cache[dfit];
+  *P_out = cacheP[dfit];
+  return MM_OK;
+}
+
+template <class VS>
+static int run_maxima(VS vs, int nvol, long long nvox_max, float sign, float thr, int delta, int* coords, int cap,
+                      int* counts, void* ws, size_t ws_bytes, hipStream_t st) {
+  if (delta < 0 || delta > 31) { mm_set_error("delta %d outside [0, 31]", delta); return MM_EINVAL; }
+  const int nb = mx_nblk(nvox_max);
+  if (nb == 0 || nvol == 0) {
+    if (nvol) MM_CHECK(hipMemsetAsync(counts, 0, sizeof(int) * nvol, st));
+    return MM_OK;
+  }
+  if (ws_bytes < mm_maxima_ws_bytes(nvox_max, nvol)) { mm_set_error("maxima workspace too small"); return MM_EINVAL; }
+  unsigned long long* masks = (unsigned long long*)ws;
+  int* bcnt = (int*)(masks + (size_t)nvol * nb * MX_WORDS);
+  hipLaunchKernelGGL((k_max_mark<VS>), dim3(nb, nvol), dim3(256), 0, st, vs, sign, thr, delta, masks, bcnt, nb);
+  MM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_max_scan, dim3(nvol), dim3(1024), 0, st, bcnt, nb, counts);
+  MM_LAUNCH_CHECK();
+  hipLaunchKernelGGL((k_max_compact<VS>), dim3(nb, nvol), dim3(256), 0, st, vs, masks, bcnt, nb, coords, cap);
+  MM_LAUNCH_CHECK();
+  return MM_OK;
+}
+
+extern "C" int mm_local_maxima(const float* img, int D, int H, int W, long long sZ, long long sX, float sign,
+                               float threshold, int delta, int* coords, int cap, int* count, void* ws, size_t ws_bytes,
+                               void* stream) {
+  if (!img || !coords || !count || D < 0 || H < 0 || W < 0) { mm_set_error("mm_local_maxima: bad argument"); return MM_EINVAL; }
+  OneVol vs{VolDesc{img, D, H, W, sZ, sX}};
+  return run_maxima(vs, 1, (long long)D * H * W, sign, threshold, delta, coords, cap, count, ws, ws_bytes,
+                    (hipStream_t)stream);
+}
+
+extern "C" int mm_delta_fit(const float* img, int D, int H, int W, long long sZ, long long sX, float sign,
+                            const void* raw, int raw_f32, long long rZ, long long rX, const int* coords,
+                            const int* count, int cap, int delta_fit, float sigmaZ, float sigmaXY, float* out8,
+                            void* stream) {
+  if (!img || !raw || !coords || !count || !out8) { mm_set_error("mm_delta_fit: null argument"); return MM_EINVAL; }
+  int* offs = nullptr;
+  int P = 0;
+  int rc = sphere_table(delta_fit, &offs, &P);
+  if (rc) return rc;
+  OneVol vs{VolDesc{img, D, H, W, sZ, sX}};
+  RawPlain rs{raw, raw_f32, rZ, rX};
+  int blocks = (cap + 3) / 4;
+  if (blocks > 2048) blocks = 2048;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL((k_fit<OneVol, RawPlain>), dim3(blocks, 1), dim3(256), 0, (hipStream_t)stream, vs, sign, rs,
+                     coords, count, cap, offs, P, sigmaZ, sigmaXY, out8);
+  MM_LAUNCH_CHECK();
+  return MM_OK;
+}
+
+extern "C" int mm_tiles_maxima(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const float* res,
+                               float threshold, int delta, int* coords, int cap, int* counts, void* ws,
+                               size_t ws_bytes, void* stream) {
+  if (!d || !tiles_dev || !res) { mm_set_error("mm_tiles_maxima: bad argument"); return MM_EINVAL; }
+  TileVols vs{(const mm_tile_dev*)tiles_dev, res, d->sz, d->T};
+  return run_maxima(vs, ntiles, (long long)d->sz * d->T * d->T, 1.0f, threshold, delta, coords, cap, counts, ws, ws_bytes,
+                    (hipStream_t)stream);
+}
+
+extern "C" int mm_tiles_fit(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const float* res,
+                            const int* coords, const int* counts, int cap, int delta_fit, float sigmaZ, float sigmaXY,
+                            float* out8, void* stream) {
+  if (!d || !tiles_dev || !res) { mm_set_error("mm_tiles_fit: bad argument"); return MM_EINVAL; }
+  if (ntiles == 0) return MM_OK;
+  const int T = d->T;
+  int* offs = nullptr;
+  int P = 0;
+  int rc = sphere_table(delta_fit, &offs, &P);
+  if (rc) return rc;
+  TileVols vs{(const mm_tile_dev*)tiles_dev, res, d->sz, T};
+  RawTile rs{(const mm_tile_dev*)tiles_dev, T};
+  int blocks = (cap + 3) / 4;
+  if (blocks > 256) blocks = 256;
+  hipLaunchKernelGGL((k_fit<TileVols, RawTile>), dim3(blocks, ntiles), dim3(256), 0, (hipStream_t)stream, vs, 1.0f,
+                     rs, coords, counts, cap, offs, P, sigmaZ, sigmaXY, out8);
+  MM_LAUNCH_CHECK();
+  return MM_OK;
+}
+
+extern "C" int mm_tiles_keep(const mm_tile_desc* tiles_dev, int ntiles, int overlap, const float* fit,
+                             const int* counts, int cap, int* kept_ws, float* Xh, int out_cap, int* total,
+                             void* stream) {
+  if (ntiles == 0) return MM_OK;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_keep_count, dim3(ntiles), dim3(256), 0, st, fit, counts, cap, (const mm_tile_dev*)tiles_dev,
+                     overlap, kept_ws);
+  MM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_keep_write, dim3(ntiles), dim3(256), 0, st, fit, counts, cap, (const mm_tile_dev*)tiles_dev,
+                     overlap, kept_ws, ntiles, Xh, out_cap, total);
+  MM_LAUNCH_CHECK();
+  return MM_OK;
+}

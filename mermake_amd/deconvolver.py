@@ -1,0 +1,286 @@
+"""Tiled Wiener deconvolution (reference ``mermake/deconvolver.py``), running the
+``mm_deconv_*`` kernels.
+
+``Deconvolver`` keeps the reference constructor and methods
+(``deconvolver.py:47-236``).  Differences a caller can observe:
+
+* ``psf_fft`` holds the half spectra ``(n_psf, Zp, T, T//2+1)`` (complex64),
+  already divided by ``Zp*T*T`` (the ``ifftn`` normalisation), instead of the
+  full ``(n_psf, Zp, T, T)`` spectra.  The PSF FFT behind it runs in float64.
+* ``tile_wise`` computes tiles in batches; the yielded views stay valid until
+  the generator computes the next batch (the reference overwrote them at every
+  step).  ``tile_pad`` / ``tile_res`` are shape-only (``meta``) tensors.
+* ``zpad < 1`` or ``overlap < 1`` raise ``ValueError`` in ``tile_wise``/``apply``
+  (the reference produces empty slices there).
+* a float64 flat field is rounded to float32 before the divide.
+"""
+import ctypes
+import os
+from itertools import cycle
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from ._dev import as_device, default_device, unit_last_stride
+
+
+def laplacian_3d(shape, device=None):
+    """``deconvolver.py:24-36``: float16; 6 at the centre, -1 at z+-1, y+-1 and only x-1."""
+    lap = torch.zeros(shape, dtype=torch.float16, device=device or default_device())
+    zc, yc, xc = shape[0] // 2, shape[1] // 2, shape[2] // 2
+    lap[zc, yc, xc] = 6
+    lap[zc - 1, yc, xc] = -1
+    lap[zc + 1, yc, xc] = -1
+    lap[zc, yc - 1, xc] = -1
+    lap[zc, yc + 1, xc] = -1
+    lap[zc, yc, xc - 1] = -1
+    return lap
+
+
+def fftn_c2c(x, inverse=False):
+    """3D complex64 FFT of a (n0, n1, n2) tensor through ``mm_fft3d_c2c`` (new tensor)."""
+    y = x.to(torch.complex64).contiguous().clone()
+    n0, n1, n2 = y.shape
+    L.call("mm_fft3d_c2c", L.ptr(y), n0, n1, n2, 7, 1 if inverse else 0, L.stream_ptr(y.device))
+    return y
+
+
+def batch_laplacian_fft(batch_size, shape):
+    """``deconvolver.py:38-44``: ``fftn(laplacian_3d(shape))[None]`` (complex64)."""
+    return fftn_c2c(laplacian_3d(shape))[None]
+
+
+class _Plan:
+    def __init__(self, sz, tile_size, overlap, zpad):
+        h = ctypes.c_void_p()
+        L.call("mm_deconv_create", sz, tile_size, overlap, zpad, ctypes.byref(h))
+        self.h = h
+        T, Zp, H = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        L.call("mm_deconv_dims", h, ctypes.byref(T), ctypes.byref(Zp), ctypes.byref(H))
+        self.T, self.Zp, self.H = T.value, Zp.value, H.value
+        self.spec_bytes = L.lib().mm_deconv_spec_bytes(h)
+
+    def __del__(self):
+        try:
+            if self.h:
+                L.lib().mm_deconv_destroy(self.h)
+        except Exception:
+            pass
+
+
+def tile_geometry(shape, tile_size, overlap, n_psf=1):
+    """Tiles of ``Deconvolver.tiled`` (``deconvolver.py:186-206``) with their padding
+    parameters (``:117-144``), truncated as ``zip`` does when several PSFs are given."""
+    sz, sx, sy = shape
+    T = tile_size + 2 * overlap
+    out = []
+    k = 0
+    for x in range(0, sx, tile_size):
+        for y in range(0, sy, tile_size):
+            if n_psf > 1 and k >= n_psf:
+                return out
+            x0, x1 = max(x - overlap, 0), min(x + tile_size + overlap, sx)
+            y0, y1 = max(y - overlap, 0), min(y + tile_size + overlap, sy)
+            xd, yd = x1 - x0, y1 - y0
+            xend = xd + (overlap if (x == 0 or xd < T) else 0)
+            yend = yd + (overlap if (y == 0 or yd < T) else 0)
+            out.append(dict(x=x, y=y, X0=x0, Y0=y0, xd=xd, yd=yd, xs=overlap if x == 0 else 0,
+                            ys=overlap if y == 0 else 0, xfirst=int(x == 0), yfirst=int(y == 0),
+                            xe=min(xend, T), ye=min(yend, T), psf=0 if n_psf == 1 else k))
+            k += 1
+    return out
+
+
+def tile_table(geom, image, flat):
+    """Device array of ``mm_tile_desc`` for ``geom`` over ``image`` (and ``flat``)."""
+    arr = np.zeros(len(geom), dtype=L.TILE_DTYPE)
+    for i, g in enumerate(geom):
+        for k in ("X0", "Y0", "xd", "yd", "xs", "ys", "xfirst", "yfirst", "xe", "ye", "psf"):
+            arr[i][k] = g[k]
+        arr[i]["ox"], arr[i]["oy"] = g["x"], g["y"]
+        arr[i]["img"] = image.data_ptr()
+        arr[i]["img_sz"], arr[i]["img_sx"] = image.stride(0), image.stride(1)
+        arr[i]["img_f32"] = 1 if image.dtype == torch.float32 else 0
+        if flat is not None:
+            arr[i]["flat"] = flat.data_ptr()
+            arr[i]["flat_sx"] = flat.stride(0)
+    return torch.from_numpy(arr.view(np.uint8).copy()).to(image.device)
+
+
+def prepare_stack(image):
+    """The stack as a device tensor the tile loader reads (uint16 or float32, unit y stride)."""
+    image = as_device(image)
+    if image.ndim != 3:
+        raise ValueError("expected a (z, x, y) stack")
+    if image.dtype not in (torch.uint16, torch.float32):
+        image = image.to(torch.float32)   # tile_pad[...] = tile casts to float32 (deconvolver.py:124)
+    return unit_last_stride(image)
+
+
+def prepare_flat(flat_field, image):
+    if flat_field is None:
+        return None
+    f = as_device(flat_field, image.device)
+    if f.ndim == 3:
+        if f.shape[0] != 1:
+            raise NotImplementedError("per-plane flat fields are not supported; pass a 2D flat")
+        f = f[0]
+    if f.ndim != 2 or tuple(f.shape) != tuple(image.shape[1:]):
+        raise ValueError(f"flat field shape {tuple(f.shape)} does not match the stack's {tuple(image.shape[1:])}")
+    if f.dtype != torch.float32:
+        f = f.to(torch.float32)
+    return unit_last_stride(f)
+
+
+class Deconvolver:
+    """``mermake.deconvolver.Deconvolver`` on gfx950."""
+
+    batch_bytes = int(os.environ.get("MERMAKE_BATCH_BYTES", str(24 << 30)))
+
+    def __init__(self, psfs, channel_shape, tile_size=300, zpad=0, overlap=89, beta=0.001, xp=None, device=None,
+                 **kwargs):
+        self.tile_size = tile_size
+        self.tile_height = channel_shape[0]
+        self.overlap = overlap
+        self.zpad = zpad
+        self.beta = beta
+        self.device = torch.device(device) if device is not None else default_device()
+        if isinstance(psfs, dict):
+            self.psf_keys = list(psfs.keys())
+            centred = [self.center_psf(np.asarray(p)) for p in psfs.values()]
+        else:
+            self.psf_keys = None
+            centred = [self.center_psf(np.asarray(psfs))]
+        self._plan = _Plan(self.tile_height, tile_size, overlap, zpad)
+        T, Zp, H = self._plan.T, self._plan.Zp, self._plan.H
+        self.psf_fft = torch.empty((len(centred), Zp, T, H), dtype=torch.complex64, device=self.device)
+        ws_bytes = L.lib().mm_deconv_wiener_ws_bytes(self._plan.h)
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=self.device)
+        st = L.stream_ptr(self.device)
+        for i, p in enumerate(centred):
+            pd = torch.from_numpy(np.ascontiguousarray(p, dtype=np.float32)).to(self.device)
+            L.call("mm_deconv_build_wiener", self._plan.h, L.ptr(pd), float(beta), L.ptr(self.psf_fft[i]),
+                   L.ptr(ws), ws_bytes, st)
+        torch.cuda.current_stream(self.device).synchronize()
+        del ws
+        self._ws = {}
+
+    # ------------------------------------------------------------ reference API
+    @property
+    def tile_pad(self):
+        T = self._plan.T
+        return torch.empty((2 * self.zpad + self.tile_height, T, T), dtype=torch.float32, device="meta")
+
+    @property
+    def tile_res(self):
+        T = self._plan.T
+        return torch.empty((self.tile_height, T, T), dtype=torch.float32, device="meta")
+
+    def center_psf(self, psf):
+        """``deconvolver.py:238-270`` (host NumPy, as in the reference)."""
+        target = np.array([self.tile_height, self.tile_size, self.tile_size])
+        shp = np.array(psf.shape)
+        out = np.zeros(tuple(target.tolist()), dtype=psf.dtype)
+        s_o = np.maximum(0, (target - shp) // 2)
+        e_o = s_o + np.minimum(target, shp)
+        s_p = np.maximum(0, (shp - target) // 2)
+        e_p = s_p + np.minimum(target, shp)
+        out[tuple(slice(int(a), int(b)) for a, b in zip(s_o, e_o))] = \
+            psf[tuple(slice(int(a), int(b)) for a, b in zip(s_p, e_p))]
+        out /= out.sum()
+        return out
+
+    def tiled(self, image):
+        """``deconvolver.py:186-206``: yields ``(x, y, view)``."""
+        sz, sx, sy = image.shape
+        ts, ov = self.tile_size, self.overlap
+        self.nx = -(-sx // ts)
+        self.ny = -(-sy // ts)
+        for x in range(0, sx, ts):
+            for y in range(0, sy, ts):
+                yield x, y, image[:, max(x - ov, 0):min(x + ts + ov, sx), max(y - ov, 0):min(y + ts + ov, sy)]
+
+    def _check(self, image):
+        if self.zpad < 1 or self.overlap < 1:
+            raise ValueError("tile processing needs zpad >= 1 and overlap >= 1")
+        if image.shape[0] != self.tile_height:
+            raise ValueError(f"stack depth {image.shape[0]} does not match the Deconvolver's {self.tile_height}")
+
+    def _workspace(self, name, nbytes):
+        t = self._ws.get(name)
+        if t is None or t.numel() < nbytes:
+            t = None
+            self._ws.pop(name, None)
+            t = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+            self._ws[name] = t
+        return t
+
+    def batch_size(self, ntiles, extra_per_tile=0):
+        T = self._plan.T
+        per = self._plan.spec_bytes + 4 * self.tile_height * T * T + extra_per_tile
+        return max(1, min(ntiles, self.batch_bytes // per))
+
+    def run_tiles(self, table, ntiles, tiles_per_block, blur_radius, res_out=None):
+        """Deconvolution (+ blur subtraction) of ``ntiles`` descriptors: ``(ntiles, sz, T, T)``."""
+        T, sz = self._plan.T, self.tile_height
+        spec = self._workspace("spec", ntiles * self._plan.spec_bytes)
+        res = res_out if res_out is not None else torch.empty((ntiles, sz, T, T), dtype=torch.float32,
+                                                               device=self.device)
+        st = L.stream_ptr(self.device)
+        L.call("mm_deconv_tiles", self._plan.h, L.ptr(table), ntiles, L.ptr(self.psf_fft), self.psf_fft.shape[0],
+               int(tiles_per_block), L.ptr(spec), L.ptr(res), st)
+        if blur_radius is not None:
+            # deconvolver.py:160-163: res -= box_y(box_x(res)); the spectrum buffer is free now
+            delta = int(blur_radius) // 2
+            L.call("mm_box_1d", L.ptr(res), L.ptr(spec), ntiles * sz, T, T, delta, 1, st)
+            L.call("mm_box_1d_sub", L.ptr(res), L.ptr(spec), L.ptr(res), ntiles * sz, T, T, delta, 2, st)
+        return res
+
+    def _batches(self, image, flat_field, blur_radius, want_raw):
+        self._check(image)
+        img = prepare_stack(image)
+        flat = prepare_flat(flat_field, img)
+        geom = tile_geometry(img.shape, self.tile_size, self.overlap, self.psf_fft.shape[0])
+        T, sz = self._plan.T, self.tile_height
+        single = self.psf_fft.shape[0] == 1
+        B = self.batch_size(len(geom), 4 * sz * T * T if want_raw else 0)
+        for b0 in range(0, len(geom), B):
+            g = geom[b0:b0 + B]
+            table = tile_table(g, img, flat)
+            res = self.run_tiles(table, len(g), len(g) if single else 1, blur_radius)
+            raw = None
+            if want_raw:
+                raw = torch.empty((len(g), sz, T, T), dtype=torch.float32, device=self.device)
+                L.call("mm_deconv_tile_raw", self._plan.h, L.ptr(table), len(g), L.ptr(raw),
+                       L.stream_ptr(self.device))
+            yield g, table, res, raw, img
+
+    def tile_wise(self, image, flat_field=None, blur_radius=None, **kwargs):
+        """``deconvolver.py:98-170``: yields ``(x, y, tile, raw)`` views of shape (sz, xend, yend)."""
+        for g, _, res, raw, _ in self._batches(image, flat_field, blur_radius, True):
+            for i, t in enumerate(g):
+                yield t["x"], t["y"], res[i, :, :t["xe"], :t["ye"]], raw[i, :, :t["xe"], :t["ye"]]
+
+    def apply(self, image, flat_field=None, blur_radius=None, output=None, **kwargs):
+        """``deconvolver.py:172-184``: stitched float32 volume."""
+        if output is None:
+            output = torch.empty(tuple(image.shape), dtype=torch.float32, device=self.device)
+        if output.dtype != torch.float32 or output.stride(-1) != 1:
+            raise ValueError("output must be float32 with unit stride along y")
+        sz, sx, sy = image.shape
+        for g, table, res, _, _ in self._batches(image, flat_field, blur_radius, False):
+            L.call("mm_deconv_stitch", self._plan.h, L.ptr(table), len(g), L.ptr(res), L.ptr(output),
+                   output.stride(0), output.stride(1), sx, sy, L.stream_ptr(self.device))
+        return output
+
+
+def full_deconv(image, psfs, flat_field=None, tile_size=300, zpad=None, overlap=1, beta=0.001):
+    """``deconvolver.py:272-287``."""
+    shape = tuple(image.shape)
+    if zpad is None:
+        zpad = shape[0]
+    d = Deconvolver(psfs, channel_shape=shape, zpad=zpad, tile_size=tile_size, overlap=overlap, beta=beta)
+    out = d.apply(image, flat_field)
+    del d
+    return out

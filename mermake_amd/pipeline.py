@@ -1,0 +1,163 @@
+"""The hyb and DAPI callers of the hot path (reference ``mermake/__main__.py:213-269``).
+
+``HybPipeline`` replaces the per-channel hyb loop (``__main__.py:242-269``):
+
+    for x, y, tile, raw in deconvolver.hybs.tile_wise(chan, flat, **hybs):
+        Xh = find_local_maxima(tile, raw=raw, **hybs)
+        keep = all(ov <= Xh[:, 1:3] < tile.shape[1:3] - ov); Xh = Xh[keep]
+        Xh[:, 1] += x - ov; Xh[:, 2] += y - ov
+    Xhf = vstack(...)
+
+with one batch of tiles per launch: deconvolution, blur subtraction, maxima,
+fit and keep-filter of every tile run as a handful of kernels, with a single
+host synchronisation per channel stack (to read the number of spots).  The
+``raw`` values the fit needs are evaluated from the stack and flat at the
+sampled voxels instead of materialising the padded tile.
+
+``plain=True`` drives the same kernels the way the reference drives its own:
+one tile at a time, host syncs after the maxima and after the fit.
+
+``DapiPipeline`` is ``__main__.py:217-225``: ``apply`` with the DAPI settings,
+RMS normalisation, maxima of +buffer and of -buffer (the negation is folded
+into the kernels' sign).
+"""
+import torch
+
+from . import _lib as L
+from .deconvolver import Deconvolver, prepare_flat, prepare_stack, tile_geometry, tile_table
+from .maxima import find_local_maxima_signed
+
+
+class HybPipeline:
+    def __init__(self, deconvolver, blur_radius=30, threshold=3600, delta=1, delta_fit=3, sigmaZ=1, sigmaXY=1.5,
+                 cap=16384, plain=False, **kwargs):
+        if delta > 5:
+            raise TypeError("Delta must be an less than or equal to 5 due to MAX_KERNEL_POINTS")
+        self.d = deconvolver
+        self.blur_radius = blur_radius
+        self.threshold = float(torch.tensor(float(threshold), dtype=torch.float32))
+        self.delta = int(delta)
+        self.delta_fit = int(delta_fit)
+        self.sigmaZ = float(sigmaZ)
+        self.sigmaXY = float(sigmaXY)
+        self.cap = int(cap)
+        self.plain = plain
+        self._tables = {}
+        self._ws = {}
+
+    @classmethod
+    def from_settings(cls, psfs, channel_shape, tile_size=500, overlap=89, beta=1e-4, zpad=None, **hybs):
+        """Deconvolver with ``zpad = sz - 1`` as in ``__main__.py:184,195``."""
+        if zpad is None:
+            zpad = channel_shape[0] - 1
+        d = Deconvolver(psfs, channel_shape, tile_size=tile_size, zpad=zpad, overlap=overlap, beta=beta)
+        return cls(d, **hybs)
+
+    def _buf(self, name, shape, dtype):
+        t = self._ws.get(name)
+        n = 1
+        for s in shape:
+            n *= s
+        if t is None or t.numel() < n or t.dtype != dtype:
+            t = torch.empty(n, dtype=dtype, device=self.d.device)
+            self._ws[name] = t
+        return t[:n].view(shape)
+
+    def _table(self, img, flat):
+        key = (img.data_ptr(), tuple(img.shape), tuple(img.stride()), img.dtype,
+               None if flat is None else flat.data_ptr())
+        t = self._tables.get(key)
+        if t is None:
+            geom = tile_geometry(img.shape, self.d.tile_size, self.d.overlap, self.d.psf_fft.shape[0])
+            t = (geom, tile_table(geom, img, flat))
+            if len(self._tables) > 8:
+                self._tables.clear()
+            self._tables[key] = t
+        return t
+
+    def _spots_batch(self, table, ntiles, single, cap):
+        """Enqueue deconv -> blur -> maxima -> fit -> keep for ``ntiles`` descriptors."""
+        d = self.d
+        T, sz = d._plan.T, d.tile_height
+        st = L.stream_ptr(d.device)
+        res = self._buf("res", (ntiles, sz, T, T), torch.float32)
+        d.run_tiles(table, ntiles, ntiles if single else 1, self.blur_radius, res_out=res)
+        ws_bytes = L.lib().mm_maxima_ws_bytes(sz * T * T, ntiles)
+        ws = self._buf("mws", (ws_bytes,), torch.uint8)
+        coords = self._buf("coords", (ntiles, cap, 3), torch.int32)
+        counts = self._buf("counts", (ntiles,), torch.int32)
+        fit = self._buf("fit", (ntiles, cap, 8), torch.float32)
+        L.call("mm_tiles_maxima", d._plan.h, L.ptr(table), ntiles, L.ptr(res), self.threshold, self.delta,
+               L.ptr(coords), cap, L.ptr(counts), L.ptr(ws), ws_bytes, st)
+        L.call("mm_tiles_fit", d._plan.h, L.ptr(table), ntiles, L.ptr(res), L.ptr(coords), L.ptr(counts), cap,
+               self.delta_fit, self.sigmaZ, self.sigmaXY, L.ptr(fit), st)
+        return counts, fit
+
+    def __call__(self, stack, flat_field=None):
+        """Spots of one channel stack: ``(N, 8)`` float32 rows in tile order, then raster order."""
+        d = self.d
+        d._check(stack)
+        img = prepare_stack(stack)
+        flat = prepare_flat(flat_field, img)
+        geom, table_all = self._table(img, flat)
+        n = len(geom)
+        single = d.psf_fft.shape[0] == 1
+        st = L.stream_ptr(d.device)
+        out_parts = []
+        B = 1 if self.plain else d.batch_size(n, self.cap * 48 + d.tile_height * d._plan.T ** 2 // 8)
+        rec = L.TILE_DTYPE.itemsize
+        for b0 in range(0, n, B):
+            nb = min(B, n - b0)
+            table = table_all[b0 * rec:(b0 + nb) * rec]
+            cap = self.cap
+            while True:
+                counts, fit = self._spots_batch(table, nb, single, cap)
+                if self.plain:
+                    mx = int(counts.max().item())       # sync after the maxima, as maxima.py:46-50
+                else:
+                    mx = None
+                if mx is not None and mx > cap:
+                    cap = mx
+                    continue
+                kept = self._buf("kept", (nb,), torch.int32)
+                total = self._buf("total", (1,), torch.int32)
+                xh = torch.empty((nb * cap, 8), dtype=torch.float32, device=d.device)
+                L.call("mm_tiles_keep", L.ptr(table), nb, d.overlap, L.ptr(fit), L.ptr(counts), cap, L.ptr(kept),
+                       L.ptr(xh), nb * cap, L.ptr(total), st)
+                stats = torch.stack([total[0], counts.max()]).cpu()   # one sync per batch
+                tot, mx = int(stats[0]), int(stats[1])
+                if mx > cap:
+                    cap = mx
+                    continue
+                out_parts.append(xh[:tot])
+                break
+        if not out_parts:
+            return torch.zeros((0, 8), dtype=torch.float32, device=d.device)
+        return out_parts[0] if len(out_parts) == 1 else torch.cat(out_parts)
+
+
+class DapiPipeline:
+    def __init__(self, deconvolver, blur_radius=50, threshold=3.0, delta=5, delta_fit=5, sigmaZ=1, sigmaXY=1.5,
+                 **kwargs):
+        if delta > 5:
+            raise TypeError("Delta must be an less than or equal to 5 due to MAX_KERNEL_POINTS")
+        self.d = deconvolver
+        self.blur_radius = blur_radius
+        self.kw = dict(threshold=float(torch.tensor(float(threshold), dtype=torch.float32)), delta=int(delta),
+                       delta_fit=int(delta_fit), sigmaZ=float(sigmaZ), sigmaXY=float(sigmaXY))
+        self._buffer = None
+
+    def __call__(self, chan, flat_field=None):
+        """``(Xh_plus, Xh_minus)`` of one DAPI stack."""
+        d = self.d
+        img = prepare_stack(chan)
+        if self._buffer is None or tuple(self._buffer.shape) != tuple(img.shape):
+            self._buffer = torch.empty(tuple(img.shape), dtype=torch.float32, device=d.device)
+        buf = d.apply(img, flat_field=flat_field, blur_radius=self.blur_radius, output=self._buffer)
+        ws = torch.empty(L.lib().mm_rms_ws_bytes(), dtype=torch.uint8, device=d.device)
+        rms = torch.empty(1, dtype=torch.float32, device=d.device)
+        L.call("mm_rms_normalize", L.ptr(buf), buf.numel(), L.ptr(ws), L.ptr(rms), L.stream_ptr(d.device))
+        raw = img
+        plus = find_local_maxima_signed(buf, raw, sign=1.0, **self.kw)
+        minus = find_local_maxima_signed(buf, raw, sign=-1.0, **self.kw)
+        return plus, minus

@@ -4,8 +4,8 @@ Follows ``mermake/maxima.py:17-98`` and the CUDA kernels
 ``mermake/maxima.cu:1-145`` (``local_maxima_count`` / ``local_maxima``) and
 ``mermake/maxima.cu:148-341`` (``delta_fit_cross_corr``).
 
-Maxima (``maxima.cu:19-62``): a voxel is a candidate when ``v > threshold``
-(strict) and ``v >= img[n]`` for every neighbour ``n`` of the sphere
+Maxima (``maxima.cu:19-62``): a voxel is rejected when ``v <= threshold`` (so a
+NaN threshold rejects nothing) and is a candidate when ``v >= img[n]`` for every neighbour ``n`` of the sphere
 ``dz^2+dx^2+dy^2 <= delta^2`` (centre excluded), so ties are maxima.  Neighbour
 indices are reflected once: ``-n`` below zero, ``2N-n-2`` above the end.  For
 axes shorter than ``delta+1`` that single reflection can still leave the axis;
@@ -64,7 +64,8 @@ def local_maxima(image, threshold, delta):
     img = np.asarray(image)
     D, H, W = img.shape
     flat = img.reshape(-1)
-    cand = np.flatnonzero(flat > np.asarray(threshold, dtype=img.dtype))
+    with np.errstate(invalid="ignore"):   # maxima.cu:19 rejects v <= threshold (NaN rejects nothing)
+        cand = np.flatnonzero(~(flat <= np.asarray(threshold, dtype=img.dtype)))
     z, rem = np.divmod(cand, H * W)
     x, y = np.divmod(rem, W)
     v = flat[cand]
@@ -89,7 +90,7 @@ def local_maxima_bruteforce(image, threshold, delta):
         for x in range(H):
             for y in range(W):
                 v = img[z, x, y]
-                if not v > threshold:
+                if v <= threshold:
                     continue
                 ismax = True
                 for dz, dx, dy in offs:
@@ -222,6 +223,9 @@ def delta_fit_scalar(image, raw, z0, x0, y0, delta_fit, sigmaZ, sigmaXY):
     return np.array([c[0], c[1], c[2], bg, a, raw[z0, x0, y0], hn, img[z0, x0, y0]])
 
 
+_delta_fit = delta_fit
+
+
 def find_local_maxima(image, threshold=None, delta=1, delta_fit=0, raw=None, sigmaZ=1, sigmaXY=1.5,
                       dtype=np.float64, **kwargs):
     """Restatement of ``find_local_maxima`` (``maxima.py:17-98``); rows in raster order."""
@@ -237,4 +241,4 @@ def find_local_maxima(image, threshold=None, delta=1, delta_fit=0, raw=None, sig
     raw = np.asarray(raw)
     if raw.dtype != np.float32:
         raw = raw.astype(np.float32)
-    return delta_fit(img, raw, z, x, y, delta_fit, np.float32(sigmaZ), np.float32(sigmaXY), dtype=dtype)
+    return _delta_fit(img, raw, z, x, y, delta_fit, np.float32(sigmaZ), np.float32(sigmaXY), dtype=dtype)

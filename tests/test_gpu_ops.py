@@ -1,0 +1,296 @@
+"""GPU parity tests of the operators (HIP path vs the oracle), through the C ABI."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import blur as oblur
+from oracle import deconv as odec
+from oracle import fill as ofill
+from oracle import maxima as omax
+
+pytestmark = pytest.mark.gpu
+
+
+def T(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+# ------------------------------------------------------------------ blur
+def test_box_1d_reference_known_answers(dev):
+    """tests/test_blur.py:62-100 of the reference: exact 3x3x3 values for size 2."""
+    from mermake_amd import blur
+    arr = torch.zeros((3, 3, 3), dtype=torch.float32, device=dev)
+    arr[0, 0, 0] = 1
+    third = np.float32(1) / np.float32(3)
+    want0 = np.zeros((3, 3, 3), np.float32); want0[0, 0, 0] = 0.5; want0[1, 0, 0] = third
+    want1 = np.zeros((3, 3, 3), np.float32); want1[0, 0, 0] = 0.5; want1[0, 1, 0] = third
+    want2 = np.zeros((3, 3, 3), np.float32); want2[0, 0, 0] = 0.5; want2[0, 0, 1] = third
+    for axis, want in ((0, want0), (1, want1), (2, want2)):
+        got = blur.box_1d(arr, 2, axis=axis).cpu().numpy()
+        assert np.array_equal(got, want), axis
+    with pytest.raises(ValueError):
+        blur.box_1d(arr, 2, axis=0, out=arr)
+
+
+@pytest.mark.parametrize("shape,axis,size", [((5, 5), 0, 3), ((5, 5), 1, 3), ((4, 4, 4), 0, 3), ((4, 4, 4), 1, 5),
+                                             ((4, 4, 4), 2, 3), ((7, 33, 40), 1, 31), ((7, 33, 40), 2, 51),
+                                             ((3, 50, 61), 1, 30), ((6, 9, 8), -1, 4)])
+def test_box_1d_bitexact(dev, shape, axis, size):
+    from mermake_amd import blur
+    rng = np.random.default_rng(1)
+    a = (rng.standard_normal(shape) * 1000).astype(np.float32)
+    got = blur.box_1d(T(a, dev), size, axis=axis).cpu().numpy()
+    want = oblur.box_1d(a, size, axis=axis)
+    assert np.array_equal(got, want)
+
+
+def test_box_multi_axis(dev):
+    from mermake_amd import blur
+    a = np.arange(64, dtype=np.float32).reshape(4, 4, 4)
+    for axes, size in (((0, 1), 3), ((1, 2), (3, 5))):
+        got = blur.box(T(a, dev), size, axes=axes).cpu().numpy()
+        assert np.array_equal(got, oblur.box(a, size, axes=axes))
+        got2 = blur.box_2d(T(a, dev), size, axes=axes).cpu().numpy()
+        assert np.array_equal(got2, got)
+    with pytest.raises(ValueError):
+        blur.box(T(a, dev), 3, axes=(0, 1, 2, 3))
+    with pytest.raises(TypeError):
+        blur.box(T(a, dev), 3, axes="bad")
+    with pytest.raises(ValueError):
+        blur.box(T(a, dev), (3, 3), axes=(0, 1, 2))
+
+
+# ------------------------------------------------------------------ fill
+def test_reflect_reference_known_answers(dev):
+    """tests/test_fill.py:22-133 of the reference."""
+    from mermake_amd.fill import reflect
+    a1 = torch.tensor([1, 2, 3, 4, 5], dtype=torch.float32, device=dev)
+    assert reflect(a1, 2, mode="out").tolist() == [1, 2, 3, 2, 1]
+    assert reflect(a1, 2, mode="in").tolist() == [5, 4, 3, 4, 5]
+    assert a1.tolist() == [1, 2, 3, 4, 5]
+    d2 = torch.arange(1, 16, dtype=torch.float32, device=dev).reshape(3, 5)
+    assert reflect(d2, 2, axis=0, mode="in").tolist() == d2.tolist()
+    assert reflect(d2, 1, axis=0, mode="in").tolist() == [[11, 12, 13, 14, 15], [6, 7, 8, 9, 10], [11, 12, 13, 14, 15]]
+    assert reflect(d2, 2, axis=1, mode="in").tolist() == [[5, 4, 3, 4, 5], [10, 9, 8, 9, 10], [15, 14, 13, 14, 15]]
+    assert reflect(d2, 3, axis=1, mode="in").tolist() == [[1, 2, 5, 4, 5], [6, 7, 10, 9, 10], [11, 12, 15, 14, 15]]
+    assert reflect(d2, 0, axis=0, mode="out").tolist() == d2.tolist()
+    assert reflect(d2, 1, axis=0, mode="out").tolist() == [[1, 2, 3, 4, 5], [6, 7, 8, 9, 10], [1, 2, 3, 4, 5]]
+    assert reflect(d2, 2, axis=1, mode="out").tolist() == [[1, 2, 3, 2, 1], [6, 7, 8, 7, 6], [11, 12, 13, 12, 11]]
+    x = a1.clone()
+    assert reflect(x, 2, mode="out", out=x) is x and x.tolist() == [1, 2, 3, 2, 1]
+    with pytest.raises(ValueError, match="output array shape"):
+        reflect(a1, 1, out=torch.zeros(4, device=dev))
+    with pytest.raises(ValueError, match="output array dtype"):
+        reflect(a1, 1, out=torch.zeros(5, dtype=torch.int32, device=dev))
+    with pytest.raises(IndexError):
+        reflect(a1, 5)
+    with pytest.raises(ValueError):
+        reflect(a1, 1, mode="invalid")
+    with pytest.raises(IndexError):
+        reflect(a1, 1, axis=1)
+
+
+def test_reflect_random_vs_oracle(dev):
+    from mermake_amd.fill import reflect
+    rng = np.random.default_rng(2)
+    a = rng.random((4, 7, 9)).astype(np.float32)
+    for axis in range(3):
+        for i in range(a.shape[axis]):
+            for mode in ("in", "out"):
+                got = reflect(T(a, dev), i, axis=axis, mode=mode).cpu().numpy()
+                assert np.array_equal(got, ofill.reflect(a, i, axis=axis, mode=mode)), (axis, i, mode)
+    # strided (non-contiguous) input
+    t = T(a, dev).transpose(0, 2)
+    got = reflect(t, 2, axis=1, mode="in").cpu().numpy()
+    assert np.array_equal(got, ofill.reflect(a.transpose(2, 1, 0), 2, axis=1, mode="in"))
+
+
+# ------------------------------------------------------------------ FFT
+@pytest.mark.parametrize("shape", [(1, 1, 8), (3, 5, 7), (2, 6, 16), (4, 29, 59), (2, 113, 3), (1, 13, 118),
+                                   (2, 3, 678), (2, 478, 2), (3, 11, 105), (1, 2, 1101)])
+def test_fft3d_vs_numpy(dev, shape):
+    from mermake_amd.deconvolver import fftn_c2c
+    rng = np.random.default_rng(3)
+    x = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    got = fftn_c2c(T(x, dev)).cpu().numpy()
+    want = np.fft.fftn(x.astype(np.complex128))
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) < 2e-6
+    inv = fftn_c2c(T(x, dev), inverse=True).cpu().numpy()
+    want_i = np.fft.ifftn(x.astype(np.complex128))
+    assert np.linalg.norm(inv - want_i) / np.linalg.norm(want_i) < 2e-6
+
+
+# ------------------------------------------------------------------ maxima + fit
+def _peaky(rng, shape, n=30):
+    img = rng.normal(0, 1, shape).astype(np.float32)
+    for _ in range(n):
+        z, x, y = [rng.integers(0, s) for s in shape]
+        img[z, x, y] += rng.uniform(5, 20)
+    return img
+
+
+@pytest.mark.parametrize("shape,delta", [((12, 20, 24), 1), ((10, 17, 13), 2), ((8, 30, 31), 3), ((12, 12, 12), 5),
+                                         ((2, 2, 2), 1), ((1, 1, 1), 1), ((10, 5, 3), 1)])
+def test_maxima_sets_exact(dev, shape, delta):
+    from mermake_amd.maxima import maxima_coords
+    rng = np.random.default_rng(4)
+    img = _peaky(rng, shape)
+    thr = np.float32(2.0)
+    got, _ = maxima_coords(T(img, dev), float(thr), delta)
+    got = got.cpu().numpy()
+    z, x, y = omax.local_maxima(img, thr, delta)
+    assert np.array_equal(got, np.stack([z, x, y], 1).astype(np.int32))
+    if np.prod(shape) <= 4000:
+        assert [tuple(r) for r in got] == omax.local_maxima_bruteforce(img, thr, delta)
+
+
+def test_maxima_ties_and_views(dev):
+    from mermake_amd.maxima import find_local_maxima
+    img = np.zeros((5, 6, 7), np.float32)
+    img[2, 2, 2] = 1.0
+    img[2, 2, 3] = 1.0      # tie: both are maxima (v >= neighbours)
+    out = find_local_maxima(T(img, dev), threshold=0.5, delta=1, delta_fit=1, raw=T(img, dev))
+    assert out.shape == (2, 8)
+    big = np.zeros((5, 10, 12), np.float32)
+    big[:, 1:7, 2:9] = img[:, :6, :7]
+    view = T(big, dev)[:, 1:7, 2:9]    # strided view: reflection at the view border
+    out2 = find_local_maxima(view, threshold=0.5, delta=1, delta_fit=1, raw=view)
+    assert torch.equal(out.cpu(), out2.cpu())
+
+
+def test_find_local_maxima_reference_behaviour(dev):
+    """tests/test_maxima.py:26-166 of the reference."""
+    from mermake_amd.maxima import find_local_maxima
+    image = torch.zeros((5, 5, 5), dtype=torch.float32, device=dev)
+    image[1:4, 1:4, 1:4] = 0.5
+    image[2, 2, 2] = 1.0
+    r = find_local_maxima(image, threshold=0.8, delta=1, delta_fit=1, raw=image.clone())
+    assert r.shape == (1, 8) and all(abs(float(r[0, k]) - 2.0) < 0.5 for k in range(3))
+    image = torch.zeros((5, 5, 5), dtype=torch.float32, device=dev)
+    image[1, 1, 1] = 0.9
+    image[3, 3, 3] = 0.6
+    assert find_local_maxima(image, threshold=0.5, delta=1, delta_fit=1, raw=image).shape[0] == 2
+    assert find_local_maxima(image, threshold=0.8, delta=1, delta_fit=1, raw=image).shape[0] == 1
+    r = find_local_maxima(image, threshold=1.5, delta=1, delta_fit=1, raw=image)
+    assert r.shape == (0, 8)
+    with pytest.raises(TypeError, match="Delta must be an less than or equal to 5"):
+        find_local_maxima(image, threshold=0.5, delta=6, delta_fit=1, raw=image)
+    ones = torch.ones((3, 3, 3), dtype=torch.float32, device=dev)
+    r1 = find_local_maxima(ones, threshold=0.5, delta=1, delta_fit=1, raw=(ones * 65535).to(torch.uint16))
+    r2 = find_local_maxima(ones, threshold=0.5, delta=1, delta_fit=1, raw=ones.double())
+    assert r1.shape == r2.shape == (27, 8)
+
+
+@pytest.mark.parametrize("dfit,sz,sxy", [(1, 1.0, 1.5), (3, 1.0, 1.5), (5, 1.0, 1.5), (2, 2.0, 2.0), (6, 1.0, 1.0)])
+def test_fit_vs_oracle(dev, dfit, sz, sxy):
+    from mermake_amd.maxima import find_local_maxima
+    rng = np.random.default_rng(5)
+    img = _peaky(rng, (14, 40, 45), 40) * 100 + 500
+    raw = (img * 1.3 + rng.normal(0, 5, img.shape)).astype(np.float32)
+    got = find_local_maxima(T(img, dev), threshold=900, delta=1, delta_fit=dfit, raw=T(raw, dev), sigmaZ=sz,
+                            sigmaXY=sxy).cpu().numpy()
+    want = omax.find_local_maxima(img, threshold=900, delta=1, delta_fit=dfit, raw=raw, sigmaZ=sz, sigmaXY=sxy)
+    assert got.shape == want.shape and got.shape[0] > 10
+    assert np.abs(got[:, :3] - want[:, :3]).max() < 1e-3                      # coordinates (px)
+    assert np.array_equal(got[:, 3], want[:, 3].astype(np.float32))          # background = min sample
+    assert np.array_equal(got[:, 5], want[:, 5].astype(np.float32))          # habs
+    assert np.array_equal(got[:, 7], want[:, 7].astype(np.float32))          # h
+    assert np.abs(got[:, [4, 6]] - want[:, [4, 6]]).max() < 1e-5             # correlations in [-1, 1]
+
+
+def test_fit_nan_when_single_sample(dev):
+    """delta_fit = 0: one sample, std_G = 0 -> a, hn NaN (maxima.cu:249-254)."""
+    from mermake_amd.maxima import find_local_maxima
+    img = np.zeros((5, 5, 5), np.float32)
+    img[2, 2, 2] = 3.0
+    r = find_local_maxima(T(img, dev), threshold=1.0, delta=1, delta_fit=0, raw=T(img, dev)).cpu().numpy()
+    assert r.shape == (1, 8) and np.isnan(r[0, 4]) and np.isnan(r[0, 6])
+    assert r[0, 0] == 2 and r[0, 1] == 2 and r[0, 2] == 2 and r[0, 7] == 3.0
+
+
+# ------------------------------------------------------------------ deconvolution
+def _rel_l2(a, b):
+    return float(np.linalg.norm(a.astype(np.float64) - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("shape,ts,ov,zpad,beta,flat,blur", [
+    ((10, 50, 50), 30, 5, 2, 1e-3, False, None),
+    ((10, 50, 50), 30, 5, 2, 1e-3, True, None),
+    ((6, 37, 53), 20, 4, 3, 1e-2, True, 5),
+    ((5, 20, 20), 50, 5, 2, 1e-3, False, None),     # single tile larger than the image
+    ((4, 45, 45), 15, 2, 1, 1e-4, False, 3),
+    ((3, 40, 40), 20, 7, 5, 1e-3, True, None),      # zpad >= sz: zero planes in the z padding
+])
+def test_deconv_vs_oracle(dev, shape, ts, ov, zpad, beta, flat, blur):
+    from mermake_amd.deconvolver import Deconvolver
+    rng = np.random.default_rng(6)
+    img = rng.integers(100, 4000, size=shape).astype(np.uint16)
+    psf = rng.random((5, 7, 7)).astype(np.float32) + np.float32(0.1)
+    fl = (0.7 + 0.6 * rng.random(shape[1:])).astype(np.float32) if flat else None
+    d = Deconvolver(psf, shape, tile_size=ts, zpad=zpad, overlap=ov, beta=beta)
+    W = odec.wiener(psf, shape[0], ts, zpad, ov, beta, mode="fp64")
+    got = list(d.tile_wise(T(img, dev), flat_field=None if fl is None else T(fl, dev), blur_radius=blur))
+    want = list(odec.tile_wise(img, W, fl, blur, ts, ov, zpad, mode="fp64"))
+    want32 = list(odec.tile_wise(img, odec.wiener(psf, shape[0], ts, zpad, ov, beta, mode="fp32"), fl, blur, ts, ov,
+                                 zpad, mode="fp32"))
+    assert len(got) == len(want)
+    for (gx, gy, gt, gr), (wx, wy, wt, wr), (_, _, _, wr32) in zip(got, want, want32):
+        assert (gx, gy) == (wx, wy) and tuple(gt.shape) == wt.shape and tuple(gr.shape) == wr.shape
+        assert _rel_l2(gt.cpu().numpy(), wt) < 1e-5
+        assert np.array_equal(gr.cpu().numpy(), wr32)          # raw views: float32 (v/f)*f semantics
+    out = d.apply(T(img, dev), flat_field=None if fl is None else T(fl, dev), blur_radius=blur).cpu().numpy()
+    ref = odec.apply(img, W, fl, blur, ts, ov, zpad, mode="fp64")
+    written = np.zeros(shape, bool)
+    for x, y, t, _ in odec.tile_wise(img, W, fl, blur, ts, ov, zpad, mode="fp64"):
+        written[:, x:x + t.shape[1] - 2 * ov, y:y + t.shape[2] - 2 * ov] = True
+    assert _rel_l2(out[written], ref[written]) < 1e-5
+
+
+def test_multi_psf_pairing(dev):
+    """dict of PSFs: tile i uses PSF i, zip truncates to the shorter list (deconvolver.py:110-116)."""
+    from mermake_amd.deconvolver import Deconvolver
+    rng = np.random.default_rng(7)
+    shape = (6, 40, 40)
+    img = rng.integers(0, 3000, size=shape).astype(np.uint16)
+    psfs = {k: rng.random((3, 5, 5)).astype(np.float32) for k in ("a", "b", "c")}
+    d = Deconvolver(psfs, shape, tile_size=20, zpad=2, overlap=4, beta=1e-3)
+    assert len(d.psf_fft) == 3
+    got = list(d.tile_wise(T(img, dev)))
+    W = odec.wiener(psfs, shape[0], 20, 2, 4, 1e-3, mode="fp64")
+    want = list(odec.tile_wise(img, W, None, None, 20, 4, 2))
+    assert len(got) == len(want) == 3
+    for (_, _, gt, _), (_, _, wt, _) in zip(got, want):
+        assert _rel_l2(gt.cpu().numpy(), wt) < 1e-5
+
+
+def test_wiener_closed_form(dev):
+    """Delta PSF: W = 1/(1 + beta |L|^2) / (Zp T T); L(0) = 1 so the DC gain is 1/(1+beta)."""
+    from mermake_amd.deconvolver import Deconvolver
+    sz, ts, ov, zpad, beta = 6, 20, 3, 2, 1e-2
+    psf = np.zeros((sz, ts, ts), np.float32)
+    psf[sz // 2, ts // 2, ts // 2] = 1
+    d = Deconvolver(psf, (sz, 40, 40), tile_size=ts, zpad=zpad, overlap=ov, beta=beta)
+    W = d.psf_fft[0].cpu().numpy().astype(np.complex128)
+    Zp, Tt = sz + 2 * zpad, ts + 2 * ov
+    L2 = odec.laplacian_power((Zp, Tt, Tt))[:, :, :Tt // 2 + 1]
+    want = 1.0 / (1.0 + beta * L2) / (Zp * Tt * Tt)
+    assert np.abs(W - want).max() / want.max() < 1e-6
+    assert abs(W[0, 0, 0].real * Zp * Tt * Tt - 1 / (1 + beta)) < 1e-6
+
+
+def test_fixture_golden_tiles(dev):
+    """The reference's golden float16 deconvolution of the real fixture
+    (tests/test_deconvolver.py:422-434) on three committed tile windows."""
+    from fixtures import OV, TS, fixture_stack, golden_accept, psf_750
+    from mermake_amd.deconvolver import Deconvolver
+    stack, wins, gz = fixture_stack()
+    psf = psf_750()
+    d = Deconvolver(psf, stack.shape, tile_size=TS, zpad=stack.shape[0], overlap=OV, beta=0.001)
+    out = d.apply(T(stack, dev)).cpu().numpy()
+    W = odec.wiener(psf, stack.shape[0], TS, stack.shape[0], OV, 0.001, mode="fp64")
+    only = set(wins)
+    for x, y, t, _ in odec.tile_wise(stack, W, None, None, TS, OV, stack.shape[0], only=only):
+        mine = out[:, x:x + TS, y:y + TS]
+        assert _rel_l2(mine, t[:, OV:OV + TS, OV:OV + TS]) < 1e-5, (x, y)
+        exact, worst = golden_accept(mine[list(gz)], wins[(x, y)][2])
+        assert exact > 0.995 and worst <= 1.0, (x, y, exact, worst)

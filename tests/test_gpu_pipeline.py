@@ -1,0 +1,157 @@
+"""GPU parity of the batched hyb / DAPI callers and of the drift."""
+import types
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import align as oalign
+from oracle import deconv as odec
+from oracle import maxima as omax
+
+pytestmark = pytest.mark.gpu
+
+HYB = dict(blur_radius=30, threshold=3600, delta=1, delta_fit=3, sigmaZ=1, sigmaXY=1.5)
+
+
+def T(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def _match(a, b, tol):
+    """Rows of b matched to rows of a by nearest (z,x,y); returns max |diff| of coordinates and the match rate."""
+    if len(a) == 0 or len(b) == 0:
+        return 0.0, float(len(a) == len(b))
+    d = np.abs(a[:, None, :3] - b[None, :, :3]).max(-1)
+    j = d.argmin(1)
+    ok = d[np.arange(len(a)), j] < tol
+    return float(d[np.arange(len(a)), j][ok].max(initial=0.0)), float(ok.mean())
+
+
+@pytest.fixture(scope="module")
+def small_hyb():
+    """Config-0-like stack (scaled down): 20 x 300 x 300, 400 puncta, flat, tile 100, overlap 30."""
+    from mermake_amd.synth import gaussian_psf, stack_np
+    raw, flat = stack_np(11, (20, 300, 300), 400, flat=True)
+    psf = gaussian_psf((21, 41, 41), 1.5, 1.2)
+    return raw, flat, psf
+
+
+def test_hyb_candidates_exact_given_same_volume(dev, small_hyb):
+    """Given the identical deconvolved tile, maxima sets are bit-exact and fits within 1e-3 px."""
+    from mermake_amd.deconvolver import Deconvolver
+    from mermake_amd.maxima import find_local_maxima, maxima_coords
+    raw, flat, psf = small_hyb
+    d = Deconvolver(psf, raw.shape, tile_size=100, zpad=19, overlap=30, beta=1e-4)
+    n = 0
+    for x, y, tile, rawv in d.tile_wise(T(raw, dev), flat_field=T(flat, dev), blur_radius=30):
+        got = find_local_maxima(tile, threshold=3600, delta=1, delta_fit=3, raw=rawv).cpu().numpy()
+        t_np, r_np = tile.cpu().numpy(), rawv.cpu().numpy()
+        z, xx, yy = omax.local_maxima(t_np, np.float32(3600), 1)
+        cand, _ = maxima_coords(tile, 3600.0, 1)
+        assert np.array_equal(cand.cpu().numpy(), np.stack([z, xx, yy], 1).astype(np.int32)), (x, y)
+        want = omax.find_local_maxima(t_np, threshold=3600, delta=1, delta_fit=3, raw=r_np)
+        assert got.shape == want.shape, (x, y)
+        if len(got):
+            assert np.abs(got[:, :3] - want[:, :3]).max() < 1e-3
+            assert np.array_equal(got[:, 7], want[:, 7].astype(np.float32))
+        n += len(got)
+    assert n > 50
+
+
+def test_hyb_pipeline_vs_oracle(dev, small_hyb):
+    """Batched hyb pipeline vs the float64 oracle of the reference loop (__main__.py:254-265)."""
+    from mermake_amd.pipeline import HybPipeline
+    raw, flat, psf = small_hyb
+    pipe = HybPipeline.from_settings(psf, raw.shape, tile_size=100, overlap=30, beta=1e-4, **HYB)
+    got = pipe(T(raw, dev), T(flat, dev)).cpu().numpy()
+    W = odec.wiener(psf, raw.shape[0], 100, raw.shape[0] - 1, 30, 1e-4, mode="fp64")
+    want = odec.hyb_spots(raw, W, flat, 100, 30, raw.shape[0] - 1, **HYB)
+    assert abs(len(got) - len(want)) <= max(2, 0.01 * len(want)), (len(got), len(want))
+    err, rate = _match(got, want, 0.5)
+    assert rate > 0.99 and err < 1e-3, (err, rate)
+    # plain (one tile at a time, host syncs) gives the same rows in the same order
+    plain = HybPipeline(pipe.d, plain=True, **HYB)(T(raw, dev), T(flat, dev)).cpu().numpy()
+    assert np.array_equal(plain, got)
+    # determinism: a second run is bit-identical
+    again = pipe(T(raw, dev), T(flat, dev)).cpu().numpy()
+    assert np.array_equal(again, got)
+
+
+def test_dapi_pipeline_vs_oracle(dev):
+    from mermake_amd.deconvolver import Deconvolver
+    from mermake_amd.pipeline import DapiPipeline
+    from mermake_amd.synth import gaussian_psf, stack_np
+    raw, flat = stack_np(12, (12, 160, 160), 60, amp=(3000, 20000), flat=True)
+    psf = gaussian_psf((12, 31, 31), 1.5, 1.2)
+    kw = dict(blur_radius=50, threshold=3.0, delta=5, delta_fit=5, sigmaZ=1, sigmaXY=1.5)
+    d = Deconvolver(psf, raw.shape, tile_size=100, zpad=(raw.shape[0] - 1) // 2, overlap=20, beta=0.01)
+    plus, minus = DapiPipeline(d, **kw)(T(raw, dev), T(flat, dev))
+    W = odec.wiener(psf, raw.shape[0], 100, (raw.shape[0] - 1) // 2, 20, 0.01, mode="fp64")
+    wp, wm = odec.dapi_spots(raw, W, flat, 100, 20, (raw.shape[0] - 1) // 2, **kw)
+    for g, w in ((plus.cpu().numpy(), wp), (minus.cpu().numpy(), wm)):
+        assert abs(len(g) - len(w)) <= max(2, 0.02 * len(w)), (len(g), len(w))
+        err, rate = _match(g, w, 0.5)
+        assert rate > 0.98 and err < 1e-3, (err, rate)
+
+
+def test_fftconvolve_vs_oracle(dev):
+    from mermake_amd.utils import fftconvolve
+    rng = np.random.default_rng(8)
+    a = rng.random((5, 17, 23)).astype(np.float32)
+    b = rng.random((3, 9, 4)).astype(np.float32)
+    for mode in ("full", "same", "valid"):
+        got = fftconvolve(T(a, dev), T(b, dev), mode=mode).cpu().numpy()
+        want = oalign.fftconvolve(a, b, mode=mode)
+        assert got.shape == want.shape
+        assert np.abs(got - want).max() < 1e-4 * np.abs(want).max()
+
+
+def _cloud(rng, n, lo=(0, 60, 60), hi=(40, 1400, 1400)):
+    X = rng.uniform(lo, hi, size=(n, 3)).astype(np.float32)
+    h = rng.uniform(4.5, 9, size=(n, 1)).astype(np.float32)
+    return np.concatenate([X, np.zeros((n, 4), np.float32), h], 1)
+
+
+@pytest.mark.parametrize("shift", [(2, -15, 25), (-3, 30, -30), (0, 0, 0), (1, 7, -12)])
+def test_drift_recovers_integer_shift(dev, shift):
+    from mermake_amd.align import Aligner
+    rng = np.random.default_rng(9)
+    ref = _cloud(rng, 800)
+    mov = ref.copy()
+    mov[:, :3] += np.array(shift, np.float32)
+    mov[:, :3] += rng.normal(0, 0.2, size=(len(mov), 3)).astype(np.float32)
+    al = Aligner(T(ref[:, :3], dev))
+    t, n = al.get_best_translation_points(T(mov[:, :3], dev), return_counts=True)
+    assert np.array_equal(np.round(t).astype(int), np.array(shift)), t
+    ot, on = oalign.Aligner(ref[:, :3]).get_best_translation_points(mov[:, :3], return_counts=True)
+    assert on == n and np.abs(ot - t).max() < 1e-4
+    assert np.array_equal(al.correlation_peak(T(mov[:, :3], dev)), oalign.Aligner(ref[:, :3]).correlation_peak(
+        mov[:, :3]))
+
+
+def test_drift_fixture_known_answer(dev):
+    """Reference drift pickle: N+ = 621, N- = 261, zero shift (y > 4 reference-threshold quirk)."""
+    from fixtures import dapi_features, drift_known_answer
+    from mermake_amd.align import DualAligner, drift
+    xp, xm = dapi_features()
+    ka = drift_known_answer()
+    img = types.SimpleNamespace(Xh_plus=T(xp, dev), Xh_minus=T(xm, dev), path="/data/H1_TEST_set1/Conv_zscan__020.zarr")
+    dual = DualAligner(img, th=4)
+    f, tp, tm, n_p, n_m = dual.get_best_translation_pointsV2(img)
+    assert (n_p, n_m) == (ka["N_plus"], ka["N_minus"])
+    assert np.allclose(f, ka["tzxyf"]) and np.allclose(tp, 0) and np.allclose(tm, 0)
+
+    class Block(list):
+        def ifov(self):
+            return 20
+
+        def iset(self):
+            return 1
+
+        def drift_file(self):
+            return "/nonexistent/drift_20--_set1.pkl"
+
+    res, path = drift(Block([img]))
+    drifts, files, ifov, refpath = res
+    assert len(drifts) == 1 and drifts[0][3] == 621 and drifts[0][4] == 261 and ifov == 20
