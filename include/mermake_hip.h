@@ -42,6 +42,13 @@ int mm_box_1d(const float* in, float* out, int size_z, int size_x, int size_y, i
 int mm_box_1d_sub(const float* res, const float* buf, float* out, int size_z, int size_x, int size_y, int delta,
                   int axis, void* stream);
 
+/* out = res - box_1d(box_1d(res, axis 1), axis 2) over nplanes (T, T) planes:
+ * the blur subtraction of deconvolver.py:160-163 for a batch of tiles.
+ * Bit-identical to the two reference passes.  `out` must not overlap `res`;
+ * delta 15 and 25 run as one fused kernel, other widths need `tmp`
+ * (nplanes*T*T floats). */
+int mm_blur_sub_xy(const float* res, float* out, float* tmp, int nplanes, int T, int delta, void* stream);
+
 /* ------------------------------------------------------------------ fill
  * Replaces the CUDA kernel `mirror` (mermake/fill.py:6-52) launched by
  * fill.reflect (mermake/fill.py:54-111), in place on `arr` (float32, up to 8

@@ -30,6 +30,7 @@ _SIGS = {
     "mm_last_error": (ctypes.c_char_p, []),
     "mm_box_1d": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _vp]),
     "mm_box_1d_sub": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
+    "mm_blur_sub_xy": (_i, [_vp, _vp, _vp, _i, _i, _i, _vp]),
     "mm_reflect": (_i, [_vp, _i, ctypes.POINTER(_ll), ctypes.POINTER(_ll), _i, _i, _i, _vp]),
     "mm_deconv_create": (_i, [_i, _i, _i, _i, ctypes.POINTER(_vp)]),
     "mm_deconv_destroy": (_i, [_vp]),

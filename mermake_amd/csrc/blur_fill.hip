@@ -69,6 +69,144 @@ extern "C" int mm_box_1d_sub(const float* res, const float* buf, float* out, int
   return box_launch(buf, res, out, size_z, size_x, size_y, delta, axis, true, stream);
 }
 
+// ------------------------------------------------------- fused blur-subtract
+// out = res - box_y(box_x(res)) on (nplanes, T, T) float32 planes, for a window
+// half-width D known at compile time (deconvolver.py:160-163 with
+// blur_radius = 2D or 2D+1).  A block owns a 64x64 output patch: it stages the
+// patch plus a D halo in LDS, computes box_x for the patch rows over the halo
+// columns, then box_y and the subtraction.  Every output keeps the sequential
+// low-to-high float32 window sum and the single division of blur.cu:20-51,
+// so results equal the two-pass form bit for bit; a thread produces 16
+// consecutive outputs from 16+2D values held in registers.
+template <int D>
+__global__ __launch_bounds__(256) void k_blur_sub_xy(const float* __restrict__ res, float* __restrict__ out, int T) {
+  constexpr int BX = 64, BY = 64, XB = 16, W = XB + 2 * D;
+  constexpr int TWc = BY + 2 * D;      // staged columns (y)
+  constexpr int THr = BX + 2 * D;      // staged rows (x)
+  constexpr int P = TWc | 1;           // odd row pitch: conflict-free column walks
+  extern __shared__ __align__(16) unsigned char smem[];
+  float* tin = (float*)smem;           // [THr][P]: rows x0-D.., cols y0-D..
+  float* tb = tin + THr * P;           // [BX][P]: box_x of rows x0.., cols y0-D..
+  const int nb = (T + BX - 1) / BX;
+  const int x0 = (blockIdx.x / nb) * BX, y0 = (blockIdx.x % nb) * BY;
+  const long long pofs = (long long)blockIdx.y * T * T;
+  const float* rp = res + pofs;
+  float* op = out + pofs;
+  // stage the patch and its halo (zeros outside the plane; never summed)
+  const int tot = THr * TWc;
+  for (int t0 = threadIdx.x; t0 < tot; t0 += 8 * blockDim.x) {
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int t = t0 + u * blockDim.x;
+      const int r = t / TWc, c = t - r * TWc;
+      const int gx = x0 - D + r, gy = y0 - D + c;
+      v[u] = (t < tot && gx >= 0 && gx < T && gy >= 0 && gy < T) ? rp[(long long)gx * T + gy] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int t = t0 + u * blockDim.x;
+      const int r = t / TWc, c = t - r * TWc;
+      if (t < tot) tin[r * P + c] = v[u];
+    }
+  }
+  __syncthreads();
+  // box along x for patch rows, all staged columns
+  for (int it = threadIdx.x; it < TWc * (BX / XB); it += blockDim.x) {
+    const int c = it % TWc, xb = it / TWc;
+    const int gy = y0 - D + c;
+    const int xbase = x0 + xb * XB;
+    if (gy < 0 || gy >= T || xbase >= T) continue;
+    float v[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) v[w] = tin[(xb * XB + w) * P + c];
+    const bool interior = xbase - D >= 0 && xbase + XB - 1 + D <= T - 1;
+#pragma unroll
+    for (int xo = 0; xo < XB; ++xo) {
+      const int gx = xbase + xo;
+      float sum = 0.0f;
+      int num;
+      if (interior) {
+#pragma unroll
+        for (int w = 0; w <= 2 * D; ++w) sum += v[xo + w];
+        num = 2 * D + 1;
+      } else {
+        const int lo = max(0, gx - D), hi = min(T - 1, gx + D);
+#pragma unroll
+        for (int w = 0; w <= 2 * D; ++w) {
+          const int gi = gx - D + w;
+          if (gi >= lo && gi <= hi) sum += v[xo + w];
+        }
+        num = hi - lo + 1;
+      }
+      if (gx < T) tb[(xb * XB + xo) * P + c] = sum / (float)num;
+    }
+  }
+  __syncthreads();
+  // box along y and subtract
+  {
+    const int yb = threadIdx.x % (BY / XB), xr = threadIdx.x / (BY / XB);
+    const int gx = x0 + xr;
+    const int ybase = y0 + yb * XB;
+    if (xr < BX && gx < T && ybase < T) {
+      float u[W];
+#pragma unroll
+      for (int w = 0; w < W; ++w) u[w] = tb[xr * P + yb * XB + w];
+      const bool interior = ybase - D >= 0 && ybase + XB - 1 + D <= T - 1;
+#pragma unroll
+      for (int yo = 0; yo < XB; ++yo) {
+        const int gy = ybase + yo;
+        float sum = 0.0f;
+        int num;
+        if (interior) {
+#pragma unroll
+          for (int w = 0; w <= 2 * D; ++w) sum += u[yo + w];
+          num = 2 * D + 1;
+        } else {
+          const int lo = max(0, gy - D), hi = min(T - 1, gy + D);
+#pragma unroll
+          for (int w = 0; w <= 2 * D; ++w) {
+            const int gi = gy - D + w;
+            if (gi >= lo && gi <= hi) sum += u[yo + w];
+          }
+          num = hi - lo + 1;
+        }
+        if (gy < T) op[(long long)gx * T + gy] = tin[(xr + D) * P + yb * XB + yo + D] - sum / (float)num;
+      }
+    }
+  }
+}
+
+template <int D>
+static int launch_blur_sub(const float* res, float* out, int nplanes, int T, hipStream_t st) {
+  constexpr int BX = 64, P = (64 + 2 * D) | 1;
+  const size_t lds = sizeof(float) * (size_t)P * ((BX + 2 * D) + BX);
+  const int nb = (T + BX - 1) / BX;
+  for (int p0 = 0; p0 < nplanes; p0 += 65535) {
+    const int np = nplanes - p0 < 65535 ? nplanes - p0 : 65535;
+    hipLaunchKernelGGL(k_blur_sub_xy<D>, dim3(nb * nb, np), dim3(256), lds, st, res + (long long)p0 * T * T,
+                       out + (long long)p0 * T * T, T);
+    MM_LAUNCH_CHECK();
+  }
+  return MM_OK;
+}
+
+// out = res - box_y(box_x(res)) over nplanes (T, T) planes; `out` must not overlap
+// `res`.  delta 15 and 25 (blur_radius 30/31 and 50/51) run fused; other widths
+// run the two single-axis passes through `tmp` (nplanes*T*T floats, unused and
+// may be NULL for the fused widths).
+extern "C" int mm_blur_sub_xy(const float* res, float* out, float* tmp, int nplanes, int T, int delta, void* stream) {
+  if (!res || !out || nplanes < 0 || T < 1 || delta < 0) { mm_set_error("mm_blur_sub_xy: bad argument"); return MM_EINVAL; }
+  if (nplanes == 0) return MM_OK;
+  hipStream_t st = (hipStream_t)stream;
+  if (delta == 15) return launch_blur_sub<15>(res, out, nplanes, T, st);
+  if (delta == 25) return launch_blur_sub<25>(res, out, nplanes, T, st);
+  if (!tmp) { mm_set_error("mm_blur_sub_xy: delta %d needs a temporary buffer", delta); return MM_EINVAL; }
+  const int rc = box_launch(res, nullptr, tmp, nplanes, T, T, delta, 1, false, stream);
+  if (rc) return rc;
+  return box_launch(tmp, res, out, nplanes, T, T, delta, 2, true, stream);
+}
+
 // ------------------------------------------------------------------ reflect
 struct NdDesc {
   int ndim;

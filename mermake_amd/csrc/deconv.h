@@ -73,6 +73,9 @@ struct mm_deconv {
   FftPlanDev<float2> pT, pZ;
   FftPlanDev<double2> pT64, pZ64;
   int gT_r2c, gT_x, gZ, gT64_r2c, gT64_x, gZ64;
+  int zconv;                // z pass as a folded z-kernel matrix (sz <= 48) instead of z FFTs
+  int zG;                   // columns per block of the z-kernel pass
+  int* zsrc_dev;            // per data plane, the padded planes that copy it (4 slots, -1 padded)
   int prof;                 // record events around the five passes of mm_deconv_tiles
   hipEvent_t ev[6];
 };

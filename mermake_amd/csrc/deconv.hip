@@ -10,10 +10,15 @@
 //                      in the loader from closed-form index maps, so only the
 //                      sz data planes are transformed (the z pads are plane copies);
 //   2. k_fft_c2c       C2C along x (in place, strided);
-//   3. k_tile_zwiener  per (kx,ky) column: gather the Zp padded planes through the
-//                      z map, forward FFT, multiply by W, inverse FFT, keep the
-//                      sz cropped planes (in place).  With one PSF, W is loaded
-//                      once per column block and reused for every tile;
+//   3. k_tile_zconv    per (kx,ky) column, the z pass (reflection padding to Zp
+//                      planes, FFT, * W, inverse FFT, crop to sz planes) is one
+//                      linear map of the sz data planes: y[t] = sum_s K[t][s] x[s]
+//                      with K[t][s] = sum over the padded copies p of plane s of
+//                      w[(zpad + t - p) mod Zp], w = inverse z-DFT of W (computed
+//                      once, in float64, when the spectra are built).  K lives in
+//                      registers and, with one PSF, is reused for every tile.
+//                      (k_tile_zwiener, the FFT form of the same pass, serves
+//                      stacks deeper than 48 planes);
 //   4. k_fft_c2c       inverse C2C along x;
 //   5. k_tile_c2r      C2R along y (two rows per complex transform) -> res (sz,T,T).
 // The 1/(Zp*T*T) normalisation of ifftn is folded into W.
@@ -22,6 +27,8 @@
 #include "mermake_hip.h"
 
 #include <vector>
+
+#define MM_UNR 8  // global loads issued back to back before the LDS stores
 
 static_assert(sizeof(mm_tile_dev) == sizeof(mm_tile_desc) && sizeof(mm_tile_desc) == 104,
               "tile descriptor layout");
@@ -33,28 +40,36 @@ __global__ __launch_bounds__(256) void k_fft_c2c(C* data, long long estride, lon
                                                  double scale) {
   typedef decltype(data[0].x) S;
   extern __shared__ __align__(16) unsigned char smem[];
-  C* tw = (C*)smem;
-  C* a = tw + P.N;
-  C* b = a + (size_t)G * P.N;
+  const FftLds<C> L = fft_lds_carve(smem, P, G);
+  C* a = L.a;
   const int N = P.N;
   const int ncb = (ncols + G - 1) / G;
   const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
   const int cb = bid % ncb;
   const int batch = bid / ncb;
-  load_twiddles(tw, P);
   C* base = data + (long long)batch * bstride + (long long)cb * G * cstride;
   const int gmax = min(G, ncols - cb * G);
   const int tot = G * N;
-  for (int t = threadIdx.x; t < tot; t += blockDim.x) {
-    int g, n;
-    if (gfast) { g = t % G; n = t / G; } else { n = t % N; g = t / N; }
-    C v;
-    v.x = 0; v.y = 0;
-    if (g < gmax) v = base[g * cstride + n * estride];
-    a[g * N + n] = v;
+  for (int t0 = threadIdx.x; t0 < tot; t0 += MM_UNR * blockDim.x) {  // MM_UNR loads in flight per thread
+    C v[MM_UNR];
+#pragma unroll
+    for (int u = 0; u < MM_UNR; ++u) {
+      const int t = t0 + u * blockDim.x;
+      int g, n;
+      if (gfast) { g = t % G; n = t / G; } else { n = t % N; g = t / N; }
+      v[u].x = 0; v[u].y = 0;
+      if (t < tot && g < gmax) v[u] = base[g * cstride + n * estride];
+    }
+#pragma unroll
+    for (int u = 0; u < MM_UNR; ++u) {
+      const int t = t0 + u * blockDim.x;
+      int g, n;
+      if (gfast) { g = t % G; n = t / G; } else { n = t % N; g = t / N; }
+      if (t < tot) a[g * N + n] = v[u];
+    }
   }
   __syncthreads();
-  C* r = fft_lds<INV>(a, b, tw, P, G);
+  C* r = fft_lds<INV>(L.a, L.b, L.tw, P, G, L.coef);
   const S sc = (S)scale;
   for (int t = threadIdx.x; t < tot; t += blockDim.x) {
     int g, n;
@@ -64,33 +79,15 @@ __global__ __launch_bounds__(256) void k_fft_c2c(C* data, long long estride, lon
 }
 
 // ---------------------------------------------------------------- R2C rows
-// Row loaders return the real value at column y of row q (rows of one batch
-// entry are numbered 0..nrows-1).
-struct TileRowLoader {  // padded tile rows (z, x), z in [0,sz), x in [0,T)
-  const mm_tile_dev* tiles;
-  int T;
-  __device__ __forceinline__ float operator()(const mm_tile_dev& td, int q, int y) const {
-    const int z = q / T;
-    const int p = q - z * T;
-    const int r = pad_map(p, td.xs, td.xd, td.xfirst, T);
-    const int c = pad_map(y, td.ys, td.yd, td.yfirst, T);
-    if (r < 0 || c < 0) return 0.0f;
-    return tile_src(td, z, r, c);
-  }
-};
-
 template <typename C>
 __global__ __launch_bounds__(256) void k_r2c_rows_plain(const float* vol, int nrows, C* out, FftPlanDev<C> P,
                                                         int G) {
   typedef decltype(out[0].x) S;
   extern __shared__ __align__(16) unsigned char smem[];
   const int N = P.N, H = N / 2 + 1;
-  C* tw = (C*)smem;
-  C* a = tw + N;
-  C* b = a + (size_t)G * N;
+  const FftLds<C> L = fft_lds_carve(smem, P, G);
   const int q0 = blockIdx.x * 2 * G;
-  load_twiddles(tw, P);
-  S* af = (S*)a;
+  S* af = (S*)L.a;
   for (int t = threadIdx.x; t < 2 * G * N; t += blockDim.x) {
     const int row = t / N, y = t - row * N, q = q0 + row;
     S v = 0;
@@ -98,7 +95,7 @@ __global__ __launch_bounds__(256) void k_r2c_rows_plain(const float* vol, int nr
     af[((row >> 1) * N + y) * 2 + (row & 1)] = v;
   }
   __syncthreads();
-  C* r = fft_lds<false>(a, b, tw, P, G);
+  C* r = fft_lds<false>(L.a, L.b, L.tw, P, G, L.coef);
   for (int t = threadIdx.x; t < G * H; t += blockDim.x) {
     const int g = t / H, k = t - g * H;
     const C Z = r[g * N + k], Zm = r[g * N + (k == 0 ? 0 : N - k)];
@@ -115,23 +112,40 @@ __global__ __launch_bounds__(256) void k_tile_r2c(const mm_tile_dev* __restrict_
                                                   float2* S, long long s_tstride, FftPlanDev<float2> P, int G) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int N = P.N, H = N / 2 + 1;
-  float2* tw = (float2*)smem;
-  float2* a = tw + N;
-  float2* b = a + (size_t)G * N;
+  const FftLds<float2> L = fft_lds_carve(smem, P, G);
   const int tile = blockIdx.y;
   const int q0 = blockIdx.x * 2 * G;
   const mm_tile_dev td = tiles[tile];
-  const TileRowLoader ld{tiles, T};
-  load_twiddles(tw, P);
-  float* af = (float*)a;
-  for (int t = threadIdx.x; t < 2 * G * N; t += blockDim.x) {
-    const int row = t / N, y = t - row * N, q = q0 + row;
-    float v = 0.0f;
-    if (q < nrows) v = ld(td, q, y);
-    af[((row >> 1) * N + y) * 2 + (row & 1)] = v;
+  float* af = (float*)L.a;
+  const int tot = 2 * G * N;
+  for (int t0 = threadIdx.x; t0 < tot; t0 += MM_UNR * blockDim.x) {  // issue all loads, then divide
+    float v[MM_UNR], f[MM_UNR];
+#pragma unroll
+    for (int u = 0; u < MM_UNR; ++u) {
+      const int t = t0 + u * blockDim.x;
+      const int row = t / N, y = t - row * N, q = q0 + row;
+      v[u] = 0.0f;
+      f[u] = 1.0f;
+      if (t < tot && q < nrows) {
+        const int z = q / T;
+        const int rr = pad_map(q - z * T, td.xs, td.xd, td.xfirst, T);
+        const int cc = pad_map(y, td.ys, td.yd, td.yfirst, T);
+        if (rr >= 0 && cc >= 0) {
+          const long long o = (long long)z * td.img_sz + (long long)(td.X0 + rr) * td.img_sx + (td.Y0 + cc);
+          v[u] = td.img_f32 ? ((const float*)td.img)[o] : (float)((const uint16_t*)td.img)[o];
+          if (td.flat) f[u] = td.flat[(long long)(td.X0 + rr) * td.flat_sx + (td.Y0 + cc)];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < MM_UNR; ++u) {
+      const int t = t0 + u * blockDim.x;
+      const int row = t / N, y = t - row * N;
+      if (t < tot) af[((row >> 1) * N + y) * 2 + (row & 1)] = td.flat ? v[u] / f[u] : v[u];
+    }
   }
   __syncthreads();
-  float2* r = fft_lds<false>(a, b, tw, P, G);
+  float2* r = fft_lds<false>(L.a, L.b, L.tw, P, G, L.coef);
   float2* out = S + (long long)tile * s_tstride;
   for (int t = threadIdx.x; t < G * H; t += blockDim.x) {
     const int g = t / H, k = t - g * H;
@@ -154,10 +168,10 @@ __global__ __launch_bounds__(256) void k_tile_zwiener(float2* S, long long s_tst
                                                       FftPlanDev<float2> P, int G) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int Zp = P.N;
-  float2* tw = (float2*)smem;
-  float2* a = tw + Zp;
-  float2* b = a + (size_t)G * Zp;
-  float2* wl = b + (size_t)G * Zp;
+  const FftLds<float2> L = fft_lds_carve(smem, P, G);
+  float2* a = L.a;
+  float2* b = L.b;
+  float2* wl = (float2*)L.extra;
   int* zm = (int*)(wl + (size_t)G * Zp);
   const int ncb = (TH + G - 1) / G;
   const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
@@ -165,7 +179,6 @@ __global__ __launch_bounds__(256) void k_tile_zwiener(float2* S, long long s_tst
   const int tg = bid / ncb;
   const int c0 = cb * G;
   const int gmax = min(G, TH - c0);
-  load_twiddles(tw, P);
   for (int i = threadIdx.x; i < Zp; i += blockDim.x) zm[i] = zmap[i];
   int cached = -1;
   const int t_end = min(ntiles, (tg + 1) * tpb);
@@ -187,10 +200,10 @@ __global__ __launch_bounds__(256) void k_tile_zwiener(float2* S, long long s_tst
       a[g * Zp + p] = (g < gmax && zs >= 0) ? St[(long long)zs * TH + g] : make_float2(0.f, 0.f);
     }
     __syncthreads();
-    float2* r = fft_lds<false>(a, b, tw, P, G);
+    float2* r = fft_lds<false>(a, b, L.tw, P, G, L.coef);
     for (int t = threadIdx.x; t < G * Zp; t += blockDim.x) r[t] = cmul(r[t], wl[t]);
     __syncthreads();
-    float2* o = fft_lds<true>(r, r == a ? b : a, tw, P, G);
+    float2* o = fft_lds<true>(r, r == a ? b : a, L.tw, P, G, L.coef);
     for (int t = threadIdx.x; t < G * sz; t += blockDim.x) {
       const int g = t % G, p = t / G;
       if (g < gmax) St[(long long)p * TH + g] = o[g * Zp + p + zpad];
@@ -203,27 +216,37 @@ __global__ __launch_bounds__(256) void k_tile_c2r(const float2* S, long long s_t
                                                   long long r_tstride, int nrows, FftPlanDev<float2> P, int G) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int N = P.N, H = N / 2 + 1;
-  float2* tw = (float2*)smem;
-  float2* a = tw + N;
-  float2* b = a + (size_t)G * N;
+  const FftLds<float2> L = fft_lds_carve(smem, P, G);
+  float2* a = L.a;
   const int tile = blockIdx.y;
   const int q0 = blockIdx.x * 2 * G;
-  load_twiddles(tw, P);
   const float2* in = S + (long long)tile * s_tstride;
-  for (int t = threadIdx.x; t < G * N; t += blockDim.x) {
-    const int g = t / N, k = t - g * N;
-    const int q = q0 + 2 * g;
-    const bool cj = k >= H;
-    const int kk = cj ? N - k : k;
-    float2 A = make_float2(0.f, 0.f), B = make_float2(0.f, 0.f);
-    if (q < nrows) A = in[(long long)q * H + kk];
-    if (q + 1 < nrows) B = in[(long long)(q + 1) * H + kk];
-    if (cj) { A.y = -A.y; B.y = -B.y; }
-    if (k == 0 || 2 * k == N) { A.y = 0.f; B.y = 0.f; }  // real part semantics of ifftn(...).real
-    a[g * N + k] = make_float2(A.x - B.y, A.y + B.x);
+  const int tot = G * N;
+  for (int t0 = threadIdx.x; t0 < tot; t0 += MM_UNR * blockDim.x) {
+    float2 A[MM_UNR], B[MM_UNR];
+#pragma unroll
+    for (int u = 0; u < MM_UNR; ++u) {
+      const int t = t0 + u * blockDim.x;
+      const int g = t / N, k = t - g * N;
+      const int q = q0 + 2 * g;
+      const int kk = k >= H ? N - k : k;
+      A[u] = make_float2(0.f, 0.f);
+      B[u] = make_float2(0.f, 0.f);
+      if (t < tot && q < nrows) A[u] = in[(long long)q * H + kk];
+      if (t < tot && q + 1 < nrows) B[u] = in[(long long)(q + 1) * H + kk];
+    }
+#pragma unroll
+    for (int u = 0; u < MM_UNR; ++u) {
+      const int t = t0 + u * blockDim.x;
+      const int g = t / N, k = t - g * N;
+      float2 a0 = A[u], b0 = B[u];
+      if (k >= H) { a0.y = -a0.y; b0.y = -b0.y; }
+      if (k == 0 || 2 * k == N) { a0.y = 0.f; b0.y = 0.f; }  // real part semantics of ifftn(...).real
+      if (t < tot) a[g * N + k] = make_float2(a0.x - b0.y, a0.y + b0.x);
+    }
   }
   __syncthreads();
-  float2* r = fft_lds<true>(a, b, tw, P, G);
+  float2* r = fft_lds<true>(L.a, L.b, L.tw, P, G, L.coef);
   float* out = res + (long long)tile * r_tstride;
   for (int t = threadIdx.x; t < 2 * G * N; t += blockDim.x) {
     const int row = t / N, y = t - row * N, q = q0 + row;
@@ -258,8 +281,8 @@ __global__ void k_psf_place(const float* __restrict__ psf, int sz, int ts, int z
 // W = conj(H) / (|H|^2 + beta |L|^2) * scale, |L|^2 in closed form for the
 // Laplacian of deconvolver.py:24-36 (6 at the centre, -1 at z+-1, x+-1 and only
 // y-1 on the last axis): |L|^2 = (6 - 2cos a - 2cos b - cos c)^2 + sin^2 c.
-__global__ void k_wiener(const double2* __restrict__ Hs, float2* __restrict__ W, int Zp, int T, int H, double beta,
-                         double scale) {
+// Computed in float64, in place over the PSF spectrum.
+__global__ void k_wiener64(double2* __restrict__ Hs, int Zp, int T, int H, double beta, double scale) {
   const long long total = (long long)Zp * T * H;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
        i += (long long)gridDim.x * blockDim.x) {
@@ -270,8 +293,116 @@ __global__ void k_wiener(const double2* __restrict__ Hs, float2* __restrict__ W,
     const double re = 6.0 - 2.0 * cospi(2.0 * kz / Zp) - 2.0 * cospi(2.0 * kx / T) - cospi(2.0 * ky / T);
     const double im = sinpi(2.0 * ky / T);
     const double den = h.x * h.x + h.y * h.y + beta * (re * re + im * im);
-    W[i] = make_float2((float)(h.x / den * scale), (float)(-h.y / den * scale));
+    Hs[i] = make_double2(h.x / den * scale, -h.y / den * scale);
   }
+}
+
+__global__ void k_d2f(const double2* __restrict__ a, float2* __restrict__ b, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    b[i] = make_float2((float)a[i].x, (float)a[i].y);
+}
+
+// ------------------------------------------------------------ z-kernel pass
+#define ZC_PF 8  // tiles whose columns are in flight per block (software pipeline depth)
+
+template <int SZM>
+__global__ __launch_bounds__(640) void k_tile_zconv(float2* S, long long s_tstride, int TH, int sz, int zpad, int Zp,
+                                                    const int* __restrict__ zsrc, const float2* __restrict__ Wz,
+                                                    long long w_stride, const mm_tile_dev* __restrict__ tiles,
+                                                    int ntiles, int tpb, int G) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  float2* wl = (float2*)smem;                        // [p][g]
+  float2* xl = wl + (size_t)G * Zp;                  // [j][s][g]
+  int* sl = (int*)(xl + (size_t)ZC_PF * sz * G);     // [s][4]
+  const int ncb = (TH + G - 1) / G;
+  const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int cb = bid % ncb, tg = bid / ncb;
+  const int c0 = cb * G;
+  const int gmax = min(G, TH - c0);
+  const int tid = threadIdx.x;
+  const int g = tid % G, t = tid / G;
+  const bool act = t < sz;
+  const bool col = act && g < gmax;
+  const int t0 = tg * tpb, t1 = min(ntiles, t0 + tpb);
+  for (int i = tid; i < sz * 4; i += blockDim.x) sl[i] = zsrc[i];
+  const float2* Wp = Wz + (long long)tiles[t0].psf * w_stride + c0;  // a group shares one PSF
+  for (int i = tid; i < G * Zp; i += blockDim.x) {
+    const int gg = i % G, p = i / G;
+    wl[p * G + gg] = gg < gmax ? Wp[(long long)p * TH + gg] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  float2 K[SZM];
+#pragma unroll
+  for (int s = 0; s < SZM; ++s) {
+    float2 acc = make_float2(0.f, 0.f);
+    if (act && s < sz) {
+      for (int q = 0; q < 4; ++q) {
+        const int p = sl[s * 4 + q];
+        if (p >= 0) {
+          int idx = zpad + t - p;
+          idx = idx < 0 ? idx + Zp : (idx >= Zp ? idx - Zp : idx);
+          const float2 w = wl[idx * G + g];
+          acc.x += w.x;
+          acc.y += w.y;
+        }
+      }
+    }
+    K[s] = acc;
+  }
+  const long long cofs = (long long)t * TH + c0 + g;
+  float2 pre[ZC_PF];
+#pragma unroll
+  for (int j = 0; j < ZC_PF; ++j)
+    pre[j] = (col && t0 + j < t1) ? S[(long long)(t0 + j) * s_tstride + cofs] : make_float2(0.f, 0.f);
+  for (int cs = t0; cs < t1; cs += ZC_PF) {
+    __syncthreads();  // the previous chunk is no longer read from xl
+    if (act) {
+#pragma unroll
+      for (int j = 0; j < ZC_PF; ++j) xl[((size_t)j * sz + t) * G + g] = pre[j];
+    }
+    __syncthreads();
+    const int nxt = cs + ZC_PF;
+#pragma unroll
+    for (int j = 0; j < ZC_PF; ++j)
+      pre[j] = (col && nxt + j < t1) ? S[(long long)(nxt + j) * s_tstride + cofs] : make_float2(0.f, 0.f);
+#pragma unroll
+    for (int j = 0; j < ZC_PF; ++j) {
+      if (col && cs + j < t1) {
+        const float2* xv = xl + (size_t)j * sz * G + g;
+        float2 y = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int s = 0; s < SZM; ++s) {
+          if (s < sz) {
+            const float2 x = xv[s * G];
+            y.x += K[s].x * x.x - K[s].y * x.y;
+            y.y += K[s].x * x.y + K[s].y * x.x;
+          }
+        }
+        S[(long long)(cs + j) * s_tstride + cofs] = y;
+      }
+    }
+  }
+}
+
+static int launch_zconv(mm_deconv* d, float2* S, long long s_ts, const float2* Wz, const mm_tile_dev* td, int ntiles,
+                        int tpb, hipStream_t st) {
+  const int sz = d->sz, Zp = d->Zp, TH = d->T * d->H, G = d->zG;
+  const int ngroups = (ntiles + tpb - 1) / tpb;
+  const long long nblk = (long long)((TH + G - 1) / G) * ngroups;
+  const int threads = ((G * sz + 63) / 64) * 64;
+  const size_t lds = sizeof(float2) * ((size_t)G * Zp + (size_t)ZC_PF * sz * G) + sizeof(int) * 4 * sz;
+#define MM_ZC(M)                                                                                                  \
+  hipLaunchKernelGGL(k_tile_zconv<M>, dim3((unsigned)nblk), dim3(threads), lds, st, S, s_ts, TH, sz, d->zpad, Zp, \
+                     d->zsrc_dev, Wz, (long long)Zp * TH, td, ntiles, tpb, G)
+  if (sz <= 8) MM_ZC(8);
+  else if (sz <= 16) MM_ZC(16);
+  else if (sz <= 24) MM_ZC(24);
+  else if (sz <= 32) MM_ZC(32);
+  else if (sz <= 40) MM_ZC(40);
+  else MM_ZC(48);
+#undef MM_ZC
+  MM_LAUNCH_CHECK();
+  return MM_OK;
 }
 
 // ------------------------------------------------------------------- host
@@ -320,12 +451,32 @@ extern "C" int mm_deconv_create(int sz, int tile_size, int overlap, int zpad, mm
     delete d;
     return MM_ENOMEM;
   }
-  d->gT_r2c = mm_fft_pick_g(d->T, 8, 65536, 0, 8);
-  d->gT_x = mm_fft_pick_g(d->T, 8, 65536, 0, 8);
-  d->gZ = mm_fft_pick_g(d->Zp, 8, 49152, 8 * d->Zp + 0, 16);
-  d->gT64_r2c = mm_fft_pick_g(d->T, 16, 98304, 0, 8);
-  d->gT64_x = mm_fft_pick_g(d->T, 16, 98304, 0, 8);
-  d->gZ64 = mm_fft_pick_g(d->Zp, 16, 98304, 0, 16);
+  // two blocks per CU (LDS <= 80 KB) for the T-length passes
+  d->gT_r2c = mm_fft_pick_g(d->pT, 81920, 0, 8);
+  d->gT_x = mm_fft_pick_g(d->pT, 81920, 0, 8);
+  d->gZ = mm_fft_pick_g(d->pZ, 65536, 8 * d->Zp, 16);
+  d->gT64_r2c = mm_fft_pick_g(d->pT64, 98304, 0, 8);
+  d->gT64_x = mm_fft_pick_g(d->pT64, 98304, 0, 8);
+  d->gZ64 = mm_fft_pick_g(d->pZ64, 98304, 0, 16);
+  // z-kernel pass: the padded planes that copy each data plane
+  d->zconv = sz <= 48;
+  if (d->zconv) {
+    std::vector<int> src(4 * sz, -1), cnt(sz, 0);
+    for (int p = 0; p < d->Zp; ++p) {
+      const int s = zm[p];
+      if (s < 0) continue;
+      if (cnt[s] >= 4) { d->zconv = 0; break; }
+      src[4 * s + cnt[s]++] = p;
+    }
+    d->zG = 640 / sz;
+    if (d->zG > 64) d->zG = 64;
+    if (d->zconv && (hipMalloc(&d->zsrc_dev, sizeof(int) * 4 * sz) != hipSuccess ||
+                     hipMemcpy(d->zsrc_dev, src.data(), sizeof(int) * 4 * sz, hipMemcpyHostToDevice) != hipSuccess)) {
+      mm_set_error("mm_deconv_create: cannot upload the z source table");
+      delete d;
+      return MM_ENOMEM;
+    }
+  }
   *out = d;
   return MM_OK;
 }
@@ -333,6 +484,7 @@ extern "C" int mm_deconv_create(int sz, int tile_size, int overlap, int zpad, mm
 extern "C" int mm_deconv_destroy(mm_deconv* d) {
   if (!d) return MM_OK;
   if (d->zmap_dev) (void)hipFree(d->zmap_dev);
+  if (d->zsrc_dev) (void)hipFree(d->zsrc_dev);
   if (d->prof)
     for (int i = 0; i < 6; ++i) (void)hipEventDestroy(d->ev[i]);
   delete d;
@@ -360,7 +512,7 @@ static int launch_c2c(C* data, long long estride, long long cstride, int ncols, 
   const int ncb = (ncols + G - 1) / G;
   const long long nblk = (long long)ncb * nbatch;
   if (nblk > 0x7fffffffLL) { mm_set_error("c2c grid too large"); return MM_EINVAL; }
-  const size_t lds = mm_fft_lds_bytes(P.N, sizeof(C), G, 0);
+  const size_t lds = fft_lds_bytes(P, G, 0);
   hipLaunchKernelGGL((k_fft_c2c<INV, C>), dim3((unsigned)nblk), dim3(256), lds, st, data, estride, cstride, ncols,
                      bstride, cstride == 1 ? 1 : 0, P, G, scale);
   MM_LAUNCH_CHECK();
@@ -381,7 +533,7 @@ extern "C" int mm_deconv_build_wiener(mm_deconv* d, const float* psf_centered, d
   {
     const int G = d->gT64_r2c;
     const int nrows = Zp * T;
-    const size_t lds = mm_fft_lds_bytes(T, 16, G, 0);
+    const size_t lds = fft_lds_bytes(d->pT64, G, 0);
     hipLaunchKernelGGL((k_r2c_rows_plain<double2>), dim3((nrows + 2 * G - 1) / (2 * G)), dim3(256), lds, st, pv,
                        nrows, Hs, d->pT64, G);
     MM_LAUNCH_CHECK();
@@ -390,7 +542,11 @@ extern "C" int mm_deconv_build_wiener(mm_deconv* d, const float* psf_centered, d
   if ((rc = launch_c2c<false>(Hs, H, 1, H, (long long)T * H, Zp, d->pT64, d->gT64_x, 1.0, st))) return rc;
   if ((rc = launch_c2c<false>(Hs, (long long)T * H, 1, T * H, 0, 1, d->pZ64, d->gZ64, 1.0, st))) return rc;
   const double scale = 1.0 / ((double)Zp * T * T);
-  hipLaunchKernelGGL(k_wiener, dim3(4096), dim3(256), 0, st, Hs, (float2*)W_out, Zp, T, H, beta, scale);
+  hipLaunchKernelGGL(k_wiener64, dim3(4096), dim3(256), 0, st, Hs, Zp, T, H, beta, scale);
+  MM_LAUNCH_CHECK();
+  if (d->zconv)  // z-domain kernels w = unnormalised inverse z-DFT of W (the 1/Zp is already in W)
+    if ((rc = launch_c2c<true>(Hs, (long long)T * H, 1, T * H, 0, 1, d->pZ64, d->gZ64, 1.0, st))) return rc;
+  hipLaunchKernelGGL(k_d2f, dim3(4096), dim3(256), 0, st, Hs, (float2*)W_out, (long long)Zp * T * H);
   MM_LAUNCH_CHECK();
   return MM_OK;
 }
@@ -417,7 +573,7 @@ extern "C" int mm_deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int 
   const int nrows = sz * T;
   {  // 1. R2C along y of the padded rows
     const int G = d->gT_r2c;
-    const size_t lds = mm_fft_lds_bytes(T, 8, G, 0);
+    const size_t lds = fft_lds_bytes(d->pT, G, 0);
     hipLaunchKernelGGL(k_tile_r2c, dim3((nrows + 2 * G - 1) / (2 * G), ntiles), dim3(256), lds, st, td, T, nrows, S,
                        s_ts, d->pT, G);
     MM_LAUNCH_CHECK();
@@ -427,13 +583,15 @@ extern "C" int mm_deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int 
   // 2. C2C along x, all tiles and planes at once
   if ((rc = launch_c2c<false>(S, H, 1, H, (long long)T * H, ntiles * sz, d->pT, d->gT_x, 1.0, st))) return rc;
   MM_MARK(2);
-  {  // 3. z pass with the Wiener multiply
+  const int tpb = tiles_per_block < 1 ? 1 : (tiles_per_block > ntiles ? ntiles : tiles_per_block);
+  if (d->zconv) {  // 3. z pass as the folded z-kernel matrix
+    if ((rc = launch_zconv(d, S, s_ts, (const float2*)W, td, ntiles, tpb, st))) return rc;
+  } else {  // 3. z pass with z FFTs and the Wiener multiply
     const int G = d->gZ;
     const int TH = T * H;
-    const int tpb = tiles_per_block < 1 ? 1 : (tiles_per_block > ntiles ? ntiles : tiles_per_block);
     const int ngroups = (ntiles + tpb - 1) / tpb;
     const long long nblk = (long long)((TH + G - 1) / G) * ngroups;
-    const size_t lds = mm_fft_lds_bytes(Zp, 8, G, 8 * Zp) + sizeof(int) * Zp;
+    const size_t lds = fft_lds_bytes(d->pZ, G, (size_t)8 * Zp * G + sizeof(int) * Zp);
     hipLaunchKernelGGL(k_tile_zwiener, dim3((unsigned)nblk), dim3(256), lds, st, S, s_ts, TH, sz, d->zpad,
                        d->zmap_dev, (const float2*)W, (long long)Zp * TH, td, ntiles, tpb, d->pZ, G);
     MM_LAUNCH_CHECK();
@@ -444,7 +602,7 @@ extern "C" int mm_deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int 
   MM_MARK(4);
   {  // 5. C2R along y into res
     const int G = d->gT_r2c;
-    const size_t lds = mm_fft_lds_bytes(T, 8, G, 0);
+    const size_t lds = fft_lds_bytes(d->pT, G, 0);
     hipLaunchKernelGGL(k_tile_c2r, dim3((nrows + 2 * G - 1) / (2 * G), ntiles), dim3(256), lds, st, S, s_ts, res,
                        (long long)sz * T * T, nrows, d->pT, G);
     MM_LAUNCH_CHECK();
@@ -539,15 +697,15 @@ extern "C" int mm_fft3d_c2c(void* data, int n0, int n1, int n2, int axes_mask, i
     if (rc) return rc;
     const double sc = inverse ? 1.0 / n[ax] : 1.0;
     if (ax == 2) {
-      const int G = mm_fft_pick_g(n2, 8, 65536, 0, 16);
+      const int G = mm_fft_pick_g(P, 98304, 0, 16);
       rc = inverse ? launch_c2c<true>(p, 1, n2, n0 * n1, 0, 1, P, G, sc, st)
                    : launch_c2c<false>(p, 1, n2, n0 * n1, 0, 1, P, G, sc, st);
     } else if (ax == 1) {
-      const int G = mm_fft_pick_g(n1, 8, 65536, 0, 16);
+      const int G = mm_fft_pick_g(P, 98304, 0, 16);
       rc = inverse ? launch_c2c<true>(p, n2, 1, n2, (long long)n1 * n2, n0, P, G, sc, st)
                    : launch_c2c<false>(p, n2, 1, n2, (long long)n1 * n2, n0, P, G, sc, st);
     } else {
-      const int G = mm_fft_pick_g(n0, 8, 65536, 0, 16);
+      const int G = mm_fft_pick_g(P, 98304, 0, 16);
       rc = inverse ? launch_c2c<true>(p, (long long)n1 * n2, 1, n1 * n2, 0, 1, P, G, sc, st)
                    : launch_c2c<false>(p, (long long)n1 * n2, 1, n1 * n2, 0, 1, P, G, sc, st);
     }

@@ -28,7 +28,10 @@ struct FftPlanDev {
   int N;
   int nst;
   int radix[MM_FFT_MAXST];
-  const C* tw;  // forward twiddles, N entries
+  const C* tw;        // forward twiddles, N entries
+  const float* coef;  // cos / sin matrices of the prime radix `coefR` (float32 plans), or nullptr
+  int coefR;          // prime handled as a register-tiled matrix product (0: none)
+  int coef_len;       // floats in `coef`: 2 * h * hp, h = (coefR-1)/2, hp = h rounded up to 8
 };
 
 template <typename C> __device__ __forceinline__ C mk(double x, double y);
@@ -231,11 +234,92 @@ __device__ __forceinline__ void stage_prime(C* __restrict__ src, C* __restrict__
   }
 }
 
+// Odd prime radix R as a matrix product (float32).  Step 1 twiddles the inputs
+// and replaces each mirrored pair (r, R-r) in place by its sum s_r and
+// difference d_r.  Step 2: for a butterfly and a tile of 8 outputs k,
+//   A_k = v0 + sum_r cos(2 pi r k / R) s_r,  B_k = sum_r sin(2 pi r k / R) d_r,
+//   X_k = A_k -/+ i B_k,  X_{R-k} = A_k +/- i B_k,
+// with the cos/sin matrices read from LDS (rows r, 8 consecutive k per row):
+// 32 FMAs per 2 data loads and 4 vector coefficient loads.
+template <bool INV>
+__device__ __forceinline__ void stage_prime_mm(float2* __restrict__ src, float2* __restrict__ dst,
+                                               const float2* __restrict__ tw, const float* __restrict__ coef,
+                                               int N, int R, int M, int Ns, int tws, int G) {
+  const int h = (R - 1) >> 1;
+  const int hp = (h + 7) & ~7;
+  const int total = G * M * h;
+  for (int t = threadIdx.x; t < total; t += blockDim.x) {
+    const int r = 1 + t % h;
+    const int rest = t / h;
+    const int j = rest % M;
+    const int g = rest / M;
+    const int k = j % Ns;
+    const int i1 = g * N + j + r * M, i2 = g * N + j + (R - r) * M;
+    const float2 a = vmul(src[i1], twl<INV>(tw, k * r * tws));
+    const float2 b = vmul(src[i2], twl<INV>(tw, k * (R - r) * tws));
+    src[i1] = vadd(a, b);
+    src[i2] = vsub(a, b);
+  }
+  __syncthreads();
+  const int nkt = (h + 7) >> 3;
+  const int total2 = G * M * (nkt + 1);
+  const float* cm = coef;
+  const float* sm = coef + h * hp;
+  for (int t = threadIdx.x; t < total2; t += blockDim.x) {
+    const int kt = t % (nkt + 1);
+    const int bfly = t / (nkt + 1);
+    const int j = bfly % M;
+    const int g = bfly / M;
+    const int kk = j % Ns;
+    const float2* in = src + g * N + j;
+    float2* out = dst + g * N + (j - kk) * R + kk;
+    const float2 v0 = in[0];
+    if (kt == nkt) {
+      float2 acc = v0;
+      for (int r = 1; r <= h; ++r) acc = vadd(acc, in[r * M]);
+      out[0] = acc;
+      continue;
+    }
+    float2 A[8], B[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) { A[q] = v0; B[q] = make_float2(0.f, 0.f); }
+    const int k0 = kt * 8;
+    for (int r = 1; r <= h; ++r) {
+      const float2 sv = in[r * M];
+      const float2 dv = in[(R - r) * M];
+      const float4 c0 = *(const float4*)(cm + (r - 1) * hp + k0);
+      const float4 c1 = *(const float4*)(cm + (r - 1) * hp + k0 + 4);
+      const float4 s0 = *(const float4*)(sm + (r - 1) * hp + k0);
+      const float4 s1 = *(const float4*)(sm + (r - 1) * hp + k0 + 4);
+      const float c[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+      const float q8[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        A[q].x += sv.x * c[q];
+        A[q].y += sv.y * c[q];
+        B[q].x += dv.x * q8[q];
+        B[q].y += dv.y * q8[q];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int kq = k0 + 1 + q;
+      if (kq <= h) {
+        const float2 iB = vmi<INV>(B[q]);
+        out[kq * Ns] = vadd(A[q], iB);
+        out[(R - kq) * Ns] = vsub(A[q], iB);
+      }
+    }
+  }
+}
+
 // Runs all passes on the G transforms in `a` (scratch `b`); returns the buffer
 // holding the result.  Must be called by every thread of the block; a
-// __syncthreads() must separate the writes to `a` from this call.
+// __syncthreads() must separate the writes to `a` from this call.  `coef` is
+// the LDS copy of P.coef (float32 plans with a matrix-product prime radix).
 template <bool INV, typename C>
-__device__ C* fft_lds(C* a, C* b, const C* __restrict__ tw, const FftPlanDev<C>& P, int G) {
+__device__ C* fft_lds(C* a, C* b, const C* __restrict__ tw, const FftPlanDev<C>& P, int G,
+                      const float* coef = nullptr) {
   const int N = P.N;
   C* src = a;
   C* dst = b;
@@ -251,7 +335,15 @@ __device__ C* fft_lds(C* a, C* b, const C* __restrict__ tw, const FftPlanDev<C>&
       case 5: stage_small<5, INV>(src, dst, tw, N, M, Ns, tws, G); break;
       case 7: stage_small<7, INV>(src, dst, tw, N, M, Ns, tws, G); break;
       case 8: stage_small<8, INV>(src, dst, tw, N, M, Ns, tws, G); break;
-      default: stage_prime<INV>(src, dst, tw, N, R, M, Ns, tws, G); break;
+      default:
+        if constexpr (sizeof(C) == 8) {
+          if (coef != nullptr && R == P.coefR) {
+            stage_prime_mm<INV>(src, dst, tw, coef, N, R, M, Ns, tws, G);
+            break;
+          }
+        }
+        stage_prime<INV>(src, dst, tw, N, R, M, Ns, tws, G);
+        break;
     }
     __syncthreads();
     C* t = src;
@@ -265,4 +357,47 @@ __device__ C* fft_lds(C* a, C* b, const C* __restrict__ tw, const FftPlanDev<C>&
 template <typename C>
 __device__ __forceinline__ void load_twiddles(C* lds_tw, const FftPlanDev<C>& P) {
   for (int i = threadIdx.x; i < P.N; i += blockDim.x) lds_tw[i] = P.tw[i];
+}
+
+// LDS layout of the FFT kernels: [twiddles N][coefficients][a: G*N][b: G*N][extra]
+template <typename C>
+struct FftLds {
+  C* tw;
+  float* coef;
+  C* a;
+  C* b;
+  unsigned char* extra;
+};
+
+__host__ __device__ __forceinline__ size_t mm_align16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+template <typename C>
+__host__ __device__ __forceinline__ int fft_coef_floats(const FftPlanDev<C>& P) {
+  return (sizeof(C) == 8 && P.coef) ? ((P.coef_len + 3) & ~3) : 0;
+}
+
+// Dynamic LDS bytes of fft_lds_carve for G transforms plus `extra` bytes.
+template <typename C>
+__host__ __device__ __forceinline__ size_t fft_lds_bytes(const FftPlanDev<C>& P, int G, size_t extra) {
+  return mm_align16((size_t)P.N * sizeof(C)) + mm_align16((size_t)fft_coef_floats(P) * 4) +
+         mm_align16(2 * (size_t)G * P.N * sizeof(C)) + extra;
+}
+
+template <typename C>
+__device__ __forceinline__ FftLds<C> fft_lds_carve(unsigned char* smem, const FftPlanDev<C>& P, int G) {
+  FftLds<C> L;
+  const int cl = fft_coef_floats(P);
+  size_t off = 0;
+  L.tw = (C*)(smem + off);
+  off += mm_align16((size_t)P.N * sizeof(C));
+  L.coef = (float*)(smem + off);
+  off += mm_align16((size_t)cl * 4);
+  L.a = (C*)(smem + off);
+  L.b = L.a + (size_t)G * P.N;
+  off += mm_align16(2 * (size_t)G * P.N * sizeof(C));
+  L.extra = smem + off;
+  for (int i = threadIdx.x; i < P.N; i += blockDim.x) L.tw[i] = P.tw[i];
+  for (int i = threadIdx.x; i < cl; i += blockDim.x) L.coef[i] = i < P.coef_len ? P.coef[i] : 0.0f;
+  if (cl == 0) L.coef = nullptr;
+  return L;
 }

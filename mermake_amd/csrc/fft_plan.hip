@@ -9,7 +9,32 @@
 namespace {
 
 std::mutex g_plan_mu;
-std::map<std::pair<int, int>, void*> g_tw;  // (device, N) -> twiddles (float2 or double2)
+std::map<std::pair<int, int>, void*> g_tw;    // (device, N) -> twiddles (float2 or double2)
+std::map<std::pair<int, int>, float*> g_coef;  // (device, R) -> cos/sin matrices of prime R
+
+// Largest coefficient table kept in LDS (bytes): primes up to 181.
+const int kCoefMaxBytes = 65536;
+
+// cos/sin(2 pi r k / R) for r, k in [1, h], rows r, row stride hp = h rounded up to 8.
+float* coef_table(int dev, int R, int* len) {
+  const int h = (R - 1) / 2, hp = (h + 7) & ~7;
+  *len = 2 * h * hp;
+  auto key = std::make_pair(dev, R);
+  auto it = g_coef.find(key);
+  if (it != g_coef.end()) return it->second;
+  std::vector<float> c((size_t)*len, 0.0f);
+  for (int r = 1; r <= h; ++r)
+    for (int k = 1; k <= h; ++k) {
+      const long double a = 2.0L * 3.141592653589793238462643383279502884L * (long double)((r * k) % R) / R;
+      c[(size_t)(r - 1) * hp + (k - 1)] = (float)cosl(a);
+      c[(size_t)h * hp + (size_t)(r - 1) * hp + (k - 1)] = (float)sinl(a);
+    }
+  float* d = nullptr;
+  if (hipMalloc(&d, sizeof(float) * c.size()) != hipSuccess) return nullptr;
+  if (hipMemcpy(d, c.data(), sizeof(float) * c.size(), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+  g_coef[key] = d;
+  return d;
+}
 
 // Radix schedule: 8s first, then one 4 or 2 for the remaining power of two,
 // then the odd primes in ascending order (3, 5, 7 are register butterflies,
@@ -62,6 +87,24 @@ int get_plan(int N, FftPlanDev<C>* out, int tag) {
   out->nst = (int)f.size();
   for (int i = 0; i < MM_FFT_MAXST; ++i) out->radix[i] = i < (int)f.size() ? f[i] : 0;
   out->tw = (const C*)tw;
+  out->coef = nullptr;
+  out->coefR = 0;
+  out->coef_len = 0;
+  if (sizeof(C) == 8) {  // float32 plans: the largest prime >= 11 whose matrices fit runs as a matrix product
+    int best = 0;
+    for (int r : f) {
+      const int h = (r - 1) / 2, hp = (h + 7) & ~7;
+      if (r >= 11 && r > best && 2 * h * hp * 4 <= kCoefMaxBytes) best = r;
+    }
+    if (best) {
+      int len = 0;
+      float* c = coef_table(dev, best, &len);
+      if (!c) { mm_set_error("cannot upload the coefficient table of radix %d", best); return MM_ENOMEM; }
+      out->coef = c;
+      out->coefR = best;
+      out->coef_len = len;
+    }
+  }
   return MM_OK;
 }
 
@@ -70,10 +113,3 @@ int get_plan(int N, FftPlanDev<C>* out, int tag) {
 int mm_get_plan_f32(int N, FftPlanDev<float2>* out) { return get_plan<float2>(N, out, 0); }
 int mm_get_plan_f64(int N, FftPlanDev<double2>* out) { return get_plan<double2>(N, out, 1); }
 
-int mm_fft_pick_g(int N, int esize, int budget, int extra_per_g, int gmax) {
-  long long per = 2LL * N * esize + extra_per_g;
-  long long g = ((long long)budget - (long long)N * esize) / per;
-  if (g > gmax) g = gmax;
-  if (g < 1) g = 1;
-  return (int)g;
-}

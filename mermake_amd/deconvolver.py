@@ -222,7 +222,9 @@ class Deconvolver:
         return max(1, min(ntiles, self.batch_bytes // per))
 
     def run_tiles(self, table, ntiles, tiles_per_block, blur_radius, res_out=None):
-        """Deconvolution (+ blur subtraction) of ``ntiles`` descriptors: ``(ntiles, sz, T, T)``."""
+        """Deconvolution (+ blur subtraction) of ``ntiles`` descriptors: ``(ntiles, sz, T, T)``.
+
+        With a blur the result lives in the spectrum workspace (valid until the next call)."""
         T, sz = self._plan.T, self.tile_height
         spec = self._workspace("spec", ntiles * self._plan.spec_bytes)
         res = res_out if res_out is not None else torch.empty((ntiles, sz, T, T), dtype=torch.float32,
@@ -230,12 +232,16 @@ class Deconvolver:
         st = L.stream_ptr(self.device)
         L.call("mm_deconv_tiles", self._plan.h, L.ptr(table), ntiles, L.ptr(self.psf_fft), self.psf_fft.shape[0],
                int(tiles_per_block), L.ptr(spec), L.ptr(res), st)
-        if blur_radius is not None:
-            # deconvolver.py:160-163: res -= box_y(box_x(res)); the spectrum buffer is free now
-            delta = int(blur_radius) // 2
-            L.call("mm_box_1d", L.ptr(res), L.ptr(spec), ntiles * sz, T, T, delta, 1, st)
-            L.call("mm_box_1d_sub", L.ptr(res), L.ptr(spec), L.ptr(res), ntiles * sz, T, T, delta, 2, st)
-        return res
+        if blur_radius is None:
+            return res
+        # deconvolver.py:160-163: res - box_y(box_x(res)); the spectrum buffer is free now
+        delta = int(blur_radius) // 2
+        out = spec[:ntiles * sz * T * T * 4].view(torch.float32).view(ntiles, sz, T, T)
+        tmp = None
+        if delta not in (15, 25):
+            tmp = self._workspace("blur_tmp", ntiles * sz * T * T * 4)
+        L.call("mm_blur_sub_xy", L.ptr(res), L.ptr(out), L.ptr(tmp), ntiles * sz, T, delta, st)
+        return out
 
     def _batches(self, image, flat_field, blur_radius, want_raw):
         self._check(image)

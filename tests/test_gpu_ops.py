@@ -270,8 +270,9 @@ def test_wiener_closed_form(dev):
     psf = np.zeros((sz, ts, ts), np.float32)
     psf[sz // 2, ts // 2, ts // 2] = 1
     d = Deconvolver(psf, (sz, 40, 40), tile_size=ts, zpad=zpad, overlap=ov, beta=beta)
-    W = d.psf_fft[0].cpu().numpy().astype(np.complex128)
     Zp, Tt = sz + 2 * zpad, ts + 2 * ov
+    W = d.psf_fft[0].cpu().numpy().astype(np.complex128)
+    W = np.fft.fft(W, axis=0) / Zp      # stacks of <= 48 planes keep w = unnormalised ifft_z(W)
     L2 = odec.laplacian_power((Zp, Tt, Tt))[:, :, :Tt // 2 + 1]
     want = 1.0 / (1.0 + beta * L2) / (Zp * Tt * Tt)
     assert np.abs(W - want).max() / want.max() < 1e-6
