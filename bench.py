@@ -81,6 +81,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-tile GPU form")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="end with torch.neg over 1 GiB of float32 (reads 1 GiB, writes 1 GiB): the reference "
+                         "kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
@@ -169,6 +172,14 @@ def main():
     if not args.no_cpu:
         log("CPU baseline (oracle, bounded sample)")
         cpu = cpu_baseline(stacks[0].cpu().numpy(), flat.cpu().numpy(), psf, cfg, len(geom))
+
+    if args.calib_copy:
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
 
     lat_sorted = sorted(lat)
     rec = {

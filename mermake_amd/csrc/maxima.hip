@@ -53,13 +53,8 @@ __device__ __forceinline__ int refl1(int n, int N) {  // maxima.cu:44-52
   return n;
 }
 
-__device__ __forceinline__ bool is_max(const VolDesc& V, long long idx, float sign, float thr, int delta) {
-  const long long hw = (long long)V.H * V.W;
-  const int z = (int)(idx / hw);
-  const int rem = (int)(idx - (long long)z * hw);
-  const int x = rem / V.W, y = rem - (rem / V.W) * V.W;
-  const float c = sign * V.p[z * V.sZ + x * V.sX + y];
-  if (c <= thr) return false;  // maxima.cu:19 (a NaN threshold rejects nothing)
+// Neighbourhood test of a voxel whose value c already passed the threshold.
+__device__ __forceinline__ bool is_max_at(const VolDesc& V, int z, int x, int y, float c, float sign, int delta) {
   const int d2 = delta * delta;
   for (int dz = -delta; dz <= delta; ++dz) {
     const int nz = refl1(z + dz, V.D);
@@ -92,12 +87,24 @@ __global__ __launch_bounds__(256) void k_max_mark(VS vs, float sign, float thr, 
   unsigned long long* mw = masks + ((long long)v * nblk + blockIdx.x) * MX_WORDS;
   __shared__ int wcnt[4];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long long hw = (long long)V.H * V.W;
+  int zz[4], xx[4], yy[4];
+  float c[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {  // all four centre loads in flight before any test
+    const long long idx = start + i * 256 + threadIdx.x;
+    zz[i] = (int)(idx / hw);
+    const int rem = (int)(idx - (long long)zz[i] * hw);
+    xx[i] = rem / V.W;
+    yy[i] = rem - xx[i] * V.W;
+    c[i] = idx < nvox ? sign * V.p[zz[i] * V.sZ + xx[i] * V.sX + yy[i]] : 0.0f;
+  }
   int cnt = 0;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const long long idx = start + i * 256 + threadIdx.x;
-    bool m = false;
-    if (idx < nvox) m = is_max(V, idx, sign, thr, delta);
+    // maxima.cu:19: v <= threshold is rejected (a NaN threshold rejects nothing)
+    const bool m = idx < nvox && !(c[i] <= thr) && is_max_at(V, zz[i], xx[i], yy[i], c[i], sign, delta);
     const unsigned long long b = __ballot(m);
     if (lane == 0) mw[i * 4 + wave] = b;
     cnt += __popcll(b);
