@@ -76,79 +76,89 @@ extern "C" int mm_box_1d_sub(const float* res, const float* buf, float* out, int
 // patch plus a D halo in LDS, computes box_x for the patch rows over the halo
 // columns, then box_y and the subtraction.  Every output keeps the sequential
 // low-to-high float32 window sum and the single division of blur.cu:20-51,
-// so results equal the two-pass form bit for bit; a thread produces 16
-// consecutive outputs from 16+2D values held in registers.
+// so results equal the two-pass form bit for bit; a thread produces 8
+// consecutive outputs from 8+2D values held in registers, and the next
+// plane's patch is loaded into registers while the current one is computed.
 template <int D>
-__global__ __launch_bounds__(256) void k_blur_sub_xy(const float* __restrict__ res, float* __restrict__ out, int T) {
-  constexpr int BX = 64, BY = 64, XB = 16, W = XB + 2 * D;
+__global__ __launch_bounds__(512, 2) void k_blur_sub_xy(const float* __restrict__ res, float* __restrict__ out, int T,
+                                                     int nplanes, int pz) {
+  constexpr int NT = 512;
+  constexpr int BX = 64, BY = 64, XB = 8, W = XB + 2 * D;
   constexpr int TWc = BY + 2 * D;      // staged columns (y)
   constexpr int THr = BX + 2 * D;      // staged rows (x)
   constexpr int P = TWc | 1;           // odd row pitch: conflict-free column walks
+  constexpr int TOT = THr * TWc;
+  constexpr int NPT = (TOT + NT - 1) / NT;  // staged values per thread
   extern __shared__ __align__(16) unsigned char smem[];
   float* tin = (float*)smem;           // [THr][P]: rows x0-D.., cols y0-D..
   float* tb = tin + THr * P;           // [BX][P]: box_x of rows x0.., cols y0-D..
   const int nb = (T + BX - 1) / BX;
-  const int x0 = (blockIdx.x / nb) * BX, y0 = (blockIdx.x % nb) * BY;
-  const long long pofs = (long long)blockIdx.y * T * T;
-  const float* rp = res + pofs;
-  float* op = out + pofs;
-  // stage the patch and its halo (zeros outside the plane; never summed)
-  const int tot = THr * TWc;
-  for (int t0 = threadIdx.x; t0 < tot; t0 += 8 * blockDim.x) {
-    float v[8];
+  // XCD-aware order: neighbouring patches of a plane group run on one XCD and share its L2 for the halos
+  const unsigned bid = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+  const int patch = bid % gridDim.x, pgrp = bid / gridDim.x;
+  const int x0 = (patch / nb) * BX, y0 = (patch % nb) * BY;
+  const int p0 = pgrp * pz, p1 = min(nplanes, p0 + pz);
+  // the next plane's patch + halo is loaded into registers while the current one is computed
+  float pf[NPT];
+  auto load = [&](int plane) {
+    const float* rp = res + (long long)plane * T * T;
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int t = t0 + u * blockDim.x;
+    for (int u = 0; u < NPT; ++u) {
+      const int t = threadIdx.x + u * NT;
       const int r = t / TWc, c = t - r * TWc;
       const int gx = x0 - D + r, gy = y0 - D + c;
-      v[u] = (t < tot && gx >= 0 && gx < T && gy >= 0 && gy < T) ? rp[(long long)gx * T + gy] : 0.0f;
+      pf[u] = (t < TOT && gx >= 0 && gx < T && gy >= 0 && gy < T) ? rp[(long long)gx * T + gy] : 0.0f;
     }
+  };
+  load(p0);
+  for (int plane = p0; plane < p1; ++plane) {
+    __syncthreads();  // the previous plane no longer reads tin / tb
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int t = t0 + u * blockDim.x;
+    for (int u = 0; u < NPT; ++u) {
+      const int t = threadIdx.x + u * NT;
       const int r = t / TWc, c = t - r * TWc;
-      if (t < tot) tin[r * P + c] = v[u];
+      if (t < TOT) tin[r * P + c] = pf[u];
     }
-  }
-  __syncthreads();
-  // box along x for patch rows, all staged columns
-  for (int it = threadIdx.x; it < TWc * (BX / XB); it += blockDim.x) {
-    const int c = it % TWc, xb = it / TWc;
-    const int gy = y0 - D + c;
-    const int xbase = x0 + xb * XB;
-    if (gy < 0 || gy >= T || xbase >= T) continue;
-    float v[W];
+    __syncthreads();
+    if (plane + 1 < p1) load(plane + 1);
+    // box along x for the patch rows, all staged columns
+    for (int it = threadIdx.x; it < TWc * (BX / XB); it += blockDim.x) {
+      const int c = it % TWc, xb = it / TWc;
+      const int gy = y0 - D + c;
+      const int xbase = x0 + xb * XB;
+      if (gy < 0 || gy >= T || xbase >= T) continue;
+      float v[W];
 #pragma unroll
-    for (int w = 0; w < W; ++w) v[w] = tin[(xb * XB + w) * P + c];
-    const bool interior = xbase - D >= 0 && xbase + XB - 1 + D <= T - 1;
+      for (int w = 0; w < W; ++w) v[w] = tin[(xb * XB + w) * P + c];
+      const bool interior = xbase - D >= 0 && xbase + XB - 1 + D <= T - 1;
 #pragma unroll
-    for (int xo = 0; xo < XB; ++xo) {
-      const int gx = xbase + xo;
-      float sum = 0.0f;
-      int num;
-      if (interior) {
+      for (int xo = 0; xo < XB; ++xo) {
+        const int gx = xbase + xo;
+        float sum = 0.0f;
+        int num;
+        if (interior) {
 #pragma unroll
-        for (int w = 0; w <= 2 * D; ++w) sum += v[xo + w];
-        num = 2 * D + 1;
-      } else {
-        const int lo = max(0, gx - D), hi = min(T - 1, gx + D);
+          for (int w = 0; w <= 2 * D; ++w) sum += v[xo + w];
+          num = 2 * D + 1;
+        } else {
+          const int lo = max(0, gx - D), hi = min(T - 1, gx + D);
 #pragma unroll
-        for (int w = 0; w <= 2 * D; ++w) {
-          const int gi = gx - D + w;
-          if (gi >= lo && gi <= hi) sum += v[xo + w];
+          for (int w = 0; w <= 2 * D; ++w) {
+            const int gi = gx - D + w;
+            if (gi >= lo && gi <= hi) sum += v[xo + w];
+          }
+          num = hi - lo + 1;
         }
-        num = hi - lo + 1;
+        if (gx < T) tb[(xb * XB + xo) * P + c] = sum / (float)num;
       }
-      if (gx < T) tb[(xb * XB + xo) * P + c] = sum / (float)num;
     }
-  }
-  __syncthreads();
-  // box along y and subtract
-  {
+    __syncthreads();
+    // box along y and subtract
     const int yb = threadIdx.x % (BY / XB), xr = threadIdx.x / (BY / XB);
     const int gx = x0 + xr;
     const int ybase = y0 + yb * XB;
     if (xr < BX && gx < T && ybase < T) {
+      float* op = out + (long long)plane * T * T + (long long)gx * T;
       float u[W];
 #pragma unroll
       for (int w = 0; w < W; ++w) u[w] = tb[xr * P + yb * XB + w];
@@ -171,7 +181,7 @@ __global__ __launch_bounds__(256) void k_blur_sub_xy(const float* __restrict__ r
           }
           num = hi - lo + 1;
         }
-        if (gy < T) op[(long long)gx * T + gy] = tin[(xr + D) * P + yb * XB + yo + D] - sum / (float)num;
+        if (gy < T) op[gy] = tin[(xr + D) * P + yb * XB + yo + D] - sum / (float)num;
       }
     }
   }
@@ -182,12 +192,11 @@ static int launch_blur_sub(const float* res, float* out, int nplanes, int T, hip
   constexpr int BX = 64, P = (64 + 2 * D) | 1;
   const size_t lds = sizeof(float) * (size_t)P * ((BX + 2 * D) + BX);
   const int nb = (T + BX - 1) / BX;
-  for (int p0 = 0; p0 < nplanes; p0 += 65535) {
-    const int np = nplanes - p0 < 65535 ? nplanes - p0 : 65535;
-    hipLaunchKernelGGL(k_blur_sub_xy<D>, dim3(nb * nb, np), dim3(256), lds, st, res + (long long)p0 * T * T,
-                       out + (long long)p0 * T * T, T);
-    MM_LAUNCH_CHECK();
-  }
+  const int pz = 8;  // planes per block
+  const int ng = (nplanes + pz - 1) / pz;
+  if (ng > 65535) { mm_set_error("mm_blur_sub_xy: too many planes"); return MM_EINVAL; }
+  hipLaunchKernelGGL(k_blur_sub_xy<D>, dim3(nb * nb, ng), dim3(512), lds, st, res, out, T, nplanes, pz);
+  MM_LAUNCH_CHECK();
   return MM_OK;
 }
 

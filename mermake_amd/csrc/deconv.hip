@@ -50,21 +50,24 @@ __global__ __launch_bounds__(256) void k_fft_c2c(C* data, long long estride, lon
   C* base = data + (long long)batch * bstride + (long long)cb * G * cstride;
   const int gmax = min(G, ncols - cb * G);
   const int tot = G * N;
+  const FastDiv fd(gfast ? G : N);
   for (int t0 = threadIdx.x; t0 < tot; t0 += MM_UNR * blockDim.x) {  // MM_UNR loads in flight per thread
     C v[MM_UNR];
 #pragma unroll
     for (int u = 0; u < MM_UNR; ++u) {
       const int t = t0 + u * blockDim.x;
-      int g, n;
-      if (gfast) { g = t % G; n = t / G; } else { n = t % N; g = t / N; }
+      int q, r, g, n;
+      fd.divmod(t, q, r);
+      if (gfast) { g = r; n = q; } else { n = r; g = q; }
       v[u].x = 0; v[u].y = 0;
       if (t < tot && g < gmax) v[u] = base[g * cstride + n * estride];
     }
 #pragma unroll
     for (int u = 0; u < MM_UNR; ++u) {
       const int t = t0 + u * blockDim.x;
-      int g, n;
-      if (gfast) { g = t % G; n = t / G; } else { n = t % N; g = t / N; }
+      int q, r, g, n;
+      fd.divmod(t, q, r);
+      if (gfast) { g = r; n = q; } else { n = r; g = q; }
       if (t < tot) a[g * N + n] = v[u];
     }
   }
@@ -72,8 +75,9 @@ __global__ __launch_bounds__(256) void k_fft_c2c(C* data, long long estride, lon
   C* r = fft_lds<INV>(L.a, L.b, L.tw, P, G, L.coef);
   const S sc = (S)scale;
   for (int t = threadIdx.x; t < tot; t += blockDim.x) {
-    int g, n;
-    if (gfast) { g = t % G; n = t / G; } else { n = t % N; g = t / N; }
+    int q, rr, g, n;
+    fd.divmod(t, q, rr);
+    if (gfast) { g = rr; n = q; } else { n = rr; g = q; }
     if (g < gmax) base[g * cstride + n * estride] = vscale(r[g * N + n], sc);
   }
 }
@@ -117,19 +121,25 @@ __global__ __launch_bounds__(256) void k_tile_r2c(const mm_tile_dev* __restrict_
   const int q0 = blockIdx.x * 2 * G;
   const mm_tile_dev td = tiles[tile];
   float* af = (float*)L.a;
+  int* ymap = (int*)L.extra;           // column -> data column (or -1), once per block
+  for (int y = threadIdx.x; y < N; y += blockDim.x) ymap[y] = pad_map(y, td.ys, td.yd, td.yfirst, T);
+  __syncthreads();
+  const FastDiv fN(N), fT(T);
   const int tot = 2 * G * N;
   for (int t0 = threadIdx.x; t0 < tot; t0 += MM_UNR * blockDim.x) {  // issue all loads, then divide
     float v[MM_UNR], f[MM_UNR];
 #pragma unroll
     for (int u = 0; u < MM_UNR; ++u) {
       const int t = t0 + u * blockDim.x;
-      const int row = t / N, y = t - row * N, q = q0 + row;
+      int row, y, z, p;
+      fN.divmod(t, row, y);
+      const int q = q0 + row;
       v[u] = 0.0f;
       f[u] = 1.0f;
       if (t < tot && q < nrows) {
-        const int z = q / T;
-        const int rr = pad_map(q - z * T, td.xs, td.xd, td.xfirst, T);
-        const int cc = pad_map(y, td.ys, td.yd, td.yfirst, T);
+        fT.divmod(q, z, p);
+        const int rr = pad_map(p, td.xs, td.xd, td.xfirst, T);
+        const int cc = ymap[y];
         if (rr >= 0 && cc >= 0) {
           const long long o = (long long)z * td.img_sz + (long long)(td.X0 + rr) * td.img_sx + (td.Y0 + cc);
           v[u] = td.img_f32 ? ((const float*)td.img)[o] : (float)((const uint16_t*)td.img)[o];
@@ -140,7 +150,8 @@ __global__ __launch_bounds__(256) void k_tile_r2c(const mm_tile_dev* __restrict_
 #pragma unroll
     for (int u = 0; u < MM_UNR; ++u) {
       const int t = t0 + u * blockDim.x;
-      const int row = t / N, y = t - row * N;
+      int row, y;
+      fN.divmod(t, row, y);
       if (t < tot) af[((row >> 1) * N + y) * 2 + (row & 1)] = td.flat ? v[u] / f[u] : v[u];
     }
   }
@@ -305,11 +316,12 @@ __global__ void k_d2f(const double2* __restrict__ a, float2* __restrict__ b, lon
 // ------------------------------------------------------------ z-kernel pass
 #define ZC_PF 8  // tiles whose columns are in flight per block (software pipeline depth)
 
-template <int SZM>
-__global__ __launch_bounds__(640) void k_tile_zconv(float2* S, long long s_tstride, int TH, int sz, int zpad, int Zp,
+template <int SZM, bool EXACT>
+__global__ __launch_bounds__(640) void k_tile_zconv(float2* S, long long s_tstride, int TH, int sz_rt, int zpad, int Zp,
                                                     const int* __restrict__ zsrc, const float2* __restrict__ Wz,
                                                     long long w_stride, const mm_tile_dev* __restrict__ tiles,
                                                     int ntiles, int tpb, int G) {
+  const int sz = EXACT ? SZM : sz_rt;
   extern __shared__ __align__(16) unsigned char smem[];
   float2* wl = (float2*)smem;                        // [p][g]
   float2* xl = wl + (size_t)G * Zp;                  // [j][s][g]
@@ -391,16 +403,21 @@ static int launch_zconv(mm_deconv* d, float2* S, long long s_ts, const float2* W
   const long long nblk = (long long)((TH + G - 1) / G) * ngroups;
   const int threads = ((G * sz + 63) / 64) * 64;
   const size_t lds = sizeof(float2) * ((size_t)G * Zp + (size_t)ZC_PF * sz * G) + sizeof(int) * 4 * sz;
-#define MM_ZC(M)                                                                                                  \
-  hipLaunchKernelGGL(k_tile_zconv<M>, dim3((unsigned)nblk), dim3(threads), lds, st, S, s_ts, TH, sz, d->zpad, Zp, \
-                     d->zsrc_dev, Wz, (long long)Zp * TH, td, ntiles, tpb, G)
-  if (sz <= 8) MM_ZC(8);
+#define MM_ZCX(M, E)                                                                                             \
+  hipLaunchKernelGGL((k_tile_zconv<M, E>), dim3((unsigned)nblk), dim3(threads), lds, st, S, s_ts, TH, sz, d->zpad, \
+                     Zp, d->zsrc_dev, Wz, (long long)Zp * TH, td, ntiles, tpb, G)
+#define MM_ZC(M) MM_ZCX(M, false)
+  if (sz == 40) MM_ZCX(40, true);
+  else if (sz == 35) MM_ZCX(35, true);
+  else if (sz == 20) MM_ZCX(20, true);
+  else if (sz <= 8) MM_ZC(8);
   else if (sz <= 16) MM_ZC(16);
   else if (sz <= 24) MM_ZC(24);
   else if (sz <= 32) MM_ZC(32);
   else if (sz <= 40) MM_ZC(40);
   else MM_ZC(48);
 #undef MM_ZC
+#undef MM_ZCX
   MM_LAUNCH_CHECK();
   return MM_OK;
 }
@@ -573,7 +590,7 @@ extern "C" int mm_deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int 
   const int nrows = sz * T;
   {  // 1. R2C along y of the padded rows
     const int G = d->gT_r2c;
-    const size_t lds = fft_lds_bytes(d->pT, G, 0);
+    const size_t lds = fft_lds_bytes(d->pT, G, sizeof(int) * T);
     hipLaunchKernelGGL(k_tile_r2c, dim3((nrows + 2 * G - 1) / (2 * G), ntiles), dim3(256), lds, st, td, T, nrows, S,
                        s_ts, d->pT, G);
     MM_LAUNCH_CHECK();

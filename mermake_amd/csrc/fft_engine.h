@@ -60,6 +60,28 @@ template <bool INV, typename C> __device__ __forceinline__ C twl(const C* tw, in
   return w;
 }
 
+// ---------------------------------------------------------------- index math
+// Division by a block-uniform divisor d through a float reciprocal and one
+// correction step: exact for 0 <= n < 2^22 (the estimate is off by at most 1).
+struct FastDiv {
+  int d;
+  float inv;
+  __device__ __forceinline__ explicit FastDiv(int dd) : d(dd), inv(1.0f / (float)dd) {}
+  __device__ __forceinline__ int div(int n) const {
+    int q = (int)((float)n * inv);
+    const int r = n - q * d;
+    if (r >= d) ++q;
+    else if (r < 0) --q;
+    return q;
+  }
+  __device__ __forceinline__ void divmod(int n, int& q, int& r) const {
+    q = (int)((float)n * inv);
+    r = n - q * d;
+    if (r >= d) { ++q; r -= d; }
+    else if (r < 0) { --q; r += d; }
+  }
+};
+
 // ---------------------------------------------------------------- butterflies
 template <bool INV, typename C>
 __device__ __forceinline__ void bfly2(C* v) {
@@ -164,10 +186,11 @@ template <int R, bool INV, typename C>
 __device__ __forceinline__ void stage_small(const C* __restrict__ src, C* __restrict__ dst, const C* __restrict__ tw,
                                             int N, int M, int Ns, int tws, int G) {
   const int total = G * M;
+  const FastDiv fM(M), fNs(Ns);
   for (int t = threadIdx.x; t < total; t += blockDim.x) {
-    const int g = t / M;
-    const int j = t - g * M;
-    const int k = j % Ns;
+    int g, j, q, k;
+    fM.divmod(t, g, j);
+    fNs.divmod(j, q, k);
     const C* in = src + g * N;
     C* out = dst + g * N;
     C v[R];
@@ -248,12 +271,13 @@ __device__ __forceinline__ void stage_prime_mm(float2* __restrict__ src, float2*
   const int h = (R - 1) >> 1;
   const int hp = (h + 7) & ~7;
   const int total = G * M * h;
+  const FastDiv fh(h), fM(M), fNs(Ns);
   for (int t = threadIdx.x; t < total; t += blockDim.x) {
-    const int r = 1 + t % h;
-    const int rest = t / h;
-    const int j = rest % M;
-    const int g = rest / M;
-    const int k = j % Ns;
+    int rest, r, g, j, q, k;
+    fh.divmod(t, rest, r);
+    ++r;
+    fM.divmod(rest, g, j);
+    fNs.divmod(j, q, k);
     const int i1 = g * N + j + r * M, i2 = g * N + j + (R - r) * M;
     const float2 a = vmul(src[i1], twl<INV>(tw, k * r * tws));
     const float2 b = vmul(src[i2], twl<INV>(tw, k * (R - r) * tws));
@@ -265,12 +289,12 @@ __device__ __forceinline__ void stage_prime_mm(float2* __restrict__ src, float2*
   const int total2 = G * M * (nkt + 1);
   const float* cm = coef;
   const float* sm = coef + h * hp;
+  const FastDiv fkt(nkt + 1);
   for (int t = threadIdx.x; t < total2; t += blockDim.x) {
-    const int kt = t % (nkt + 1);
-    const int bfly = t / (nkt + 1);
-    const int j = bfly % M;
-    const int g = bfly / M;
-    const int kk = j % Ns;
+    int bfly, kt, g, j, q, kk;
+    fkt.divmod(t, bfly, kt);
+    fM.divmod(bfly, g, j);
+    fNs.divmod(j, q, kk);
     const float2* in = src + g * N + j;
     float2* out = dst + g * N + (j - kk) * R + kk;
     const float2 v0 = in[0];
