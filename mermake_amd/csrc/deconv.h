@@ -1,6 +1,6 @@
 // Device-side descriptors shared by the deconvolution, maxima and fit kernels.
 #pragma once
-#include "fft_engine.h"
+#include "fft_pf.h"
 
 // One overlapped tile of a z-stack (reference: deconvolver.py:186-206 for the
 // window, :118-148 for the placement and reflections).  Filled on the host.
@@ -76,6 +76,8 @@ struct mm_deconv {
   int zconv;                // z pass as a folded z-kernel matrix (sz <= 48) instead of z FFTs
   int zG;                   // columns per block of the z-kernel pass
   int* zsrc_dev;            // per data plane, the padded planes that copy it (4 slots, -1 padded)
+  int pfM, pfG;             // prime-first T passes: T = p * pfM, pfG transforms per block (pfM = 0: off)
+  PfShape pfS;
   int prof;                 // record events around the five passes of mm_deconv_tiles
   hipEvent_t ev[6];
 };

@@ -23,9 +23,11 @@
 //   5. k_tile_c2r      C2R along y (two rows per complex transform) -> res (sz,T,T).
 // The 1/(Zp*T*T) normalisation of ifftn is folded into W.
 #include "deconv.h"
+#include "fft_pf.h"
 #include "fft_plan.h"
 #include "mermake_hip.h"
 
+#include <type_traits>
 #include <vector>
 
 #define MM_UNR 8  // global loads issued back to back before the LDS stores
@@ -268,6 +270,299 @@ __global__ __launch_bounds__(256) void k_tile_c2r(const float2* S, long long s_t
   }
 }
 
+// ------------------------------------------------ prime-first tile passes
+// The same three passes (R2C rows, C2R rows, C2C along x) for T = p * M with a
+// matrix-product prime p: load fused with the prime pre-step, one LDS buffer,
+// radix-M final stage straight to global memory.  Persistent blocks: each
+// loops over groups of G transforms (group index stepped by the grid size,
+// XCD-remapped so that neighbouring groups share an L2) and prefetches the
+// next group's inputs into registers during the current group's matrix stage.
+template <bool INV, int M>
+__global__ __launch_bounds__(256) void k_pf_c2c(float2* data, long long estride, long long cstride, int ncols,
+                                                long long bstride, int nbatch, PfShape S,
+                                                const float2* __restrict__ tw, const float* __restrict__ coef, int G) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const PfLds L = pf_carve(smem, S, M, G, tw, coef);
+  const int ncb = (ncols + G - 1) / G;
+  const long long ngrp = (long long)ncb * nbatch;
+  const int es = (int)estride;  // element offsets within one batch fit in 32 bits
+  const int cs = (int)cstride;
+  auto setup = [&](long long gi, float2*& base, int& gmax) {
+    const long long batch = gi / ncb;
+    const int cb = (int)(gi - batch * ncb);
+    base = data + batch * bstride + (long long)cb * G * cstride;
+    gmax = min(G, ncols - cb * G);
+  };
+  auto fetch = [&](PfRegs& R, const float2* base, int gmax) {
+    pf_fetch<M, true>(
+        S, G, R,
+        [&](int g, int j, int r, float2& a, float2& b) {
+          if (g >= gmax) return;
+          const float2* col = base + (g * cs + j * es);
+          a = col[r * M * es];
+          b = col[(S.p - r) * M * es];
+        },
+        [&](int g, int j) { return g < gmax ? base[g * cs + j * es] : make_float2(0.f, 0.f); });
+  };
+  long long grp = xcd_remap(blockIdx.x, gridDim.x);
+  PfRegs R;
+  float2* base = nullptr;
+  int gmax = 0;
+  if (grp < ngrp) {
+    setup(grp, base, gmax);
+    fetch(R, base, gmax);
+  }
+  for (; grp < ngrp; grp += gridDim.x) {
+    pf_put<M, true>(S, G, L.sd, R);
+    __syncthreads();
+    float2* nbase = nullptr;
+    int ngmax = 0;
+    if (grp + gridDim.x < ngrp) {
+      setup(grp + gridDim.x, nbase, ngmax);
+      fetch(R, nbase, ngmax);
+    }
+    pf_matrix<M, INV>(S, L.sd, L.coef, G);
+    pf_final<M, INV>(S, (const float2*)L.sd, L.tw, G, [&](int g, int j, const float2* v) {
+      if (g >= gmax) return;
+      float2* col = base + (g * cs + j * es);
+#pragma unroll
+      for (int r = 0; r < M; ++r) col[r * S.p * es] = v[r];
+    });
+    __syncthreads();
+    base = nbase;
+    gmax = ngmax;
+  }
+}
+
+// v / f with a reciprocal and one correction step (within 1 ulp of the IEEE
+// quotient, equal to it for nearly all operands).
+__device__ __forceinline__ float div_nr(float v, float f) {
+  const float r = __builtin_amdgcn_rcpf(f);
+  const float q = v * r;
+  return fmaf(fmaf(-q, f, v), r, q);
+}
+
+// R2C row tables of one group: ymap (T ints: padded column -> data column or
+// -1) and, per padded row, the stack offset of its data row (-1: zero row)
+// and the flat-field offset.
+struct R2cTab {
+  int* ymap;
+  long long* roff;
+  long long* foff;
+};
+
+template <int M, bool F32, bool FLAT>
+__device__ __forceinline__ void r2c_fetch(const PfShape& S, int G, PfRegs& R, const mm_tile_dev& td, const R2cTab& tb) {
+  typedef typename std::conditional<F32, float, uint16_t>::type PixT;
+  const PixT* img = (const PixT*)td.img;
+  const float* fl = td.flat;
+  auto px = [&](long long o, long long fo, int c) -> float {
+    if (o < 0 || c < 0) return 0.0f;
+    const float v = (float)img[o + c];
+    return FLAT ? div_nr(v, fl[fo + c]) : v;
+  };
+  pf_fetch<M, false>(
+      S, G, R,
+      [&](int g, int j, int r, float2& a, float2& b) {
+        const long long o0 = tb.roff[2 * g], o1 = tb.roff[2 * g + 1];
+        const long long f0 = FLAT ? tb.foff[2 * g] : 0, f1 = FLAT ? tb.foff[2 * g + 1] : 0;
+        const int c1 = tb.ymap[j + r * M];
+        const int c2 = tb.ymap[j + (S.p - r) * M];
+        a = make_float2(px(o0, f0, c1), px(o1, f1, c1));
+        b = make_float2(px(o0, f0, c2), px(o1, f1, c2));
+      },
+      [&](int g, int j) {
+        const int c = tb.ymap[j];
+        return make_float2(px(tb.roff[2 * g], FLAT ? tb.foff[2 * g] : 0, c),
+                           px(tb.roff[2 * g + 1], FLAT ? tb.foff[2 * g + 1] : 0, c));
+      });
+}
+
+// Two padded rows (z, x) per transform: re = row 2g, im = row 2g + 1 of the
+// flat-divided, reflection-padded tile; half spectra written to S.  A group is
+// 2G consecutive rows of one tile.
+template <int M>
+__global__ __launch_bounds__(256) void k_pf_r2c(const mm_tile_dev* __restrict__ tiles, int ntiles, int T, int nrows,
+                                                float2* Sout, long long s_tstride, PfShape S,
+                                                const float2* __restrict__ tw, const float* __restrict__ coef, int G) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const PfLds L = pf_carve(smem, S, M, G, tw, coef);
+  R2cTab tab0, tab1;  // double-buffered: the next group's tables are built during the current group
+  {
+    unsigned char* e = L.extra;
+    R2cTab* tt[2] = {&tab0, &tab1};
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      tt[k]->ymap = (int*)e;
+      e += mm_align16((size_t)T * 4);
+      tt[k]->roff = (long long*)e;
+      tt[k]->foff = tt[k]->roff + 2 * G;
+      e += (size_t)4 * G * 8;
+    }
+  }
+  const int nrb = (nrows + 2 * G - 1) / (2 * G);
+  const long long ngrp = (long long)nrb * ntiles;
+  auto tables = [&](long long gi, const R2cTab& tb) {
+    const int tile = (int)(gi / nrb);
+    const int q0 = (int)(gi - (long long)tile * nrb) * 2 * G;
+    const mm_tile_dev& td = tiles[tile];
+    for (int y = threadIdx.x; y < T; y += blockDim.x) tb.ymap[y] = pad_map(y, td.ys, td.yd, td.yfirst, T);
+    for (int i = threadIdx.x; i < 2 * G; i += blockDim.x) {
+      const int q = q0 + i;
+      long long ro = -1, fo = 0;
+      if (q < nrows) {
+        const int z = q / T, x = q - (q / T) * T;
+        const int rr = pad_map(x, td.xs, td.xd, td.xfirst, T);
+        if (rr >= 0) {
+          ro = (long long)z * td.img_sz + (long long)(td.X0 + rr) * td.img_sx + td.Y0;
+          fo = (long long)(td.X0 + rr) * td.flat_sx + td.Y0;
+        }
+      }
+      tb.roff[i] = ro;
+      tb.foff[i] = fo;
+    }
+  };
+  auto fetch = [&](PfRegs& R, long long gi, const R2cTab& tb) {
+    const mm_tile_dev& td = tiles[(int)(gi / nrb)];
+    // pixel type and flat presence are uniform over a group: one code path each
+    if (td.img_f32) {
+      if (td.flat) r2c_fetch<M, true, true>(S, G, R, td, tb);
+      else r2c_fetch<M, true, false>(S, G, R, td, tb);
+    } else {
+      if (td.flat) r2c_fetch<M, false, true>(S, G, R, td, tb);
+      else r2c_fetch<M, false, false>(S, G, R, td, tb);
+    }
+  };
+  const int H = S.N / 2 + 1;
+  long long grp = xcd_remap(blockIdx.x, gridDim.x);
+  int set = 0;
+  PfRegs R;
+  if (grp < ngrp) {
+    tables(grp, tab0);
+    __syncthreads();
+    fetch(R, grp, tab0);
+  }
+  for (; grp < ngrp; grp += gridDim.x) {
+    const long long nxt = grp + gridDim.x;
+    const R2cTab& tn = set ? tab0 : tab1;
+    if (nxt < ngrp) tables(nxt, tn);
+    pf_put<M, false>(S, G, L.sd, R);
+    __syncthreads();
+    if (nxt < ngrp) fetch(R, nxt, tn);
+    pf_matrix<M, false>(S, L.sd, L.coef, G);
+    const int tile = (int)(grp / nrb);
+    const int q0 = (int)(grp - (long long)tile * nrb) * 2 * G;
+    float2* out = Sout + (long long)tile * s_tstride;
+    pf_final_r2c<M>(S, (const float2*)L.sd, L.tw, G, [&](int g, float2*& pa, float2*& pb) {
+      const int q = q0 + 2 * g;
+      pa = q < nrows ? out + (long long)q * H : nullptr;
+      pb = q + 1 < nrows ? out + (long long)(q + 1) * H : nullptr;
+    });
+    __syncthreads();
+    set ^= 1;
+  }
+}
+
+// Two half-spectrum rows per transform -> two real rows of res.  A group is
+// 2G consecutive rows of one tile.
+template <int M>
+__global__ __launch_bounds__(256) void k_pf_c2r(const float2* Sin, long long s_tstride, float* res,
+                                                long long r_tstride, int ntiles, int nrows, PfShape S,
+                                                const float2* __restrict__ tw, const float* __restrict__ coef, int G) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const PfLds L = pf_carve(smem, S, M, G, tw, coef);
+  const int N = S.N, H = N / 2 + 1;
+  const int nrb = (nrows + 2 * G - 1) / (2 * G);
+  const long long ngrp = (long long)nrb * ntiles;
+  auto spec = [&](const float2* ra, const float2* rb, int n) -> float2 {
+    const bool hi = n >= H;
+    const int kk = hi ? N - n : n;
+    float2 a = ra ? ra[kk] : make_float2(0.f, 0.f);
+    float2 b = rb ? rb[kk] : make_float2(0.f, 0.f);
+    if (hi) { a.y = -a.y; b.y = -b.y; }
+    if (n == 0 || 2 * n == N) { a.y = 0.f; b.y = 0.f; }  // real part semantics of ifftn(...).real
+    return make_float2(a.x - b.y, a.y + b.x);
+  };
+  auto fetch = [&](PfRegs& R, long long gi) {
+    const int tile = (int)(gi / nrb);
+    const int q0 = (int)(gi - (long long)tile * nrb) * 2 * G;
+    const float2* in = Sin + (long long)tile * s_tstride;
+    auto rows = [&](int g, const float2*& ra, const float2*& rb) {
+      const int q = q0 + 2 * g;
+      ra = q < nrows ? in + (long long)q * H : nullptr;
+      rb = q + 1 < nrows ? in + (long long)(q + 1) * H : nullptr;
+    };
+    pf_fetch<M, false>(
+        S, G, R,
+        [&](int g, int j, int r, float2& a, float2& b) {
+          const float2 *ra, *rb;
+          rows(g, ra, rb);
+          a = spec(ra, rb, j + r * M);
+          b = spec(ra, rb, j + (S.p - r) * M);
+        },
+        [&](int g, int j) {
+          const float2 *ra, *rb;
+          rows(g, ra, rb);
+          return spec(ra, rb, j);
+        });
+  };
+  long long grp = xcd_remap(blockIdx.x, gridDim.x);
+  PfRegs R;
+  if (grp < ngrp) fetch(R, grp);
+  for (; grp < ngrp; grp += gridDim.x) {
+    pf_put<M, false>(S, G, L.sd, R);
+    __syncthreads();
+    if (grp + gridDim.x < ngrp) fetch(R, grp + gridDim.x);
+    pf_matrix<M, true>(S, L.sd, L.coef, G);
+    const int tile = (int)(grp / nrb);
+    const int q0 = (int)(grp - (long long)tile * nrb) * 2 * G;
+    float* out = res + (long long)tile * r_tstride;
+    pf_final<M, true>(S, (const float2*)L.sd, L.tw, G, [&](int g, int j, const float2* v) {
+      const int q = q0 + 2 * g;
+      float* oa = out + (long long)q * N + j;
+#pragma unroll
+      for (int r = 0; r < M; ++r) {
+        if (q < nrows) oa[r * S.p] = v[r].x;
+        if (q + 1 < nrows) oa[N + r * S.p] = v[r].y;
+      }
+    });
+    __syncthreads();
+  }
+}
+
+#define PF_SWITCH(M, CALL)                    \
+  switch (M) {                                \
+    case 2: { constexpr int MM = 2; CALL; } break; \
+    case 3: { constexpr int MM = 3; CALL; } break; \
+    case 4: { constexpr int MM = 4; CALL; } break; \
+    case 6: { constexpr int MM = 6; CALL; } break; \
+    default: mm_set_error("prime-first FFT: radix %d", M); return MM_EINVAL; \
+  }
+
+static size_t pf_r2c_extra(int T, int G) { return 2 * (mm_align16((size_t)T * 4) + (size_t)4 * G * 8); }
+
+// Grid of a persistent prime-first pass: as many blocks as fit on the chip at
+// once (LDS bound), at most one per group.
+static unsigned pf_grid(long long ngrp, size_t lds) {
+  long long per_cu = (long long)(MM_LDS_MAX / (lds ? lds : 1));
+  if (per_cu < 1) per_cu = 1;
+  if (per_cu > 4) per_cu = 4;
+  const long long g = per_cu * 256;
+  return (unsigned)(ngrp < g ? ngrp : g);
+}
+
+template <bool INV>
+static int launch_pf_c2c(const mm_deconv* d, float2* data, long long estride, long long cstride, int ncols,
+                         long long bstride, int nbatch, hipStream_t st) {
+  const int G = d->pfG;
+  const long long ngrp = (long long)((ncols + G - 1) / G) * nbatch;
+  const size_t lds = pf_lds_bytes(d->pfS, d->pfM, G, 0);
+  PF_SWITCH(d->pfM, hipLaunchKernelGGL((k_pf_c2c<INV, MM>), dim3(pf_grid(ngrp, lds)), dim3(256), lds, st, data,
+                                       estride, cstride, ncols, bstride, nbatch, d->pfS, d->pT.tw, d->pT.coef, G));
+  MM_LAUNCH_CHECK();
+  return MM_OK;
+}
+
 // ------------------------------------------------------------ Wiener spectra
 // Centred PSF (sz, ts, ts) -> zero padded by (zpad, ov, ov) and rolled by
 // (-shape)//2 (deconvolver.py:60-62): out[i] = padded[(i + ceil(n/2)) mod n].
@@ -468,13 +763,25 @@ extern "C" int mm_deconv_create(int sz, int tile_size, int overlap, int zpad, mm
     delete d;
     return MM_ENOMEM;
   }
-  // two blocks per CU (LDS <= 80 KB) for the T-length passes
-  d->gT_r2c = mm_fft_pick_g(d->pT, 81920, 0, 8);
-  d->gT_x = mm_fft_pick_g(d->pT, 81920, 0, 8);
+  // two blocks per CU (LDS <= 80 KB) for the T-length passes when the radix tables allow it
+  const int budgetT = fft_lds_bytes(d->pT, 2, (size_t)sizeof(int) * d->T) <= 81920 ? 81920 : MM_LDS_MAX;
+  d->gT_r2c = mm_fft_pick_g(d->pT, budgetT - (int)sizeof(int) * d->T, 0, 8);
+  d->gT_x = mm_fft_pick_g(d->pT, budgetT, 0, 8);
   d->gZ = mm_fft_pick_g(d->pZ, 65536, 8 * d->Zp, 16);
   d->gT64_r2c = mm_fft_pick_g(d->pT64, 98304, 0, 8);
   d->gT64_x = mm_fft_pick_g(d->pT64, 98304, 0, 8);
   d->gZ64 = mm_fft_pick_g(d->pZ64, 98304, 0, 16);
+  // prime-first passes when T = p * M with the matrix-product prime p
+  d->pfM = 0;
+  if (d->pT.coefR > 0 && d->T % d->pT.coefR == 0) {
+    const int M = d->T / d->pT.coefR;
+    if (M == 2 || M == 3 || M == 4 || M == 6) {
+      const PfShape S(d->T, d->pT.coefR);
+      int G = 256 / (M * S.nkt);
+      while (G > 0 && (pf_lds_bytes(S, M, G, pf_r2c_extra(d->T, G)) > MM_LDS_MAX || G * M * S.h > PF_NI * 256)) --G;
+      if (G > 0) { d->pfM = M; d->pfG = G; d->pfS = S; }
+    }
+  }
   // z-kernel pass: the padded planes that copy each data plane
   d->zconv = sz <= 48;
   if (d->zconv) {
@@ -588,7 +895,14 @@ extern "C" int mm_deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int 
   MM_MARK(0);
   const long long s_ts = (long long)sz * T * H;
   const int nrows = sz * T;
-  {  // 1. R2C along y of the padded rows
+  if (d->pfM) {  // 1. R2C along y of the padded rows (prime-first form)
+    const int G = d->pfG;
+    const size_t lds = pf_lds_bytes(d->pfS, d->pfM, G, pf_r2c_extra(T, G));
+    const long long ngrp = (long long)((nrows + 2 * G - 1) / (2 * G)) * ntiles;
+    PF_SWITCH(d->pfM, hipLaunchKernelGGL(k_pf_r2c<MM>, dim3(pf_grid(ngrp, lds)), dim3(256), lds, st, td, ntiles, T,
+                                         nrows, S, s_ts, d->pfS, d->pT.tw, d->pT.coef, G));
+    MM_LAUNCH_CHECK();
+  } else {  // 1. R2C along y of the padded rows
     const int G = d->gT_r2c;
     const size_t lds = fft_lds_bytes(d->pT, G, sizeof(int) * T);
     hipLaunchKernelGGL(k_tile_r2c, dim3((nrows + 2 * G - 1) / (2 * G), ntiles), dim3(256), lds, st, td, T, nrows, S,
@@ -598,7 +912,9 @@ extern "C" int mm_deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int 
   MM_MARK(1);
   int rc;
   // 2. C2C along x, all tiles and planes at once
-  if ((rc = launch_c2c<false>(S, H, 1, H, (long long)T * H, ntiles * sz, d->pT, d->gT_x, 1.0, st))) return rc;
+  if (d->pfM) rc = launch_pf_c2c<false>(d, S, H, 1, H, (long long)T * H, ntiles * sz, st);
+  else rc = launch_c2c<false>(S, H, 1, H, (long long)T * H, ntiles * sz, d->pT, d->gT_x, 1.0, st);
+  if (rc) return rc;
   MM_MARK(2);
   const int tpb = tiles_per_block < 1 ? 1 : (tiles_per_block > ntiles ? ntiles : tiles_per_block);
   if (d->zconv) {  // 3. z pass as the folded z-kernel matrix
@@ -615,9 +931,18 @@ extern "C" int mm_deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int 
   }
   MM_MARK(3);
   // 4. inverse C2C along x
-  if ((rc = launch_c2c<true>(S, H, 1, H, (long long)T * H, ntiles * sz, d->pT, d->gT_x, 1.0, st))) return rc;
+  if (d->pfM) rc = launch_pf_c2c<true>(d, S, H, 1, H, (long long)T * H, ntiles * sz, st);
+  else rc = launch_c2c<true>(S, H, 1, H, (long long)T * H, ntiles * sz, d->pT, d->gT_x, 1.0, st);
+  if (rc) return rc;
   MM_MARK(4);
-  {  // 5. C2R along y into res
+  if (d->pfM) {  // 5. C2R along y into res (prime-first form)
+    const int G = d->pfG;
+    const size_t lds = pf_lds_bytes(d->pfS, d->pfM, G, 0);
+    const long long ngrp = (long long)((nrows + 2 * G - 1) / (2 * G)) * ntiles;
+    PF_SWITCH(d->pfM, hipLaunchKernelGGL(k_pf_c2r<MM>, dim3(pf_grid(ngrp, lds)), dim3(256), lds, st, S, s_ts, res,
+                                         (long long)sz * T * T, ntiles, nrows, d->pfS, d->pT.tw, d->pT.coef, G));
+    MM_LAUNCH_CHECK();
+  } else {  // 5. C2R along y into res
     const int G = d->gT_r2c;
     const size_t lds = fft_lds_bytes(d->pT, G, 0);
     hipLaunchKernelGGL(k_tile_c2r, dim3((nrows + 2 * G - 1) / (2 * G), ntiles), dim3(256), lds, st, S, s_ts, res,

@@ -12,8 +12,8 @@ std::mutex g_plan_mu;
 std::map<std::pair<int, int>, void*> g_tw;    // (device, N) -> twiddles (float2 or double2)
 std::map<std::pair<int, int>, float*> g_coef;  // (device, R) -> cos/sin matrices of prime R
 
-// Largest coefficient table kept in LDS (bytes): primes up to 181.
-const int kCoefMaxBytes = 65536;
+// Largest coefficient table kept in LDS (bytes): primes up to 241 (239 for T = 478).
+const int kCoefMaxBytes = 122880;
 
 // cos/sin(2 pi r k / R) for r, k in [1, h], rows r, row stride hp = h rounded up to 8.
 float* coef_table(int dev, int R, int* len) {

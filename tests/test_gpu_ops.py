@@ -220,6 +220,10 @@ def _rel_l2(a, b):
     ((5, 20, 20), 50, 5, 2, 1e-3, False, None),     # single tile larger than the image
     ((4, 45, 45), 15, 2, 1, 1e-4, False, 3),
     ((3, 40, 40), 20, 7, 5, 1e-3, True, None),      # zpad >= sz: zero planes in the z padding
+    ((4, 70, 70), 50, 8, 2, 1e-3, True, 5),         # T = 66 = 11 * 6: prime-first passes, radix-6 final stage
+    ((4, 50, 50), 23, 5, 2, 1e-3, False, None),     # T = 33 = 11 * 3
+    ((4, 60, 60), 30, 7, 2, 1e-3, True, None),      # T = 44 = 11 * 4
+    ((3, 600, 600), 500, 89, 3, 1e-4, True, 15),    # T = 678 = 113 * 6, the full-FOV tile
 ])
 def test_deconv_vs_oracle(dev, shape, ts, ov, zpad, beta, flat, blur):
     from mermake_amd.deconvolver import Deconvolver
