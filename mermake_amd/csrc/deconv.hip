@@ -276,7 +276,9 @@ __global__ __launch_bounds__(256) void k_tile_c2r(const float2* S, long long s_t
 // radix-M final stage straight to global memory.  Persistent blocks: each
 // loops over groups of G transforms (group index stepped by the grid size,
 // XCD-remapped so that neighbouring groups share an L2) and prefetches the
-// next group's inputs into registers during the current group's matrix stage.
+// next group's raw inputs into registers during the current group's matrix
+// stage; the loads are unconditional (clamped addresses) so they are all in
+// flight at once.
 template <bool INV, int M>
 __global__ __launch_bounds__(256) void k_pf_c2c(float2* data, long long estride, long long cstride, int ncols,
                                                 long long bstride, int nbatch, PfShape S,
@@ -293,36 +295,39 @@ __global__ __launch_bounds__(256) void k_pf_c2c(float2* data, long long estride,
     base = data + batch * bstride + (long long)cb * G * cstride;
     gmax = min(G, ncols - cb * G);
   };
-  auto fetch = [&](PfRegs& R, const float2* base, int gmax) {
-    pf_fetch<M, true>(
-        S, G, R,
-        [&](int g, int j, int r, float2& a, float2& b) {
-          if (g >= gmax) return;
-          const float2* col = base + (g * cs + j * es);
-          a = col[r * M * es];
-          b = col[(S.p - r) * M * es];
+  PfItems I;
+  pf_items<M, true>(S, G, I);
+  float2 A[PF_NI], B[PF_NI], V0;
+  auto fetch = [&](const float2* base, int gmax) {  // absent columns read the last present one
+    pf_fetch<M>(
+        G, I,
+        [&](int u, int g, int n1, int n2) {
+          const float2* col = base + min(g, gmax - 1) * cs;
+          A[u] = col[n1 * es];
+          B[u] = col[n2 * es];
         },
-        [&](int g, int j) { return g < gmax ? base[g * cs + j * es] : make_float2(0.f, 0.f); });
+        [&](int g, int j) { V0 = base[min(g, gmax - 1) * cs + j * es]; });
   };
   long long grp = xcd_remap(blockIdx.x, gridDim.x);
-  PfRegs R;
   float2* base = nullptr;
   int gmax = 0;
   if (grp < ngrp) {
     setup(grp, base, gmax);
-    fetch(R, base, gmax);
+    fetch(base, gmax);
   }
   for (; grp < ngrp; grp += gridDim.x) {
-    pf_put<M, true>(S, G, L.sd, R);
+    pf_put<M>(
+        S, G, L.sd, I, [&](int u, int, int, int, float2& a, float2& b) { a = A[u]; b = B[u]; },
+        [&](int, int) { return V0; });
     __syncthreads();
     float2* nbase = nullptr;
     int ngmax = 0;
     if (grp + gridDim.x < ngrp) {
       setup(grp + gridDim.x, nbase, ngmax);
-      fetch(R, nbase, ngmax);
+      fetch(nbase, ngmax);
     }
     pf_matrix<M, INV>(S, L.sd, L.coef, G);
-    pf_final<M, INV>(S, (const float2*)L.sd, L.tw, G, [&](int g, int j, const float2* v) {
+    pf_final<M, INV, true>(S, (const float2*)L.sd, L.tw, G, [&](int g, int j, const float2* v) {
       if (g >= gmax) return;
       float2* col = base + (g * cs + j * es);
 #pragma unroll
@@ -351,30 +356,82 @@ struct R2cTab {
   long long* foff;
 };
 
+// Raw prefetch registers of the R2C pass: pixel bits (uint16 zero-extended or
+// float32 bits) and flat values of the 4 samples of each item (rows 2g, 2g+1
+// at columns n1, n2), with their validity bits (4 per item).
+struct R2cRaw {
+  unsigned P[PF_NI][4];
+  float F[PF_NI][4];
+  unsigned msk;
+  unsigned P0[2];
+  float F0[2];
+  unsigned m0;
+};
+
 template <int M, bool F32, bool FLAT>
-__device__ __forceinline__ void r2c_fetch(const PfShape& S, int G, PfRegs& R, const mm_tile_dev& td, const R2cTab& tb) {
-  typedef typename std::conditional<F32, float, uint16_t>::type PixT;
-  const PixT* img = (const PixT*)td.img;
+__device__ __forceinline__ void r2c_fetch(int G, const PfItems& I, R2cRaw& R, const mm_tile_dev& td,
+                                          const R2cTab& tb) {
   const float* fl = td.flat;
-  auto px = [&](long long o, long long fo, int c) -> float {
-    if (o < 0 || c < 0) return 0.0f;
-    const float v = (float)img[o + c];
-    return FLAT ? div_nr(v, fl[fo + c]) : v;
+  auto raw = [&](long long o) -> unsigned {
+    return F32 ? __float_as_uint(((const float*)td.img)[o]) : (unsigned)((const uint16_t*)td.img)[o];
   };
-  pf_fetch<M, false>(
-      S, G, R,
-      [&](int g, int j, int r, float2& a, float2& b) {
+  R.msk = 0;
+  R.m0 = 0;
+  pf_fetch<M>(
+      G, I,
+      [&](int u, int g, int n1, int n2) {
         const long long o0 = tb.roff[2 * g], o1 = tb.roff[2 * g + 1];
-        const long long f0 = FLAT ? tb.foff[2 * g] : 0, f1 = FLAT ? tb.foff[2 * g + 1] : 0;
-        const int c1 = tb.ymap[j + r * M];
-        const int c2 = tb.ymap[j + (S.p - r) * M];
-        a = make_float2(px(o0, f0, c1), px(o1, f1, c1));
-        b = make_float2(px(o0, f0, c2), px(o1, f1, c2));
+        const int c1 = tb.ymap[n1], c2 = tb.ymap[n2];
+        const long long oo[4] = {o0 + c1, o1 + c1, o0 + c2, o1 + c2};
+        const bool ok[4] = {o0 >= 0 && c1 >= 0, o1 >= 0 && c1 >= 0, o0 >= 0 && c2 >= 0, o1 >= 0 && c2 >= 0};
+        unsigned m = 0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          R.P[u][e] = raw(ok[e] ? oo[e] : 0);
+          m |= ok[e] ? 1u << e : 0u;
+        }
+        if (FLAT) {
+          const long long f0 = tb.foff[2 * g], f1 = tb.foff[2 * g + 1];
+          const long long ff[4] = {f0 + c1, f1 + c1, f0 + c2, f1 + c2};
+#pragma unroll
+          for (int e = 0; e < 4; ++e) R.F[u][e] = fl[ok[e] ? ff[e] : 0];
+        }
+        R.msk |= m << (4 * u);
       },
       [&](int g, int j) {
         const int c = tb.ymap[j];
-        return make_float2(px(tb.roff[2 * g], FLAT ? tb.foff[2 * g] : 0, c),
-                           px(tb.roff[2 * g + 1], FLAT ? tb.foff[2 * g + 1] : 0, c));
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const long long o = tb.roff[2 * g + e];
+          const bool ok = o >= 0 && c >= 0;
+          R.P0[e] = raw(ok ? o + c : 0);
+          if (FLAT) R.F0[e] = fl[ok ? tb.foff[2 * g + e] + c : 0];
+          R.m0 |= ok ? 1u << e : 0u;
+        }
+      });
+}
+
+template <bool F32, bool FLAT>
+__device__ __forceinline__ float r2c_val(unsigned bits, float f, bool ok) {
+  float x = F32 ? __uint_as_float(bits) : (float)bits;
+  if (FLAT) x = div_nr(x, f);
+  return ok ? x : 0.0f;
+}
+
+template <int M, bool F32, bool FLAT>
+__device__ __forceinline__ void r2c_put(const PfShape& S, int G, float4* sd, const PfItems& I, const R2cRaw& R) {
+  pf_put<M>(
+      S, G, sd, I,
+      [&](int u, int, int, int, float2& a, float2& b) {
+        float v[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) v[e] = r2c_val<F32, FLAT>(R.P[u][e], R.F[u][e], (R.msk >> (4 * u + e)) & 1u);
+        a = make_float2(v[0], v[1]);
+        b = make_float2(v[2], v[3]);
+      },
+      [&](int, int) {
+        return make_float2(r2c_val<F32, FLAT>(R.P0[0], R.F0[0], R.m0 & 1u),
+                           r2c_val<F32, FLAT>(R.P0[1], R.F0[1], (R.m0 >> 1) & 1u));
       });
 }
 
@@ -422,33 +479,45 @@ __global__ __launch_bounds__(256) void k_pf_r2c(const mm_tile_dev* __restrict__ 
       tb.foff[i] = fo;
     }
   };
-  auto fetch = [&](PfRegs& R, long long gi, const R2cTab& tb) {
+  PfItems I;
+  pf_items<M, false>(S, G, I);
+  R2cRaw R;
+  // pixel type and flat presence are uniform over a group (one tile): one code path each
+  auto fetch = [&](long long gi, const R2cTab& tb) {
     const mm_tile_dev& td = tiles[(int)(gi / nrb)];
-    // pixel type and flat presence are uniform over a group: one code path each
     if (td.img_f32) {
-      if (td.flat) r2c_fetch<M, true, true>(S, G, R, td, tb);
-      else r2c_fetch<M, true, false>(S, G, R, td, tb);
+      if (td.flat) r2c_fetch<M, true, true>(G, I, R, td, tb);
+      else r2c_fetch<M, true, false>(G, I, R, td, tb);
     } else {
-      if (td.flat) r2c_fetch<M, false, true>(S, G, R, td, tb);
-      else r2c_fetch<M, false, false>(S, G, R, td, tb);
+      if (td.flat) r2c_fetch<M, false, true>(G, I, R, td, tb);
+      else r2c_fetch<M, false, false>(G, I, R, td, tb);
+    }
+  };
+  auto put = [&](long long gi) {
+    const mm_tile_dev& td = tiles[(int)(gi / nrb)];
+    if (td.img_f32) {
+      if (td.flat) r2c_put<M, true, true>(S, G, L.sd, I, R);
+      else r2c_put<M, true, false>(S, G, L.sd, I, R);
+    } else {
+      if (td.flat) r2c_put<M, false, true>(S, G, L.sd, I, R);
+      else r2c_put<M, false, false>(S, G, L.sd, I, R);
     }
   };
   const int H = S.N / 2 + 1;
   long long grp = xcd_remap(blockIdx.x, gridDim.x);
   int set = 0;
-  PfRegs R;
   if (grp < ngrp) {
     tables(grp, tab0);
     __syncthreads();
-    fetch(R, grp, tab0);
+    fetch(grp, tab0);
   }
   for (; grp < ngrp; grp += gridDim.x) {
     const long long nxt = grp + gridDim.x;
     const R2cTab& tn = set ? tab0 : tab1;
     if (nxt < ngrp) tables(nxt, tn);
-    pf_put<M, false>(S, G, L.sd, R);
+    put(grp);
     __syncthreads();
-    if (nxt < ngrp) fetch(R, nxt, tn);
+    if (nxt < ngrp) fetch(nxt, tn);
     pf_matrix<M, false>(S, L.sd, L.coef, G);
     const int tile = (int)(grp / nrb);
     const int q0 = (int)(grp - (long long)tile * nrb) * 2 * G;
@@ -474,48 +543,65 @@ __global__ __launch_bounds__(256) void k_pf_c2r(const float2* Sin, long long s_t
   const int N = S.N, H = N / 2 + 1;
   const int nrb = (nrows + 2 * G - 1) / (2 * G);
   const long long ngrp = (long long)nrb * ntiles;
-  auto spec = [&](const float2* ra, const float2* rb, int n) -> float2 {
-    const bool hi = n >= H;
-    const int kk = hi ? N - n : n;
-    float2 a = ra ? ra[kk] : make_float2(0.f, 0.f);
-    float2 b = rb ? rb[kk] : make_float2(0.f, 0.f);
-    if (hi) { a.y = -a.y; b.y = -b.y; }
+  PfItems I;
+  pf_items<M, false>(S, G, I);
+  // raw half-spectrum values: [u][0] rows (2g, 2g+1) at bin kk(n1), [u][1] at kk(n2)
+  float2 RA[PF_NI][2], RB[PF_NI][2], R0[2];
+  auto kk = [&](int n) { return n >= H ? N - n : n; };
+  // Z[n] of the packed transform from the two rows' bins (absent rows read as 0)
+  auto spec = [&](float2 a, float2 b, int n, bool va, bool vb) -> float2 {
+    if (!va) a = make_float2(0.f, 0.f);
+    if (!vb) b = make_float2(0.f, 0.f);
+    if (n >= H) { a.y = -a.y; b.y = -b.y; }
     if (n == 0 || 2 * n == N) { a.y = 0.f; b.y = 0.f; }  // real part semantics of ifftn(...).real
     return make_float2(a.x - b.y, a.y + b.x);
   };
-  auto fetch = [&](PfRegs& R, long long gi) {
+  auto fetch = [&](long long gi) {
     const int tile = (int)(gi / nrb);
     const int q0 = (int)(gi - (long long)tile * nrb) * 2 * G;
     const float2* in = Sin + (long long)tile * s_tstride;
-    auto rows = [&](int g, const float2*& ra, const float2*& rb) {
+    auto rows = [&](int g, const float2*& ra, const float2*& rb) {  // absent rows read the last row
       const int q = q0 + 2 * g;
-      ra = q < nrows ? in + (long long)q * H : nullptr;
-      rb = q + 1 < nrows ? in + (long long)(q + 1) * H : nullptr;
+      ra = in + (long long)min(q, nrows - 1) * H;
+      rb = in + (long long)min(q + 1, nrows - 1) * H;
     };
-    pf_fetch<M, false>(
-        S, G, R,
-        [&](int g, int j, int r, float2& a, float2& b) {
+    pf_fetch<M>(
+        G, I,
+        [&](int u, int g, int n1, int n2) {
           const float2 *ra, *rb;
           rows(g, ra, rb);
-          a = spec(ra, rb, j + r * M);
-          b = spec(ra, rb, j + (S.p - r) * M);
+          const int k1 = kk(n1), k2 = kk(n2);
+          RA[u][0] = ra[k1];
+          RB[u][0] = rb[k1];
+          RA[u][1] = ra[k2];
+          RB[u][1] = rb[k2];
         },
         [&](int g, int j) {
           const float2 *ra, *rb;
           rows(g, ra, rb);
-          return spec(ra, rb, j);
+          R0[0] = ra[j];
+          R0[1] = rb[j];
         });
   };
   long long grp = xcd_remap(blockIdx.x, gridDim.x);
-  PfRegs R;
-  if (grp < ngrp) fetch(R, grp);
+  if (grp < ngrp) fetch(grp);
   for (; grp < ngrp; grp += gridDim.x) {
-    pf_put<M, false>(S, G, L.sd, R);
-    __syncthreads();
-    if (grp + gridDim.x < ngrp) fetch(R, grp + gridDim.x);
-    pf_matrix<M, true>(S, L.sd, L.coef, G);
     const int tile = (int)(grp / nrb);
     const int q0 = (int)(grp - (long long)tile * nrb) * 2 * G;
+    pf_put<M>(
+        S, G, L.sd, I,
+        [&](int u, int g, int n1, int n2, float2& a, float2& b) {
+          const int q = q0 + 2 * g;
+          a = spec(RA[u][0], RB[u][0], n1, q < nrows, q + 1 < nrows);
+          b = spec(RA[u][1], RB[u][1], n2, q < nrows, q + 1 < nrows);
+        },
+        [&](int g, int j) {
+          const int q = q0 + 2 * g;
+          return spec(R0[0], R0[1], j, q < nrows, q + 1 < nrows);
+        });
+    __syncthreads();
+    if (grp + gridDim.x < ngrp) fetch(grp + gridDim.x);
+    pf_matrix<M, true>(S, L.sd, L.coef, G);
     float* out = res + (long long)tile * r_tstride;
     pf_final<M, true>(S, (const float2*)L.sd, L.tw, G, [&](int g, int j, const float2* v) {
       const int q = q0 + 2 * g;
@@ -615,12 +701,14 @@ template <int SZM, bool EXACT>
 __global__ __launch_bounds__(640) void k_tile_zconv(float2* S, long long s_tstride, int TH, int sz_rt, int zpad, int Zp,
                                                     const int* __restrict__ zsrc, const float2* __restrict__ Wz,
                                                     long long w_stride, const mm_tile_dev* __restrict__ tiles,
-                                                    int ntiles, int tpb, int G) {
+                                                    int ntiles, int tpb, int G_rt) {
   const int sz = EXACT ? SZM : sz_rt;
+  const int G = EXACT ? 640 / SZM : G_rt;  // columns per block
+  const int SP = EXACT ? SZM + 1 : sz_rt + 1;  // LDS stride of one column (odd: conflict-free across columns)
   extern __shared__ __align__(16) unsigned char smem[];
   float2* wl = (float2*)smem;                        // [p][g]
-  float2* xl = wl + (size_t)G * Zp;                  // [j][s][g]
-  int* sl = (int*)(xl + (size_t)ZC_PF * sz * G);     // [s][4]
+  float2* xl = wl + (size_t)G * Zp;                  // [j][g][s], column stride SP
+  int* sl = (int*)(xl + (size_t)ZC_PF * G * SP);     // [s][4]
   const int ncb = (TH + G - 1) / G;
   const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
   const int cb = bid % ncb, tg = bid / ncb;
@@ -665,7 +753,7 @@ __global__ __launch_bounds__(640) void k_tile_zconv(float2* S, long long s_tstri
     __syncthreads();  // the previous chunk is no longer read from xl
     if (act) {
 #pragma unroll
-      for (int j = 0; j < ZC_PF; ++j) xl[((size_t)j * sz + t) * G + g] = pre[j];
+      for (int j = 0; j < ZC_PF; ++j) xl[((size_t)j * G + g) * SP + t] = pre[j];
     }
     __syncthreads();
     const int nxt = cs + ZC_PF;
@@ -675,14 +763,16 @@ __global__ __launch_bounds__(640) void k_tile_zconv(float2* S, long long s_tstri
 #pragma unroll
     for (int j = 0; j < ZC_PF; ++j) {
       if (col && cs + j < t1) {
-        const float2* xv = xl + (size_t)j * sz * G + g;
+        const float2* xv = xl + ((size_t)j * G + g) * SP;
         float2 y = make_float2(0.f, 0.f);
 #pragma unroll
         for (int s = 0; s < SZM; ++s) {
           if (s < sz) {
-            const float2 x = xv[s * G];
-            y.x += K[s].x * x.x - K[s].y * x.y;
-            y.y += K[s].x * x.y + K[s].y * x.x;
+            const float2 x = xv[s];
+            y.x = fmaf(K[s].x, x.x, y.x);
+            y.x = fmaf(-K[s].y, x.y, y.x);
+            y.y = fmaf(K[s].x, x.y, y.y);
+            y.y = fmaf(K[s].y, x.x, y.y);
           }
         }
         S[(long long)(cs + j) * s_tstride + cofs] = y;
@@ -693,11 +783,13 @@ __global__ __launch_bounds__(640) void k_tile_zconv(float2* S, long long s_tstri
 
 static int launch_zconv(mm_deconv* d, float2* S, long long s_ts, const float2* Wz, const mm_tile_dev* td, int ntiles,
                         int tpb, hipStream_t st) {
-  const int sz = d->sz, Zp = d->Zp, TH = d->T * d->H, G = d->zG;
+  const int sz = d->sz, Zp = d->Zp, TH = d->T * d->H;
+  const bool exact = sz == 40 || sz == 35 || sz == 20;
+  const int G = exact ? 640 / sz : d->zG;
   const int ngroups = (ntiles + tpb - 1) / tpb;
   const long long nblk = (long long)((TH + G - 1) / G) * ngroups;
   const int threads = ((G * sz + 63) / 64) * 64;
-  const size_t lds = sizeof(float2) * ((size_t)G * Zp + (size_t)ZC_PF * sz * G) + sizeof(int) * 4 * sz;
+  const size_t lds = sizeof(float2) * ((size_t)G * Zp + (size_t)ZC_PF * G * (sz + 1)) + sizeof(int) * 4 * sz;
 #define MM_ZCX(M, E)                                                                                             \
   hipLaunchKernelGGL((k_tile_zconv<M, E>), dim3((unsigned)nblk), dim3(threads), lds, st, S, s_ts, TH, sz, d->zpad, \
                      Zp, d->zsrc_dev, Wz, (long long)Zp * TH, td, ntiles, tpb, G)

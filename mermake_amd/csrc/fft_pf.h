@@ -117,63 +117,78 @@ __device__ __forceinline__ void pf_bfly(float2* v) {
 // consecutive transforms (columns of a strided pass).  Otherwise (j, r, g):
 // consecutive lanes read consecutive elements of one row.
 #define PF_NI 8  // items per thread: G * M * h <= PF_NI * blockDim.x
-struct PfRegs {
-  float2 a[PF_NI], b[PF_NI], v0;
+
+// Per-thread item table, the same for every group (decoded once per kernel):
+// nn = n1 | n2 << 16 with n1 = j + r M, n2 = j + (p - r) M; sg = LDS slot | g << 16.
+struct PfItems {
+  unsigned nn[PF_NI], sg[PF_NI];
+  int nval;  // items of this thread (a prefix of u = 0..PF_NI-1)
 };
 
 template <int M, bool GFAST>
-__device__ __forceinline__ void pf_item(const FastDiv& fG, const FastDiv& fh, int t, int& g, int& j, int& r) {
-  if (GFAST) {  // t = ((r - 1) M + j) G + g
-    int q;
-    fG.divmod(t, q, g);
-    r = q / M;
-    j = q - r * M;
-  } else {      // t = (g h + r - 1) M + j
-    const int q = t / M;
-    j = t - q * M;
-    fh.divmod(q, g, r);
-  }
-  ++r;
-}
-
-// ld(g, j, r, a, b) loads the pair of item (g, j, r); ld0(g, j) returns x_g[j].
-template <int M, bool GFAST, class Ld, class Ld0>
-__device__ __forceinline__ void pf_fetch(const PfShape& S, int G, PfRegs& R, Ld ld, Ld0 ld0) {
+__device__ __forceinline__ void pf_items(const PfShape& S, int G, PfItems& I) {
   const int tot = G * M * S.h;
   const FastDiv fG(G), fh(S.h);
+  I.nval = 0;
 #pragma unroll
   for (int u = 0; u < PF_NI; ++u) {
     const int t = threadIdx.x + u * blockDim.x;
-    R.a[u] = make_float2(0.f, 0.f);
-    R.b[u] = make_float2(0.f, 0.f);
+    I.nn[u] = 0;
+    I.sg[u] = 0;
     if (t < tot) {
       int g, j, r;
-      pf_item<M, GFAST>(fG, fh, t, g, j, r);
-      ld(g, j, r, R.a[u], R.b[u]);
+      if (GFAST) {  // t = ((r - 1) M + j) G + g
+        int q;
+        fG.divmod(t, q, g);
+        r = q / M;
+        j = q - r * M;
+      } else {      // t = (g h + r - 1) M + j
+        const int q = t / M;
+        j = t - q * M;
+        fh.divmod(q, g, r);
+      }
+      ++r;
+      I.nn[u] = (unsigned)(j + r * M) | ((unsigned)(j + (S.p - r) * M) << 16);
+      I.sg[u] = (unsigned)((g * M + j) * S.sb + r) | ((unsigned)g << 16);
+      I.nval = u + 1;
     }
   }
-  R.v0 = make_float2(0.f, 0.f);
+}
+
+// Issues the loads of every item of this thread: ld(u, g, n1, n2) for item u
+// (the kernel keeps the raw values in registers of its own, indexed by the
+// compile-time u), and ld0(g, j) on threads t = g M + j < G M for x_g[j].
+// Loads must be unconditional (clamped addresses, validity kept aside) so
+// that none of them waits for another.
+template <int M, class Ld, class Ld0>
+__device__ __forceinline__ void pf_fetch(int G, const PfItems& I, Ld ld, Ld0 ld0) {
+#pragma unroll
+  for (int u = 0; u < PF_NI; ++u)
+    if (u < I.nval) ld(u, (int)(I.sg[u] >> 16), (int)(I.nn[u] & 0xffffu), (int)(I.nn[u] >> 16));
   if ((int)threadIdx.x < G * M) {
     const int g = threadIdx.x / M;
-    R.v0 = ld0(g, (int)threadIdx.x - g * M);
+    ld0(g, (int)threadIdx.x - g * M);
   }
 }
 
-template <int M, bool GFAST>
-__device__ __forceinline__ void pf_put(const PfShape& S, int G, float4* __restrict__ sd, const PfRegs& R) {
-  const int tot = G * M * S.h;
-  const FastDiv fG(G), fh(S.h);
+// get(u, g, n1, n2, a, b) turns the raw registers of item u into a = x_g[n1],
+// b = x_g[n2]; get0(g, j) returns x_g[j].  Writes (a + b, a - b) per item.
+template <int M, class Get, class Get0>
+__device__ __forceinline__ void pf_put(const PfShape& S, int G, float4* __restrict__ sd, const PfItems& I, Get get,
+                                       Get0 get0) {
 #pragma unroll
   for (int u = 0; u < PF_NI; ++u) {
-    const int t = threadIdx.x + u * blockDim.x;
-    if (t < tot) {
-      int g, j, r;
-      pf_item<M, GFAST>(fG, fh, t, g, j, r);
-      const float2 a = R.a[u], b = R.b[u];
-      sd[(g * M + j) * S.sb + r] = make_float4(a.x + b.x, a.y + b.y, a.x - b.x, a.y - b.y);
+    if (u < I.nval) {
+      float2 a, b;
+      get(u, (int)(I.sg[u] >> 16), (int)(I.nn[u] & 0xffffu), (int)(I.nn[u] >> 16), a, b);
+      sd[I.sg[u] & 0xffffu] = make_float4(a.x + b.x, a.y + b.y, a.x - b.x, a.y - b.y);
     }
   }
-  if ((int)threadIdx.x < G * M) sd[threadIdx.x * S.sb] = make_float4(R.v0.x, R.v0.y, 0.f, 0.f);
+  if ((int)threadIdx.x < G * M) {
+    const int g = threadIdx.x / M;
+    const float2 v0 = get0(g, (int)threadIdx.x - g * M);
+    sd[threadIdx.x * S.sb] = make_float4(v0.x, v0.y, 0.f, 0.f);
+  }
 }
 
 // ------------------------------------------------------------ stage 2
@@ -238,14 +253,17 @@ __device__ __forceinline__ void pf_matrix(const PfShape& S, float4* __restrict__
 // ------------------------------------------------------------ stage 3
 // st(g, j, v) stores the M results v[r] = X_g[j + r p] of butterfly j, which
 // reads Y[g N + j + r p].
-template <int M, bool INV, class St>
+// GFAST: consecutive lanes take consecutive transforms (the stores of a
+// strided pass then cover G adjacent columns of one row).
+template <int M, bool INV, bool GFAST = false, class St>
 __device__ __forceinline__ void pf_final(const PfShape& S, const float2* __restrict__ Y, const float2* __restrict__ tw,
                                          int G, St st) {
   const int tot = G * S.p;
-  const FastDiv fp(S.p);
+  const FastDiv fp(S.p), fG(G);
   for (int t = threadIdx.x; t < tot; t += blockDim.x) {
     int g, j;
-    fp.divmod(t, g, j);
+    if (GFAST) fG.divmod(t, j, g);
+    else fp.divmod(t, g, j);
     const float2* y = Y + (size_t)g * S.N + j;
     float2 v[M];
     v[0] = y[0];
