@@ -72,116 +72,99 @@ extern "C" int mm_box_1d_sub(const float* res, const float* buf, float* out, int
 // ------------------------------------------------------- fused blur-subtract
 // out = res - box_y(box_x(res)) on (nplanes, T, T) float32 planes, for a window
 // half-width D known at compile time (deconvolver.py:160-163 with
-// blur_radius = 2D or 2D+1).  A block owns a 64x64 output patch: it stages the
-// patch plus a D halo in LDS, computes box_x for the patch rows over the halo
-// columns, then box_y and the subtraction.  Every output keeps the sequential
-// low-to-high float32 window sum and the single division of blur.cu:20-51,
-// so results equal the two-pass form bit for bit; a thread produces 8
-// consecutive outputs from 8+2D values held in registers, and the next
-// plane's patch is loaded into registers while the current one is computed.
+// blur_radius = 2D or 2D+1).  Every output keeps the arithmetic of
+// blur.cu:20-51 -- the window summed sequentially from its lowest to its
+// highest index in float32, starting from +0, then one correctly rounded
+// division by the element count -- so results equal the two-pass reference
+// form bit for bit.  Two facts make the kernel branch-free:
+//   * windows clipped at the plane border are summed over zero padding: the
+//     sequential sum meets the +0 terms before (or after) the in-plane ones,
+//     and +0 + x = x, x + 0 = x exactly, so the sum is unchanged;
+//   * the division uses q = s*y, r = fma(-q, n, s), q' = fma(r, y, q) with
+//     y = RN(1/n), which equals RN(s/n) whenever |s| >= 2^-100 (checked
+//     exhaustively over whole binades for n = 1..64); smaller |s| takes the
+//     IEEE division.
+// A block owns BX = 16 consecutive rows of one plane across all T columns.
+// x pass: a thread takes a column, reads its BX + 2D rows from global memory
+// into registers and writes the BX box_x values into an LDS row buffer with D
+// zero columns on each side; y pass: a thread takes 9 consecutive outputs of
+// one row from that buffer and writes res - box_y.
+__device__ __forceinline__ float div_count(float s, float n, float y) {
+  if (__builtin_expect(fabsf(s) < 0x1p-100f, 0)) return s / n;
+  const float q = s * y;
+  return fmaf(fmaf(-q, n, s), y, q);
+}
+
+#define BLUR_BX 16
+#define BLUR_YB 9
 template <int D>
-__global__ __launch_bounds__(512, 2) void k_blur_sub_xy(const float* __restrict__ res, float* __restrict__ out, int T,
-                                                     int nplanes, int pz) {
-  constexpr int NT = 512;
-  constexpr int BX = 64, BY = 64, XB = 8, W = XB + 2 * D;
-  constexpr int TWc = BY + 2 * D;      // staged columns (y)
-  constexpr int THr = BX + 2 * D;      // staged rows (x)
-  constexpr int P = TWc | 1;           // odd row pitch: conflict-free column walks
-  constexpr int TOT = THr * TWc;
-  constexpr int NPT = (TOT + NT - 1) / NT;  // staged values per thread
-  extern __shared__ __align__(16) unsigned char smem[];
-  float* tin = (float*)smem;           // [THr][P]: rows x0-D.., cols y0-D..
-  float* tb = tin + THr * P;           // [BX][P]: box_x of rows x0.., cols y0-D..
-  const int nb = (T + BX - 1) / BX;
-  // XCD-aware order: neighbouring patches of a plane group run on one XCD and share its L2 for the halos
-  const unsigned bid = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
-  const int patch = bid % gridDim.x, pgrp = bid / gridDim.x;
-  const int x0 = (patch / nb) * BX, y0 = (patch % nb) * BY;
-  const int p0 = pgrp * pz, p1 = min(nplanes, p0 + pz);
-  // the next plane's patch + halo is loaded into registers while the current one is computed
-  float pf[NPT];
-  auto load = [&](int plane) {
-    const float* rp = res + (long long)plane * T * T;
+__global__ __launch_bounds__(256) void k_blur_sub_xy(const float* __restrict__ res, float* __restrict__ out, int T,
+                                                     int nplanes, int npatch) {
+  constexpr int BX = BLUR_BX, NR = BX + 2 * D, YB = BLUR_YB, W = YB + 2 * D;
+  const int nyb = (T + YB - 1) / YB;
+  const int pitch = ((nyb * YB + 2 * D) | 1);  // every y chunk's window stays inside its row
+  extern __shared__ __align__(16) float smem_f[];
+  float* rcp = smem_f;                 // [64]: RN(1/n)
+  float* tb = smem_f + 64;             // [BX][pitch]: box_x of the patch rows, D zero columns each side
+  // neighbouring patches of a plane run on one XCD and share its L2 for the halo rows
+  const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int plane = bid / npatch, patch = bid - (bid / npatch) * npatch;
+  if (plane >= nplanes) return;
+  const int x0 = patch * BX;
+  const float* rp = res + (long long)plane * T * T;
+  if (threadIdx.x < 64) rcp[threadIdx.x] = 1.0f / (float)(threadIdx.x ? threadIdx.x : 1);
+  const int nz = pitch - T;  // zero columns per row: [0, D) and [D + T, pitch)
+  for (int i = threadIdx.x; i < BX * nz; i += blockDim.x) {
+    const int r = i / nz, c = i - r * nz;
+    tb[r * pitch + (c < D ? c : T + c)] = 0.0f;
+  }
+  __syncthreads();
+  // x pass: row windows are uniform over the block
+  float yx[BX], nx[BX];
 #pragma unroll
-    for (int u = 0; u < NPT; ++u) {
-      const int t = threadIdx.x + u * NT;
-      const int r = t / TWc, c = t - r * TWc;
-      const int gx = x0 - D + r, gy = y0 - D + c;
-      pf[u] = (t < TOT && gx >= 0 && gx < T && gy >= 0 && gy < T) ? rp[(long long)gx * T + gy] : 0.0f;
+  for (int xo = 0; xo < BX; ++xo) {
+    const int gx = x0 + xo;
+    const int n = min(T - 1, gx + D) - max(0, gx - D) + 1;
+    nx[xo] = (float)n;
+    yx[xo] = rcp[n];
+  }
+  for (int c = threadIdx.x; c < T; c += blockDim.x) {
+    float v[NR];
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+      const int gx = x0 - D + i;
+      v[i] = (gx >= 0 && gx < T) ? rp[(long long)gx * T + c] : 0.0f;
     }
-  };
-  load(p0);
-  for (int plane = p0; plane < p1; ++plane) {
-    __syncthreads();  // the previous plane no longer reads tin / tb
 #pragma unroll
-    for (int u = 0; u < NPT; ++u) {
-      const int t = threadIdx.x + u * NT;
-      const int r = t / TWc, c = t - r * TWc;
-      if (t < TOT) tin[r * P + c] = pf[u];
+    for (int xo = 0; xo < BX; ++xo) {
+      float sum = 0.0f;
+#pragma unroll
+      for (int w = 0; w <= 2 * D; ++w) sum += v[xo + w];
+      if (x0 + xo < T) tb[xo * pitch + D + c] = div_count(sum, nx[xo], yx[xo]);
     }
-    __syncthreads();
-    if (plane + 1 < p1) load(plane + 1);
-    // box along x for the patch rows, all staged columns
-    for (int it = threadIdx.x; it < TWc * (BX / XB); it += blockDim.x) {
-      const int c = it % TWc, xb = it / TWc;
-      const int gy = y0 - D + c;
-      const int xbase = x0 + xb * XB;
-      if (gy < 0 || gy >= T || xbase >= T) continue;
-      float v[W];
+  }
+  __syncthreads();
+  // y pass
+  for (int it = threadIdx.x; it < BX * nyb; it += blockDim.x) {
+    const int xo = it / nyb, yb = it - (it / nyb) * nyb;
+    const int gx = x0 + xo;
+    if (gx >= T) continue;
+    const int y0 = yb * YB;
+    const float* trow = tb + xo * pitch + y0;  // trow[w] = box_x at column y0 - D + w
+    float u[W];
 #pragma unroll
-      for (int w = 0; w < W; ++w) v[w] = tin[(xb * XB + w) * P + c];
-      const bool interior = xbase - D >= 0 && xbase + XB - 1 + D <= T - 1;
+    for (int w = 0; w < W; ++w) u[w] = trow[w];
+    const float* rrow = rp + (long long)gx * T;
+    float* orow = out + (long long)plane * T * T + (long long)gx * T;
 #pragma unroll
-      for (int xo = 0; xo < XB; ++xo) {
-        const int gx = xbase + xo;
+    for (int yo = 0; yo < YB; ++yo) {
+      const int gy = y0 + yo;
+      if (gy < T) {
         float sum = 0.0f;
-        int num;
-        if (interior) {
 #pragma unroll
-          for (int w = 0; w <= 2 * D; ++w) sum += v[xo + w];
-          num = 2 * D + 1;
-        } else {
-          const int lo = max(0, gx - D), hi = min(T - 1, gx + D);
-#pragma unroll
-          for (int w = 0; w <= 2 * D; ++w) {
-            const int gi = gx - D + w;
-            if (gi >= lo && gi <= hi) sum += v[xo + w];
-          }
-          num = hi - lo + 1;
-        }
-        if (gx < T) tb[(xb * XB + xo) * P + c] = sum / (float)num;
-      }
-    }
-    __syncthreads();
-    // box along y and subtract
-    const int yb = threadIdx.x % (BY / XB), xr = threadIdx.x / (BY / XB);
-    const int gx = x0 + xr;
-    const int ybase = y0 + yb * XB;
-    if (xr < BX && gx < T && ybase < T) {
-      float* op = out + (long long)plane * T * T + (long long)gx * T;
-      float u[W];
-#pragma unroll
-      for (int w = 0; w < W; ++w) u[w] = tb[xr * P + yb * XB + w];
-      const bool interior = ybase - D >= 0 && ybase + XB - 1 + D <= T - 1;
-#pragma unroll
-      for (int yo = 0; yo < XB; ++yo) {
-        const int gy = ybase + yo;
-        float sum = 0.0f;
-        int num;
-        if (interior) {
-#pragma unroll
-          for (int w = 0; w <= 2 * D; ++w) sum += u[yo + w];
-          num = 2 * D + 1;
-        } else {
-          const int lo = max(0, gy - D), hi = min(T - 1, gy + D);
-#pragma unroll
-          for (int w = 0; w <= 2 * D; ++w) {
-            const int gi = gy - D + w;
-            if (gi >= lo && gi <= hi) sum += u[yo + w];
-          }
-          num = hi - lo + 1;
-        }
-        if (gy < T) op[gy] = tin[(xr + D) * P + yb * XB + yo + D] - sum / (float)num;
+        for (int w = 0; w <= 2 * D; ++w) sum += u[yo + w];
+        const int n = min(T - 1, gy + D) - max(0, gy - D) + 1;
+        orow[gy] = rrow[gy] - div_count(sum, (float)n, rcp[n]);
       }
     }
   }
@@ -189,13 +172,14 @@ __global__ __launch_bounds__(512, 2) void k_blur_sub_xy(const float* __restrict_
 
 template <int D>
 static int launch_blur_sub(const float* res, float* out, int nplanes, int T, hipStream_t st) {
-  constexpr int BX = 64, P = (64 + 2 * D) | 1;
-  const size_t lds = sizeof(float) * (size_t)P * ((BX + 2 * D) + BX);
-  const int nb = (T + BX - 1) / BX;
-  const int pz = 8;  // planes per block
-  const int ng = (nplanes + pz - 1) / pz;
-  if (ng > 65535) { mm_set_error("mm_blur_sub_xy: too many planes"); return MM_EINVAL; }
-  hipLaunchKernelGGL(k_blur_sub_xy<D>, dim3(nb * nb, ng), dim3(512), lds, st, res, out, T, nplanes, pz);
+  if (2 * D + 1 > 63) { mm_set_error("mm_blur_sub_xy: window too wide"); return MM_EINVAL; }
+  const int nyb = (T + BLUR_YB - 1) / BLUR_YB;
+  const size_t lds = sizeof(float) * (64 + (size_t)BLUR_BX * ((nyb * BLUR_YB + 2 * D) | 1));
+  if (lds > 163840) { mm_set_error("mm_blur_sub_xy: plane of %d columns too wide", T); return MM_EINVAL; }
+  const int npatch = (T + BLUR_BX - 1) / BLUR_BX;
+  const long long nblk = (long long)npatch * nplanes;
+  if (nblk > 0x7fffffffLL) { mm_set_error("mm_blur_sub_xy: too many planes"); return MM_EINVAL; }
+  hipLaunchKernelGGL(k_blur_sub_xy<D>, dim3((unsigned)nblk), dim3(256), lds, st, res, out, T, nplanes, npatch);
   MM_LAUNCH_CHECK();
   return MM_OK;
 }

@@ -44,6 +44,26 @@ def test_box_1d_bitexact(dev, shape, axis, size):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("T_,delta,scale", [(678, 15, 1000.0), (100, 15, 1.0), (40, 15, 1e-3), (17, 15, 1000.0),
+                                             (678, 25, 1000.0), (53, 25, 1.0), (31, 25, 1e-30)])
+def test_blur_sub_fused_bitexact(dev, T_, delta, scale):
+    """The fused blur-subtract (deconvolver.py:160-163) equals res - box_1d(box_1d(res, axis 1), axis 2) bit for bit,
+    including windows clipped at the borders, planes narrower than the window and tiny sums."""
+    from mermake_amd import _lib as L
+    rng = np.random.default_rng(T_ + delta)
+    nplanes = 3
+    res = (rng.standard_normal((nplanes, T_, T_)) * scale).astype(np.float32)
+    res[0, :5, :5] = 0.0                     # zero windows
+    res[1, 3, :] = -0.0
+    size = 2 * delta
+    want = res - oblur.box_1d(oblur.box_1d(res, size, axis=1), size, axis=2)
+    r = T(res, dev)
+    out = torch.empty_like(r)
+    L.call("mm_blur_sub_xy", L.ptr(r), L.ptr(out), None, nplanes, T_, delta, L.stream_ptr(dev))
+    got = out.cpu().numpy()
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
 def test_box_multi_axis(dev):
     from mermake_amd import blur
     a = np.arange(64, dtype=np.float32).reshape(4, 4, 4)
