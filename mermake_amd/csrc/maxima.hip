@@ -76,6 +76,33 @@ __device__ __forceinline__ bool is_max_at(const VolDesc& V, int z, int x, int y,
   return true;
 }
 
+// delta = 1 (the hyb setting): the 6 face neighbours, all loads issued at once.
+// An axis too short for the single reflection leaves the neighbour out, as above.
+__device__ __forceinline__ bool is_max_d1(const VolDesc& V, int z, int x, int y, float c, float sign) {
+  const int nz[2] = {refl1(z - 1, V.D), refl1(z + 1, V.D)};
+  const int nx[2] = {refl1(x - 1, V.H), refl1(x + 1, V.H)};
+  const int ny[2] = {refl1(y - 1, V.W), refl1(y + 1, V.W)};
+  const long long base = z * V.sZ + x * V.sX + y;
+  long long o[6];
+  bool ok[6];
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    ok[s] = nz[s] >= 0 && nz[s] < V.D;
+    o[s] = base + (nz[s] - z) * V.sZ;
+    ok[2 + s] = nx[s] >= 0 && nx[s] < V.H;
+    o[2 + s] = base + (nx[s] - x) * V.sX;
+    ok[4 + s] = ny[s] >= 0 && ny[s] < V.W;
+    o[4 + s] = base + (ny[s] - y);
+  }
+  float n[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) n[k] = V.p[ok[k] ? o[k] : base];  // the centre stands in for a missing neighbour
+  bool m = true;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) m = m && !(c < sign * n[k]);
+  return m;
+}
+
 template <class VS>
 __global__ __launch_bounds__(256) void k_max_mark(VS vs, float sign, float thr, int delta,
                                                   unsigned long long* __restrict__ masks, int* __restrict__ bcnt,
@@ -104,7 +131,9 @@ __global__ __launch_bounds__(256) void k_max_mark(VS vs, float sign, float thr, 
   for (int i = 0; i < 4; ++i) {
     const long long idx = start + i * 256 + threadIdx.x;
     // maxima.cu:19: v <= threshold is rejected (a NaN threshold rejects nothing)
-    const bool m = idx < nvox && !(c[i] <= thr) && is_max_at(V, zz[i], xx[i], yy[i], c[i], sign, delta);
+    const bool m = idx < nvox && !(c[i] <= thr) &&
+                   (delta == 1 ? is_max_d1(V, zz[i], xx[i], yy[i], c[i], sign)
+                               : is_max_at(V, zz[i], xx[i], yy[i], c[i], sign, delta));
     const unsigned long long b = __ballot(m);
     if (lane == 0) mw[i * 4 + wave] = b;
     cnt += __popcll(b);
