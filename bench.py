@@ -74,6 +74,46 @@ def cpu_baseline(stack_np, flat_np, psf, cfg, ntiles_per_stack):
                       f"{ntiles_per_stack} tiles x {cfg['channels']} channels"}
 
 
+def serve_leg(pipe, stacks, flat, cfg, args, dev):
+    """The request loop on mixed FOV sizes (40 x {2048, 2304, 2800}^2, 3 channels each, host inputs in
+    pinned memory): Poisson arrivals at ``--serve-rate``, then all requests at once (saturation)."""
+    import torch
+    from mermake_amd.serve import FovRequest, HybServer, poisson_arrivals
+    from mermake_amd.synth import stack_torch
+    fovs = {}
+    for side in (2048, 2304, 2800):
+        if side == cfg["sx"]:
+            chans, fl = list(stacks), flat
+        else:
+            chans, fl = [], None
+            for c in range(cfg["channels"]):
+                s_, f_ = stack_torch(2000 + 10 * side + c, (cfg["sz"], side, side), cfg["density"], dev, flat=True)
+                chans.append(s_)
+                fl = f_
+        fovs[side] = ([c.cpu().pin_memory() for c in chans], fl.cpu().pin_memory())
+        torch.cuda.synchronize()
+    sides = [2048, 2304, 2800]
+    rng = np.random.default_rng(7)
+    order = [sides[i] for i in rng.integers(0, 3, size=args.serve_requests)]
+    srv = HybServer(pipe, batch_tiles=36, slots=2)
+    out = {"fov_sides": order, "inputs": "host pinned uint16 stacks + float32 flat (H2D inside the loop)"}
+    for name, arr in (("poisson", poisson_arrivals(len(order), args.serve_rate, seed=11)),
+                      ("saturated", np.zeros(len(order)))):
+        srv.run([FovRequest(-1 - i, *fovs[side]) for i, side in enumerate(order[:2])])  # warm the slots
+        reqs = [FovRequest(i, *fovs[side], arrival=float(a)) for i, (side, a) in enumerate(zip(order, arr))]
+        srv.run(reqs)
+        st = dict(srv.stats)
+        vox = sum(cfg["channels"] * cfg["sz"] * s_ * s_ for s_ in order)
+        st["voxels_per_s"] = vox / st["wall_s"]
+        if name == "poisson":
+            st["rate_fov_per_s"] = args.serve_rate
+        log(f"served {name}: {st['fovs_per_s']:.2f} FOV/s, p50 {st['p50_latency_s'] * 1e3:.1f} ms, "
+            f"p99 {st['p99_latency_s'] * 1e3:.1f} ms")
+        out[name] = st
+    del srv
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +121,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-tile GPU form")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served (request loop) leg")
+    ap.add_argument("--serve-requests", type=int, default=12)
+    ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate of the served leg (FOV/s)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="end with torch.neg over 1 GiB of float32 (reads 1 GiB, writes 1 GiB): the reference "
                          "kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -168,6 +211,10 @@ def main():
         pms = (time.perf_counter() - a) * 1e3
         plain = {"ms_per_fov": pms, "fovs_per_s": 1e3 / pms, "batched_speedup": pms / ms}
 
+    served = None
+    if not args.no_serve:
+        served = serve_leg(pipe, stacks, flat, cfg, args, dev)
+
     cpu = None
     if not args.no_cpu:
         log("CPU baseline (oracle, bounded sample)")
@@ -196,6 +243,7 @@ def main():
         "pass_ms": pt, "stack_ms": stack_ms,
         "latency_ms": {"p50": 1e3 * lat_sorted[len(lat) // 2], "p99": 1e3 * lat_sorted[-1]},
         "plain_gpu": plain,
+        "served": served,
         "cpu_baseline": cpu,
     }
     print(json.dumps(rec), flush=True)

@@ -94,6 +94,11 @@ def tile_geometry(shape, tile_size, overlap, n_psf=1):
 
 def tile_table(geom, image, flat):
     """Device array of ``mm_tile_desc`` for ``geom`` over ``image`` (and ``flat``)."""
+    return torch.from_numpy(tile_records(geom, image, flat).view(np.uint8).copy()).to(image.device)
+
+
+def tile_records(geom, image, flat):
+    """Host array of ``mm_tile_desc`` records for ``geom`` over device tensors ``image`` / ``flat``."""
     arr = np.zeros(len(geom), dtype=L.TILE_DTYPE)
     for i, g in enumerate(geom):
         for k in ("X0", "Y0", "xd", "yd", "xs", "ys", "xfirst", "yfirst", "xe", "ye", "psf"):
@@ -105,7 +110,7 @@ def tile_table(geom, image, flat):
         if flat is not None:
             arr[i]["flat"] = flat.data_ptr()
             arr[i]["flat_sx"] = flat.stride(0)
-    return torch.from_numpy(arr.view(np.uint8).copy()).to(image.device)
+    return arr
 
 
 def prepare_stack(image):

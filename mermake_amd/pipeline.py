@@ -21,6 +21,7 @@ one tile at a time, host syncs after the maxima and after the fit.
 RMS normalisation, maxima of +buffer and of -buffer (the negation is folded
 into the kernels' sign).
 """
+import numpy as np
 import torch
 
 from . import _lib as L
@@ -93,17 +94,52 @@ class HybPipeline:
                self.delta_fit, self.sigmaZ, self.sigmaXY, L.ptr(fit), st)
         return counts, fit
 
-    def __call__(self, stack, flat_field=None):
-        """Spots of one channel stack: ``(N, 8)`` float32 rows in tile order, then raster order."""
+    def tile_rows(self, table, n):
+        """Kept spot rows of each of ``n`` tile descriptors (a device table), as host arrays;
+        the candidate cap grows until no tile overflows it."""
         d = self.d
-        d._check(stack)
-        img = prepare_stack(stack)
-        flat = prepare_flat(flat_field, img)
-        geom, table_all = self._table(img, flat)
-        n = len(geom)
         single = d.psf_fft.shape[0] == 1
         st = L.stream_ptr(d.device)
-        out_parts = []
+        cap = self.cap
+        while True:
+            counts, fit = self._spots_batch(table, n, single, cap)
+            kept = self._buf("kept", (n,), torch.int32)
+            total = self._buf("total", (1,), torch.int32)
+            xh = torch.empty((n * cap, 8), dtype=torch.float32, device=d.device)
+            L.call("mm_tiles_keep", L.ptr(table), n, d.overlap, L.ptr(fit), L.ptr(counts), cap, L.ptr(kept),
+                   L.ptr(xh), n * cap, L.ptr(total), st)
+            stats = torch.cat([counts.max().view(1), kept]).cpu()
+            if int(stats[0]) > cap:
+                cap = int(stats[0])
+                continue
+            k = stats[1:].numpy()
+            rows = xh[:int(k.sum())].cpu().numpy()
+            offs = np.concatenate([[0], np.cumsum(k)])
+            return [rows[offs[i]:offs[i + 1]].copy() for i in range(n)]
+
+    def __call__(self, stack, flat_field=None):
+        """Spots of one channel stack: ``(N, 8)`` float32 rows in tile order, then raster order."""
+        return self.run_many([stack], [flat_field])[0]
+
+    def run_many(self, stacks, flat_fields=None):
+        """Spots of several channel stacks batched together: tiles of different stacks (of any x, y size,
+        the same depth) share every launch.  One ``(N, 8)`` tensor per stack, equal to ``self(stack, flat)``."""
+        d = self.d
+        if flat_fields is None:
+            flat_fields = [None] * len(stacks)
+        tables, owner = [], []
+        for si, (stack, flat_field) in enumerate(zip(stacks, flat_fields)):
+            d._check(stack)
+            img = prepare_stack(stack)
+            flat = prepare_flat(flat_field, img)
+            geom, tab = self._table(img, flat)
+            tables.append(tab)
+            owner += [si] * len(geom)
+        n = len(owner)
+        table_all = tables[0] if len(tables) == 1 else torch.cat(tables)
+        single = d.psf_fft.shape[0] == 1
+        st = L.stream_ptr(d.device)
+        parts = [[] for _ in stacks]
         B = 1 if self.plain else d.batch_size(n, self.cap * 48 + d.tile_height * d._plan.T ** 2 // 8)
         rec = L.TILE_DTYPE.itemsize
         for b0 in range(0, n, B):
@@ -124,16 +160,30 @@ class HybPipeline:
                 xh = torch.empty((nb * cap, 8), dtype=torch.float32, device=d.device)
                 L.call("mm_tiles_keep", L.ptr(table), nb, d.overlap, L.ptr(fit), L.ptr(counts), cap, L.ptr(kept),
                        L.ptr(xh), nb * cap, L.ptr(total), st)
-                stats = torch.stack([total[0], counts.max()]).cpu()   # one sync per batch
-                tot, mx = int(stats[0]), int(stats[1])
+                stats = torch.cat([counts.max().view(1), kept]).cpu()   # one sync per batch
+                mx = int(stats[0])
                 if mx > cap:
                     cap = mx
                     continue
-                out_parts.append(xh[:tot])
+                kept_h = stats[1:].tolist()
+                off = 0
+                for i in range(nb):                    # tiles of one stack are contiguous: split the rows by owner
+                    si = owner[b0 + i]
+                    if kept_h[i]:
+                        if parts[si] and parts[si][-1][2] is xh and parts[si][-1][1] == off:
+                            parts[si][-1][1] = off + kept_h[i]
+                        else:
+                            parts[si].append([off, off + kept_h[i], xh])
+                    off += kept_h[i]
                 break
-        if not out_parts:
-            return torch.zeros((0, 8), dtype=torch.float32, device=d.device)
-        return out_parts[0] if len(out_parts) == 1 else torch.cat(out_parts)
+        out = []
+        for si in range(len(stacks)):
+            views = [x[a:b] for a, b, x in parts[si]]
+            if not views:
+                out.append(torch.zeros((0, 8), dtype=torch.float32, device=d.device))
+            else:
+                out.append(views[0] if len(views) == 1 else torch.cat(views))
+        return out
 
 
 class DapiPipeline:
