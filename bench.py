@@ -53,6 +53,25 @@ def pass_bytes(geom, sz, T, Zp, single_psf=True, flat=True):
             "c2r": n * spec + n * 4 * sz * T * T}
 
 
+PASS_KERNEL = {"r2c": "k_pf_r2c", "x_fwd": "k_pf_c2c", "z_wiener": "k_tile_zconv", "x_inv": "k_pf_c2c",
+               "c2r": "k_pf_c2r"}
+
+
+def measured_traffic(pass_name):
+    """HBM bytes per launch of the pass's kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    (profiles/hyb_fov_hbm_traffic.md), or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hyb_fov_hbm_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("roofline-traffic:"):
+                _, name, val = line.split()
+                if name == PASS_KERNEL.get(pass_name):
+                    return float(val)
+    except OSError:
+        return None
+    return None
+
+
 def cpu_baseline(stack_np, flat_np, psf, cfg, ntiles_per_stack):
     """Oracle (float32 mode) on a tile sample: corner, two edges, interior of one channel."""
     from oracle import deconv as odec
@@ -237,7 +256,7 @@ def main():
                                "118x678x678 per channel), 1 PSF 40x64x64, beta 1e-4, blur 30, thr 3600, delta 1, "
                                "delta_fit 3", "fov_voxels": fov_vox, "spots_per_channel": nspots},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None, "kernel": dom,
+                     "frac": achieved / HBM_PEAK, "traffic": measured_traffic(dom), "kernel": dom,
                      "kernel_ms": pt[dom], "kernel_algorithmic_bytes": pb[dom]},
         "m1_roofline_frac": m1_frac,
         "pass_ms": pt, "stack_ms": stack_ms,
