@@ -130,9 +130,10 @@ int mm_fft3d_c2c(void* data, int n0, int n1, int n2, int axes_mask, int inverse,
  * sign*v > threshold and sign*v >= every neighbour of the delta sphere
  * (single reflection at the borders), written as int32 (z, x, y) triplets in
  * raster order; *count (device) receives the number found (entries beyond
- * cap are dropped).  ws: mm_maxima_ws_bytes(D*H*W, 1) bytes.
+ * cap are dropped).  ws: mm_maxima_ws_bytes(D, H, W, 1) bytes.
  */
-size_t mm_maxima_ws_bytes(long long nvox_max, int nvol);
+/* workspace bytes for nvol volumes of at most D x H x W voxels (row bit masks and row counts) */
+size_t mm_maxima_ws_bytes(int D, int H, int W, int nvol);
 int mm_local_maxima(const float* img, int D, int H, int W, long long sZ, long long sX, float sign, float threshold,
                     int delta, int* coords, int cap, int* count, void* ws, size_t ws_bytes, void* stream);
 /* Replaces delta_fit_cross_corr (mermake/maxima.cu:148-341): out8 rows
@@ -144,7 +145,7 @@ int mm_delta_fit(const float* img, int D, int H, int W, long long sZ, long long 
 
 /* Batched forms over the tile views res[t][:, :xe, :ye] of mm_deconv_tiles:
  * the hyb loop of mermake/__main__.py:254-265.  coords: ntiles*cap*3,
- * counts: ntiles, out8: ntiles*cap*8, ws: mm_maxima_ws_bytes(sz*T*T, ntiles). */
+ * counts: ntiles, out8: ntiles*cap*8, ws: mm_maxima_ws_bytes(sz, T, T, ntiles). */
 int mm_tiles_maxima(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const float* res, float threshold,
                     int delta, int* coords, int cap, int* counts, void* ws, size_t ws_bytes, void* stream);
 int mm_tiles_fit(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const float* res, const int* coords,

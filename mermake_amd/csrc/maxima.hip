@@ -5,10 +5,11 @@
 // mermake/maxima.py:17-98 (driver), mermake/__main__.py:254-265 (keep-filter).
 //
 // Maxima are found in three passes that never use atomics, so the candidate
-// list is in raster order and identical from run to run:
-//   k_max_mark     one bit per voxel (64-voxel ballot words) + per-block counts
-//   k_max_scan     exclusive scan of the block counts of each volume
-//   k_max_compact  popcount ranks -> (z, x, y) at block offset + rank
+// list is in raster order and identical from run to run.  The unit is a row
+// (z, x) of the volume, handled by one wavefront with the lanes along y:
+//   k_max_mark     one bit per voxel (64-voxel ballot words per row) + row counts
+//   k_max_scan     exclusive scan of the row counts of each volume
+//   k_max_compact  popcount ranks -> (z, x, y) at row offset + rank
 // The fit runs one wavefront per candidate: the lanes sample the delta_fit
 // sphere, and the statistics are wave reductions (two-pass mean/variance).
 #include "deconv.h"
@@ -44,8 +45,7 @@ struct TileVols {  // view res[:, :xe, :ye] of every tile (deconvolver.py:170)
   }
 };
 
-#define MX_BLOCK_VOX 1024
-#define MX_WORDS 16
+#define MX_CHUNK 8  // row words whose centre loads are in flight together
 
 __device__ __forceinline__ int refl1(int n, int N) {  // maxima.cu:44-52
   if (n < 0) return -n;
@@ -103,44 +103,44 @@ __device__ __forceinline__ bool is_max_d1(const VolDesc& V, int z, int x, int y,
   return m;
 }
 
+// Rows r = z H + x of volume blockIdx.y, four per block (one per wavefront);
+// ww words of 64 voxels per row.  Wavefront-only work: no block barrier.
 template <class VS>
 __global__ __launch_bounds__(256) void k_max_mark(VS vs, float sign, float thr, int delta,
-                                                  unsigned long long* __restrict__ masks, int* __restrict__ bcnt,
-                                                  int nblk) {
+                                                  unsigned long long* __restrict__ masks, int* __restrict__ rcnt,
+                                                  int nrow_max, int ww) {
   const int v = blockIdx.y;
   const VolDesc V = vs(v);
-  const long long nvox = (long long)V.D * V.H * V.W;
-  const long long start = (long long)blockIdx.x * MX_BLOCK_VOX;
-  unsigned long long* mw = masks + ((long long)v * nblk + blockIdx.x) * MX_WORDS;
-  __shared__ int wcnt[4];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const long long hw = (long long)V.H * V.W;
-  int zz[4], xx[4], yy[4];
-  float c[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {  // all four centre loads in flight before any test
-    const long long idx = start + i * 256 + threadIdx.x;
-    zz[i] = (int)(idx / hw);
-    const int rem = (int)(idx - (long long)zz[i] * hw);
-    xx[i] = rem / V.W;
-    yy[i] = rem - xx[i] * V.W;
-    c[i] = idx < nvox ? sign * V.p[zz[i] * V.sZ + xx[i] * V.sX + yy[i]] : 0.0f;
-  }
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= nrow_max) return;
+  const long long ridx = (long long)v * nrow_max + row;
   int cnt = 0;
+  if (row < V.D * V.H) {
+    const int z = row / V.H, x = row - (row / V.H) * V.H;
+    const float* rp = V.p + z * V.sZ + x * V.sX;
+    unsigned long long* mw = masks + ridx * ww;
+    for (int k0 = 0; k0 < ww; k0 += MX_CHUNK) {
+      float c[MX_CHUNK];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const long long idx = start + i * 256 + threadIdx.x;
-    // maxima.cu:19: v <= threshold is rejected (a NaN threshold rejects nothing)
-    const bool m = idx < nvox && !(c[i] <= thr) &&
-                   (delta == 1 ? is_max_d1(V, zz[i], xx[i], yy[i], c[i], sign)
-                               : is_max_at(V, zz[i], xx[i], yy[i], c[i], sign, delta));
-    const unsigned long long b = __ballot(m);
-    if (lane == 0) mw[i * 4 + wave] = b;
-    cnt += __popcll(b);
+      for (int k = 0; k < MX_CHUNK; ++k) {  // the chunk's centre loads in flight before any test
+        const int y = (k0 + k) * 64 + lane;
+        c[k] = y < V.W ? sign * rp[y] : 0.0f;
+      }
+#pragma unroll
+      for (int k = 0; k < MX_CHUNK; ++k) {
+        if (k0 + k >= ww) break;
+        const int y = (k0 + k) * 64 + lane;
+        // maxima.cu:19: v <= threshold is rejected (a NaN threshold rejects nothing)
+        const bool m = y < V.W && !(c[k] <= thr) &&
+                       (delta == 1 ? is_max_d1(V, z, x, y, c[k], sign) : is_max_at(V, z, x, y, c[k], sign, delta));
+        const unsigned long long b = __ballot(m);
+        if (lane == 0) mw[k0 + k] = b;
+        cnt += __popcll(b);
+      }
+    }
   }
-  if (lane == 0) wcnt[wave] = cnt;
-  __syncthreads();
-  if (threadIdx.x == 0) bcnt[(long long)v * nblk + blockIdx.x] = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+  if (lane == 0) rcnt[ridx] = cnt;
 }
 
 // exclusive scan of the block counts of volume blockIdx.x; total -> counts[v]
@@ -172,40 +172,36 @@ __global__ __launch_bounds__(1024) void k_max_scan(int* __restrict__ bcnt, int n
   if (threadIdx.x == 0) counts[v] = carry;
 }
 
+// Row r of volume blockIdx.y writes its maxima at roff[r] + rank (raster order).
 template <class VS>
 __global__ __launch_bounds__(256) void k_max_compact(VS vs, const unsigned long long* __restrict__ masks,
-                                                     const int* __restrict__ boff, int nblk, int* __restrict__ coords,
-                                                     int cap) {
+                                                     const int* __restrict__ roff, const int* __restrict__ counts,
+                                                     int nrow_max, int ww, int* __restrict__ coords, int cap) {
   const int v = blockIdx.y;
   const VolDesc V = vs(v);
-  __shared__ unsigned long long w[MX_WORDS];
-  __shared__ int pre[MX_WORDS];
-  const unsigned long long* mw = masks + ((long long)v * nblk + blockIdx.x) * MX_WORDS;
-  if (threadIdx.x < MX_WORDS) w[threadIdx.x] = mw[threadIdx.x];
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int s = 0;
-    for (int k = 0; k < MX_WORDS; ++k) { pre[k] = s; s += __popcll(w[k]); }
-  }
-  __syncthreads();
-  const int base = boff[(long long)v * nblk + blockIdx.x];
-  const long long start = (long long)blockIdx.x * MX_BLOCK_VOX;
-  const long long hw = (long long)V.H * V.W;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int k = wave; k < MX_WORDS; k += 4) {
-    const unsigned long long m = w[k];
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int nrow = V.D * V.H;
+  if (row >= nrow) return;
+  const long long ridx = (long long)v * nrow_max + row;
+  const int base = roff[ridx];
+  const int next = row + 1 < nrow_max ? roff[ridx + 1] : counts[v];
+  if (next == base || base >= cap) return;  // no maxima in this row, or all beyond the cap
+  const int z = row / V.H, x = row - (row / V.H) * V.H;
+  const unsigned long long* mw = masks + ridx * ww;
+  int pre = 0;
+  for (int k = 0; k < ww; ++k) {
+    const unsigned long long m = mw[k];
     if ((m >> lane) & 1ull) {
-      const int pos = base + pre[k] + __popcll(m & ((1ull << lane) - 1ull));
+      const int pos = base + pre + __popcll(m & ((1ull << lane) - 1ull));
       if (pos < cap) {
-        const long long idx = start + 64LL * k + lane;
-        const int z = (int)(idx / hw);
-        const int rem = (int)(idx - (long long)z * hw);
         int* o = coords + ((long long)v * cap + pos) * 3;
         o[0] = z;
-        o[1] = rem / V.W;
-        o[2] = rem - (rem / V.W) * V.W;
+        o[1] = x;
+        o[2] = k * 64 + lane;
       }
     }
+    pre += __popcll(m);
   }
 }
 
@@ -460,11 +456,10 @@ __global__ __launch_bounds__(256) void k_keep_write(const float* __restrict__ fi
 }
 
 // ------------------------------------------------------------------ host
-static inline int mx_nblk(long long nvox) { return (int)((nvox + MX_BLOCK_VOX - 1) / MX_BLOCK_VOX); }
-
-extern "C" size_t mm_maxima_ws_bytes(long long nvox_max, int nvol) {
-  const long long nb = mx_nblk(nvox_max);
-  return (size_t)nvol * nb * (MX_WORDS * 8 + 4) + 256;
+extern "C" size_t mm_maxima_ws_bytes(int D, int H, int W, int nvol) {
+  if (D <= 0 || H <= 0 || W <= 0 || nvol <= 0) return 256;
+  const size_t ww = ((size_t)W + 63) / 64;
+  return (size_t)nvol * D * H * (ww * 8 + 4) + 256;
 }
 
 // sphere offsets of delta_fit in dz -> dx -> dy order, packed (dz+128)<<16 | (dx+128)<<8 | (dy+128)
@@ -496,23 +491,30 @@ static int sphere_table(int dfit, int** dev_out, int* P_out) {
   return MM_OK;
 }
 
+// D, H, W: bounds of every volume's extents (the workspace layout).
 template <class VS>
-static int run_maxima(VS vs, int nvol, long long nvox_max, float sign, float thr, int delta, int* coords, int cap,
+static int run_maxima(VS vs, int nvol, int D, int H, int W, float sign, float thr, int delta, int* coords, int cap,
                       int* counts, void* ws, size_t ws_bytes, hipStream_t st) {
   if (delta < 0 || delta > 31) { mm_set_error("delta %d outside [0, 31]", delta); return MM_EINVAL; }
-  const int nb = mx_nblk(nvox_max);
-  if (nb == 0 || nvol == 0) {
-    if (nvol) MM_CHECK(hipMemsetAsync(counts, 0, sizeof(int) * nvol, st));
+  if (nvol == 0) return MM_OK;
+  if (ws_bytes < mm_maxima_ws_bytes(D, H, W, nvol)) { mm_set_error("maxima workspace too small"); return MM_EINVAL; }
+  const int nrow = D * H;
+  const int ww = W > 0 ? (W + 63) / 64 : 0;
+  unsigned long long* masks = (unsigned long long*)ws;
+  int* rcnt = (int*)(masks + (size_t)nvol * nrow * ww);
+  const unsigned nbx = (unsigned)((nrow + 3) / 4);
+  if (nbx == 0 || ww == 0) {  // empty volumes: zero counts without a memset node
+    hipLaunchKernelGGL(k_max_scan, dim3(nvol), dim3(1024), 0, st, rcnt, 0, counts);
+    MM_LAUNCH_CHECK();
     return MM_OK;
   }
-  if (ws_bytes < mm_maxima_ws_bytes(nvox_max, nvol)) { mm_set_error("maxima workspace too small"); return MM_EINVAL; }
-  unsigned long long* masks = (unsigned long long*)ws;
-  int* bcnt = (int*)(masks + (size_t)nvol * nb * MX_WORDS);
-  hipLaunchKernelGGL((k_max_mark<VS>), dim3(nb, nvol), dim3(256), 0, st, vs, sign, thr, delta, masks, bcnt, nb);
+  hipLaunchKernelGGL((k_max_mark<VS>), dim3(nbx, nvol), dim3(256), 0, st, vs, sign, thr, delta, masks, rcnt, nrow,
+                     ww);
   MM_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_max_scan, dim3(nvol), dim3(1024), 0, st, bcnt, nb, counts);
+  hipLaunchKernelGGL(k_max_scan, dim3(nvol), dim3(1024), 0, st, rcnt, nrow, counts);
   MM_LAUNCH_CHECK();
-  hipLaunchKernelGGL((k_max_compact<VS>), dim3(nb, nvol), dim3(256), 0, st, vs, masks, bcnt, nb, coords, cap);
+  hipLaunchKernelGGL((k_max_compact<VS>), dim3(nbx, nvol), dim3(256), 0, st, vs, masks, rcnt, counts, nrow, ww,
+                     coords, cap);
   MM_LAUNCH_CHECK();
   return MM_OK;
 }
@@ -522,7 +524,7 @@ extern "C" int mm_local_maxima(const float* img, int D, int H, int W, long long 
                                void* stream) {
   if (!img || !coords || !count || D < 0 || H < 0 || W < 0) { mm_set_error("mm_local_maxima: bad argument"); return MM_EINVAL; }
   OneVol vs{VolDesc{img, D, H, W, sZ, sX}};
-  return run_maxima(vs, 1, (long long)D * H * W, sign, threshold, delta, coords, cap, count, ws, ws_bytes,
+  return run_maxima(vs, 1, D, H, W, sign, threshold, delta, coords, cap, count, ws, ws_bytes,
                     (hipStream_t)stream);
 }
 
@@ -551,7 +553,7 @@ extern "C" int mm_tiles_maxima(mm_deconv* d, const mm_tile_desc* tiles_dev, int 
                                size_t ws_bytes, void* stream) {
   if (!d || !tiles_dev || !res) { mm_set_error("mm_tiles_maxima: bad argument"); return MM_EINVAL; }
   TileVols vs{(const mm_tile_dev*)tiles_dev, res, d->sz, d->T};
-  return run_maxima(vs, ntiles, (long long)d->sz * d->T * d->T, 1.0f, threshold, delta, coords, cap, counts, ws, ws_bytes,
+  return run_maxima(vs, ntiles, d->sz, d->T, d->T, 1.0f, threshold, delta, coords, cap, counts, ws, ws_bytes,
                     (hipStream_t)stream);
 }
 

@@ -38,7 +38,7 @@ def maxima_coords(image, threshold, delta, sign=1.0, cap=None):
     nvox = D * H * W
     dev = image.device
     st = L.stream_ptr(dev)
-    ws_bytes = L.lib().mm_maxima_ws_bytes(max(nvox, 1), 1)
+    ws_bytes = L.lib().mm_maxima_ws_bytes(D, H, W, 1)
     ws = _workspace(dev, ws_bytes)
     count = torch.zeros(1, dtype=torch.int32, device=dev)
     cap = int(min(nvox, 1 << 16) if cap is None else cap)

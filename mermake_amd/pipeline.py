@@ -83,7 +83,7 @@ class HybPipeline:
         st = L.stream_ptr(d.device)
         res = self._buf("res", (ntiles, sz, T, T), torch.float32)
         res = d.run_tiles(table, ntiles, ntiles if single else 1, self.blur_radius, res_out=res)
-        ws_bytes = L.lib().mm_maxima_ws_bytes(sz * T * T, ntiles)
+        ws_bytes = L.lib().mm_maxima_ws_bytes(sz, T, T, ntiles)
         ws = self._buf("mws", (ws_bytes,), torch.uint8)
         coords = self._buf("coords", (ntiles, cap, 3), torch.int32)
         counts = self._buf("counts", (ntiles,), torch.int32)

@@ -75,7 +75,7 @@ class _Slot:
         self.tmp = None
         if pipe.blur_radius is not None and int(pipe.blur_radius) // 2 not in (15, 25):
             self.tmp = torch.empty((B, sz, T, T), dtype=torch.float32, device=device)
-        self.mws_bytes = L.lib().mm_maxima_ws_bytes(sz * T * T, B)
+        self.mws_bytes = L.lib().mm_maxima_ws_bytes(sz, T, T, B)
         self.mws = torch.empty(self.mws_bytes, dtype=torch.uint8, device=device)
         self.coords = torch.empty((B, cap, 3), dtype=torch.int32, device=device)
         self.fit = torch.empty((B, cap, 8), dtype=torch.float32, device=device)
