@@ -76,6 +76,7 @@ struct mm_deconv {
   int zconv;                // z pass as a folded z-kernel matrix (sz <= 48) instead of z FFTs
   int zG;                   // columns per block of the z-kernel pass
   int* zsrc_dev;            // per data plane, the padded planes that copy it (4 slots, -1 padded)
+  int zsrc_h[4 * 48];       // host copy (kernel argument of the z-kernel pass)
   int pfM, pfG;             // prime-first T passes: T = p * pfM, pfG transforms per block (pfM = 0: off)
   PfShape pfS;
   int prof;                 // record events around the five passes of mm_deconv_tiles

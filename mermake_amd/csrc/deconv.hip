@@ -695,20 +695,30 @@ __global__ void k_d2f(const double2* __restrict__ a, float2* __restrict__ b, lon
 }
 
 // ------------------------------------------------------------ z-kernel pass
-#define ZC_PF 8  // tiles whose columns are in flight per block (software pipeline depth)
+#define ZC_CH 12  // tiles per chunk: K is rebuilt once per chunk, the chunk's columns stream through LDS
 
-template <int SZM, bool EXACT>
-__global__ __launch_bounds__(640) void k_tile_zconv(float2* S, long long s_tstride, int TH, int sz_rt, int zpad, int Zp,
-                                                    const int* __restrict__ zsrc, const float2* __restrict__ Wz,
-                                                    long long w_stride, const mm_tile_dev* __restrict__ tiles,
-                                                    int ntiles, int tpb, int G_rt) {
+// Padded planes that copy each data plane (4 slots, -1 empty), passed by value:
+// uniform over the grid, read with scalar loads.
+struct ZcSrc {
+  int p[4 * 48];
+};
+
+// Thread (g, t) = (column, output plane).  Per chunk of CH tiles and per data
+// plane s it forms K[t][s] = sum over the copies p of s of w[(zpad + t - p) mod Zp]
+// from w in LDS (the same order of additions for every chunk, so K is
+// identical), then adds K[t][s] x_j[s] to the CH accumulators.  Keeping the
+// accumulators instead of a K row in registers leaves room for two blocks per CU.
+template <int SZM, bool EXACT, int CH>
+__global__ __launch_bounds__(640, 5) void k_tile_zconv(float2* S, long long s_tstride, int TH, int sz_rt, int zpad, int Zp,
+                                                    ZcSrc src, const float2* __restrict__ Wz, long long w_stride,
+                                                    const mm_tile_dev* __restrict__ tiles, int ntiles, int tpb,
+                                                    int G_rt) {
   const int sz = EXACT ? SZM : sz_rt;
   const int G = EXACT ? 640 / SZM : G_rt;  // columns per block
   const int SP = EXACT ? SZM + 1 : sz_rt + 1;  // LDS stride of one column (odd: conflict-free across columns)
   extern __shared__ __align__(16) unsigned char smem[];
-  float2* wl = (float2*)smem;                        // [p][g]
-  float2* xl = wl + (size_t)G * Zp;                  // [j][g][s], column stride SP
-  int* sl = (int*)(xl + (size_t)ZC_PF * G * SP);     // [s][4]
+  float2* wl = (float2*)smem;            // [p][g]
+  float2* xl = wl + (size_t)G * Zp;      // [j][g][s], column stride SP
   const int ncb = (TH + G - 1) / G;
   const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
   const int cb = bid % ncb, tg = bid / ncb;
@@ -719,65 +729,58 @@ __global__ __launch_bounds__(640) void k_tile_zconv(float2* S, long long s_tstri
   const bool act = t < sz;
   const bool col = act && g < gmax;
   const int t0 = tg * tpb, t1 = min(ntiles, t0 + tpb);
-  for (int i = tid; i < sz * 4; i += blockDim.x) sl[i] = zsrc[i];
   const float2* Wp = Wz + (long long)tiles[t0].psf * w_stride + c0;  // a group shares one PSF
   for (int i = tid; i < G * Zp; i += blockDim.x) {
     const int gg = i % G, p = i / G;
     wl[p * G + gg] = gg < gmax ? Wp[(long long)p * TH + gg] : make_float2(0.f, 0.f);
   }
-  __syncthreads();
-  float2 K[SZM];
-#pragma unroll
-  for (int s = 0; s < SZM; ++s) {
-    float2 acc = make_float2(0.f, 0.f);
-    if (act && s < sz) {
-      for (int q = 0; q < 4; ++q) {
-        const int p = sl[s * 4 + q];
-        if (p >= 0) {
-          int idx = zpad + t - p;
-          idx = idx < 0 ? idx + Zp : (idx >= Zp ? idx - Zp : idx);
-          const float2 w = wl[idx * G + g];
-          acc.x += w.x;
-          acc.y += w.y;
-        }
-      }
-    }
-    K[s] = acc;
-  }
   const long long cofs = (long long)t * TH + c0 + g;
-  float2 pre[ZC_PF];
+  for (int cs = t0; cs < t1; cs += CH) {
+    __syncthreads();  // wl is written / the previous chunk is no longer read from xl
+    if (act) {  // the chunk's columns (the other resident block computes meanwhile)
+      float2 v[CH];
 #pragma unroll
-  for (int j = 0; j < ZC_PF; ++j)
-    pre[j] = (col && t0 + j < t1) ? S[(long long)(t0 + j) * s_tstride + cofs] : make_float2(0.f, 0.f);
-  for (int cs = t0; cs < t1; cs += ZC_PF) {
-    __syncthreads();  // the previous chunk is no longer read from xl
-    if (act) {
+      for (int j = 0; j < CH; ++j)
+        v[j] = (col && cs + j < t1) ? S[(long long)(cs + j) * s_tstride + cofs] : make_float2(0.f, 0.f);
 #pragma unroll
-      for (int j = 0; j < ZC_PF; ++j) xl[((size_t)j * G + g) * SP + t] = pre[j];
+      for (int j = 0; j < CH; ++j) xl[((size_t)j * G + g) * SP + t] = v[j];
     }
     __syncthreads();
-    const int nxt = cs + ZC_PF;
+    float2 y[CH];
 #pragma unroll
-    for (int j = 0; j < ZC_PF; ++j)
-      pre[j] = (col && nxt + j < t1) ? S[(long long)(nxt + j) * s_tstride + cofs] : make_float2(0.f, 0.f);
+    for (int j = 0; j < CH; ++j) y[j] = make_float2(0.f, 0.f);
+    if (act) {
+      // not fully unrolled: K[t][s] must stay per-plane work inside the chunk loop, not be
+      // hoisted out of it as a whole register row
+#pragma unroll 2
+      for (int s = 0; s < sz; ++s) {
+        {
+          float2 k = make_float2(0.f, 0.f);
 #pragma unroll
-    for (int j = 0; j < ZC_PF; ++j) {
-      if (col && cs + j < t1) {
-        const float2* xv = xl + ((size_t)j * G + g) * SP;
-        float2 y = make_float2(0.f, 0.f);
+          for (int q = 0; q < 4; ++q) {
+            const int p = src.p[s * 4 + q];
+            if (p >= 0) {
+              int idx = zpad + t - p;
+              idx = idx < 0 ? idx + Zp : (idx >= Zp ? idx - Zp : idx);
+              const float2 w = wl[idx * G + g];
+              k.x += w.x;
+              k.y += w.y;
+            }
+          }
 #pragma unroll
-        for (int s = 0; s < SZM; ++s) {
-          if (s < sz) {
-            const float2 x = xv[s];
-            y.x = fmaf(K[s].x, x.x, y.x);
-            y.x = fmaf(-K[s].y, x.y, y.x);
-            y.y = fmaf(K[s].x, x.y, y.y);
-            y.y = fmaf(K[s].y, x.x, y.y);
+          for (int j = 0; j < CH; ++j) {
+            const float2 x = xl[((size_t)j * G + g) * SP + s];
+            y[j].x = fmaf(k.x, x.x, y[j].x);
+            y[j].x = fmaf(-k.y, x.y, y[j].x);
+            y[j].y = fmaf(k.x, x.y, y[j].y);
+            y[j].y = fmaf(k.y, x.x, y[j].y);
           }
         }
-        S[(long long)(cs + j) * s_tstride + cofs] = y;
       }
     }
+#pragma unroll
+    for (int j = 0; j < CH; ++j)
+      if (col && cs + j < t1) S[(long long)(cs + j) * s_tstride + cofs] = y[j];
   }
 }
 
@@ -789,10 +792,19 @@ static int launch_zconv(mm_deconv* d, float2* S, long long s_ts, const float2* W
   const int ngroups = (ntiles + tpb - 1) / tpb;
   const long long nblk = (long long)((TH + G - 1) / G) * ngroups;
   const int threads = ((G * sz + 63) / 64) * 64;
-  const size_t lds = sizeof(float2) * ((size_t)G * Zp + (size_t)ZC_PF * G * (sz + 1)) + sizeof(int) * 4 * sz;
-#define MM_ZCX(M, E)                                                                                             \
-  hipLaunchKernelGGL((k_tile_zconv<M, E>), dim3((unsigned)nblk), dim3(threads), lds, st, S, s_ts, TH, sz, d->zpad, \
-                     Zp, d->zsrc_dev, Wz, (long long)Zp * TH, td, ntiles, tpb, G)
+  ZcSrc src;
+  for (int i = 0; i < 4 * 48; ++i) src.p[i] = i < 4 * sz ? d->zsrc_h[i] : -1;
+  const int ch = tpb >= ZC_CH ? ZC_CH : 1;  // one PSF per tile: no chunk of tiles shares K
+  const size_t lds = sizeof(float2) * ((size_t)G * Zp + (size_t)ch * G * (sz + 1));
+#define MM_ZCX(M, E)                                                                                            \
+  do {                                                                                                          \
+    if (ch == ZC_CH)                                                                                            \
+      hipLaunchKernelGGL((k_tile_zconv<M, E, ZC_CH>), dim3((unsigned)nblk), dim3(threads), lds, st, S, s_ts, TH, \
+                         sz, d->zpad, Zp, src, Wz, (long long)Zp * TH, td, ntiles, tpb, G);                      \
+    else                                                                                                        \
+      hipLaunchKernelGGL((k_tile_zconv<M, E, 1>), dim3((unsigned)nblk), dim3(threads), lds, st, S, s_ts, TH, sz, \
+                         d->zpad, Zp, src, Wz, (long long)Zp * TH, td, ntiles, tpb, G);                          \
+  } while (0)
 #define MM_ZC(M) MM_ZCX(M, false)
   if (sz == 40) MM_ZCX(40, true);
   else if (sz == 35) MM_ZCX(35, true);
@@ -886,6 +898,7 @@ extern "C" int mm_deconv_create(int sz, int tile_size, int overlap, int zpad, mm
     }
     d->zG = 640 / sz;
     if (d->zG > 64) d->zG = 64;
+    for (int i = 0; i < 4 * sz; ++i) d->zsrc_h[i] = src[i];
     if (d->zconv && (hipMalloc(&d->zsrc_dev, sizeof(int) * 4 * sz) != hipSuccess ||
                      hipMemcpy(d->zsrc_dev, src.data(), sizeof(int) * 4 * sz, hipMemcpyHostToDevice) != hipSuccess)) {
       mm_set_error("mm_deconv_create: cannot upload the z source table");

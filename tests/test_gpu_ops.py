@@ -45,7 +45,8 @@ def test_box_1d_bitexact(dev, shape, axis, size):
 
 
 @pytest.mark.parametrize("T_,delta,scale", [(678, 15, 1000.0), (100, 15, 1.0), (40, 15, 1e-3), (17, 15, 1000.0),
-                                             (678, 25, 1000.0), (53, 25, 1.0), (31, 25, 1e-30)])
+                                             (300, 15, 10.0), (200, 15, 1.0), (1000, 15, 100.0),
+                                             (678, 25, 1000.0), (478, 25, 1.0), (53, 25, 1.0), (31, 25, 1e-30)])
 def test_blur_sub_fused_bitexact(dev, T_, delta, scale):
     """The fused blur-subtract (deconvolver.py:160-163) equals res - box_1d(box_1d(res, axis 1), axis 2) bit for bit,
     including windows clipped at the borders, planes narrower than the window and tiny sums."""
