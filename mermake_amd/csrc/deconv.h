@@ -1,6 +1,6 @@
 // Device-side descriptors shared by the deconvolution, maxima and fit kernels.
 #pragma once
-#include "fft_pf.h"
+#include "fft_rader.h"
 
 // One overlapped tile of a z-stack (reference: deconvolver.py:186-206 for the
 // window, :118-148 for the placement and reflections).  Filled on the host.
@@ -79,6 +79,8 @@ struct mm_deconv {
   int zsrc_h[4 * 48];       // host copy (kernel argument of the z-kernel pass)
   int pfM, pfG;             // prime-first T passes: T = p * pfM, pfG transforms per block (pfM = 0: off)
   PfShape pfS;
+  int rd;                   // the prime p = 113 runs by Rader's algorithm (tables rdF / rdI)
+  RdTab rdF, rdI;
   int prof;                 // record events around the five passes of mm_deconv_tiles
   hipEvent_t ev[6];
 };

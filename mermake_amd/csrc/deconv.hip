@@ -616,6 +616,242 @@ __global__ __launch_bounds__(256) void k_pf_c2r(const float2* Sin, long long s_t
   }
 }
 
+// ------------------------------------------------ Rader (p = 113) tile passes
+// The prime-first passes with the prime stage of fft_rader.h (T = 113 * M).
+// Same persistent structure: the next group's raw elements are prefetched
+// into registers during the current group's Rader and final stages.
+template <bool INV, int M>
+__global__ __launch_bounds__(256, 2) void k_rd_c2c(float2* data, long long estride, long long cstride, int ncols,
+                                                long long bstride, int nbatch, RdTab tab,
+                                                const float2* __restrict__ tw, int G) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int N = RD_P * M;
+  const RdLds L = rd_carve(smem, N, M, G, tw, tab);
+  __syncthreads();  // slot_in is read by rd_items
+  const int ncb = (ncols + G - 1) / G;
+  const long long ngrp = (long long)ncb * nbatch;
+  const int es = (int)estride;
+  const int cs = (int)cstride;
+  auto setup = [&](long long gi, float2*& base, int& gmax) {
+    const long long batch = gi / ncb;
+    const int cb = (int)(gi - batch * ncb);
+    base = data + batch * bstride + (long long)cb * G * cstride;
+    gmax = min(G, ncols - cb * G);
+  };
+  RdItems I;
+  rd_items<M, true>(G, L.slot_in, I);
+  float2 A[RD_NI];
+  auto fetch = [&](const float2* base, int gmax) {  // absent columns read the last present one
+    rd_fetch(I, [&](int u, int g, int n) { A[u] = base[min(g, gmax - 1) * cs + n * es]; });
+  };
+  long long grp = xcd_remap(blockIdx.x, gridDim.x);
+  float2* base = nullptr;
+  int gmax = 0;
+  if (grp < ngrp) {
+    setup(grp, base, gmax);
+    fetch(base, gmax);
+  }
+  for (; grp < ngrp; grp += gridDim.x) {
+    rd_put(L.bufA, I, [&](int u, int, int) { return A[u]; });
+    __syncthreads();
+    float2* nbase = nullptr;
+    int ngmax = 0;
+    if (grp + gridDim.x < ngrp) {
+      setup(grp + gridDim.x, nbase, ngmax);
+      fetch(nbase, ngmax);
+    }
+    rd_stages<M, INV>(L, G);
+    pf_final<M, INV, true>(PfShape(N, RD_P), (const float2*)L.bufB, L.tw, G, [&](int g, int j, const float2* v) {
+      if (g >= gmax) return;
+      float2* col = base + (g * cs + j * es);
+#pragma unroll
+      for (int r = 0; r < M; ++r) col[r * RD_P * es] = v[r];
+    });
+    base = nbase;
+    gmax = ngmax;
+  }
+}
+
+// Raw prefetch registers of the Rader R2C pass: pixel bits and flat values of
+// the two samples of each item (rows 2g, 2g+1 at column n), 2 validity bits per item.
+struct RdR2cRaw {
+  unsigned P[RD_NI][2];
+  float F[RD_NI][2];
+  unsigned msk;
+};
+
+template <int M, bool F32, bool FLAT>
+__device__ __forceinline__ void rd_r2c_fetch(const RdItems& I, RdR2cRaw& R, const mm_tile_dev& td, const R2cTab& tb) {
+  const float* fl = td.flat;
+  R.msk = 0;
+  rd_fetch(I, [&](int u, int g, int n) {
+    const int c = tb.ymap[n];
+    unsigned m = 0;
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const long long o = tb.roff[2 * g + e];
+      const bool ok = o >= 0 && c >= 0;
+      R.P[u][e] = F32 ? __float_as_uint(((const float*)td.img)[ok ? o + c : 0])
+                      : (unsigned)((const uint16_t*)td.img)[ok ? o + c : 0];
+      if (FLAT) R.F[u][e] = fl[ok ? tb.foff[2 * g + e] + c : 0];
+      m |= ok ? 1u << e : 0u;
+    }
+    R.msk |= m << (2 * u);
+  });
+}
+
+template <int M, bool F32, bool FLAT>
+__device__ __forceinline__ void rd_r2c_put(float2* bufA, const RdItems& I, const RdR2cRaw& R) {
+  rd_put(bufA, I, [&](int u, int, int) {
+    return make_float2(r2c_val<F32, FLAT>(R.P[u][0], R.F[u][0], (R.msk >> (2 * u)) & 1u),
+                       r2c_val<F32, FLAT>(R.P[u][1], R.F[u][1], (R.msk >> (2 * u + 1)) & 1u));
+  });
+}
+
+template <int M>
+__global__ __launch_bounds__(256, 2) void k_rd_r2c(const mm_tile_dev* __restrict__ tiles, int ntiles, int T, int nrows,
+                                                float2* Sout, long long s_tstride, RdTab tab,
+                                                const float2* __restrict__ tw, int G) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int N = RD_P * M;
+  const RdLds L = rd_carve(smem, N, M, G, tw, tab);
+  R2cTab tab0, tab1;
+  {
+    unsigned char* e = L.extra;
+    R2cTab* tt[2] = {&tab0, &tab1};
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      tt[k]->ymap = (int*)e;
+      e += mm_align16((size_t)T * 4);
+      tt[k]->roff = (long long*)e;
+      tt[k]->foff = tt[k]->roff + 2 * G;
+      e += (size_t)4 * G * 8;
+    }
+  }
+  const int nrb = (nrows + 2 * G - 1) / (2 * G);
+  const long long ngrp = (long long)nrb * ntiles;
+  auto tables = [&](long long gi, const R2cTab& tb) {
+    const int tile = (int)(gi / nrb);
+    const int q0 = (int)(gi - (long long)tile * nrb) * 2 * G;
+    const mm_tile_dev& td = tiles[tile];
+    for (int y = threadIdx.x; y < T; y += blockDim.x) tb.ymap[y] = pad_map(y, td.ys, td.yd, td.yfirst, T);
+    for (int i = threadIdx.x; i < 2 * G; i += blockDim.x) {
+      const int q = q0 + i;
+      long long ro = -1, fo = 0;
+      if (q < nrows) {
+        const int z = q / T, x = q - (q / T) * T;
+        const int rr = pad_map(x, td.xs, td.xd, td.xfirst, T);
+        if (rr >= 0) {
+          ro = (long long)z * td.img_sz + (long long)(td.X0 + rr) * td.img_sx + td.Y0;
+          fo = (long long)(td.X0 + rr) * td.flat_sx + td.Y0;
+        }
+      }
+      tb.roff[i] = ro;
+      tb.foff[i] = fo;
+    }
+  };
+  long long grp = xcd_remap(blockIdx.x, gridDim.x);
+  if (grp < ngrp) tables(grp, tab0);
+  __syncthreads();  // slot_in and the first tables
+  RdItems I;
+  rd_items<M, false>(G, L.slot_in, I);
+  RdR2cRaw R;
+  auto fetch = [&](long long gi, const R2cTab& tb) {
+    const mm_tile_dev& td = tiles[(int)(gi / nrb)];
+    if (td.img_f32) {
+      if (td.flat) rd_r2c_fetch<M, true, true>(I, R, td, tb);
+      else rd_r2c_fetch<M, true, false>(I, R, td, tb);
+    } else {
+      if (td.flat) rd_r2c_fetch<M, false, true>(I, R, td, tb);
+      else rd_r2c_fetch<M, false, false>(I, R, td, tb);
+    }
+  };
+  auto put = [&](long long gi) {
+    const mm_tile_dev& td = tiles[(int)(gi / nrb)];
+    if (td.img_f32) {
+      if (td.flat) rd_r2c_put<M, true, true>(L.bufA, I, R);
+      else rd_r2c_put<M, true, false>(L.bufA, I, R);
+    } else {
+      if (td.flat) rd_r2c_put<M, false, true>(L.bufA, I, R);
+      else rd_r2c_put<M, false, false>(L.bufA, I, R);
+    }
+  };
+  const int H = N / 2 + 1;
+  int set = 0;
+  if (grp < ngrp) fetch(grp, tab0);
+  for (; grp < ngrp; grp += gridDim.x) {
+    const long long nxt = grp + gridDim.x;
+    const R2cTab& tn = set ? tab0 : tab1;
+    if (nxt < ngrp) tables(nxt, tn);
+    put(grp);
+    __syncthreads();
+    if (nxt < ngrp) fetch(nxt, tn);
+    rd_stages<M, false>(L, G);
+    const int tile = (int)(grp / nrb);
+    const int q0 = (int)(grp - (long long)tile * nrb) * 2 * G;
+    float2* out = Sout + (long long)tile * s_tstride;
+    pf_final_r2c<M>(PfShape(N, RD_P), (const float2*)L.bufB, L.tw, G, [&](int g, float2*& pa, float2*& pb) {
+      const int q = q0 + 2 * g;
+      pa = q < nrows ? out + (long long)q * H : nullptr;
+      pb = q + 1 < nrows ? out + (long long)(q + 1) * H : nullptr;
+    });
+    set ^= 1;
+  }
+}
+
+template <int M>
+__global__ __launch_bounds__(256, 2) void k_rd_c2r(const float2* Sin, long long s_tstride, float* res,
+                                                long long r_tstride, int ntiles, int nrows, RdTab tab,
+                                                const float2* __restrict__ tw, int G) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int N = RD_P * M, H = N / 2 + 1;
+  const RdLds L = rd_carve(smem, N, M, G, tw, tab);
+  __syncthreads();
+  const int nrb = (nrows + 2 * G - 1) / (2 * G);
+  const long long ngrp = (long long)nrb * ntiles;
+  RdItems I;
+  rd_items<M, false>(G, L.slot_in, I);
+  float2 RA[RD_NI], RB[RD_NI];  // the two rows' bins kk(n) of each item
+  auto fetch = [&](long long gi) {
+    const int tile = (int)(gi / nrb);
+    const int q0 = (int)(gi - (long long)tile * nrb) * 2 * G;
+    const float2* in = Sin + (long long)tile * s_tstride;
+    rd_fetch(I, [&](int u, int g, int n) {  // absent rows read the last row
+      const int q = q0 + 2 * g;
+      const int kk = n >= H ? N - n : n;
+      RA[u] = in[(long long)min(q, nrows - 1) * H + kk];
+      RB[u] = in[(long long)min(q + 1, nrows - 1) * H + kk];
+    });
+  };
+  long long grp = xcd_remap(blockIdx.x, gridDim.x);
+  if (grp < ngrp) fetch(grp);
+  for (; grp < ngrp; grp += gridDim.x) {
+    const int tile = (int)(grp / nrb);
+    const int q0 = (int)(grp - (long long)tile * nrb) * 2 * G;
+    rd_put(L.bufA, I, [&](int u, int g, int n) {
+      const int q = q0 + 2 * g;
+      float2 a = q < nrows ? RA[u] : make_float2(0.f, 0.f);
+      float2 b = q + 1 < nrows ? RB[u] : make_float2(0.f, 0.f);
+      if (n >= H) { a.y = -a.y; b.y = -b.y; }
+      if (n == 0 || 2 * n == N) { a.y = 0.f; b.y = 0.f; }  // real part semantics of ifftn(...).real
+      return make_float2(a.x - b.y, a.y + b.x);
+    });
+    __syncthreads();
+    if (grp + gridDim.x < ngrp) fetch(grp + gridDim.x);
+    rd_stages<M, true>(L, G);
+    float* out = res + (long long)tile * r_tstride;
+    pf_final<M, true>(PfShape(N, RD_P), (const float2*)L.bufB, L.tw, G, [&](int g, int j, const float2* v) {
+      const int q = q0 + 2 * g;
+      float* oa = out + (long long)q * N + j;
+#pragma unroll
+      for (int r = 0; r < M; ++r) {
+        if (q < nrows) oa[r * RD_P] = v[r].x;
+        if (q + 1 < nrows) oa[N + r * RD_P] = v[r].y;
+      }
+    });
+  }
+}
+
 #define PF_SWITCH(M, CALL)                    \
   switch (M) {                                \
     case 2: { constexpr int MM = 2; CALL; } break; \
@@ -645,6 +881,18 @@ static int launch_pf_c2c(const mm_deconv* d, float2* data, long long estride, lo
   const size_t lds = pf_lds_bytes(d->pfS, d->pfM, G, 0);
   PF_SWITCH(d->pfM, hipLaunchKernelGGL((k_pf_c2c<INV, MM>), dim3(pf_grid(ngrp, lds)), dim3(256), lds, st, data,
                                        estride, cstride, ncols, bstride, nbatch, d->pfS, d->pT.tw, d->pT.coef, G));
+  MM_LAUNCH_CHECK();
+  return MM_OK;
+}
+
+template <bool INV>
+static int launch_rd_c2c(const mm_deconv* d, float2* data, long long estride, long long cstride, int ncols,
+                         long long bstride, int nbatch, hipStream_t st) {
+  const int G = d->pfG;
+  const long long ngrp = (long long)((ncols + G - 1) / G) * nbatch;
+  const size_t lds = rd_lds_bytes(d->T, d->pfM, G, 0);
+  PF_SWITCH(d->pfM, hipLaunchKernelGGL((k_rd_c2c<INV, MM>), dim3(pf_grid(ngrp, lds)), dim3(256), lds, st, data,
+                                       estride, cstride, ncols, bstride, nbatch, INV ? d->rdI : d->rdF, d->pT.tw, G));
   MM_LAUNCH_CHECK();
   return MM_OK;
 }
@@ -886,6 +1134,20 @@ extern "C" int mm_deconv_create(int sz, int tile_size, int overlap, int zpad, mm
       if (G > 0) { d->pfM = M; d->pfG = G; d->pfS = S; }
     }
   }
+  // Rader form of the prime 113 (the full-FOV tile T = 678 = 113 * 6)
+  d->rd = 0;
+  if (d->pfM && d->pfS.p == RD_P) {
+    const int M = d->pfM;
+    int G = 256 / (7 * M);
+    while (G > 0 && (rd_lds_bytes(d->T, M, G, pf_r2c_extra(d->T, G)) > 81920 || G * M * RD_P > RD_NI * 256)) --G;
+    if (G > 0 && (rc = mm_get_rader113(&d->rdF, &d->rdI)) == MM_OK) {
+      d->rd = 1;
+      d->pfG = G;
+    } else if (rc) {
+      delete d;
+      return rc;
+    }
+  }
   // z-kernel pass: the padded planes that copy each data plane
   d->zconv = sz <= 48;
   if (d->zconv) {
@@ -1000,7 +1262,14 @@ extern "C" int mm_deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int 
   MM_MARK(0);
   const long long s_ts = (long long)sz * T * H;
   const int nrows = sz * T;
-  if (d->pfM) {  // 1. R2C along y of the padded rows (prime-first form)
+  if (d->rd) {  // 1. R2C along y of the padded rows (prime-first form, Rader prime stage)
+    const int G = d->pfG;
+    const size_t lds = rd_lds_bytes(T, d->pfM, G, pf_r2c_extra(T, G));
+    const long long ngrp = (long long)((nrows + 2 * G - 1) / (2 * G)) * ntiles;
+    PF_SWITCH(d->pfM, hipLaunchKernelGGL(k_rd_r2c<MM>, dim3(pf_grid(ngrp, lds)), dim3(256), lds, st, td, ntiles, T,
+                                         nrows, S, s_ts, d->rdF, d->pT.tw, G));
+    MM_LAUNCH_CHECK();
+  } else if (d->pfM) {  // 1. R2C along y of the padded rows (prime-first form)
     const int G = d->pfG;
     const size_t lds = pf_lds_bytes(d->pfS, d->pfM, G, pf_r2c_extra(T, G));
     const long long ngrp = (long long)((nrows + 2 * G - 1) / (2 * G)) * ntiles;
@@ -1017,7 +1286,8 @@ extern "C" int mm_deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int 
   MM_MARK(1);
   int rc;
   // 2. C2C along x, all tiles and planes at once
-  if (d->pfM) rc = launch_pf_c2c<false>(d, S, H, 1, H, (long long)T * H, ntiles * sz, st);
+  if (d->rd) rc = launch_rd_c2c<false>(d, S, H, 1, H, (long long)T * H, ntiles * sz, st);
+  else if (d->pfM) rc = launch_pf_c2c<false>(d, S, H, 1, H, (long long)T * H, ntiles * sz, st);
   else rc = launch_c2c<false>(S, H, 1, H, (long long)T * H, ntiles * sz, d->pT, d->gT_x, 1.0, st);
   if (rc) return rc;
   MM_MARK(2);
@@ -1036,11 +1306,19 @@ extern "C" int mm_deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int 
   }
   MM_MARK(3);
   // 4. inverse C2C along x
-  if (d->pfM) rc = launch_pf_c2c<true>(d, S, H, 1, H, (long long)T * H, ntiles * sz, st);
+  if (d->rd) rc = launch_rd_c2c<true>(d, S, H, 1, H, (long long)T * H, ntiles * sz, st);
+  else if (d->pfM) rc = launch_pf_c2c<true>(d, S, H, 1, H, (long long)T * H, ntiles * sz, st);
   else rc = launch_c2c<true>(S, H, 1, H, (long long)T * H, ntiles * sz, d->pT, d->gT_x, 1.0, st);
   if (rc) return rc;
   MM_MARK(4);
-  if (d->pfM) {  // 5. C2R along y into res (prime-first form)
+  if (d->rd) {  // 5. C2R along y into res (prime-first form, Rader prime stage)
+    const int G = d->pfG;
+    const size_t lds = rd_lds_bytes(T, d->pfM, G, 0);
+    const long long ngrp = (long long)((nrows + 2 * G - 1) / (2 * G)) * ntiles;
+    PF_SWITCH(d->pfM, hipLaunchKernelGGL(k_rd_c2r<MM>, dim3(pf_grid(ngrp, lds)), dim3(256), lds, st, S, s_ts, res,
+                                         (long long)sz * T * T, ntiles, nrows, d->rdI, d->pT.tw, G));
+    MM_LAUNCH_CHECK();
+  } else if (d->pfM) {  // 5. C2R along y into res (prime-first form)
     const int G = d->pfG;
     const size_t lds = pf_lds_bytes(d->pfS, d->pfM, G, 0);
     const long long ngrp = (long long)((nrows + 2 * G - 1) / (2 * G)) * ntiles;

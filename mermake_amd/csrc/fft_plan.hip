@@ -108,7 +108,70 @@ int get_plan(int N, FftPlanDev<C>* out, int tag) {
   return MM_OK;
 }
 
+std::map<int, void*> g_rader;  // device -> Rader tables of p = 113
+
 }  // namespace
+
+int mm_get_rader113(RdTab* fwd, RdTab* inv) {
+  int dev = 0;
+  MM_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_plan_mu);
+  void* buf = nullptr;
+  auto it = g_rader.find(dev);
+  if (it != g_rader.end()) {
+    buf = it->second;
+  } else {
+    const int p = RD_P, L = RD_L;
+    int gam = 0;  // smallest primitive root of p
+    for (int c = 2; c < p && !gam; ++c) {
+      int x = 1, ord = 0;
+      do { x = x * c % p; ++ord; } while (x != 1);
+      if (ord == p - 1) gam = c;
+    }
+    int ginv = 1;
+    for (int e = 0; e < p - 2; ++e) ginv = ginv * gam % p;  // gam^(p-2) = gam^-1
+    std::vector<int> perm_out(L), slot_in(p, 0);
+    int x = 1, y = 1;
+    for (int a = 0; a < L; ++a) {
+      perm_out[a] = x;   // gam^a
+      slot_in[y] = a;    // gam^-a -> a
+      x = x * gam % p;
+      y = y * ginv % p;
+    }
+    slot_in[0] = L;  // x_0
+    const long double PI = 3.141592653589793238462643383279502884L;
+    std::vector<float2> h(3 * L);
+    for (int m = 0; m < L; ++m) {
+      const long double a = -2.0L * PI * m / L;
+      h[m] = make_float2((float)cosl(a), (float)sinl(a));
+    }
+    for (int dir = 0; dir < 2; ++dir) {  // ghat = DFT_112(g) / 112, g_n = exp(-/+ 2 pi i gam^n / p)
+      const long double sg = dir ? 1.0L : -1.0L;
+      for (int k = 0; k < L; ++k) {
+        long double re = 0, im = 0;
+        for (int n = 0; n < L; ++n) {
+          const long double ag = sg * 2.0L * PI * perm_out[n] / p;
+          const long double aw = -2.0L * PI * (long double)((n * k) % L) / L;
+          re += cosl(ag + aw);
+          im += sinl(ag + aw);
+        }
+        h[(1 + dir) * L + k] = make_float2((float)(re / L), (float)(im / L));
+      }
+    }
+    const size_t fbytes = sizeof(float2) * 3 * L, ibytes = sizeof(int) * (L + p);
+    MM_CHECK(hipMalloc(&buf, fbytes + ibytes));
+    std::vector<int> ints(perm_out);
+    ints.insert(ints.end(), slot_in.begin(), slot_in.end());
+    MM_CHECK(hipMemcpy(buf, h.data(), fbytes, hipMemcpyHostToDevice));
+    MM_CHECK(hipMemcpy((char*)buf + fbytes, ints.data(), ibytes, hipMemcpyHostToDevice));
+    g_rader[dev] = buf;
+  }
+  const float2* f = (const float2*)buf;
+  const int* ip = (const int*)(f + 3 * RD_L);
+  *fwd = RdTab{f, f + RD_L, ip, ip + RD_L};
+  *inv = RdTab{f, f + 2 * RD_L, ip, ip + RD_L};
+  return MM_OK;
+}
 
 int mm_get_plan_f32(int N, FftPlanDev<float2>* out) { return get_plan<float2>(N, out, 0); }
 int mm_get_plan_f64(int N, FftPlanDev<double2>* out) { return get_plan<double2>(N, out, 1); }

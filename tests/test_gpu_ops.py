@@ -244,7 +244,10 @@ def _rel_l2(a, b):
     ((4, 70, 70), 50, 8, 2, 1e-3, True, 5),         # T = 66 = 11 * 6: prime-first passes, radix-6 final stage
     ((4, 50, 50), 23, 5, 2, 1e-3, False, None),     # T = 33 = 11 * 3
     ((4, 60, 60), 30, 7, 2, 1e-3, True, None),      # T = 44 = 11 * 4
-    ((3, 600, 600), 500, 89, 3, 1e-4, True, 15),    # T = 678 = 113 * 6, the full-FOV tile
+    ((3, 600, 600), 500, 89, 3, 1e-4, True, 15),    # T = 678 = 113 * 6, the full-FOV tile (Rader prime stage)
+    ((3, 300, 300), 200, 13, 2, 1e-3, True, None),  # T = 226 = 113 * 2 (Rader, radix-2 final stage)
+    ((3, 340, 340), 301, 19, 2, 1e-3, False, 5),    # T = 339 = 113 * 3
+    ((2, 500, 500), 400, 26, 2, 1e-3, True, None),  # T = 452 = 113 * 4
 ])
 def test_deconv_vs_oracle(dev, shape, ts, ov, zpad, beta, flat, blur):
     from mermake_amd.deconvolver import Deconvolver
