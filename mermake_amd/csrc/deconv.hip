@@ -640,6 +640,7 @@ __global__ __launch_bounds__(256, 2) void k_rd_c2c(float2* data, long long estri
   };
   RdItems I;
   rd_items<M, true>(G, L.slot_in, I);
+  const RdMid mid = rd_mid(L);
   float2 A[RD_NI];
   auto fetch = [&](const float2* base, int gmax) {  // absent columns read the last present one
     rd_fetch(I, [&](int u, int g, int n) { A[u] = base[min(g, gmax - 1) * cs + n * es]; });
@@ -660,8 +661,8 @@ __global__ __launch_bounds__(256, 2) void k_rd_c2c(float2* data, long long estri
       setup(grp + gridDim.x, nbase, ngmax);
       fetch(nbase, ngmax);
     }
-    rd_stages<M, INV>(L, G);
-    pf_final<M, INV, true>(PfShape(N, RD_P), (const float2*)L.bufB, L.tw, G, [&](int g, int j, const float2* v) {
+    rd_stages<M, true, true>(L, G, mid);
+    rd_final<M, INV>((const float2*)L.bufB, L.tw, L.perm_out, G, [&](int g, int j, const float2* v) {
       if (g >= gmax) return;
       float2* col = base + (g * cs + j * es);
 #pragma unroll
@@ -672,173 +673,198 @@ __global__ __launch_bounds__(256, 2) void k_rd_c2c(float2* data, long long estri
   }
 }
 
-// Raw prefetch registers of the Rader R2C pass: pixel bits and flat values of
-// the two samples of each item (rows 2g, 2g+1 at column n), 2 validity bits per item.
-struct RdR2cRaw {
-  unsigned P[RD_NI][2];
-  float F[RD_NI][2];
-  unsigned msk;
+// Row passes (R2C, C2R) by columns: thread tid loads columns n = tid + k * 256
+// (k < NC) of each of the 2G rows of its group, so every load address is a
+// uniform row pointer plus a per-lane column offset fixed for the kernel.
+// Groups of G transforms: 32 to 36 row samples per thread; the 2G rows' validity
+// bits fit one 32-bit mask.
+constexpr int rd_row_g(int M) { return M == 2 ? 16 : (M == 6 ? 6 : 9); }
+constexpr int rd_row_nc(int M) { return (RD_P * M + 255) / 256; }
+
+template <int NC>
+struct RdCols {
+  int n[NC];   // column
+  int cs[NC];  // LDS slot of the column within one transform's M * RD_S block, -1 for n >= N
 };
 
-template <int M, bool F32, bool FLAT>
-__device__ __forceinline__ void rd_r2c_fetch(const RdItems& I, RdR2cRaw& R, const mm_tile_dev& td, const R2cTab& tb) {
-  const float* fl = td.flat;
-  R.msk = 0;
-  rd_fetch(I, [&](int u, int g, int n) {
-    const int c = tb.ymap[n];
-    unsigned m = 0;
+template <int M, int NC>
+__device__ __forceinline__ RdCols<NC> rd_cols(const int* __restrict__ slot_in) {
+  RdCols<NC> c;
 #pragma unroll
-    for (int e = 0; e < 2; ++e) {
-      const long long o = tb.roff[2 * g + e];
-      const bool ok = o >= 0 && c >= 0;
-      R.P[u][e] = F32 ? __float_as_uint(((const float*)td.img)[ok ? o + c : 0])
-                      : (unsigned)((const uint16_t*)td.img)[ok ? o + c : 0];
-      if (FLAT) R.F[u][e] = fl[ok ? tb.foff[2 * g + e] + c : 0];
-      m |= ok ? 1u << e : 0u;
-    }
-    R.msk |= m << (2 * u);
-  });
+  for (int k = 0; k < NC; ++k) {
+    const int n = threadIdx.x + k * 256;
+    c.n[k] = n;
+    c.cs[k] = n < RD_P * M ? (n % M) * RD_S + slot_in[n / M] : -1;
+  }
+  return c;
 }
 
-template <int M, bool F32, bool FLAT>
-__device__ __forceinline__ void rd_r2c_put(float2* bufA, const RdItems& I, const RdR2cRaw& R) {
-  rd_put(bufA, I, [&](int u, int, int) {
-    return make_float2(r2c_val<F32, FLAT>(R.P[u][0], R.F[u][0], (R.msk >> (2 * u)) & 1u),
-                       r2c_val<F32, FLAT>(R.P[u][1], R.F[u][1], (R.msk >> (2 * u + 1)) & 1u));
-  });
+// Raw samples of the R2C group: pixel bits and flat values of row i (2G rows of
+// the padded tile from q0), column k; row i exists and maps to data <=> bit i of
+// rowok (uniform), column k maps to data <=> bit k of colok.
+template <int NR, int NC>
+struct RdR2cRaw {
+  unsigned P[NR][NC];
+  float F[NR][NC];
+  unsigned rowok, colok;
+  static_assert(NR <= 32 && NC <= 32, "validity masks are 32-bit");
+};
+
+template <int M, int NR, int NC, bool F32, bool FLAT>
+__device__ __forceinline__ void rd_r2c_load(RdR2cRaw<NR, NC>& R, const mm_tile_dev& td, const RdCols<NC>& C, int T,
+                                            int nrows, int q0) {
+  int c[NC];
+  R.colok = 0;
+#pragma unroll
+  for (int k = 0; k < NC; ++k) {
+    const int cc = C.n[k] < RD_P * M ? pad_map(C.n[k], td.ys, td.yd, td.yfirst, T) : -1;
+    R.colok |= (cc >= 0 ? 1u : 0u) << k;
+    c[k] = cc < 0 ? 0 : cc;  // absent columns read data column 0 of the row
+  }
+  int z = q0 / T, x = q0 - z * T;
+  R.rowok = 0;
+#pragma unroll
+  for (int i = 0; i < NR; ++i) {
+    const int rr = q0 + i < nrows ? pad_map(x, td.xs, td.xd, td.xfirst, T) : -1;
+    R.rowok |= (rr >= 0 ? 1u : 0u) << i;
+    // absent rows read the stack's first row
+    const long long ro = rr >= 0 ? (long long)z * td.img_sz + (long long)(td.X0 + rr) * td.img_sx + td.Y0 : 0;
+    const long long fo = rr >= 0 ? (long long)(td.X0 + rr) * td.flat_sx + td.Y0 : 0;
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      R.P[i][k] = F32 ? __float_as_uint(((const float*)td.img)[ro + c[k]])
+                      : (unsigned)((const uint16_t*)td.img)[ro + c[k]];
+      if (FLAT) R.F[i][k] = td.flat[fo + c[k]];
+    }
+    if (++x == T) {
+      x = 0;
+      ++z;
+    }
+  }
+}
+
+template <int M, int NR, int NC, bool F32, bool FLAT>
+__device__ __forceinline__ void rd_r2c_store(float2* __restrict__ bufA, const RdR2cRaw<NR, NC>& R,
+                                             const RdCols<NC>& C) {
+#pragma unroll
+  for (int g = 0; g < NR / 2; ++g)
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      if (C.cs[k] < 0) continue;
+      const unsigned ok = (R.colok >> k) & 1u;
+      bufA[g * M * RD_S + C.cs[k]] =
+          make_float2(r2c_val<F32, FLAT>(R.P[2 * g][k], R.F[2 * g][k], ok & (R.rowok >> (2 * g))),
+                      r2c_val<F32, FLAT>(R.P[2 * g + 1][k], R.F[2 * g + 1][k], ok & (R.rowok >> (2 * g + 1))));
+    }
 }
 
 template <int M>
 __global__ __launch_bounds__(256, 2) void k_rd_r2c(const mm_tile_dev* __restrict__ tiles, int ntiles, int T, int nrows,
                                                 float2* Sout, long long s_tstride, RdTab tab,
-                                                const float2* __restrict__ tw, int G) {
+                                                const float2* __restrict__ tw) {
+  constexpr int G = rd_row_g(M), NC = rd_row_nc(M), NR = 2 * G;
   extern __shared__ __align__(16) unsigned char smem[];
   const int N = RD_P * M;
   const RdLds L = rd_carve(smem, N, M, G, tw, tab);
-  R2cTab tab0, tab1;
-  {
-    unsigned char* e = L.extra;
-    R2cTab* tt[2] = {&tab0, &tab1};
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      tt[k]->ymap = (int*)e;
-      e += mm_align16((size_t)T * 4);
-      tt[k]->roff = (long long*)e;
-      tt[k]->foff = tt[k]->roff + 2 * G;
-      e += (size_t)4 * G * 8;
-    }
-  }
-  const int nrb = (nrows + 2 * G - 1) / (2 * G);
+  __syncthreads();  // slot_in
+  const RdCols<NC> C = rd_cols<M, NC>(L.slot_in);
+  const int nrb = (nrows + NR - 1) / NR;
   const long long ngrp = (long long)nrb * ntiles;
-  auto tables = [&](long long gi, const R2cTab& tb) {
+  RdR2cRaw<NR, NC> R;
+  auto fetch = [&](long long gi) {
     const int tile = (int)(gi / nrb);
-    const int q0 = (int)(gi - (long long)tile * nrb) * 2 * G;
+    const int q0 = (int)(gi - (long long)tile * nrb) * NR;
     const mm_tile_dev& td = tiles[tile];
-    for (int y = threadIdx.x; y < T; y += blockDim.x) tb.ymap[y] = pad_map(y, td.ys, td.yd, td.yfirst, T);
-    for (int i = threadIdx.x; i < 2 * G; i += blockDim.x) {
-      const int q = q0 + i;
-      long long ro = -1, fo = 0;
-      if (q < nrows) {
-        const int z = q / T, x = q - (q / T) * T;
-        const int rr = pad_map(x, td.xs, td.xd, td.xfirst, T);
-        if (rr >= 0) {
-          ro = (long long)z * td.img_sz + (long long)(td.X0 + rr) * td.img_sx + td.Y0;
-          fo = (long long)(td.X0 + rr) * td.flat_sx + td.Y0;
-        }
-      }
-      tb.roff[i] = ro;
-      tb.foff[i] = fo;
-    }
-  };
-  long long grp = xcd_remap(blockIdx.x, gridDim.x);
-  if (grp < ngrp) tables(grp, tab0);
-  __syncthreads();  // slot_in and the first tables
-  RdItems I;
-  rd_items<M, false>(G, L.slot_in, I);
-  RdR2cRaw R;
-  auto fetch = [&](long long gi, const R2cTab& tb) {
-    const mm_tile_dev& td = tiles[(int)(gi / nrb)];
     if (td.img_f32) {
-      if (td.flat) rd_r2c_fetch<M, true, true>(I, R, td, tb);
-      else rd_r2c_fetch<M, true, false>(I, R, td, tb);
+      if (td.flat) rd_r2c_load<M, NR, NC, true, true>(R, td, C, T, nrows, q0);
+      else rd_r2c_load<M, NR, NC, true, false>(R, td, C, T, nrows, q0);
     } else {
-      if (td.flat) rd_r2c_fetch<M, false, true>(I, R, td, tb);
-      else rd_r2c_fetch<M, false, false>(I, R, td, tb);
+      if (td.flat) rd_r2c_load<M, NR, NC, false, true>(R, td, C, T, nrows, q0);
+      else rd_r2c_load<M, NR, NC, false, false>(R, td, C, T, nrows, q0);
     }
   };
   auto put = [&](long long gi) {
     const mm_tile_dev& td = tiles[(int)(gi / nrb)];
     if (td.img_f32) {
-      if (td.flat) rd_r2c_put<M, true, true>(L.bufA, I, R);
-      else rd_r2c_put<M, true, false>(L.bufA, I, R);
+      if (td.flat) rd_r2c_store<M, NR, NC, true, true>(L.bufA, R, C);
+      else rd_r2c_store<M, NR, NC, true, false>(L.bufA, R, C);
     } else {
-      if (td.flat) rd_r2c_put<M, false, true>(L.bufA, I, R);
-      else rd_r2c_put<M, false, false>(L.bufA, I, R);
+      if (td.flat) rd_r2c_store<M, NR, NC, false, true>(L.bufA, R, C);
+      else rd_r2c_store<M, NR, NC, false, false>(L.bufA, R, C);
     }
   };
   const int H = N / 2 + 1;
-  int set = 0;
-  if (grp < ngrp) fetch(grp, tab0);
+  long long grp = xcd_remap(blockIdx.x, gridDim.x);
+  if (grp < ngrp) fetch(grp);
   for (; grp < ngrp; grp += gridDim.x) {
-    const long long nxt = grp + gridDim.x;
-    const R2cTab& tn = set ? tab0 : tab1;
-    if (nxt < ngrp) tables(nxt, tn);
     put(grp);
     __syncthreads();
-    if (nxt < ngrp) fetch(nxt, tn);
-    rd_stages<M, false>(L, G);
+    if (grp + gridDim.x < ngrp) fetch(grp + gridDim.x);
+    rd_stages<M, false, false>(L, G, RdMid{});
     const int tile = (int)(grp / nrb);
-    const int q0 = (int)(grp - (long long)tile * nrb) * 2 * G;
+    const int q0 = (int)(grp - (long long)tile * nrb) * NR;
     float2* out = Sout + (long long)tile * s_tstride;
     pf_final_r2c<M>(PfShape(N, RD_P), (const float2*)L.bufB, L.tw, G, [&](int g, float2*& pa, float2*& pb) {
       const int q = q0 + 2 * g;
       pa = q < nrows ? out + (long long)q * H : nullptr;
       pb = q + 1 < nrows ? out + (long long)(q + 1) * H : nullptr;
     });
-    set ^= 1;
   }
 }
 
 template <int M>
 __global__ __launch_bounds__(256, 2) void k_rd_c2r(const float2* Sin, long long s_tstride, float* res,
                                                 long long r_tstride, int ntiles, int nrows, RdTab tab,
-                                                const float2* __restrict__ tw, int G) {
+                                                const float2* __restrict__ tw) {
+  constexpr int G = rd_row_g(M), NC = rd_row_nc(M), NR = 2 * G;
   extern __shared__ __align__(16) unsigned char smem[];
   const int N = RD_P * M, H = N / 2 + 1;
   const RdLds L = rd_carve(smem, N, M, G, tw, tab);
   __syncthreads();
-  const int nrb = (nrows + 2 * G - 1) / (2 * G);
+  const RdCols<NC> C = rd_cols<M, NC>(L.slot_in);
+  int kk[NC];  // half-spectrum bin of column n (0 for the absent columns n >= N)
+#pragma unroll
+  for (int k = 0; k < NC; ++k) kk[k] = C.n[k] >= N ? 0 : (C.n[k] >= H ? N - C.n[k] : C.n[k]);
+  const int nrb = (nrows + NR - 1) / NR;
   const long long ngrp = (long long)nrb * ntiles;
-  RdItems I;
-  rd_items<M, false>(G, L.slot_in, I);
-  float2 RA[RD_NI], RB[RD_NI];  // the two rows' bins kk(n) of each item
-  auto fetch = [&](long long gi) {
+  float2 RA[G][NC], RB[G][NC];  // bins kk of the two rows of each transform
+  auto fetch = [&](long long gi) {  // absent rows read the last row
     const int tile = (int)(gi / nrb);
-    const int q0 = (int)(gi - (long long)tile * nrb) * 2 * G;
+    const int q0 = (int)(gi - (long long)tile * nrb) * NR;
     const float2* in = Sin + (long long)tile * s_tstride;
-    rd_fetch(I, [&](int u, int g, int n) {  // absent rows read the last row
-      const int q = q0 + 2 * g;
-      const int kk = n >= H ? N - n : n;
-      RA[u] = in[(long long)min(q, nrows - 1) * H + kk];
-      RB[u] = in[(long long)min(q + 1, nrows - 1) * H + kk];
-    });
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const float2* ra = in + (long long)min(q0 + 2 * g, nrows - 1) * H;
+      const float2* rb = in + (long long)min(q0 + 2 * g + 1, nrows - 1) * H;
+#pragma unroll
+      for (int k = 0; k < NC; ++k) {
+        RA[g][k] = ra[kk[k]];
+        RB[g][k] = rb[kk[k]];
+      }
+    }
   };
   long long grp = xcd_remap(blockIdx.x, gridDim.x);
   if (grp < ngrp) fetch(grp);
   for (; grp < ngrp; grp += gridDim.x) {
     const int tile = (int)(grp / nrb);
-    const int q0 = (int)(grp - (long long)tile * nrb) * 2 * G;
-    rd_put(L.bufA, I, [&](int u, int g, int n) {
+    const int q0 = (int)(grp - (long long)tile * nrb) * NR;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
       const int q = q0 + 2 * g;
-      float2 a = q < nrows ? RA[u] : make_float2(0.f, 0.f);
-      float2 b = q + 1 < nrows ? RB[u] : make_float2(0.f, 0.f);
-      if (n >= H) { a.y = -a.y; b.y = -b.y; }
-      if (n == 0 || 2 * n == N) { a.y = 0.f; b.y = 0.f; }  // real part semantics of ifftn(...).real
-      return make_float2(a.x - b.y, a.y + b.x);
-    });
+#pragma unroll
+      for (int k = 0; k < NC; ++k) {
+        if (C.cs[k] < 0) continue;
+        const int n = C.n[k];
+        float2 a = q < nrows ? RA[g][k] : make_float2(0.f, 0.f);
+        float2 b = q + 1 < nrows ? RB[g][k] : make_float2(0.f, 0.f);
+        if (n >= H) { a.y = -a.y; b.y = -b.y; }
+        if (n == 0 || 2 * n == N) { a.y = 0.f; b.y = 0.f; }  // real part semantics of ifftn(...).real
+        L.bufA[g * M * RD_S + C.cs[k]] = make_float2(a.x - b.y, a.y + b.x);
+      }
+    }
     __syncthreads();
     if (grp + gridDim.x < ngrp) fetch(grp + gridDim.x);
-    rd_stages<M, true>(L, G);
+    rd_stages<M, false, false>(L, G, RdMid{});
     float* out = res + (long long)tile * r_tstride;
     pf_final<M, true>(PfShape(N, RD_P), (const float2*)L.bufB, L.tw, G, [&](int g, int j, const float2* v) {
       const int q = q0 + 2 * g;
@@ -895,6 +921,26 @@ static int launch_rd_c2c(const mm_deconv* d, float2* data, long long estride, lo
                                        estride, cstride, ncols, bstride, nbatch, INV ? d->rdI : d->rdF, d->pT.tw, G));
   MM_LAUNCH_CHECK();
   return MM_OK;
+}
+
+template <int M>
+static void launch_rd_r2c(const mm_tile_dev* td, int ntiles, int T, int nrows, float2* S, long long s_ts,
+                          const RdTab& tab, const float2* tw, hipStream_t st) {
+  constexpr int G = rd_row_g(M);
+  const size_t lds = rd_lds_bytes(T, M, G, 0);
+  const long long ngrp = (long long)((nrows + 2 * G - 1) / (2 * G)) * ntiles;
+  hipLaunchKernelGGL(k_rd_r2c<M>, dim3(pf_grid(ngrp, lds)), dim3(256), lds, st, td, ntiles, T, nrows, S, s_ts, tab,
+                     tw);
+}
+
+template <int M>
+static void launch_rd_c2r(const float2* S, long long s_ts, float* res, long long r_ts, int ntiles, int nrows,
+                          const RdTab& tab, const float2* tw, hipStream_t st) {
+  constexpr int G = rd_row_g(M);
+  const size_t lds = rd_lds_bytes(RD_P * M, M, G, 0);
+  const long long ngrp = (long long)((nrows + 2 * G - 1) / (2 * G)) * ntiles;
+  hipLaunchKernelGGL(k_rd_c2r<M>, dim3(pf_grid(ngrp, lds)), dim3(256), lds, st, S, s_ts, res, r_ts, ntiles, nrows,
+                     tab, tw);
 }
 
 // ------------------------------------------------------------ Wiener spectra
@@ -1139,7 +1185,7 @@ extern "C" int mm_deconv_create(int sz, int tile_size, int overlap, int zpad, mm
   if (d->pfM && d->pfS.p == RD_P) {
     const int M = d->pfM;
     int G = 256 / (7 * M);
-    while (G > 0 && (rd_lds_bytes(d->T, M, G, pf_r2c_extra(d->T, G)) > 81920 || G * M * RD_P > RD_NI * 256)) --G;
+    while (G > 0 && (rd_lds_bytes(d->T, M, G, 0) > 81920 || G * M * RD_P > RD_NI * 256)) --G;
     if (G > 0 && (rc = mm_get_rader113(&d->rdF, &d->rdI)) == MM_OK) {
       d->rd = 1;
       d->pfG = G;
@@ -1263,11 +1309,7 @@ extern "C" int mm_deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int 
   const long long s_ts = (long long)sz * T * H;
   const int nrows = sz * T;
   if (d->rd) {  // 1. R2C along y of the padded rows (prime-first form, Rader prime stage)
-    const int G = d->pfG;
-    const size_t lds = rd_lds_bytes(T, d->pfM, G, pf_r2c_extra(T, G));
-    const long long ngrp = (long long)((nrows + 2 * G - 1) / (2 * G)) * ntiles;
-    PF_SWITCH(d->pfM, hipLaunchKernelGGL(k_rd_r2c<MM>, dim3(pf_grid(ngrp, lds)), dim3(256), lds, st, td, ntiles, T,
-                                         nrows, S, s_ts, d->rdF, d->pT.tw, G));
+    PF_SWITCH(d->pfM, launch_rd_r2c<MM>(td, ntiles, T, nrows, S, s_ts, d->rdF, d->pT.tw, st));
     MM_LAUNCH_CHECK();
   } else if (d->pfM) {  // 1. R2C along y of the padded rows (prime-first form)
     const int G = d->pfG;
@@ -1312,11 +1354,7 @@ extern "C" int mm_deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int 
   if (rc) return rc;
   MM_MARK(4);
   if (d->rd) {  // 5. C2R along y into res (prime-first form, Rader prime stage)
-    const int G = d->pfG;
-    const size_t lds = rd_lds_bytes(T, d->pfM, G, 0);
-    const long long ngrp = (long long)((nrows + 2 * G - 1) / (2 * G)) * ntiles;
-    PF_SWITCH(d->pfM, hipLaunchKernelGGL(k_rd_c2r<MM>, dim3(pf_grid(ngrp, lds)), dim3(256), lds, st, S, s_ts, res,
-                                         (long long)sz * T * T, ntiles, nrows, d->rdI, d->pT.tw, G));
+    PF_SWITCH(d->pfM, launch_rd_c2r<MM>(S, s_ts, res, (long long)sz * T * T, ntiles, nrows, d->rdI, d->pT.tw, st));
     MM_LAUNCH_CHECK();
   } else if (d->pfM) {  // 5. C2R along y into res (prime-first form)
     const int G = d->pfG;

@@ -14,8 +14,10 @@
 // About 2.6 times fewer operations than the direct symmetric DFT.
 //
 // LDS per transform: two buffers of M * RD_S complex (slot 112 holds x_0,
-// slot 113 holds X_0), ping-ponged; the second one finally holds
-// Y[j p + k] in the layout the radix-M final stage of fft_pf.h reads.
+// slot 113 holds X_0), ping-ponged; the second one finally holds the
+// transforms X of the M sub-sequences, in natural order for the radix-M final
+// stage of fft_pf.h (row passes) or in Rader's output order for rd_final
+// (strided pass: no scattered LDS stores).
 #pragma once
 #include "fft_pf.h"
 
@@ -175,11 +177,34 @@ __device__ __forceinline__ void rd_put(float2* __restrict__ bufA, const RdItems&
 }
 
 // ------------------------------------------------------------ Rader stages
+// Per-thread constants of the middle stage.  Its butterfly q = it & 15 equals
+// threadIdx.x & 15 in every round (blockDim.x is a multiple of 16), so its
+// twiddles w112^{q r} (r = 1..6) and G_{q + 16 r} (r = 0..6) can stay in
+// registers (REGC; kernels short of registers read them from LDS).
+struct RdMid {
+  float2 tw[6];
+  float2 gh[7];
+};
+
+__device__ __forceinline__ RdMid rd_mid(const RdLds& L) {
+  const int q = threadIdx.x & 15;
+  RdMid c;
+#pragma unroll
+  for (int r = 1; r < 7; ++r) c.tw[r - 1] = L.w112[q * r];
+#pragma unroll
+  for (int r = 0; r < 7; ++r) c.gh[r] = L.ghat[q + 16 * r];
+  return c;
+}
+
 // After rd_put and a barrier: bufA holds f (slots 0..111) and x_0 (slot 112) of
-// the G*M DFTs.  Ends (after its last barrier) with Y in bufB:
-// Y[g N + j p + k] = X_k of DFT (g, j), N = p M.
-template <int M, bool INV>
-__device__ __forceinline__ void rd_stages(const RdLds& L, int G) {
+// the G*M DFTs.  Ends (after its last barrier) with Y in bufB, at d = g M + j
+// (N = p M):
+//   BORD false: Y[d p + k] = X_k (natural order),
+//   BORD true:  Y[d p] = X_0, Y[d p + 1 + b] = X_{gamma^b} (rd_final reads it).
+// The first stage writes its output (q, k1) at k1 * 7 + q, so that both its
+// stores and the middle stage's loads run across the banks.
+template <int M, bool BORD, bool REGC>
+__device__ __forceinline__ void rd_stages(const RdLds& L, int G, const RdMid& mid) {
   const int ndft = G * M;
   // forward FFT-112, radix 16 at stride 1: butterfly q < 7 reads f[q + 7 r]
   for (int it = threadIdx.x; it < ndft * 7; it += blockDim.x) {
@@ -189,23 +214,23 @@ __device__ __forceinline__ void rd_stages(const RdLds& L, int G) {
 #pragma unroll
     for (int r = 0; r < 16; ++r) v[r] = in[7 * r];
     bfly16<false>(v);
-    float2* out = L.bufB + d * RD_S + q * 16;
+    float2* out = L.bufB + d * RD_S + q;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) out[r] = v[r];
+    for (int r = 0; r < 16; ++r) out[7 * r] = v[r];
   }
   __syncthreads();
   // radix 7 at stride 16 (twiddles w112^{q r}) -> F, times ghat, inverse radix 7 at stride 1
   for (int it = threadIdx.x; it < ndft * 16; it += blockDim.x) {
     const int d = it >> 4, q = it & 15;
-    const float2* in = L.bufB + d * RD_S + q;
+    const float2* in = L.bufB + d * RD_S + q * 7;
     float2 v[7];
     v[0] = in[0];
 #pragma unroll
-    for (int r = 1; r < 7; ++r) v[r] = vmul(in[16 * r], L.w112[q * r]);
+    for (int r = 1; r < 7; ++r) v[r] = vmul(in[r], REGC ? mid.tw[r - 1] : L.w112[q * r]);
     bfly_odd<7, false>(v, kCos7, kSin7);
     if (q == 0) L.bufA[d * RD_S + 113] = vadd(L.bufA[d * RD_S + 112], v[0]);  // X_0 = x_0 + F_0
 #pragma unroll
-    for (int r = 0; r < 7; ++r) v[r] = vmul(v[r], L.ghat[q + 16 * r]);
+    for (int r = 0; r < 7; ++r) v[r] = vmul(v[r], REGC ? mid.gh[r] : L.ghat[q + 16 * r]);
     bfly_odd<7, true>(v, kCos7, kSin7);
     float2* out = L.bufA + d * RD_S + q * 7;
 #pragma unroll
@@ -223,12 +248,40 @@ __device__ __forceinline__ void rd_stages(const RdLds& L, int G) {
     for (int r = 1; r < 16; ++r) v[r] = vmul(in[7 * r], vconj(L.w112[q * r]));
     bfly16<true>(v);
     const float2 x0 = L.bufA[d * RD_S + 112];
-    const int g = d / M, j = d - (d / M) * M;
-    float2* Y = L.bufB + g * (RD_P * M) + j * RD_P;
+    float2* Y = L.bufB + d * RD_P;
+    if (BORD) {
 #pragma unroll
-    for (int r = 0; r < 16; ++r) Y[L.perm_out[q + 7 * r]] = vadd(x0, v[r]);
+      for (int r = 0; r < 16; ++r) Y[1 + q + 7 * r] = vadd(x0, v[r]);
+    } else {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) Y[L.perm_out[q + 7 * r]] = vadd(x0, v[r]);
+    }
     if (q == 0) Y[0] = L.bufA[d * RD_S + 113];
   }
   __syncthreads();
-  (void)INV;
+}
+
+// Radix-M final stage over Y in the order BORD of rd_stages, consecutive
+// lanes on consecutive transforms (strided pass): the item kk reads its
+// contiguous slot kk of the M sub-transforms and takes k = 0 (kk = 0) or
+// gamma^(kk - 1) for the twiddles and the output rows; st(g, k, v) stores
+// v[r] = Z[k + r p].
+template <int M, bool INV, class St>
+__device__ __forceinline__ void rd_final(const float2* __restrict__ Y, const float2* __restrict__ tw,
+                                         const int* __restrict__ perm_out, int G, St st) {
+  const int N = RD_P * M;
+  const int tot = G * RD_P;
+  const FastDiv fG(G);
+  for (int t = threadIdx.x; t < tot; t += blockDim.x) {
+    int g, kk;
+    fG.divmod(t, kk, g);
+    const int k = kk == 0 ? 0 : perm_out[kk - 1];
+    const float2* y = Y + (size_t)g * N + kk;
+    float2 v[M];
+    v[0] = y[0];
+#pragma unroll
+    for (int r = 1; r < M; ++r) v[r] = vmul(y[r * RD_P], twl<INV>(tw, k * r));
+    pf_bfly<M, INV>(v);
+    st(g, k, v);
+  }
 }
