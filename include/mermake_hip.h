@@ -48,6 +48,15 @@ int mm_box_1d_sub(const float* res, const float* buf, float* out, int size_z, in
  * delta 15 and 25 run as one fused kernel, other widths need `tmp`
  * (nplanes*T*T floats). */
 int mm_blur_sub_xy(const float* res, float* out, float* tmp, int nplanes, int T, int delta, void* stream);
+/* mm_blur_sub_xy with outputs only at rows and columns [lo, hi) of every plane
+ * (0 <= lo < hi <= T); the box windows still clip at the plane edges and read
+ * every input they cover, so those outputs are bit-identical, and `out` is left
+ * untouched elsewhere.  No counterpart in the reference: for the hyb caller,
+ * whose keep filter (mermake/__main__.py:256-261) discards every spot fitted
+ * outside the overlap margins, so the margins' blur is never read.  Widths
+ * without the fused kernel compute whole planes. */
+int mm_blur_sub_xy_window(const float* res, float* out, float* tmp, int nplanes, int T, int delta, int lo, int hi,
+                          void* stream);
 
 /* ------------------------------------------------------------------ fill
  * Replaces the CUDA kernel `mirror` (mermake/fill.py:6-52) launched by
@@ -148,6 +157,15 @@ int mm_delta_fit(const float* img, int D, int H, int W, long long sZ, long long 
  * counts: ntiles, out8: ntiles*cap*8, ws: mm_maxima_ws_bytes(sz, T, T, ntiles). */
 int mm_tiles_maxima(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const float* res, float threshold,
                     int delta, int* coords, int cap, int* counts, void* ws, size_t ws_bytes, void* stream);
+/* mm_tiles_maxima with candidates searched at x, y in [lo, hi) only (every z);
+ * the neighbour tests read res at [lo - delta, hi + delta).  No counterpart in
+ * the reference: for the hyb caller, whose keep filter discards candidates
+ * whose fitted x, y leave [ov, extent - ov) (a fit moves a candidate by at most
+ * delta_fit), so candidates outside [ov - delta_fit - 1, T - ov + delta_fit + 1)
+ * are never kept. */
+int mm_tiles_maxima_window(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const float* res,
+                           float threshold, int delta, int lo, int hi, int* coords, int cap, int* counts, void* ws,
+                           size_t ws_bytes, void* stream);
 int mm_tiles_fit(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const float* res, const int* coords,
                  const int* counts, int cap, int delta_fit, float sigmaZ, float sigmaXY, float* out8, void* stream);
 /* keep-filter on the fitted x,y (ov <= . < view extent - ov), offset to stack

@@ -31,6 +31,7 @@ _SIGS = {
     "mm_box_1d": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _vp]),
     "mm_box_1d_sub": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
     "mm_blur_sub_xy": (_i, [_vp, _vp, _vp, _i, _i, _i, _vp]),
+    "mm_blur_sub_xy_window": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
     "mm_reflect": (_i, [_vp, _i, ctypes.POINTER(_ll), ctypes.POINTER(_ll), _i, _i, _i, _vp]),
     "mm_deconv_create": (_i, [_i, _i, _i, _i, ctypes.POINTER(_vp)]),
     "mm_deconv_destroy": (_i, [_vp]),
@@ -50,6 +51,7 @@ _SIGS = {
     "mm_local_maxima": (_i, [_vp, _i, _i, _i, _ll, _ll, _f, _f, _i, _vp, _i, _vp, _vp, _sz, _vp]),
     "mm_delta_fit": (_i, [_vp, _i, _i, _i, _ll, _ll, _f, _vp, _i, _ll, _ll, _vp, _vp, _i, _i, _f, _f, _vp, _vp]),
     "mm_tiles_maxima": (_i, [_vp, _vp, _i, _vp, _f, _i, _vp, _i, _vp, _vp, _sz, _vp]),
+    "mm_tiles_maxima_window": (_i, [_vp, _vp, _i, _vp, _f, _i, _i, _i, _vp, _i, _vp, _vp, _sz, _vp]),
     "mm_tiles_fit": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _i, _i, _f, _f, _vp, _vp]),
     "mm_tiles_keep": (_i, [_vp, _i, _i, _vp, _vp, _i, _vp, _vp, _i, _vp, _vp]),
     "mm_points_round_bounds": (_i, [_vp, _i, _i, _f, _vp, _vp, _vp]),

@@ -97,12 +97,20 @@ __device__ __forceinline__ float div_count(float s, float n, float y) {
 
 #define BLUR_BX 16
 #define BLUR_YB 9
+// Outputs are computed for rows and columns in [lo, hi) of each plane (the
+// whole plane: lo = 0, hi = T); the windows still read every input they cover.
+__host__ __device__ __forceinline__ int blur_pitch(int T, int D, int lo, int hi) {
+  const int nyb = (hi - lo + BLUR_YB - 1) / BLUR_YB;
+  const int need = lo + nyb * BLUR_YB + 2 * D;  // every y chunk's window stays inside its row
+  return (need > T + 2 * D ? need : T + 2 * D) | 1;
+}
+
 template <int D>
 __global__ __launch_bounds__(256) void k_blur_sub_xy(const float* __restrict__ res, float* __restrict__ out, int T,
-                                                     int nplanes, int npatch) {
+                                                     int nplanes, int npatch, int lo, int hi) {
   constexpr int BX = BLUR_BX, NR = BX + 2 * D, YB = BLUR_YB, W = YB + 2 * D;
-  const int nyb = (T + YB - 1) / YB;
-  const int pitch = ((nyb * YB + 2 * D) | 1);  // every y chunk's window stays inside its row
+  const int nyb = (hi - lo + YB - 1) / YB;
+  const int pitch = blur_pitch(T, D, lo, hi);
   extern __shared__ __align__(16) float smem_f[];
   float* rcp = smem_f;                 // [64]: RN(1/n)
   float* tb = smem_f + 64;             // [BX][pitch]: box_x of the patch rows, D zero columns each side
@@ -110,7 +118,7 @@ __global__ __launch_bounds__(256) void k_blur_sub_xy(const float* __restrict__ r
   const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
   const int plane = bid / npatch, patch = bid - (bid / npatch) * npatch;
   if (plane >= nplanes) return;
-  const int x0 = patch * BX;
+  const int x0 = lo + patch * BX;
   const float* rp = res + (long long)plane * T * T;
   if (threadIdx.x < 64) rcp[threadIdx.x] = 1.0f / (float)(threadIdx.x ? threadIdx.x : 1);
   const int nz = pitch - T;  // zero columns per row: [0, D) and [D + T, pitch)
@@ -128,7 +136,8 @@ __global__ __launch_bounds__(256) void k_blur_sub_xy(const float* __restrict__ r
     nx[xo] = (float)n;
     yx[xo] = rcp[n];
   }
-  for (int c = threadIdx.x; c < T; c += blockDim.x) {
+  const int chi = min(T, hi + D);  // columns of box_x the y windows of [lo, hi) read
+  for (int c = max(0, lo - D) + threadIdx.x; c < chi; c += blockDim.x) {
     float v[NR];
 #pragma unroll
     for (int i = 0; i < NR; ++i) {
@@ -148,8 +157,8 @@ __global__ __launch_bounds__(256) void k_blur_sub_xy(const float* __restrict__ r
   for (int it = threadIdx.x; it < BX * nyb; it += blockDim.x) {
     const int xo = it / nyb, yb = it - (it / nyb) * nyb;
     const int gx = x0 + xo;
-    if (gx >= T) continue;
-    const int y0 = yb * YB;
+    if (gx >= hi) continue;
+    const int y0 = lo + yb * YB;
     const float* trow = tb + xo * pitch + y0;  // trow[w] = box_x at column y0 - D + w
     float u[W];
 #pragma unroll
@@ -159,7 +168,7 @@ __global__ __launch_bounds__(256) void k_blur_sub_xy(const float* __restrict__ r
 #pragma unroll
     for (int yo = 0; yo < YB; ++yo) {
       const int gy = y0 + yo;
-      if (gy < T) {
+      if (gy < hi) {
         float sum = 0.0f;
 #pragma unroll
         for (int w = 0; w <= 2 * D; ++w) sum += u[yo + w];
@@ -171,15 +180,15 @@ __global__ __launch_bounds__(256) void k_blur_sub_xy(const float* __restrict__ r
 }
 
 template <int D>
-static int launch_blur_sub(const float* res, float* out, int nplanes, int T, hipStream_t st) {
+static int launch_blur_sub(const float* res, float* out, int nplanes, int T, int lo, int hi, hipStream_t st) {
   if (2 * D + 1 > 63) { mm_set_error("mm_blur_sub_xy: window too wide"); return MM_EINVAL; }
-  const int nyb = (T + BLUR_YB - 1) / BLUR_YB;
-  const size_t lds = sizeof(float) * (64 + (size_t)BLUR_BX * ((nyb * BLUR_YB + 2 * D) | 1));
+  const size_t lds = sizeof(float) * (64 + (size_t)BLUR_BX * blur_pitch(T, D, lo, hi));
   if (lds > 163840) { mm_set_error("mm_blur_sub_xy: plane of %d columns too wide", T); return MM_EINVAL; }
-  const int npatch = (T + BLUR_BX - 1) / BLUR_BX;
+  const int npatch = (hi - lo + BLUR_BX - 1) / BLUR_BX;
   const long long nblk = (long long)npatch * nplanes;
   if (nblk > 0x7fffffffLL) { mm_set_error("mm_blur_sub_xy: too many planes"); return MM_EINVAL; }
-  hipLaunchKernelGGL(k_blur_sub_xy<D>, dim3((unsigned)nblk), dim3(256), lds, st, res, out, T, nplanes, npatch);
+  hipLaunchKernelGGL(k_blur_sub_xy<D>, dim3((unsigned)nblk), dim3(256), lds, st, res, out, T, nplanes, npatch, lo,
+                     hi);
   MM_LAUNCH_CHECK();
   return MM_OK;
 }
@@ -192,12 +201,29 @@ extern "C" int mm_blur_sub_xy(const float* res, float* out, float* tmp, int npla
   if (!res || !out || nplanes < 0 || T < 1 || delta < 0) { mm_set_error("mm_blur_sub_xy: bad argument"); return MM_EINVAL; }
   if (nplanes == 0) return MM_OK;
   hipStream_t st = (hipStream_t)stream;
-  if (delta == 15) return launch_blur_sub<15>(res, out, nplanes, T, st);
-  if (delta == 25) return launch_blur_sub<25>(res, out, nplanes, T, st);
+  if (delta == 15) return launch_blur_sub<15>(res, out, nplanes, T, 0, T, st);
+  if (delta == 25) return launch_blur_sub<25>(res, out, nplanes, T, 0, T, st);
   if (!tmp) { mm_set_error("mm_blur_sub_xy: delta %d needs a temporary buffer", delta); return MM_EINVAL; }
   const int rc = box_launch(res, nullptr, tmp, nplanes, T, T, delta, 1, false, stream);
   if (rc) return rc;
   return box_launch(tmp, res, out, nplanes, T, T, delta, 2, true, stream);
+}
+
+// mm_blur_sub_xy restricted to rows and columns [lo, hi) of every plane: out
+// holds res - box_y(box_x(res)) there (the box windows still clip at the plane
+// edges 0 and T) and is left untouched elsewhere.  Widths without a fused
+// kernel compute the whole plane.
+extern "C" int mm_blur_sub_xy_window(const float* res, float* out, float* tmp, int nplanes, int T, int delta, int lo,
+                                     int hi, void* stream) {
+  if (!res || !out || nplanes < 0 || T < 1 || delta < 0 || lo < 0 || hi > T || lo >= hi) {
+    mm_set_error("mm_blur_sub_xy_window: bad argument (lo %d hi %d T %d)", lo, hi, T);
+    return MM_EINVAL;
+  }
+  if (nplanes == 0) return MM_OK;
+  hipStream_t st = (hipStream_t)stream;
+  if (delta == 15) return launch_blur_sub<15>(res, out, nplanes, T, lo, hi, st);
+  if (delta == 25) return launch_blur_sub<25>(res, out, nplanes, T, lo, hi, st);
+  return mm_blur_sub_xy(res, out, tmp, nplanes, T, delta, stream);
 }
 
 // ------------------------------------------------------------------ reflect

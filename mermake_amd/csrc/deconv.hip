@@ -1037,7 +1037,7 @@ __global__ __launch_bounds__(640, 5) void k_tile_zconv(float2* S, long long s_ts
       for (int j = 0; j < CH; ++j)
         v[j] = (col && cs + j < t1) ? S[(long long)(cs + j) * s_tstride + cofs] : make_float2(0.f, 0.f);
 #pragma unroll
-      for (int j = 0; j < CH; ++j) xl[((size_t)j * G + g) * SP + t] = v[j];
+      for (int j = 0; j < CH; ++j) xl[(j * G + g) * SP + t] = v[j];
     }
     __syncthreads();
     float2 y[CH];
@@ -1046,7 +1046,7 @@ __global__ __launch_bounds__(640, 5) void k_tile_zconv(float2* S, long long s_ts
     if (act) {
       // not fully unrolled: K[t][s] must stay per-plane work inside the chunk loop, not be
       // hoisted out of it as a whole register row
-#pragma unroll 2
+#pragma unroll(EXACT ? 2 : 1)
       for (int s = 0; s < sz; ++s) {
         {
           float2 k = make_float2(0.f, 0.f);
@@ -1063,7 +1063,7 @@ __global__ __launch_bounds__(640, 5) void k_tile_zconv(float2* S, long long s_ts
           }
 #pragma unroll
           for (int j = 0; j < CH; ++j) {
-            const float2 x = xl[((size_t)j * G + g) * SP + s];
+            const float2 x = xl[(j * G + g) * SP + s];
             y[j].x = fmaf(k.x, x.x, y[j].x);
             y[j].x = fmaf(-k.y, x.y, y[j].x);
             y[j].y = fmaf(k.x, x.y, y[j].y);
@@ -1090,6 +1090,10 @@ static int launch_zconv(mm_deconv* d, float2* S, long long s_ts, const float2* W
   for (int i = 0; i < 4 * 48; ++i) src.p[i] = i < 4 * sz ? d->zsrc_h[i] : -1;
   const int ch = tpb >= ZC_CH ? ZC_CH : 1;  // one PSF per tile: no chunk of tiles shares K
   const size_t lds = sizeof(float2) * ((size_t)G * Zp + (size_t)ch * G * (sz + 1));
+  if (lds > 163840) {
+    mm_set_error("z pass: %zu bytes of LDS for %d planes, Zp %d", lds, sz, Zp);
+    return MM_EINVAL;
+  }
 #define MM_ZCX(M, E)                                                                                            \
   do {                                                                                                          \
     if (ch == ZC_CH)                                                                                            \
@@ -1206,6 +1210,8 @@ extern "C" int mm_deconv_create(int sz, int tile_size, int overlap, int zpad, mm
     }
     d->zG = 640 / sz;
     if (d->zG > 64) d->zG = 64;
+    // LDS of a block: w of its columns and a chunk of ZC_CH tiles' columns
+    if (sizeof(float2) * ((size_t)d->zG * d->Zp + (size_t)ZC_CH * d->zG * (sz + 1)) > 163840) d->zconv = 0;
     for (int i = 0; i < 4 * sz; ++i) d->zsrc_h[i] = src[i];
     if (d->zconv && (hipMalloc(&d->zsrc_dev, sizeof(int) * 4 * sz) != hipSuccess ||
                      hipMemcpy(d->zsrc_dev, src.data(), sizeof(int) * 4 * sz, hipMemcpyHostToDevice) != hipSuccess)) {

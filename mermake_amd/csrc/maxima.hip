@@ -22,6 +22,7 @@ struct VolDesc {
   const float* p;
   int D, H, W;
   long long sZ, sX;
+  int w0, w1;  // candidates are searched at x and y in [w0, w1) only
 };
 
 struct OneVol {
@@ -33,6 +34,7 @@ struct TileVols {  // view res[:, :xe, :ye] of every tile (deconvolver.py:170)
   const mm_tile_dev* tiles;
   const float* res;
   int sz, T;
+  int w0 = 0, w1 = 0x7fffffff;  // candidate window of every tile
   __device__ __forceinline__ VolDesc operator()(int v) const {
     VolDesc d;
     d.p = res + (long long)v * sz * T * T;
@@ -41,6 +43,8 @@ struct TileVols {  // view res[:, :xe, :ye] of every tile (deconvolver.py:170)
     d.W = tiles[v].ye;
     d.sZ = (long long)T * T;
     d.sX = T;
+    d.w0 = w0;
+    d.w1 = w1;
     return d;
   }
 };
@@ -116,23 +120,24 @@ __global__ __launch_bounds__(256) void k_max_mark(VS vs, float sign, float thr, 
   if (row >= nrow_max) return;
   const long long ridx = (long long)v * nrow_max + row;
   int cnt = 0;
-  if (row < V.D * V.H) {
-    const int z = row / V.H, x = row - (row / V.H) * V.H;
+  const int z = row / V.H, x = row - (row / V.H) * V.H;
+  if (row < V.D * V.H && x >= V.w0 && x < V.w1) {  // rows outside the window keep a zero count
     const float* rp = V.p + z * V.sZ + x * V.sX;
     unsigned long long* mw = masks + ridx * ww;
+    const int y0 = max(V.w0, 0), y1 = min(V.W, V.w1);
     for (int k0 = 0; k0 < ww; k0 += MX_CHUNK) {
       float c[MX_CHUNK];
 #pragma unroll
       for (int k = 0; k < MX_CHUNK; ++k) {  // the chunk's centre loads in flight before any test
         const int y = (k0 + k) * 64 + lane;
-        c[k] = y < V.W ? sign * rp[y] : 0.0f;
+        c[k] = y >= y0 && y < y1 ? sign * rp[y] : 0.0f;
       }
 #pragma unroll
       for (int k = 0; k < MX_CHUNK; ++k) {
         if (k0 + k >= ww) break;
         const int y = (k0 + k) * 64 + lane;
         // maxima.cu:19: v <= threshold is rejected (a NaN threshold rejects nothing)
-        const bool m = y < V.W && !(c[k] <= thr) &&
+        const bool m = y >= y0 && y < y1 && !(c[k] <= thr) &&
                        (delta == 1 ? is_max_d1(V, z, x, y, c[k], sign) : is_max_at(V, z, x, y, c[k], sign, delta));
         const unsigned long long b = __ballot(m);
         if (lane == 0) mw[k0 + k] = b;
@@ -523,7 +528,7 @@ extern "C" int mm_local_maxima(const float* img, int D, int H, int W, long long 
                                float threshold, int delta, int* coords, int cap, int* count, void* ws, size_t ws_bytes,
                                void* stream) {
   if (!img || !coords || !count || D < 0 || H < 0 || W < 0) { mm_set_error("mm_local_maxima: bad argument"); return MM_EINVAL; }
-  OneVol vs{VolDesc{img, D, H, W, sZ, sX}};
+  OneVol vs{VolDesc{img, D, H, W, sZ, sX, 0, 0x7fffffff}};
   return run_maxima(vs, 1, D, H, W, sign, threshold, delta, coords, cap, count, ws, ws_bytes,
                     (hipStream_t)stream);
 }
@@ -537,7 +542,7 @@ extern "C" int mm_delta_fit(const float* img, int D, int H, int W, long long sZ,
   int P = 0;
   int rc = sphere_table(delta_fit, &offs, &P);
   if (rc) return rc;
-  OneVol vs{VolDesc{img, D, H, W, sZ, sX}};
+  OneVol vs{VolDesc{img, D, H, W, sZ, sX, 0, 0x7fffffff}};
   RawPlain rs{raw, raw_f32, rZ, rX};
   int blocks = (cap + 3) / 4;
   if (blocks > 2048) blocks = 2048;
@@ -553,6 +558,18 @@ extern "C" int mm_tiles_maxima(mm_deconv* d, const mm_tile_desc* tiles_dev, int 
                                size_t ws_bytes, void* stream) {
   if (!d || !tiles_dev || !res) { mm_set_error("mm_tiles_maxima: bad argument"); return MM_EINVAL; }
   TileVols vs{(const mm_tile_dev*)tiles_dev, res, d->sz, d->T};
+  return run_maxima(vs, ntiles, d->sz, d->T, d->T, 1.0f, threshold, delta, coords, cap, counts, ws, ws_bytes,
+                    (hipStream_t)stream);
+}
+
+extern "C" int mm_tiles_maxima_window(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const float* res,
+                                      float threshold, int delta, int lo, int hi, int* coords, int cap, int* counts,
+                                      void* ws, size_t ws_bytes, void* stream) {
+  if (!d || !tiles_dev || !res || lo < 0 || hi <= lo) {
+    mm_set_error("mm_tiles_maxima_window: bad argument (lo %d hi %d)", lo, hi);
+    return MM_EINVAL;
+  }
+  TileVols vs{(const mm_tile_dev*)tiles_dev, res, d->sz, d->T, lo, hi};
   return run_maxima(vs, ntiles, d->sz, d->T, d->T, 1.0f, threshold, delta, coords, cap, counts, ws, ws_bytes,
                     (hipStream_t)stream);
 }

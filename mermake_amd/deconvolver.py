@@ -226,10 +226,12 @@ class Deconvolver:
         per = self._plan.spec_bytes + 4 * self.tile_height * T * T + extra_per_tile
         return max(1, min(ntiles, self.batch_bytes // per))
 
-    def run_tiles(self, table, ntiles, tiles_per_block, blur_radius, res_out=None):
+    def run_tiles(self, table, ntiles, tiles_per_block, blur_radius, res_out=None, window=None):
         """Deconvolution (+ blur subtraction) of ``ntiles`` descriptors: ``(ntiles, sz, T, T)``.
 
-        With a blur the result lives in the spectrum workspace (valid until the next call)."""
+        With a blur the result lives in the spectrum workspace (valid until the next call).
+        ``window=(lo, hi)`` computes the blur subtraction only at rows and columns [lo, hi)
+        (``mm_blur_sub_xy_window``; the rest of the result is left undefined)."""
         T, sz = self._plan.T, self.tile_height
         spec = self._workspace("spec", ntiles * self._plan.spec_bytes)
         res = res_out if res_out is not None else torch.empty((ntiles, sz, T, T), dtype=torch.float32,
@@ -245,7 +247,11 @@ class Deconvolver:
         tmp = None
         if delta not in (15, 25):
             tmp = self._workspace("blur_tmp", ntiles * sz * T * T * 4)
-        L.call("mm_blur_sub_xy", L.ptr(res), L.ptr(out), L.ptr(tmp), ntiles * sz, T, delta, st)
+        if window is None:
+            L.call("mm_blur_sub_xy", L.ptr(res), L.ptr(out), L.ptr(tmp), ntiles * sz, T, delta, st)
+        else:
+            L.call("mm_blur_sub_xy_window", L.ptr(res), L.ptr(out), L.ptr(tmp), ntiles * sz, T, delta,
+                   int(window[0]), int(window[1]), st)
         return out
 
     def _batches(self, image, flat_field, blur_radius, want_raw):

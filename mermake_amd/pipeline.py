@@ -31,7 +31,7 @@ from .maxima import find_local_maxima_signed
 
 class HybPipeline:
     def __init__(self, deconvolver, blur_radius=30, threshold=3600, delta=1, delta_fit=3, sigmaZ=1, sigmaXY=1.5,
-                 cap=16384, plain=False, **kwargs):
+                 cap=16384, plain=False, window=True, **kwargs):
         if delta > 5:
             raise TypeError("Delta must be an less than or equal to 5 due to MAX_KERNEL_POINTS")
         self.d = deconvolver
@@ -43,6 +43,7 @@ class HybPipeline:
         self.sigmaXY = float(sigmaXY)
         self.cap = int(cap)
         self.plain = plain
+        self.window = window
         self._tables = {}
         self._ws = {}
 
@@ -76,20 +77,45 @@ class HybPipeline:
             self._tables[key] = t
         return t
 
+    def windows(self):
+        """(blur window, candidate window) as (lo, hi) pairs over x and y, or None for whole tiles.
+
+        The keep filter drops every spot whose fitted x or y leaves [ov, extent - ov), and the fit
+        moves a candidate by at most delta_fit (the centroid is a weighted mean of sphere offsets
+        with weights >= 0), so candidates outside [ov - delta_fit - 1, T - ov + delta_fit + 1) are
+        never kept.  Their neighbour tests and fits read the blurred tile up to max(delta,
+        delta_fit) further out, which is where the blur has to be computed."""
+        if not self.window:
+            return None
+        d = self.d
+        T, ov = d._plan.T, d.overlap
+        c0, c1 = ov - self.delta_fit - 1, T - ov + self.delta_fit + 1
+        r = max(self.delta, self.delta_fit)
+        b0, b1 = c0 - r, c1 + r
+        if b0 <= 0 or b1 >= T or c0 >= c1:
+            return None
+        return (b0, b1), (c0, c1)
+
     def _spots_batch(self, table, ntiles, single, cap):
         """Enqueue deconv -> blur -> maxima -> fit -> keep for ``ntiles`` descriptors."""
         d = self.d
         T, sz = d._plan.T, d.tile_height
         st = L.stream_ptr(d.device)
+        win = self.windows()
         res = self._buf("res", (ntiles, sz, T, T), torch.float32)
-        res = d.run_tiles(table, ntiles, ntiles if single else 1, self.blur_radius, res_out=res)
+        res = d.run_tiles(table, ntiles, ntiles if single else 1, self.blur_radius, res_out=res,
+                          window=None if win is None else win[0])
         ws_bytes = L.lib().mm_maxima_ws_bytes(sz, T, T, ntiles)
         ws = self._buf("mws", (ws_bytes,), torch.uint8)
         coords = self._buf("coords", (ntiles, cap, 3), torch.int32)
         counts = self._buf("counts", (ntiles,), torch.int32)
         fit = self._buf("fit", (ntiles, cap, 8), torch.float32)
-        L.call("mm_tiles_maxima", d._plan.h, L.ptr(table), ntiles, L.ptr(res), self.threshold, self.delta,
-               L.ptr(coords), cap, L.ptr(counts), L.ptr(ws), ws_bytes, st)
+        if win is None:
+            L.call("mm_tiles_maxima", d._plan.h, L.ptr(table), ntiles, L.ptr(res), self.threshold, self.delta,
+                   L.ptr(coords), cap, L.ptr(counts), L.ptr(ws), ws_bytes, st)
+        else:
+            L.call("mm_tiles_maxima_window", d._plan.h, L.ptr(table), ntiles, L.ptr(res), self.threshold,
+                   self.delta, win[1][0], win[1][1], L.ptr(coords), cap, L.ptr(counts), L.ptr(ws), ws_bytes, st)
         L.call("mm_tiles_fit", d._plan.h, L.ptr(table), ntiles, L.ptr(res), L.ptr(coords), L.ptr(counts), cap,
                self.delta_fit, self.sigmaZ, self.sigmaXY, L.ptr(fit), st)
         return counts, fit

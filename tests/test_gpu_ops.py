@@ -65,6 +65,29 @@ def test_blur_sub_fused_bitexact(dev, T_, delta, scale):
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
 
 
+@pytest.mark.parametrize("T_,delta,lo,hi", [(678, 15, 82, 596), (678, 15, 1, 677), (100, 15, 23, 77), (100, 15, 40, 41),
+                                           (200, 25, 10, 190), (60, 15, 5, 55)])
+def test_blur_sub_window_bitexact(dev, T_, delta, lo, hi):
+    """The windowed blur-subtract equals the whole-plane one at rows and columns [lo, hi) and leaves the rest of
+    the output untouched."""
+    from mermake_amd import _lib as L
+    rng = np.random.default_rng(T_ + lo)
+    nplanes = 2
+    res = (rng.standard_normal((nplanes, T_, T_)) * 100).astype(np.float32)
+    size = 2 * delta
+    want = res - oblur.box_1d(oblur.box_1d(res, size, axis=1), size, axis=2)
+    r = T(res, dev)
+    out = torch.full_like(r, 12345.0)
+    L.call("mm_blur_sub_xy_window", L.ptr(r), L.ptr(out), None, nplanes, T_, delta, lo, hi, L.stream_ptr(dev))
+    got = out.cpu().numpy()
+    assert np.array_equal(got[:, lo:hi, lo:hi].view(np.uint32), want[:, lo:hi, lo:hi].view(np.uint32))
+    inside = np.zeros((T_, T_), bool)
+    inside[lo:hi, lo:hi] = True
+    assert np.all(got[:, ~inside] == 12345.0)
+    with pytest.raises(L.MermakeLibraryError):
+        L.call("mm_blur_sub_xy_window", L.ptr(r), L.ptr(out), None, nplanes, T_, delta, 5, T_ + 1, L.stream_ptr(dev))
+
+
 def test_box_multi_axis(dev):
     from mermake_amd import blur
     a = np.arange(64, dtype=np.float32).reshape(4, 4, 4)

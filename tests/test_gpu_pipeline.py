@@ -78,6 +78,50 @@ def test_hyb_pipeline_vs_oracle(dev, small_hyb):
     assert np.array_equal(again, got)
 
 
+@pytest.mark.parametrize("delta,delta_fit", [(1, 3), (2, 1), (1, 0)])
+def test_hyb_window_equals_whole_tiles(dev, small_hyb, delta, delta_fit):
+    """Blur and maxima restricted to the margins the keep filter can use give the same kept rows, bit for bit,
+    as whole tiles; the windowed candidate lists are the whole-tile lists restricted to the window."""
+    from mermake_amd import _lib as L
+    from mermake_amd.pipeline import HybPipeline
+    raw, flat, psf = small_hyb
+    hyb = dict(HYB, delta=delta, delta_fit=delta_fit, threshold=2000)
+    win = HybPipeline.from_settings(psf, raw.shape, tile_size=100, overlap=30, beta=1e-4, **hyb)
+    assert win.windows() is not None
+    full = HybPipeline(win.d, window=False, **hyb)
+    a = win(T(raw, dev), T(flat, dev)).cpu().numpy()
+    b = full(T(raw, dev), T(flat, dev)).cpu().numpy()
+    assert len(a) > 50 and np.array_equal(a.view(np.uint32), b.view(np.uint32))  # delta_fit 0 fits NaN columns
+    # candidates: whole-tile maxima restricted to the candidate window
+    d = win.d
+    geom, table = full._table(*_prepared(d, raw, flat, dev))
+    n = len(geom)
+    (b0, b1), (c0, c1) = win.windows()
+    Tt, sz = d._plan.T, d.tile_height
+    res = d.run_tiles(table, n, n, 30).clone()
+    ws_bytes = L.lib().mm_maxima_ws_bytes(sz, Tt, Tt, n)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+    cap = 8192
+    out = []
+    for name, extra in (("mm_tiles_maxima", ()), ("mm_tiles_maxima_window", (c0, c1))):
+        coords = torch.zeros((n, cap, 3), dtype=torch.int32, device=dev)
+        counts = torch.zeros(n, dtype=torch.int32, device=dev)
+        L.call(name, d._plan.h, L.ptr(table), n, L.ptr(res), float(hyb["threshold"]), delta, *extra, L.ptr(coords),
+               cap, L.ptr(counts), L.ptr(ws), ws_bytes, L.stream_ptr(dev))
+        out.append((coords.cpu().numpy(), counts.cpu().numpy()))
+    (cf, nf), (cw, nw) = out
+    for t in range(n):
+        f = cf[t, :nf[t]]
+        keep = (f[:, 1] >= c0) & (f[:, 1] < c1) & (f[:, 2] >= c0) & (f[:, 2] < c1)
+        assert np.array_equal(cw[t, :nw[t]], f[keep]), t
+
+
+def _prepared(d, raw, flat, dev):
+    from mermake_amd.deconvolver import prepare_flat, prepare_stack
+    img = prepare_stack(T(raw, dev))
+    return img, prepare_flat(T(flat, dev), img)
+
+
 def test_dapi_pipeline_vs_oracle(dev):
     from mermake_amd.deconvolver import Deconvolver
     from mermake_amd.pipeline import DapiPipeline
