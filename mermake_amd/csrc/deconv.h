@@ -2,6 +2,12 @@
 #pragma once
 #include "fft_rader.h"
 
+// Pointer into global memory.  Pointers read from descriptors are generic, and
+// loads through them compile to flat instructions; a cast to this type makes
+// them global loads.
+template <class X>
+using gptr = X __attribute__((address_space(1)))*;
+
 // One overlapped tile of a z-stack (reference: deconvolver.py:186-206 for the
 // window, :118-148 for the placement and reflections).  Filled on the host.
 struct mm_tile_dev {

@@ -713,13 +713,13 @@ struct RdR2cRaw {
 template <int M, int NR, int NC, bool F32, bool FLAT>
 __device__ __forceinline__ void rd_r2c_load(RdR2cRaw<NR, NC>& R, const mm_tile_dev& td, const RdCols<NC>& C, int T,
                                             int nrows, int q0) {
-  int c[NC];
+  unsigned c[NC];  // column offsets in elements (absent columns read data column 0 of the row)
   R.colok = 0;
 #pragma unroll
   for (int k = 0; k < NC; ++k) {
     const int cc = C.n[k] < RD_P * M ? pad_map(C.n[k], td.ys, td.yd, td.yfirst, T) : -1;
     R.colok |= (cc >= 0 ? 1u : 0u) << k;
-    c[k] = cc < 0 ? 0 : cc;  // absent columns read data column 0 of the row
+    c[k] = cc < 0 ? 0u : (unsigned)cc;
   }
   int z = q0 / T, x = q0 - z * T;
   R.rowok = 0;
@@ -730,11 +730,14 @@ __device__ __forceinline__ void rd_r2c_load(RdR2cRaw<NR, NC>& R, const mm_tile_d
     // absent rows read the stack's first row
     const long long ro = rr >= 0 ? (long long)z * td.img_sz + (long long)(td.X0 + rr) * td.img_sx + td.Y0 : 0;
     const long long fo = rr >= 0 ? (long long)(td.X0 + rr) * td.flat_sx + td.Y0 : 0;
+    // uniform row pointers in the global address space: global loads with a
+    // scalar base and a 32-bit lane offset
+    const gptr<const char> rb = (gptr<const char>)td.img + ro * (F32 ? 4 : 2);
+    const gptr<const char> fb = (gptr<const char>)td.flat + fo * 4;
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
-      R.P[i][k] = F32 ? __float_as_uint(((const float*)td.img)[ro + c[k]])
-                      : (unsigned)((const uint16_t*)td.img)[ro + c[k]];
-      if (FLAT) R.F[i][k] = td.flat[fo + c[k]];
+      R.P[i][k] = F32 ? *(gptr<const unsigned>)(rb + c[k] * 4u) : (unsigned)*(gptr<const uint16_t>)(rb + c[k] * 2u);
+      if (FLAT) R.F[i][k] = *(gptr<const float>)(fb + c[k] * 4u);
     }
     if (++x == T) {
       x = 0;

@@ -130,35 +130,36 @@ int mm_get_rader113(RdTab* fwd, RdTab* inv) {
     }
     int ginv = 1;
     for (int e = 0; e < p - 2; ++e) ginv = ginv * gam % p;  // gam^(p-2) = gam^-1
-    std::vector<int> perm_out(L), slot_in(p, 0);
+    // Good-Thomas maps of 112 = 16 * 7: n = 7 n1 + 16 n2, k = 49 k1 + 64 k2 (mod 112)
+    std::vector<int> pw(L), perm_out(L), slot_in(p, 0);
     int x = 1, y = 1;
     for (int a = 0; a < L; ++a) {
-      perm_out[a] = x;   // gam^a
-      slot_in[y] = a;    // gam^-a -> a
+      pw[a] = x;                                   // gam^a
+      slot_in[y] = (4 * a) % 7 + 7 * ((7 * a) % 16);  // gam^-a -> slot n2 + 7 n1 of f_a
       x = x * gam % p;
       y = y * ginv % p;
     }
     slot_in[0] = L;  // x_0
+    for (int m1 = 0; m1 < 16; ++m1)
+      for (int m2 = 0; m2 < 7; ++m2) perm_out[m2 + 7 * m1] = pw[(7 * m1 + 16 * m2) % L];
     const long double PI = 3.141592653589793238462643383279502884L;
-    std::vector<float2> h(3 * L);
-    for (int m = 0; m < L; ++m) {
-      const long double a = -2.0L * PI * m / L;
-      h[m] = make_float2((float)cosl(a), (float)sinl(a));
-    }
+    std::vector<float2> h(2 * L);
     for (int dir = 0; dir < 2; ++dir) {  // ghat = DFT_112(g) / 112, g_n = exp(-/+ 2 pi i gam^n / p)
       const long double sg = dir ? 1.0L : -1.0L;
-      for (int k = 0; k < L; ++k) {
-        long double re = 0, im = 0;
-        for (int n = 0; n < L; ++n) {
-          const long double ag = sg * 2.0L * PI * perm_out[n] / p;
-          const long double aw = -2.0L * PI * (long double)((n * k) % L) / L;
-          re += cosl(ag + aw);
-          im += sinl(ag + aw);
+      for (int k1 = 0; k1 < 16; ++k1)
+        for (int k2 = 0; k2 < 7; ++k2) {
+          const int k = (49 * k1 + 64 * k2) % L;
+          long double re = 0, im = 0;
+          for (int n = 0; n < L; ++n) {
+            const long double ag = sg * 2.0L * PI * pw[n] / p;
+            const long double aw = -2.0L * PI * (long double)((n * k) % L) / L;
+            re += cosl(ag + aw);
+            im += sinl(ag + aw);
+          }
+          h[dir * L + k1 + 16 * k2] = make_float2((float)(re / L), (float)(im / L));
         }
-        h[(1 + dir) * L + k] = make_float2((float)(re / L), (float)(im / L));
-      }
     }
-    const size_t fbytes = sizeof(float2) * 3 * L, ibytes = sizeof(int) * (L + p);
+    const size_t fbytes = sizeof(float2) * 2 * L, ibytes = sizeof(int) * (L + p);
     MM_CHECK(hipMalloc(&buf, fbytes + ibytes));
     std::vector<int> ints(perm_out);
     ints.insert(ints.end(), slot_in.begin(), slot_in.end());
@@ -167,9 +168,9 @@ int mm_get_rader113(RdTab* fwd, RdTab* inv) {
     g_rader[dev] = buf;
   }
   const float2* f = (const float2*)buf;
-  const int* ip = (const int*)(f + 3 * RD_L);
-  *fwd = RdTab{f, f + RD_L, ip, ip + RD_L};
-  *inv = RdTab{f, f + 2 * RD_L, ip, ip + RD_L};
+  const int* ip = (const int*)(f + 2 * RD_L);
+  *fwd = RdTab{f, ip, ip + RD_L};
+  *inv = RdTab{f + RD_L, ip, ip + RD_L};
   return MM_OK;
 }
 

@@ -6,12 +6,17 @@
 // is, with a primitive root gamma (3 for 113), r = gamma^-a and k = gamma^b,
 //   X_{gamma^b} = x_0 + (f (*) g)_b,  f_a = x_{gamma^-a},  g_n = w^{gamma^n},
 //   X_0 = x_0 + sum_a f_a,
-// a cyclic convolution of length L = p - 1 = 112 = 16 * 7.  It runs as a
-// forward Stockham FFT-112 (radix 16 at stride 1, then radix 7 at stride 16),
-// a product with G = FFT(g) / L (precomputed in float64), and an inverse
-// FFT-112 (radix 7 at stride 1, then radix 16 at stride 7).  The two radix-7
-// passes act on the same 7 elements, so they run back to back in registers.
-// About 2.6 times fewer operations than the direct symmetric DFT.
+// a cyclic convolution of length L = p - 1 = 112 = 16 * 7.  16 and 7 are
+// coprime, so the FFT-112 runs in the Good-Thomas (prime-factor) form without
+// twiddles: with n = 7 n1 + 16 n2 and k = 49 k1 + 64 k2 (mod 112),
+//   W112^{n k} = W16^{n1 k1} W7^{n2 k2}.
+// Forward: DFT-16 over n1, then DFT-7 over n2; the product with
+// G = FFT(g) / L (precomputed in float64, stored in the (k1, k2) order); the
+// inverse: DFT-7 over k2, then DFT-16 over k1, whose outputs (m1, m2) are
+// C_b at b = 7 m1 + 16 m2.  The index maps live in the tables: slot_in places
+// f_a at its (n1, n2) slot, perm_out maps (m1, m2) straight to gamma^b.  The
+// two radix-7 passes act on the same 7 elements, so they run back to back in
+// registers.  About 2.6 times fewer operations than the direct symmetric DFT.
 //
 // LDS per transform: two buffers of M * RD_S complex (slot 112 holds x_0,
 // slot 113 holds X_0), ping-ponged; the second one finally holds the
@@ -26,25 +31,25 @@
 #define RD_S 114
 #define RD_NI 16  // fetch items (single elements) per thread: G * M * p <= RD_NI * blockDim.x
 
-// Device tables (float64-accurate, float32 storage): w112[112], ghat forward /
-// inverse [112] each (already divided by 112), then perm_out[112] = gamma^b mod p
-// and slot_in[113]: the slot a of element r (gamma^-a = r), 112 for r = 0.
+// Device tables (float64-accurate, float32 storage): ghat forward / inverse
+// [112] each (already divided by 112) at k1 + 16 k2; perm_out[m2 + 7 m1] =
+// gamma^b mod p (b = 7 m1 + 16 m2 mod 112); slot_in[113]: the slot
+// n2 + 7 n1 of element r = gamma^-a (n1 = 7 a mod 16, n2 = 4 a mod 7), 112
+// for r = 0.
 struct RdTab {
-  const float2* w112;
   const float2* ghat;  // of the transform direction
   const int* perm_out;
   const int* slot_in;
 };
 
-// LDS bytes: twiddles of N, w112, ghat, perm_out, slot_in, two data buffers.
+// LDS bytes: twiddles of N, ghat, perm_out, slot_in, two data buffers.
 __host__ __device__ __forceinline__ size_t rd_lds_bytes(int N, int M, int G, size_t extra) {
-  return mm_align16((size_t)N * 8) + (size_t)RD_L * 8 * 2 + mm_align16((size_t)(RD_L + RD_P) * 4) +
+  return mm_align16((size_t)N * 8) + (size_t)RD_L * 8 + mm_align16((size_t)(RD_L + RD_P) * 4) +
          (size_t)2 * G * M * RD_S * 8 + extra;
 }
 
 struct RdLds {
   float2* tw;
-  float2* w112;
   float2* ghat;
   int* perm_out;
   int* slot_in;
@@ -59,8 +64,6 @@ __device__ __forceinline__ RdLds rd_carve(unsigned char* smem, int N, int M, int
   size_t off = 0;
   L.tw = (float2*)smem;
   off += mm_align16((size_t)N * 8);
-  L.w112 = (float2*)(smem + off);
-  off += RD_L * 8;
   L.ghat = (float2*)(smem + off);
   off += RD_L * 8;
   L.perm_out = (int*)(smem + off);
@@ -73,7 +76,6 @@ __device__ __forceinline__ RdLds rd_carve(unsigned char* smem, int N, int M, int
   L.extra = smem + off;
   for (int i = threadIdx.x; i < N; i += blockDim.x) L.tw[i] = tw[i];
   for (int i = threadIdx.x; i < RD_L; i += blockDim.x) {
-    L.w112[i] = T.w112[i];
     L.ghat[i] = T.ghat[i];
     L.perm_out[i] = T.perm_out[i];
   }
@@ -179,18 +181,15 @@ __device__ __forceinline__ void rd_put(float2* __restrict__ bufA, const RdItems&
 // ------------------------------------------------------------ Rader stages
 // Per-thread constants of the middle stage.  Its butterfly q = it & 15 equals
 // threadIdx.x & 15 in every round (blockDim.x is a multiple of 16), so its
-// twiddles w112^{q r} (r = 1..6) and G_{q + 16 r} (r = 0..6) can stay in
-// registers (REGC; kernels short of registers read them from LDS).
+// G_{q + 16 r} (r = 0..6) can stay in registers (REGC; kernels short of
+// registers read them from LDS).
 struct RdMid {
-  float2 tw[6];
   float2 gh[7];
 };
 
 __device__ __forceinline__ RdMid rd_mid(const RdLds& L) {
   const int q = threadIdx.x & 15;
   RdMid c;
-#pragma unroll
-  for (int r = 1; r < 7; ++r) c.tw[r - 1] = L.w112[q * r];
 #pragma unroll
   for (int r = 0; r < 7; ++r) c.gh[r] = L.ghat[q + 16 * r];
   return c;
@@ -200,13 +199,14 @@ __device__ __forceinline__ RdMid rd_mid(const RdLds& L) {
 // the G*M DFTs.  Ends (after its last barrier) with Y in bufB, at d = g M + j
 // (N = p M):
 //   BORD false: Y[d p + k] = X_k (natural order),
-//   BORD true:  Y[d p] = X_0, Y[d p + 1 + b] = X_{gamma^b} (rd_final reads it).
-// The first stage writes its output (q, k1) at k1 * 7 + q, so that both its
+//   BORD true:  Y[d p] = X_0, Y[d p + 1 + s] = X_{perm_out[s]}, s = m2 + 7 m1
+//               (rd_final reads it).
+// The first stage writes its output (n2, k1) at k1 * 7 + n2, so that both its
 // stores and the middle stage's loads run across the banks.
 template <int M, bool BORD, bool REGC>
 __device__ __forceinline__ void rd_stages(const RdLds& L, int G, const RdMid& mid) {
   const int ndft = G * M;
-  // forward FFT-112, radix 16 at stride 1: butterfly q < 7 reads f[q + 7 r]
+  // forward DFT-16 over n1: butterfly q = n2 < 7 reads slots q + 7 n1
   for (int it = threadIdx.x; it < ndft * 7; it += blockDim.x) {
     const int d = it / 7, q = it - (it / 7) * 7;
     const float2* in = L.bufA + d * RD_S + q;
@@ -219,14 +219,14 @@ __device__ __forceinline__ void rd_stages(const RdLds& L, int G, const RdMid& mi
     for (int r = 0; r < 16; ++r) out[7 * r] = v[r];
   }
   __syncthreads();
-  // radix 7 at stride 16 (twiddles w112^{q r}) -> F, times ghat, inverse radix 7 at stride 1
+  // DFT-7 over n2 -> F(k1 = q, k2), times ghat, inverse DFT-7 over k2
   for (int it = threadIdx.x; it < ndft * 16; it += blockDim.x) {
     const int d = it >> 4, q = it & 15;
     const float2* in = L.bufB + d * RD_S + q * 7;
     float2 v[7];
     v[0] = in[0];
 #pragma unroll
-    for (int r = 1; r < 7; ++r) v[r] = vmul(in[r], REGC ? mid.tw[r - 1] : L.w112[q * r]);
+    for (int r = 1; r < 7; ++r) v[r] = in[r];
     bfly_odd<7, false>(v, kCos7, kSin7);
     if (q == 0) L.bufA[d * RD_S + 113] = vadd(L.bufA[d * RD_S + 112], v[0]);  // X_0 = x_0 + F_0
 #pragma unroll
@@ -237,7 +237,7 @@ __device__ __forceinline__ void rd_stages(const RdLds& L, int G, const RdMid& mi
     for (int r = 0; r < 7; ++r) out[r] = v[r];
   }
   __syncthreads();
-  // inverse radix 16 at stride 7 (twiddles conj w112^{q r}) -> C_b, b = q + 7 r;
+  // inverse DFT-16 over k1: butterfly q = m2 < 7 -> C_b, b = 7 m1 + 16 m2;
   // X_{gamma^b} = x_0 + C_b
   for (int it = threadIdx.x; it < ndft * 7; it += blockDim.x) {
     const int d = it / 7, q = it - (it / 7) * 7;
@@ -245,7 +245,7 @@ __device__ __forceinline__ void rd_stages(const RdLds& L, int G, const RdMid& mi
     float2 v[16];
     v[0] = in[0];
 #pragma unroll
-    for (int r = 1; r < 16; ++r) v[r] = vmul(in[7 * r], vconj(L.w112[q * r]));
+    for (int r = 1; r < 16; ++r) v[r] = in[7 * r];
     bfly16<true>(v);
     const float2 x0 = L.bufA[d * RD_S + 112];
     float2* Y = L.bufB + d * RD_P;
@@ -264,7 +264,7 @@ __device__ __forceinline__ void rd_stages(const RdLds& L, int G, const RdMid& mi
 // Radix-M final stage over Y in the order BORD of rd_stages, consecutive
 // lanes on consecutive transforms (strided pass): the item kk reads its
 // contiguous slot kk of the M sub-transforms and takes k = 0 (kk = 0) or
-// gamma^(kk - 1) for the twiddles and the output rows; st(g, k, v) stores
+// perm_out[kk - 1] for the twiddles and the output rows; st(g, k, v) stores
 // v[r] = Z[k + r p].
 template <int M, bool INV, class St>
 __device__ __forceinline__ void rd_final(const float2* __restrict__ Y, const float2* __restrict__ tw,
