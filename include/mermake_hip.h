@@ -96,17 +96,26 @@ typedef struct mm_tile_desc {
 int mm_deconv_create(int sz, int tile_size, int overlap, int zpad, mm_deconv** out);
 int mm_deconv_destroy(mm_deconv* d);
 int mm_deconv_dims(const mm_deconv* d, int* T, int* Zp, int* H);
+/* In-plane layout of the half spectra (T rows x, H columns ky) in W and in the
+ * spectrum workspace: blk = 0 is natural (x * H + ky); blk > 0 is
+ * column-blocked, (ky / blk) * T * blk + x * blk + ky % blk with Hs = H rounded
+ * up to blk columns (zero padding), which makes every x transform group one
+ * contiguous run.  A plane holds T * Hs complex64.  No counterpart in the
+ * reference (its spectra are cupy arrays). */
+int mm_deconv_spec_layout(const mm_deconv* d, int* blk, int* Hs);
 size_t mm_deconv_wiener_ws_bytes(const mm_deconv* d);
-/* W = conj(H)/(H conj(H) + beta |L|^2) / (Zp*T*T) as a (Zp, T, H) complex64
- * half spectrum, from one centred PSF (sz, tile_size, tile_size) float32
- * (deconvolver.py:60-74; the PSF FFT runs in float64). */
+/* W = conj(H)/(H conj(H) + beta |L|^2) / (Zp*T*T) as Zp planes of complex64
+ * half spectra in the mm_deconv_spec_layout layout (for stacks of up to 48
+ * planes, the z-domain kernels ifft_z(W) instead), from one centred PSF
+ * (sz, tile_size, tile_size) float32 (deconvolver.py:60-74; the PSF FFT runs
+ * in float64). */
 int mm_deconv_build_wiener(mm_deconv* d, const float* psf_centered, double beta, void* W_out, void* ws,
                            size_t ws_bytes, void* stream);
-/* bytes of the per-tile half-spectrum workspace (sz*T*H complex64) */
+/* bytes of the per-tile half-spectrum workspace (sz planes of T*Hs complex64) */
 size_t mm_deconv_spec_bytes(const mm_deconv* d);
 /* Replaces the FFT part of Deconvolver.tile_wise (deconvolver.py:116-157) for
  * a batch of tiles: res[t] (sz, T, T) float32 = ifftn(fftn(pad_t) * W)[zpad:-zpad].real.
- * tiles_dev: device array of mm_tile_desc.  W: n_psf spectra of (Zp,T,H).
+ * tiles_dev: device array of mm_tile_desc.  W: n_psf outputs of mm_deconv_build_wiener.
  * tiles_per_block: tiles sharing one load of W (use ntiles when all tiles use
  * the same PSF, 1 otherwise).  spec_ws: ntiles * mm_deconv_spec_bytes. */
 int mm_deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const void* W, int n_psf,

@@ -8,6 +8,18 @@
 template <class X>
 using gptr = X __attribute__((address_space(1)))*;
 
+// In-plane layout of the tile spectra (T rows x, H half-spectrum columns ky):
+// natural, x * H + ky (blk = 0), or column-blocked, (ky / blk) * T * blk +
+// x * blk + ky % blk, where the x transforms of one block of blk columns are a
+// single contiguous run.  `plane` is the element stride of one (tile, z) plane.
+struct SpecLayout {
+  int T, H, blk;
+  long long plane;
+  __host__ __device__ __forceinline__ long long row(int x) const {
+    return blk ? (long long)x * blk : (long long)x * H;
+  }
+};
+
 // One overlapped tile of a z-stack (reference: deconvolver.py:186-206 for the
 // window, :118-148 for the placement and reflections).  Filled on the host.
 struct mm_tile_dev {
@@ -86,6 +98,8 @@ struct mm_deconv {
   int pfM, pfG;             // prime-first T passes: T = p * pfM, pfG transforms per block (pfM = 0: off)
   PfShape pfS;
   int rd;                   // the prime p = 113 runs by Rader's algorithm (tables rdF / rdI)
+  int blk;                  // spectra column-blocked by blk columns (0: natural layout)
+  int Hs;                   // columns of one spectrum row in memory (H rounded up to blk)
   RdTab rdF, rdI;
   int prof;                 // record events around the five passes of mm_deconv_tiles
   hipEvent_t ev[6];

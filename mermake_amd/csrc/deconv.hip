@@ -622,7 +622,7 @@ __global__ __launch_bounds__(256) void k_pf_c2r(const float2* Sin, long long s_t
 // into registers during the current group's Rader and final stages.
 template <bool INV, int M>
 __global__ __launch_bounds__(256, 2) void k_rd_c2c(float2* data, long long estride, long long cstride, int ncols,
-                                                long long bstride, int nbatch, RdTab tab,
+                                                long long bstride, int nbatch, long long gstride, RdTab tab,
                                                 const float2* __restrict__ tw, int G) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int N = RD_P * M;
@@ -635,7 +635,7 @@ __global__ __launch_bounds__(256, 2) void k_rd_c2c(float2* data, long long estri
   auto setup = [&](long long gi, float2*& base, int& gmax) {
     const long long batch = gi / ncb;
     const int cb = (int)(gi - batch * ncb);
-    base = data + batch * bstride + (long long)cb * G * cstride;
+    base = data + batch * bstride + cb * gstride;
     gmax = min(G, ncols - cb * G);
   };
   RdItems I;
@@ -763,7 +763,7 @@ __device__ __forceinline__ void rd_r2c_store(float2* __restrict__ bufA, const Rd
 
 template <int M>
 __global__ __launch_bounds__(256, 2) void k_rd_r2c(const mm_tile_dev* __restrict__ tiles, int ntiles, int T, int nrows,
-                                                float2* Sout, long long s_tstride, RdTab tab,
+                                                float2* Sout, long long s_tstride, SpecLayout Ls, RdTab tab,
                                                 const float2* __restrict__ tw) {
   constexpr int G = rd_row_g(M), NC = rd_row_nc(M), NR = 2 * G;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -807,16 +807,29 @@ __global__ __launch_bounds__(256, 2) void k_rd_r2c(const mm_tile_dev* __restrict
     const int tile = (int)(grp / nrb);
     const int q0 = (int)(grp - (long long)tile * nrb) * NR;
     float2* out = Sout + (long long)tile * s_tstride;
-    pf_final_r2c<M>(PfShape(N, RD_P), (const float2*)L.bufB, L.tw, G, [&](int g, float2*& pa, float2*& pb) {
+    auto rows = [&](int g, float2*& pa, float2*& pb) {
       const int q = q0 + 2 * g;
-      pa = q < nrows ? out + (long long)q * H : nullptr;
-      pb = q + 1 < nrows ? out + (long long)(q + 1) * H : nullptr;
-    });
+      const int za = q / T, xa = q - za * T;
+      const int zb = (q + 1) / T, xb = q + 1 - zb * T;
+      pa = q < nrows ? out + za * Ls.plane + Ls.row(xa) : nullptr;
+      pb = q + 1 < nrows ? out + zb * Ls.plane + Ls.row(xb) : nullptr;
+    };
+    if (Ls.blk) {
+      const FastDiv fb(Ls.blk);
+      const int bstr = T * Ls.blk;
+      pf_final_r2c<M>(PfShape(N, RD_P), (const float2*)L.bufB, L.tw, G, rows, [&](int n) {
+        int b, r;
+        fb.divmod(n, b, r);
+        return b * bstr + r;
+      });
+    } else {
+      pf_final_r2c<M>(PfShape(N, RD_P), (const float2*)L.bufB, L.tw, G, rows);
+    }
   }
 }
 
 template <int M>
-__global__ __launch_bounds__(256, 2) void k_rd_c2r(const float2* Sin, long long s_tstride, float* res,
+__global__ __launch_bounds__(256, 2) void k_rd_c2r(const float2* Sin, long long s_tstride, SpecLayout Ls, float* res,
                                                 long long r_tstride, int ntiles, int nrows, RdTab tab,
                                                 const float2* __restrict__ tw) {
   constexpr int G = rd_row_g(M), NC = rd_row_nc(M), NR = 2 * G;
@@ -825,9 +838,12 @@ __global__ __launch_bounds__(256, 2) void k_rd_c2r(const float2* Sin, long long 
   const RdLds L = rd_carve(smem, N, M, G, tw, tab);
   __syncthreads();
   const RdCols<NC> C = rd_cols<M, NC>(L.slot_in);
-  int kk[NC];  // half-spectrum bin of column n (0 for the absent columns n >= N)
+  int kk[NC];  // in-row offset of the half-spectrum bin of column n (bin 0 for the absent columns n >= N)
 #pragma unroll
-  for (int k = 0; k < NC; ++k) kk[k] = C.n[k] >= N ? 0 : (C.n[k] >= H ? N - C.n[k] : C.n[k]);
+  for (int k = 0; k < NC; ++k) {
+    const int b = C.n[k] >= N ? 0 : (C.n[k] >= H ? N - C.n[k] : C.n[k]);
+    kk[k] = Ls.blk ? (b / Ls.blk) * N * Ls.blk + b % Ls.blk : b;
+  }
   const int nrb = (nrows + NR - 1) / NR;
   const long long ngrp = (long long)nrb * ntiles;
   float2 RA[G][NC], RB[G][NC];  // bins kk of the two rows of each transform
@@ -837,8 +853,10 @@ __global__ __launch_bounds__(256, 2) void k_rd_c2r(const float2* Sin, long long 
     const float2* in = Sin + (long long)tile * s_tstride;
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-      const float2* ra = in + (long long)min(q0 + 2 * g, nrows - 1) * H;
-      const float2* rb = in + (long long)min(q0 + 2 * g + 1, nrows - 1) * H;
+      const int qa = min(q0 + 2 * g, nrows - 1), qb = min(q0 + 2 * g + 1, nrows - 1);
+      const int za = qa / N, zb = qb / N;
+      const float2* ra = in + za * Ls.plane + Ls.row(qa - za * N);
+      const float2* rb = in + zb * Ls.plane + Ls.row(qb - zb * N);
 #pragma unroll
       for (int k = 0; k < NC; ++k) {
         RA[g][k] = ra[kk[k]];
@@ -915,35 +933,38 @@ static int launch_pf_c2c(const mm_deconv* d, float2* data, long long estride, lo
 }
 
 template <bool INV>
-static int launch_rd_c2c(const mm_deconv* d, float2* data, long long estride, long long cstride, int ncols,
-                         long long bstride, int nbatch, hipStream_t st) {
+// x transforms of every (tile, z) plane of the spectra in layout Ls (the
+// groups of G columns are the layout's column blocks when it is blocked).
+static int launch_rd_c2c(const mm_deconv* d, float2* data, const SpecLayout& Ls, int nbatch, hipStream_t st) {
   const int G = d->pfG;
-  const long long ngrp = (long long)((ncols + G - 1) / G) * nbatch;
+  const long long ngrp = (long long)((Ls.H + G - 1) / G) * nbatch;
   const size_t lds = rd_lds_bytes(d->T, d->pfM, G, 0);
-  PF_SWITCH(d->pfM, hipLaunchKernelGGL((k_rd_c2c<INV, MM>), dim3(pf_grid(ngrp, lds)), dim3(256), lds, st, data,
-                                       estride, cstride, ncols, bstride, nbatch, INV ? d->rdI : d->rdF, d->pT.tw, G));
+  const long long es = Ls.blk ? Ls.blk : Ls.H;
+  const long long gs = Ls.blk ? (long long)Ls.T * Ls.blk : G;
+  PF_SWITCH(d->pfM, hipLaunchKernelGGL((k_rd_c2c<INV, MM>), dim3(pf_grid(ngrp, lds)), dim3(256), lds, st, data, es,
+                                       1LL, Ls.H, Ls.plane, nbatch, gs, INV ? d->rdI : d->rdF, d->pT.tw, G));
   MM_LAUNCH_CHECK();
   return MM_OK;
 }
 
 template <int M>
 static void launch_rd_r2c(const mm_tile_dev* td, int ntiles, int T, int nrows, float2* S, long long s_ts,
-                          const RdTab& tab, const float2* tw, hipStream_t st) {
+                          const SpecLayout& Ls, const RdTab& tab, const float2* tw, hipStream_t st) {
   constexpr int G = rd_row_g(M);
   const size_t lds = rd_lds_bytes(T, M, G, 0);
   const long long ngrp = (long long)((nrows + 2 * G - 1) / (2 * G)) * ntiles;
-  hipLaunchKernelGGL(k_rd_r2c<M>, dim3(pf_grid(ngrp, lds)), dim3(256), lds, st, td, ntiles, T, nrows, S, s_ts, tab,
-                     tw);
+  hipLaunchKernelGGL(k_rd_r2c<M>, dim3(pf_grid(ngrp, lds)), dim3(256), lds, st, td, ntiles, T, nrows, S, s_ts, Ls,
+                     tab, tw);
 }
 
 template <int M>
-static void launch_rd_c2r(const float2* S, long long s_ts, float* res, long long r_ts, int ntiles, int nrows,
-                          const RdTab& tab, const float2* tw, hipStream_t st) {
+static void launch_rd_c2r(const float2* S, long long s_ts, const SpecLayout& Ls, float* res, long long r_ts,
+                          int ntiles, int nrows, const RdTab& tab, const float2* tw, hipStream_t st) {
   constexpr int G = rd_row_g(M);
   const size_t lds = rd_lds_bytes(RD_P * M, M, G, 0);
   const long long ngrp = (long long)((nrows + 2 * G - 1) / (2 * G)) * ntiles;
-  hipLaunchKernelGGL(k_rd_c2r<M>, dim3(pf_grid(ngrp, lds)), dim3(256), lds, st, S, s_ts, res, r_ts, ntiles, nrows,
-                     tab, tw);
+  hipLaunchKernelGGL(k_rd_c2r<M>, dim3(pf_grid(ngrp, lds)), dim3(256), lds, st, S, s_ts, Ls, res, r_ts, ntiles,
+                     nrows, tab, tw);
 }
 
 // ------------------------------------------------------------ Wiener spectra
@@ -989,6 +1010,25 @@ __global__ void k_wiener64(double2* __restrict__ Hs, int Zp, int T, int H, doubl
 __global__ void k_d2f(const double2* __restrict__ a, float2* __restrict__ b, long long n) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     b[i] = make_float2((float)a[i].x, (float)a[i].y);
+}
+
+// The same into the column-blocked layout Ls (planes of Ls.plane; the padding
+// columns of the last block are zero).
+__global__ void k_d2f_blocked(const double2* __restrict__ a, float2* __restrict__ b, int Zp, SpecLayout Ls) {
+  const long long n = (long long)Zp * Ls.plane;
+  const int bw = Ls.T * Ls.blk;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const long long p = i / Ls.plane;
+    const int c = (int)(i - p * Ls.plane);
+    const int kb = c / bw, x = (c - kb * bw) / Ls.blk;
+    const int ky = kb * Ls.blk + (c - kb * bw - x * Ls.blk);
+    float2 v = make_float2(0.f, 0.f);
+    if (ky < Ls.H) {
+      const double2 w = a[(p * Ls.T + x) * Ls.H + ky];
+      v = make_float2((float)w.x, (float)w.y);
+    }
+    b[i] = v;
+  }
 }
 
 // ------------------------------------------------------------ z-kernel pass
@@ -1081,9 +1121,10 @@ __global__ __launch_bounds__(640, 5) void k_tile_zconv(float2* S, long long s_ts
   }
 }
 
-static int launch_zconv(mm_deconv* d, float2* S, long long s_ts, const float2* Wz, const mm_tile_dev* td, int ntiles,
-                        int tpb, hipStream_t st) {
-  const int sz = d->sz, Zp = d->Zp, TH = d->T * d->H;
+// S and W in the same in-plane layout Ls (planes of TH columns).
+static int launch_zconv(mm_deconv* d, float2* S, long long s_ts, const SpecLayout& Ls, const float2* Wz,
+                        const mm_tile_dev* td, int ntiles, int tpb, hipStream_t st) {
+  const int sz = d->sz, Zp = d->Zp, TH = (int)Ls.plane;
   const bool exact = sz == 40 || sz == 35 || sz == 20;
   const int G = exact ? 640 / sz : d->zG;
   const int ngroups = (ntiles + tpb - 1) / tpb;
@@ -1101,10 +1142,10 @@ static int launch_zconv(mm_deconv* d, float2* S, long long s_ts, const float2* W
   do {                                                                                                          \
     if (ch == ZC_CH)                                                                                            \
       hipLaunchKernelGGL((k_tile_zconv<M, E, ZC_CH>), dim3((unsigned)nblk), dim3(threads), lds, st, S, s_ts, TH, \
-                         sz, d->zpad, Zp, src, Wz, (long long)Zp * TH, td, ntiles, tpb, G);                      \
+                         sz, d->zpad, Zp, src, Wz, (long long)Zp * TH, td, ntiles, tpb, G);            \
     else                                                                                                        \
       hipLaunchKernelGGL((k_tile_zconv<M, E, 1>), dim3((unsigned)nblk), dim3(threads), lds, st, S, s_ts, TH, sz, \
-                         d->zpad, Zp, src, Wz, (long long)Zp * TH, td, ntiles, tpb, G);                          \
+                         d->zpad, Zp, src, Wz, (long long)Zp * TH, td, ntiles, tpb, G);                \
   } while (0)
 #define MM_ZC(M) MM_ZCX(M, false)
   if (sz == 40) MM_ZCX(40, true);
@@ -1223,6 +1264,10 @@ extern "C" int mm_deconv_create(int sz, int tile_size, int overlap, int zpad, mm
       return MM_ENOMEM;
     }
   }
+  // Rader passes with the z-kernel pass: spectra column-blocked by the x-pass group
+  // width, so that every x transform group is one contiguous run
+  d->blk = d->rd && d->zconv ? d->pfG : 0;
+  d->Hs = d->blk ? (d->H + d->blk - 1) / d->blk * d->blk : d->H;
   *out = d;
   return MM_OK;
 }
@@ -1292,13 +1337,25 @@ extern "C" int mm_deconv_build_wiener(mm_deconv* d, const float* psf_centered, d
   MM_LAUNCH_CHECK();
   if (d->zconv)  // z-domain kernels w = unnormalised inverse z-DFT of W (the 1/Zp is already in W)
     if ((rc = launch_c2c<true>(Hs, (long long)T * H, 1, T * H, 0, 1, d->pZ64, d->gZ64, 1.0, st))) return rc;
-  hipLaunchKernelGGL(k_d2f, dim3(4096), dim3(256), 0, st, Hs, (float2*)W_out, (long long)Zp * T * H);
+  if (d->blk) {
+    const SpecLayout Ls{T, H, d->blk, (long long)T * d->Hs};
+    hipLaunchKernelGGL(k_d2f_blocked, dim3(4096), dim3(256), 0, st, Hs, (float2*)W_out, Zp, Ls);
+  } else {
+    hipLaunchKernelGGL(k_d2f, dim3(4096), dim3(256), 0, st, Hs, (float2*)W_out, (long long)Zp * T * H);
+  }
   MM_LAUNCH_CHECK();
   return MM_OK;
 }
 
+extern "C" int mm_deconv_spec_layout(const mm_deconv* d, int* blk, int* Hs) {
+  if (!d) return MM_EINVAL;
+  if (blk) *blk = d->blk;
+  if (Hs) *Hs = d->Hs;
+  return MM_OK;
+}
+
 extern "C" size_t mm_deconv_spec_bytes(const mm_deconv* d) {
-  return d ? (size_t)d->sz * d->T * d->H * sizeof(float2) : 0;
+  return d ? (size_t)d->sz * d->T * d->Hs * sizeof(float2) : 0;
 }
 
 extern "C" int mm_deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const void* W, int n_psf,
@@ -1315,10 +1372,11 @@ extern "C" int mm_deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int 
 #define MM_MARK(i) \
   if (d->prof) MM_CHECK(hipEventRecord(d->ev[i], st))
   MM_MARK(0);
-  const long long s_ts = (long long)sz * T * H;
+  const SpecLayout Ls{T, H, d->blk, (long long)T * d->Hs};  // natural unless blk
+  const long long s_ts = (long long)sz * Ls.plane;
   const int nrows = sz * T;
   if (d->rd) {  // 1. R2C along y of the padded rows (prime-first form, Rader prime stage)
-    PF_SWITCH(d->pfM, launch_rd_r2c<MM>(td, ntiles, T, nrows, S, s_ts, d->rdF, d->pT.tw, st));
+    PF_SWITCH(d->pfM, launch_rd_r2c<MM>(td, ntiles, T, nrows, S, s_ts, Ls, d->rdF, d->pT.tw, st));
     MM_LAUNCH_CHECK();
   } else if (d->pfM) {  // 1. R2C along y of the padded rows (prime-first form)
     const int G = d->pfG;
@@ -1337,14 +1395,14 @@ extern "C" int mm_deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int 
   MM_MARK(1);
   int rc;
   // 2. C2C along x, all tiles and planes at once
-  if (d->rd) rc = launch_rd_c2c<false>(d, S, H, 1, H, (long long)T * H, ntiles * sz, st);
+  if (d->rd) rc = launch_rd_c2c<false>(d, S, Ls, ntiles * sz, st);
   else if (d->pfM) rc = launch_pf_c2c<false>(d, S, H, 1, H, (long long)T * H, ntiles * sz, st);
   else rc = launch_c2c<false>(S, H, 1, H, (long long)T * H, ntiles * sz, d->pT, d->gT_x, 1.0, st);
   if (rc) return rc;
   MM_MARK(2);
   const int tpb = tiles_per_block < 1 ? 1 : (tiles_per_block > ntiles ? ntiles : tiles_per_block);
   if (d->zconv) {  // 3. z pass as the folded z-kernel matrix
-    if ((rc = launch_zconv(d, S, s_ts, (const float2*)W, td, ntiles, tpb, st))) return rc;
+    if ((rc = launch_zconv(d, S, s_ts, Ls, (const float2*)W, td, ntiles, tpb, st))) return rc;
   } else {  // 3. z pass with z FFTs and the Wiener multiply
     const int G = d->gZ;
     const int TH = T * H;
@@ -1357,13 +1415,14 @@ extern "C" int mm_deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int 
   }
   MM_MARK(3);
   // 4. inverse C2C along x
-  if (d->rd) rc = launch_rd_c2c<true>(d, S, H, 1, H, (long long)T * H, ntiles * sz, st);
+  if (d->rd) rc = launch_rd_c2c<true>(d, S, Ls, ntiles * sz, st);
   else if (d->pfM) rc = launch_pf_c2c<true>(d, S, H, 1, H, (long long)T * H, ntiles * sz, st);
   else rc = launch_c2c<true>(S, H, 1, H, (long long)T * H, ntiles * sz, d->pT, d->gT_x, 1.0, st);
   if (rc) return rc;
   MM_MARK(4);
   if (d->rd) {  // 5. C2R along y into res (prime-first form, Rader prime stage)
-    PF_SWITCH(d->pfM, launch_rd_c2r<MM>(S, s_ts, res, (long long)sz * T * T, ntiles, nrows, d->rdI, d->pT.tw, st));
+    PF_SWITCH(d->pfM,
+              launch_rd_c2r<MM>(S, s_ts, Ls, res, (long long)sz * T * T, ntiles, nrows, d->rdI, d->pT.tw, st));
     MM_LAUNCH_CHECK();
   } else if (d->pfM) {  // 5. C2R along y into res (prime-first form)
     const int G = d->pfG;

@@ -277,10 +277,14 @@ __device__ __forceinline__ void pf_final(const PfShape& S, const float2* __restr
 // Final stage for the packed R2C of two real rows: butterflies j and p - j in
 // one thread, so that Z[n] and Z[N - n] meet.  rows(g, pa, pb) gives the
 // output rows of transform g (real-part row, imaginary-part row; nullptr for
-// an absent row); the half-spectrum bins k <= N/2 are written there.
-template <int M, class Rows>
+// an absent row); the half-spectrum bin k <= N/2 goes to row[coff(k)].
+struct PfContig {
+  __device__ __forceinline__ int operator()(int k) const { return k; }
+};
+
+template <int M, class Rows, class Coff = PfContig>
 __device__ __forceinline__ void pf_final_r2c(const PfShape& S, const float2* __restrict__ Y,
-                                             const float2* __restrict__ tw, int G, Rows rows) {
+                                             const float2* __restrict__ tw, int G, Rows rows, Coff coff = Coff()) {
   const int np = (S.p + 1) >> 1;
   const int tot = G * np;
   const int half = S.N >> 1;
@@ -310,15 +314,17 @@ __device__ __forceinline__ void pf_final_r2c(const PfShape& S, const float2* __r
       const float2 Z = u[r];
       const float2 Zm = j1 == 0 ? u[(M - r) % M] : w[M - 1 - r];
       if (n <= half) {
-        if (pa) pa[n] = make_float2((Z.x + Zm.x) * 0.5f, (Z.y - Zm.y) * 0.5f);
-        if (pb) pb[n] = make_float2((Z.y + Zm.y) * 0.5f, (Zm.x - Z.x) * 0.5f);
+        const int o = coff(n);
+        if (pa) pa[o] = make_float2((Z.x + Zm.x) * 0.5f, (Z.y - Zm.y) * 0.5f);
+        if (pb) pb[o] = make_float2((Z.y + Zm.y) * 0.5f, (Zm.x - Z.x) * 0.5f);
       }
       if (j1 > 0) {
         const int n2 = j2 + r * S.p;
         const float2 Z2 = w[r], Z2m = u[M - 1 - r];
         if (n2 <= half) {
-          if (pa) pa[n2] = make_float2((Z2.x + Z2m.x) * 0.5f, (Z2.y - Z2m.y) * 0.5f);
-          if (pb) pb[n2] = make_float2((Z2.y + Z2m.y) * 0.5f, (Z2m.x - Z2.x) * 0.5f);
+          const int o = coff(n2);
+          if (pa) pa[o] = make_float2((Z2.x + Z2m.x) * 0.5f, (Z2.y - Z2m.y) * 0.5f);
+          if (pb) pb[o] = make_float2((Z2.y + Z2m.y) * 0.5f, (Z2m.x - Z2.x) * 0.5f);
         }
       }
     }

@@ -7,6 +7,9 @@
 * ``psf_fft`` holds the half spectra ``(n_psf, Zp, T, T//2+1)`` (complex64),
   already divided by ``Zp*T*T`` (the ``ifftn`` normalisation), instead of the
   full ``(n_psf, Zp, T, T)`` spectra.  The PSF FFT behind it runs in float64.
+  Stacks of up to 48 planes hold the z-domain kernels ``ifft_z`` of those
+  spectra, and tiles T = 113*M hold them column-blocked,
+  ``(n_psf, Zp, Hs/blk, T, blk)`` (``mm_deconv_spec_layout``).
 * ``tile_wise`` computes tiles in batches; the yielded views stay valid until
   the generator computes the next batch (the reference overwrote them at every
   step).  ``tile_pad`` / ``tile_res`` are shape-only (``meta``) tensors.
@@ -59,7 +62,17 @@ class _Plan:
         T, Zp, H = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         L.call("mm_deconv_dims", h, ctypes.byref(T), ctypes.byref(Zp), ctypes.byref(H))
         self.T, self.Zp, self.H = T.value, Zp.value, H.value
+        blk, Hs = ctypes.c_int(), ctypes.c_int()
+        L.call("mm_deconv_spec_layout", h, ctypes.byref(blk), ctypes.byref(Hs))
+        self.blk, self.Hs = blk.value, Hs.value
         self.spec_bytes = L.lib().mm_deconv_spec_bytes(h)
+
+    def spectrum_shape(self, n):
+        """Shape of ``n`` Wiener operators in the plan's layout: (n, Zp, T, H) natural, or
+        (n, Zp, Hs // blk, T, blk) column-blocked."""
+        if self.blk:
+            return (n, self.Zp, self.Hs // self.blk, self.T, self.blk)
+        return (n, self.Zp, self.T, self.H)
 
     def __del__(self):
         try:
@@ -158,8 +171,8 @@ class Deconvolver:
             self.psf_keys = None
             centred = [self.center_psf(np.asarray(psfs))]
         self._plan = _Plan(self.tile_height, tile_size, overlap, zpad)
-        T, Zp, H = self._plan.T, self._plan.Zp, self._plan.H
-        self.psf_fft = torch.empty((len(centred), Zp, T, H), dtype=torch.complex64, device=self.device)
+        self.psf_fft = torch.empty(self._plan.spectrum_shape(len(centred)), dtype=torch.complex64,
+                                   device=self.device)
         ws_bytes = L.lib().mm_deconv_wiener_ws_bytes(self._plan.h)
         ws = torch.empty(ws_bytes, dtype=torch.uint8, device=self.device)
         st = L.stream_ptr(self.device)
