@@ -137,12 +137,22 @@ __global__ __launch_bounds__(256) void k_blur_sub_xy(const float* __restrict__ r
     yx[xo] = rcp[n];
   }
   const int chi = min(T, hi + D);  // columns of box_x the y windows of [lo, hi) read
+  // every row of the strip inside the plane (uniform): unconditional loads from
+  // one scalar strip base with 32-bit lane offsets
+  const bool rows_in = x0 - D >= 0 && x0 + BX + D <= T;
+  const float* strip = rp + (long long)(rows_in ? x0 - D : 0) * T;
   for (int c = max(0, lo - D) + threadIdx.x; c < chi; c += blockDim.x) {
     float v[NR];
+    if (rows_in) {
 #pragma unroll
-    for (int i = 0; i < NR; ++i) {
-      const int gx = x0 - D + i;
-      v[i] = (gx >= 0 && gx < T) ? rp[(long long)gx * T + c] : 0.0f;
+      for (int i = 0; i < NR; ++i) v[i] = strip[(unsigned)(i * T + c)];
+    } else {  // strips crossing the plane edge: clamped rows, zeros outside
+#pragma unroll
+      for (int i = 0; i < NR; ++i) {
+        const int gx = x0 - D + i;
+        const float x = rp[(unsigned)(min(max(gx, 0), T - 1) * T + c)];
+        v[i] = (gx >= 0 && gx < T) ? x : 0.0f;
+      }
     }
 #pragma unroll
     for (int xo = 0; xo < BX; ++xo) {
@@ -165,16 +175,18 @@ __global__ __launch_bounds__(256) void k_blur_sub_xy(const float* __restrict__ r
     for (int w = 0; w < W; ++w) u[w] = trow[w];
     const float* rrow = rp + (long long)gx * T;
     float* orow = out + (long long)plane * T * T + (long long)gx * T;
+    float rv[YB];  // the item's res values, in flight during the sums (columns clamped into the window)
+#pragma unroll
+    for (int yo = 0; yo < YB; ++yo) rv[yo] = rrow[min(y0 + yo, hi - 1)];
 #pragma unroll
     for (int yo = 0; yo < YB; ++yo) {
       const int gy = y0 + yo;
-      if (gy < hi) {
-        float sum = 0.0f;
+      float sum = 0.0f;
 #pragma unroll
-        for (int w = 0; w <= 2 * D; ++w) sum += u[yo + w];
-        const int n = min(T - 1, gy + D) - max(0, gy - D) + 1;
-        orow[gy] = rrow[gy] - div_count(sum, (float)n, rcp[n]);
-      }
+      for (int w = 0; w <= 2 * D; ++w) sum += u[yo + w];
+      const int n = min(T - 1, gy + D) - max(0, gy - D) + 1;
+      const float q = div_count(sum, (float)n, rcp[min(n, 63)]);
+      if (gy < hi) orow[gy] = rv[yo] - q;
     }
   }
 }
