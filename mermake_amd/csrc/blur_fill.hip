@@ -95,6 +95,25 @@ __device__ __forceinline__ float div_count(float s, float n, float y) {
   return fmaf(fmaf(-q, n, s), y, q);
 }
 
+// div_count of K sums at once: the rare tiny sums take the IEEE division in one
+// branch for the whole group instead of one branch per sum.
+template <int K>
+__device__ __forceinline__ void div_counts(const float (&s)[K], const float (&n)[K], const float (&y)[K],
+                                           float (&q)[K]) {
+  bool tiny = false;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const float t = s[k] * y[k];
+    q[k] = fmaf(fmaf(-t, n[k], s[k]), y[k], t);
+    tiny |= fabsf(s[k]) < 0x1p-100f;
+  }
+  if (__builtin_expect(tiny, 0)) {
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (fabsf(s[k]) < 0x1p-100f) q[k] = s[k] / n[k];
+  }
+}
+
 #define BLUR_BX 16
 #define BLUR_YB 9
 // Outputs are computed for rows and columns in [lo, hi) of each plane (the
@@ -105,15 +124,86 @@ __host__ __device__ __forceinline__ int blur_pitch(int T, int D, int lo, int hi)
   return (need > T + 2 * D ? need : T + 2 * D) | 1;
 }
 
+// x pass of column c: the BX box_x values of the patch rows into tb.  IN: every
+// row of the strip and every window lies inside the plane (counts 2D + 1).
+template <int D, bool IN>
+__device__ __forceinline__ void blur_x_col(const float* __restrict__ rp, const float* __restrict__ strip, int T,
+                                           int x0, int c, int pitch, const float* __restrict__ rown,
+                                           const float* __restrict__ rowy, float* __restrict__ tb) {
+  constexpr int BX = BLUR_BX, NR = BX + 2 * D;
+  float v[NR];
+  if (IN) {  // unconditional loads from one scalar strip base with 32-bit lane offsets
+#pragma unroll
+    for (int i = 0; i < NR; ++i) v[i] = strip[(unsigned)(i * T + c)];
+  } else {  // strips crossing the plane edge: clamped rows, zeros outside
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+      const int gx = x0 - D + i;
+      const float x = rp[(unsigned)(min(max(gx, 0), T - 1) * T + c)];
+      v[i] = (gx >= 0 && gx < T) ? x : 0.0f;
+    }
+  }
+  float sum[BX], n[BX], y[BX], q[BX];
+#pragma unroll
+  for (int xo = 0; xo < BX; ++xo) {
+    sum[xo] = 0.0f;
+#pragma unroll
+    for (int w = 0; w <= 2 * D; ++w) sum[xo] += v[xo + w];
+    n[xo] = IN ? (float)(2 * D + 1) : rown[xo];
+    y[xo] = IN ? 1.0f / (float)(2 * D + 1) : rowy[xo];
+  }
+  div_counts<BX>(sum, n, y, q);
+#pragma unroll
+  for (int xo = 0; xo < BX; ++xo)
+    if (IN || x0 + xo < T) tb[xo * pitch + D + c] = q[xo];
+}
+
+// y pass of one item: outputs y0 .. y0 + YB - 1 (< hi) of row gx.  IN: every
+// window inside the row (counts 2D + 1).
+template <int D, bool IN>
+__device__ __forceinline__ void blur_y_item(const float* __restrict__ trow, const float* __restrict__ rrow,
+                                            float* __restrict__ orow, int T, int y0, int hi,
+                                            const float* __restrict__ rcp) {
+  constexpr int YB = BLUR_YB, W = YB + 2 * D;
+  float u[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) u[w] = trow[w];
+  float rv[YB];  // the item's res values, in flight during the sums (columns clamped into the window)
+#pragma unroll
+  for (int yo = 0; yo < YB; ++yo) rv[yo] = rrow[min(y0 + yo, hi - 1)];
+  float sum[YB], n[YB], y[YB], q[YB];
+#pragma unroll
+  for (int yo = 0; yo < YB; ++yo) {
+    sum[yo] = 0.0f;
+#pragma unroll
+    for (int w = 0; w <= 2 * D; ++w) sum[yo] += u[yo + w];
+    if (IN) {
+      n[yo] = (float)(2 * D + 1);
+      y[yo] = 1.0f / (float)(2 * D + 1);
+    } else {
+      const int gy = y0 + yo;
+      const int nn = min(T - 1, gy + D) - max(0, gy - D) + 1;
+      n[yo] = (float)nn;
+      y[yo] = rcp[min(nn, 63)];
+    }
+  }
+  div_counts<YB>(sum, n, y, q);
+#pragma unroll
+  for (int yo = 0; yo < YB; ++yo)
+    if (y0 + yo < hi) orow[y0 + yo] = rv[yo] - q[yo];
+}
+
 template <int D>
 __global__ __launch_bounds__(256) void k_blur_sub_xy(const float* __restrict__ res, float* __restrict__ out, int T,
                                                      int nplanes, int npatch, int lo, int hi) {
-  constexpr int BX = BLUR_BX, NR = BX + 2 * D, YB = BLUR_YB, W = YB + 2 * D;
+  constexpr int BX = BLUR_BX, YB = BLUR_YB;
   const int nyb = (hi - lo + YB - 1) / YB;
   const int pitch = blur_pitch(T, D, lo, hi);
   extern __shared__ __align__(16) float smem_f[];
   float* rcp = smem_f;                 // [64]: RN(1/n)
-  float* tb = smem_f + 64;             // [BX][pitch]: box_x of the patch rows, D zero columns each side
+  float* rown = smem_f + 64;           // [BX]: window counts of the patch rows
+  float* rowy = smem_f + 64 + BX;      // [BX]: their RN(1/n)
+  float* tb = smem_f + 64 + 2 * BX;    // [BX][pitch]: box_x of the patch rows, D zero columns each side
   // neighbouring patches of a plane run on one XCD and share its L2 for the halo rows
   const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
   const int plane = bid / npatch, patch = bid - (bid / npatch) * npatch;
@@ -121,46 +211,28 @@ __global__ __launch_bounds__(256) void k_blur_sub_xy(const float* __restrict__ r
   const int x0 = lo + patch * BX;
   const float* rp = res + (long long)plane * T * T;
   if (threadIdx.x < 64) rcp[threadIdx.x] = 1.0f / (float)(threadIdx.x ? threadIdx.x : 1);
+  if (threadIdx.x < BX) {
+    const int gx = x0 + threadIdx.x;
+    const int n = min(T - 1, gx + D) - max(0, gx - D) + 1;
+    rown[threadIdx.x] = (float)n;
+    rowy[threadIdx.x] = 1.0f / (float)n;
+  }
   const int nz = pitch - T;  // zero columns per row: [0, D) and [D + T, pitch)
   for (int i = threadIdx.x; i < BX * nz; i += blockDim.x) {
     const int r = i / nz, c = i - r * nz;
     tb[r * pitch + (c < D ? c : T + c)] = 0.0f;
   }
   __syncthreads();
-  // x pass: row windows are uniform over the block
-  float yx[BX], nx[BX];
-#pragma unroll
-  for (int xo = 0; xo < BX; ++xo) {
-    const int gx = x0 + xo;
-    const int n = min(T - 1, gx + D) - max(0, gx - D) + 1;
-    nx[xo] = (float)n;
-    yx[xo] = rcp[n];
-  }
-  const int chi = min(T, hi + D);  // columns of box_x the y windows of [lo, hi) read
-  // every row of the strip inside the plane (uniform): unconditional loads from
-  // one scalar strip base with 32-bit lane offsets
-  const bool rows_in = x0 - D >= 0 && x0 + BX + D <= T;
+  // x pass over the columns the y windows of [lo, hi) read
+  const int chi = min(T, hi + D);
+  const bool rows_in = x0 - D >= 0 && x0 + BX + D <= T;  // uniform
   const float* strip = rp + (long long)(rows_in ? x0 - D : 0) * T;
-  for (int c = max(0, lo - D) + threadIdx.x; c < chi; c += blockDim.x) {
-    float v[NR];
-    if (rows_in) {
-#pragma unroll
-      for (int i = 0; i < NR; ++i) v[i] = strip[(unsigned)(i * T + c)];
-    } else {  // strips crossing the plane edge: clamped rows, zeros outside
-#pragma unroll
-      for (int i = 0; i < NR; ++i) {
-        const int gx = x0 - D + i;
-        const float x = rp[(unsigned)(min(max(gx, 0), T - 1) * T + c)];
-        v[i] = (gx >= 0 && gx < T) ? x : 0.0f;
-      }
-    }
-#pragma unroll
-    for (int xo = 0; xo < BX; ++xo) {
-      float sum = 0.0f;
-#pragma unroll
-      for (int w = 0; w <= 2 * D; ++w) sum += v[xo + w];
-      if (x0 + xo < T) tb[xo * pitch + D + c] = div_count(sum, nx[xo], yx[xo]);
-    }
+  if (rows_in) {
+    for (int c = max(0, lo - D) + threadIdx.x; c < chi; c += blockDim.x)
+      blur_x_col<D, true>(rp, strip, T, x0, c, pitch, rown, rowy, tb);
+  } else {
+    for (int c = max(0, lo - D) + threadIdx.x; c < chi; c += blockDim.x)
+      blur_x_col<D, false>(rp, strip, T, x0, c, pitch, rown, rowy, tb);
   }
   __syncthreads();
   // y pass
@@ -170,31 +242,17 @@ __global__ __launch_bounds__(256) void k_blur_sub_xy(const float* __restrict__ r
     if (gx >= hi) continue;
     const int y0 = lo + yb * YB;
     const float* trow = tb + xo * pitch + y0;  // trow[w] = box_x at column y0 - D + w
-    float u[W];
-#pragma unroll
-    for (int w = 0; w < W; ++w) u[w] = trow[w];
     const float* rrow = rp + (long long)gx * T;
     float* orow = out + (long long)plane * T * T + (long long)gx * T;
-    float rv[YB];  // the item's res values, in flight during the sums (columns clamped into the window)
-#pragma unroll
-    for (int yo = 0; yo < YB; ++yo) rv[yo] = rrow[min(y0 + yo, hi - 1)];
-#pragma unroll
-    for (int yo = 0; yo < YB; ++yo) {
-      const int gy = y0 + yo;
-      float sum = 0.0f;
-#pragma unroll
-      for (int w = 0; w <= 2 * D; ++w) sum += u[yo + w];
-      const int n = min(T - 1, gy + D) - max(0, gy - D) + 1;
-      const float q = div_count(sum, (float)n, rcp[min(n, 63)]);
-      if (gy < hi) orow[gy] = rv[yo] - q;
-    }
+    if (y0 - D >= 0 && y0 + YB - 1 + D <= T - 1) blur_y_item<D, true>(trow, rrow, orow, T, y0, hi, rcp);
+    else blur_y_item<D, false>(trow, rrow, orow, T, y0, hi, rcp);
   }
 }
 
 template <int D>
 static int launch_blur_sub(const float* res, float* out, int nplanes, int T, int lo, int hi, hipStream_t st) {
   if (2 * D + 1 > 63) { mm_set_error("mm_blur_sub_xy: window too wide"); return MM_EINVAL; }
-  const size_t lds = sizeof(float) * (64 + (size_t)BLUR_BX * blur_pitch(T, D, lo, hi));
+  const size_t lds = sizeof(float) * (64 + 2 * BLUR_BX + (size_t)BLUR_BX * blur_pitch(T, D, lo, hi));
   if (lds > 163840) { mm_set_error("mm_blur_sub_xy: plane of %d columns too wide", T); return MM_EINVAL; }
   const int npatch = (hi - lo + BLUR_BX - 1) / BLUR_BX;
   const long long nblk = (long long)npatch * nplanes;
