@@ -120,6 +120,15 @@ size_t mm_deconv_spec_bytes(const mm_deconv* d);
  * the same PSF, 1 otherwise).  spec_ws: ntiles * mm_deconv_spec_bytes. */
 int mm_deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const void* W, int n_psf,
                     int tiles_per_block, void* spec_ws, float* res, void* stream);
+/* mm_deconv_tiles with res computed only at rows and columns [lo, hi) of every
+ * plane (0 <= lo < hi <= T; the rest of res is left undefined), bit-identical
+ * there.  No counterpart in the reference: for the hyb caller, whose blur and
+ * maxima read only the margins its keep filter can use (see
+ * mm_blur_sub_xy_window).  Tiles T = 113*M skip the other rows in the final
+ * inverse y transform and the other columns' stores; other sizes compute
+ * whole planes. */
+int mm_deconv_tiles_window(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const void* W, int n_psf,
+                           int tiles_per_block, void* spec_ws, float* res, int lo, int hi, void* stream);
 /* Timing of mm_deconv_tiles with HIP events recorded on its stream around
  * its five passes (no counterpart in the reference; used by the benchmark):
  * ms5 = r2c, x forward, z Wiener, x inverse, c2r of the last profiled call. */

@@ -41,6 +41,7 @@ _SIGS = {
     "mm_deconv_build_wiener": (_i, [_vp, _vp, _d, _vp, _vp, _sz, _vp]),
     "mm_deconv_spec_bytes": (_sz, [_vp]),
     "mm_deconv_tiles": (_i, [_vp, _vp, _i, _vp, _i, _i, _vp, _vp, _vp]),
+    "mm_deconv_tiles_window": (_i, [_vp, _vp, _i, _vp, _i, _i, _vp, _vp, _i, _i, _vp]),
     "mm_deconv_tile_raw": (_i, [_vp, _vp, _i, _vp, _vp]),
     "mm_deconv_set_profiling": (_i, [_vp, _i]),
     "mm_deconv_pass_times": (_i, [_vp, ctypes.POINTER(_f)]),

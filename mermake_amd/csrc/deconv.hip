@@ -828,10 +828,12 @@ __global__ __launch_bounds__(256, 2) void k_rd_r2c(const mm_tile_dev* __restrict
   }
 }
 
+// Rows (z, x) with x in [xlo, xlo + nx) only (nrows = sz * nx of them, taken
+// in that order), output columns in [clo, chi) only.
 template <int M>
 __global__ __launch_bounds__(256, 2) void k_rd_c2r(const float2* Sin, long long s_tstride, SpecLayout Ls, float* res,
-                                                long long r_tstride, int ntiles, int nrows, RdTab tab,
-                                                const float2* __restrict__ tw) {
+                                                long long r_tstride, int ntiles, int nrows, int xlo, int nx, int clo,
+                                                int chi, RdTab tab, const float2* __restrict__ tw) {
   constexpr int G = rd_row_g(M), NC = rd_row_nc(M), NR = 2 * G;
   extern __shared__ __align__(16) unsigned char smem[];
   const int N = RD_P * M, H = N / 2 + 1;
@@ -854,9 +856,9 @@ __global__ __launch_bounds__(256, 2) void k_rd_c2r(const float2* Sin, long long 
 #pragma unroll
     for (int g = 0; g < G; ++g) {
       const int qa = min(q0 + 2 * g, nrows - 1), qb = min(q0 + 2 * g + 1, nrows - 1);
-      const int za = qa / N, zb = qb / N;
-      const float2* ra = in + za * Ls.plane + Ls.row(qa - za * N);
-      const float2* rb = in + zb * Ls.plane + Ls.row(qb - zb * N);
+      const int za = qa / nx, zb = qb / nx;
+      const float2* ra = in + za * Ls.plane + Ls.row(xlo + qa - za * nx);
+      const float2* rb = in + zb * Ls.plane + Ls.row(xlo + qb - zb * nx);
 #pragma unroll
       for (int k = 0; k < NC; ++k) {
         RA[g][k] = ra[kk[k]];
@@ -889,11 +891,14 @@ __global__ __launch_bounds__(256, 2) void k_rd_c2r(const float2* Sin, long long 
     float* out = res + (long long)tile * r_tstride;
     pf_final<M, true>(PfShape(N, RD_P), (const float2*)L.bufB, L.tw, G, [&](int g, int j, const float2* v) {
       const int q = q0 + 2 * g;
-      float* oa = out + (long long)q * N + j;
+      const int za = q / nx, zb = (q + 1) / nx;
+      float* oa = out + ((long long)za * N + xlo + q - za * nx) * N + j;
+      float* ob = out + ((long long)zb * N + xlo + q + 1 - zb * nx) * N + j;
 #pragma unroll
       for (int r = 0; r < M; ++r) {
-        if (q < nrows) oa[r * RD_P] = v[r].x;
-        if (q + 1 < nrows) oa[N + r * RD_P] = v[r].y;
+        const bool cin = j + r * RD_P >= clo && j + r * RD_P < chi;
+        if (cin && q < nrows) oa[r * RD_P] = v[r].x;
+        if (cin && q + 1 < nrows) ob[r * RD_P] = v[r].y;
       }
     });
   }
@@ -959,12 +964,13 @@ static void launch_rd_r2c(const mm_tile_dev* td, int ntiles, int T, int nrows, f
 
 template <int M>
 static void launch_rd_c2r(const float2* S, long long s_ts, const SpecLayout& Ls, float* res, long long r_ts,
-                          int ntiles, int nrows, const RdTab& tab, const float2* tw, hipStream_t st) {
+                          int ntiles, int sz, int lo, int hi, const RdTab& tab, const float2* tw, hipStream_t st) {
   constexpr int G = rd_row_g(M);
   const size_t lds = rd_lds_bytes(RD_P * M, M, G, 0);
+  const int nrows = sz * (hi - lo);
   const long long ngrp = (long long)((nrows + 2 * G - 1) / (2 * G)) * ntiles;
   hipLaunchKernelGGL(k_rd_c2r<M>, dim3(pf_grid(ngrp, lds)), dim3(256), lds, st, S, s_ts, Ls, res, r_ts, ntiles,
-                     nrows, tab, tw);
+                     nrows, lo, hi - lo, lo, hi, tab, tw);
 }
 
 // ------------------------------------------------------------ Wiener spectra
@@ -1358,11 +1364,23 @@ extern "C" size_t mm_deconv_spec_bytes(const mm_deconv* d) {
   return d ? (size_t)d->sz * d->T * d->Hs * sizeof(float2) : 0;
 }
 
-extern "C" int mm_deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const void* W, int n_psf,
-                               int tiles_per_block, void* spec_ws, float* res, void* stream) {
-  if (!d || !tiles_dev || !W || !spec_ws || !res || ntiles < 0 || n_psf < 1) {
+// res rows and columns [lo, hi) of every plane (the Rader C2R computes only
+// those; the other forms compute whole planes).
+static int deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const void* W, int n_psf,
+                        int tiles_per_block, void* spec_ws, float* res, int lo, int hi, void* stream) {
+  if (!d || !tiles_dev || !W || !spec_ws || !res || ntiles < 0 || n_psf < 1 || lo < 0 || hi > d->T || lo >= hi) {
     mm_set_error("mm_deconv_tiles: bad argument");
     return MM_EINVAL;
+  }
+  // The C2R packs rows 2m and 2m + 1 (of z * T + x) into one complex transform,
+  // and a row's rounding depends on its partner: the window keeps whole pairs
+  // (T even: lo and hi even; T odd: pairs straddle planes, whole planes).
+  if (d->T & 1) {
+    lo = 0;
+    hi = d->T;
+  } else {
+    lo &= ~1;
+    hi = min(d->T, (hi + 1) & ~1);
   }
   if (ntiles == 0) return MM_OK;
   hipStream_t st = (hipStream_t)stream;
@@ -1422,7 +1440,7 @@ extern "C" int mm_deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int 
   MM_MARK(4);
   if (d->rd) {  // 5. C2R along y into res (prime-first form, Rader prime stage)
     PF_SWITCH(d->pfM,
-              launch_rd_c2r<MM>(S, s_ts, Ls, res, (long long)sz * T * T, ntiles, nrows, d->rdI, d->pT.tw, st));
+              launch_rd_c2r<MM>(S, s_ts, Ls, res, (long long)sz * T * T, ntiles, sz, lo, hi, d->rdI, d->pT.tw, st));
     MM_LAUNCH_CHECK();
   } else if (d->pfM) {  // 5. C2R along y into res (prime-first form)
     const int G = d->pfG;
@@ -1441,6 +1459,17 @@ extern "C" int mm_deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int 
   MM_MARK(5);
 #undef MM_MARK
   return MM_OK;
+}
+
+extern "C" int mm_deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const void* W, int n_psf,
+                               int tiles_per_block, void* spec_ws, float* res, void* stream) {
+  return deconv_tiles(d, tiles_dev, ntiles, W, n_psf, tiles_per_block, spec_ws, res, 0, d ? d->T : 1, stream);
+}
+
+extern "C" int mm_deconv_tiles_window(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const void* W,
+                                      int n_psf, int tiles_per_block, void* spec_ws, float* res, int lo, int hi,
+                                      void* stream) {
+  return deconv_tiles(d, tiles_dev, ntiles, W, n_psf, tiles_per_block, spec_ws, res, lo, hi, stream);
 }
 
 extern "C" int mm_deconv_set_profiling(mm_deconv* d, int on) {

@@ -244,18 +244,23 @@ class Deconvolver:
 
         With a blur the result lives in the spectrum workspace (valid until the next call).
         ``window=(lo, hi)`` computes the blur subtraction only at rows and columns [lo, hi)
-        (``mm_blur_sub_xy_window``; the rest of the result is left undefined)."""
+        (``mm_blur_sub_xy_window``), and the deconvolution only where those boxes read
+        (``mm_deconv_tiles_window``); the rest of the result is left undefined."""
         T, sz = self._plan.T, self.tile_height
         spec = self._workspace("spec", ntiles * self._plan.spec_bytes)
         res = res_out if res_out is not None else torch.empty((ntiles, sz, T, T), dtype=torch.float32,
                                                                device=self.device)
         st = L.stream_ptr(self.device)
-        L.call("mm_deconv_tiles", self._plan.h, L.ptr(table), ntiles, L.ptr(self.psf_fft), self.psf_fft.shape[0],
-               int(tiles_per_block), L.ptr(spec), L.ptr(res), st)
+        delta = None if blur_radius is None else int(blur_radius) // 2
+        args = (self._plan.h, L.ptr(table), ntiles, L.ptr(self.psf_fft), self.psf_fft.shape[0],
+                int(tiles_per_block), L.ptr(spec), L.ptr(res))
+        if window is None or delta is None:
+            L.call("mm_deconv_tiles", *args, st)
+        else:
+            L.call("mm_deconv_tiles_window", *args, max(0, window[0] - delta), min(T, window[1] + delta), st)
         if blur_radius is None:
             return res
         # deconvolver.py:160-163: res - box_y(box_x(res)); the spectrum buffer is free now
-        delta = int(blur_radius) // 2
         out = spec[:ntiles * sz * T * T * 4].view(torch.float32).view(ntiles, sz, T, T)
         tmp = None
         if delta not in (15, 25):

@@ -297,6 +297,34 @@ def test_deconv_vs_oracle(dev, shape, ts, ov, zpad, beta, flat, blur):
     assert _rel_l2(out[written], ref[written]) < 1e-5
 
 
+@pytest.mark.parametrize("shape,ts,ov,lo,hi", [((3, 300, 300), 200, 13, 5, 215),   # T = 226 (Rader, blocked spectra)
+                                                 ((3, 760, 760), 500, 89, 67, 611),  # T = 678
+                                                 ((4, 120, 120), 60, 20, 9, 91)])    # T = 100
+def test_deconv_window_bitexact(dev, shape, ts, ov, lo, hi):
+    """mm_deconv_tiles_window equals mm_deconv_tiles at rows and columns [lo, hi) of every plane, bit for bit."""
+    from mermake_amd import _lib as L
+    from mermake_amd.deconvolver import Deconvolver, prepare_stack, tile_geometry, tile_table
+    rng = np.random.default_rng(hi)
+    img = rng.integers(100, 4000, size=shape).astype(np.uint16)
+    psf = rng.random((shape[0], 9, 9)).astype(np.float32)
+    d = Deconvolver(psf, shape, tile_size=ts, zpad=2, overlap=ov, beta=1e-3)
+    x = prepare_stack(T(img, dev))
+    geom = tile_geometry(tuple(x.shape), ts, ov, 1)
+    table = tile_table(geom, x, None)
+    n, Tt = len(geom), d._plan.T
+    full = d.run_tiles(table, n, n, None).clone()
+    spec = torch.empty(n * d._plan.spec_bytes, dtype=torch.uint8, device=dev)
+    res = torch.full_like(full, 7.0)
+    L.call("mm_deconv_tiles_window", d._plan.h, L.ptr(table), n, L.ptr(d.psf_fft), 1, n, L.ptr(spec), L.ptr(res),
+           lo, hi, L.stream_ptr(dev))
+    a = res[:, :, lo:hi, lo:hi].cpu().numpy()
+    b = full[:, :, lo:hi, lo:hi].cpu().numpy()
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    with pytest.raises(L.MermakeLibraryError):
+        L.call("mm_deconv_tiles_window", d._plan.h, L.ptr(table), n, L.ptr(d.psf_fft), 1, n, L.ptr(spec),
+               L.ptr(res), 0, Tt + 1, L.stream_ptr(dev))
+
+
 def test_multi_psf_pairing(dev):
     """dict of PSFs: tile i uses PSF i, zip truncates to the shorter list (deconvolver.py:110-116)."""
     from mermake_amd.deconvolver import Deconvolver

@@ -78,15 +78,16 @@ def test_hyb_pipeline_vs_oracle(dev, small_hyb):
     assert np.array_equal(again, got)
 
 
-@pytest.mark.parametrize("delta,delta_fit", [(1, 3), (2, 1), (1, 0)])
-def test_hyb_window_equals_whole_tiles(dev, small_hyb, delta, delta_fit):
-    """Blur and maxima restricted to the margins the keep filter can use give the same kept rows, bit for bit,
-    as whole tiles; the windowed candidate lists are the whole-tile lists restricted to the window."""
+@pytest.mark.parametrize("delta,delta_fit,ts,ov", [(1, 3, 100, 30), (2, 1, 100, 30), (1, 0, 100, 30),
+                                                   (1, 3, 200, 13)])  # T = 226: Rader passes, blocked spectra
+def test_hyb_window_equals_whole_tiles(dev, small_hyb, delta, delta_fit, ts, ov):
+    """Deconvolution, blur and maxima restricted to the margins the keep filter can use give the same kept rows,
+    bit for bit, as whole tiles; the windowed candidate lists are the whole-tile lists restricted to the window."""
     from mermake_amd import _lib as L
     from mermake_amd.pipeline import HybPipeline
     raw, flat, psf = small_hyb
     hyb = dict(HYB, delta=delta, delta_fit=delta_fit, threshold=2000)
-    win = HybPipeline.from_settings(psf, raw.shape, tile_size=100, overlap=30, beta=1e-4, **hyb)
+    win = HybPipeline.from_settings(psf, raw.shape, tile_size=ts, overlap=ov, beta=1e-4, **hyb)
     assert win.windows() is not None
     full = HybPipeline(win.d, window=False, **hyb)
     a = win(T(raw, dev), T(flat, dev)).cpu().numpy()
