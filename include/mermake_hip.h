@@ -100,15 +100,18 @@ int mm_deconv_dims(const mm_deconv* d, int* T, int* Zp, int* H);
  * spectrum workspace: blk = 0 is natural (x * H + ky); blk > 0 is
  * column-blocked, (ky / blk) * T * blk + x * blk + ky % blk with Hs = H rounded
  * up to blk columns (zero padding), which makes every x transform group one
- * contiguous run.  A plane holds T * Hs complex64.  No counterpart in the
- * reference (its spectra are cupy arrays). */
-int mm_deconv_spec_layout(const mm_deconv* d, int* blk, int* Hs);
+ * contiguous run.  A plane holds T * Hs complex64; W holds wplanes of them.
+ * No counterpart in the reference (its spectra are cupy arrays). */
+int mm_deconv_spec_layout(const mm_deconv* d, int* blk, int* Hs, int* wplanes);
 size_t mm_deconv_wiener_ws_bytes(const mm_deconv* d);
 /* W = conj(H)/(H conj(H) + beta |L|^2) / (Zp*T*T) as Zp planes of complex64
- * half spectra in the mm_deconv_spec_layout layout (for stacks of up to 48
- * planes, the z-domain kernels ifft_z(W) instead), from one centred PSF
+ * half spectra in the mm_deconv_spec_layout layout, from one centred PSF
  * (sz, tile_size, tile_size) float32 (deconvolver.py:60-74; the PSF FFT runs
- * in float64). */
+ * in float64).  For stacks of up to 48 planes it holds the z-domain kernels
+ * w = ifft_z(W) instead, and with zpad = sz - 1 (sz <= 40) the 240 planes of
+ * the length-80 z-pass operator built from w (A = DFT80 of w on |m| < sz, G =
+ * DFT80 of w[n] + w[n - 2(sz - 1)] on n < 2 sz - 1, both / 80; w[t] and
+ * w[t - sz + 1] for t < 40). */
 int mm_deconv_build_wiener(mm_deconv* d, const float* psf_centered, double beta, void* W_out, void* ws,
                            size_t ws_bytes, void* stream);
 /* bytes of the per-tile half-spectrum workspace (sz planes of T*Hs complex64) */

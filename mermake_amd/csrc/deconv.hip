@@ -23,6 +23,7 @@
 //   5. k_tile_c2r      C2R along y (two rows per complex transform) -> res (sz,T,T).
 // The 1/(Zp*T*T) normalisation of ifftn is folded into W.
 #include "deconv.h"
+#include "zfft80.h"
 #include "fft_pf.h"
 #include "fft_plan.h"
 #include "mermake_hip.h"
@@ -1127,6 +1128,165 @@ __global__ __launch_bounds__(640, 5) void k_tile_zconv(float2* S, long long s_ts
   }
 }
 
+// ------------------------------------------------------- z pass (FFT-80)
+// Reflection padding with zpad = sz - 1 (the hyb setting), sz <= 40: the
+// z-kernel matrix of a column is a Toeplitz part (the data copy) plus a Hankel
+// part (the two reflections),
+//   K[t][s] = w[t - s] + g[t + s] - [s = 0] w[t] - [s = sz - 1] w[t - sz + 1],
+//   g[n] = w[n] + w[n - 2 (sz - 1)]   (indices of w mod Zp),
+// and both are circular convolutions of length 80 without aliasing:
+//   y = IDFT80(A X + G X~)[t] - c0[t] x[0] - c1[t] x[sz - 1],  t < sz,
+// X = DFT80(x zero padded), X~[k] = X[-k], A = DFT80(a) / 80 with
+// a[m mod 80] = w[m] (|m| < sz), G = DFT80(g) / 80, c0[t] = w[t],
+// c1[t] = w[t - sz + 1].  A column's operator is ZF_OPS planes (A, G, c0, c1),
+// built in float64 with W; a thread transforms one column of one tile in
+// registers, reading the operator from LDS.
+#define ZF_G 32      // columns per block
+#define ZF_OPS 240   // operator planes: A[80], G[80], c0[40], c1[40]
+
+template <int SZ>  // 0: depth sz at run time
+__global__ __launch_bounds__(256, SZ ? 2 : 1) void k_tile_zfft(float2* S, long long s_tstride, int TH, int sz_rt,
+                                                   const float2* __restrict__ Ops, long long ops_stride,
+                                                   const mm_tile_dev* __restrict__ tiles, int ntiles, int tpb) {
+  const int sz = SZ ? SZ : sz_rt;
+  extern __shared__ __align__(16) float4 zsm[];
+  float4* AG = zsm;              // [80][ZF_G]: (A[k], G[k])
+  float4* CC = zsm + 80 * ZF_G;  // [40][ZF_G]: (c0[t], c1[t])
+  const int ncb = (TH + ZF_G - 1) / ZF_G;
+  const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int cb = bid % ncb, tg = bid / ncb;
+  const int c0 = cb * ZF_G;
+  const int gmax = min(ZF_G, TH - c0);
+  const int t0 = tg * tpb, t1 = min(ntiles, t0 + tpb);
+  const float2* Op = Ops + (long long)tiles[t0].psf * ops_stride + c0;  // a group shares one PSF
+  for (int i = threadIdx.x; i < 120 * ZF_G; i += blockDim.x) {
+    const int q = i / ZF_G, g = i - (i / ZF_G) * ZF_G;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (g < gmax) {
+      const int pa = q < 80 ? q : 80 + q;        // A[k] | c0[t]
+      const int pb = q < 80 ? 80 + q : 120 + q;  // G[k] | c1[t]
+      const float2 a = Op[(long long)pa * TH + g], b = Op[(long long)pb * TH + g];
+      v = make_float4(a.x, a.y, b.x, b.y);
+    }
+    zsm[i] = v;
+  }
+  __syncthreads();
+  const int ntask = ZF_G * (t1 - t0);
+  for (int task = threadIdx.x; task < ntask; task += blockDim.x) {
+    const int g = task % ZF_G, j = t0 + task / ZF_G;
+    if (g >= gmax) continue;
+    float2* col = S + (long long)j * s_tstride + c0 + g;
+    float2 x[40];
+    {  // a running pointer: no per-plane offsets held in scalar registers
+      const float2* p = col;
+#pragma unroll
+      for (int s = 0; s < 40; ++s) {
+        x[s] = s < sz ? *p : make_float2(0.f, 0.f);
+        p += TH;
+        asm volatile("" : "+v"(p));
+      }
+    }
+    const float2 xf = x[0];
+    float2 xl = x[SZ ? SZ - 1 : 0];  // x[sz - 1]
+    if (!SZ) {
+#pragma unroll
+      for (int s = 1; s < 40; ++s) xl = s == sz - 1 ? x[s] : xl;
+    }
+    float2 U[5][16];
+    zf_fwd_stage1(x, U);
+    auto AGk = [&](int k) { return AG[k * ZF_G + g]; };
+    zf_middle<0>(U, AGk);
+    zf_middle<1>(U, AGk);
+    zf_middle<2>(U, AGk);
+    float2 y[40];
+    zf_inv_stage3(U, y);
+    // the store addresses are formed here, not kept in registers from the loads
+    asm volatile("" : "+v"(col));
+#pragma unroll
+    for (int t = 0; t < 40; ++t) {
+      if (t < sz) {
+        const float4 c = CC[t * ZF_G + g];
+        *col = make_float2(y[t].x - (c.x * xf.x - c.y * xf.y) - (c.z * xl.x - c.w * xl.y),
+                           y[t].y - (c.x * xf.y + c.y * xf.x) - (c.z * xl.y + c.w * xl.x));
+      }
+      col += TH;
+      asm volatile("" : "+v"(col));
+    }
+  }
+}
+
+static int launch_zfft(mm_deconv* d, float2* S, long long s_ts, const SpecLayout& Ls, const float2* Ops,
+                       const mm_tile_dev* td, int ntiles, int tpb, hipStream_t st) {
+  const int TH = (int)Ls.plane;
+  const int ngroups = (ntiles + tpb - 1) / tpb;
+  const long long nblk = (long long)((TH + ZF_G - 1) / ZF_G) * ngroups;
+  if (nblk > 0x7fffffffLL) { mm_set_error("z pass: too many blocks"); return MM_EINVAL; }
+  const size_t lds = sizeof(float4) * 120 * ZF_G;
+  if (d->sz == 40)
+    hipLaunchKernelGGL(k_tile_zfft<40>, dim3((unsigned)nblk), dim3(256), lds, st, S, s_ts, TH, d->sz, Ops,
+                       (long long)ZF_OPS * TH, td, ntiles, tpb);
+  else
+    hipLaunchKernelGGL(k_tile_zfft<0>, dim3((unsigned)nblk), dim3(256), lds, st, S, s_ts, TH, d->sz, Ops,
+                       (long long)ZF_OPS * TH, td, ntiles, tpb);
+  MM_LAUNCH_CHECK();
+  return MM_OK;
+}
+
+// Operator planes of the FFT-80 z pass from the z-domain kernels w (Zp planes
+// of T x H natural-layout columns, float64), written in layout Ls.
+__global__ void k_zfft_ops(const double2* __restrict__ w, int Zp, int sz, SpecLayout Ls, float2* __restrict__ out) {
+  __shared__ double2 tw[80];  // exp(-2 pi i m / 80)
+  for (int m = threadIdx.x; m < 80; m += blockDim.x) {
+    double sn, cs;
+    sincospi(-2.0 * m / 80.0, &sn, &cs);
+    tw[m] = make_double2(cs, sn);
+  }
+  __syncthreads();
+  const long long ncol = (long long)Ls.T * Ls.H;
+  const long long n = Ls.plane * ZF_OPS;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const int q = (int)(i / Ls.plane);
+    const int o = (int)(i - (long long)q * Ls.plane);  // column in layout Ls
+    int x = o / Ls.H, ky = o - x * Ls.H;
+    if (Ls.blk) {
+      const int bw = Ls.T * Ls.blk, kb = o / bw;
+      x = (o - kb * bw) / Ls.blk;
+      ky = kb * Ls.blk + (o - kb * bw - x * Ls.blk);
+    }
+    if (ky >= Ls.H) {  // padding of the last column block
+      out[i] = make_float2(0.f, 0.f);
+      continue;
+    }
+    const long long c = (long long)x * Ls.H + ky;  // natural column
+    auto W = [&](int m) { m %= Zp; if (m < 0) m += Zp; return w[(long long)m * ncol + c]; };
+    double re = 0, im = 0;
+    if (q < 160) {  // A[k] or G[k]
+      const int k = q % 80;
+      const bool gpart = q >= 80;
+      const int m0 = gpart ? 0 : -(sz - 1), m1 = gpart ? 2 * sz - 2 : sz - 1;
+      for (int m = m0; m <= m1; ++m) {
+        double2 v = W(m);
+        if (gpart) {
+          const double2 u = W(m - 2 * (sz - 1));
+          v.x += u.x;
+          v.y += u.y;
+        }
+        const double2 e = tw[(((m * k) % 80) + 80) % 80];
+        re += v.x * e.x - v.y * e.y;
+        im += v.x * e.y + v.y * e.x;
+      }
+      re /= 80.0;
+      im /= 80.0;
+    } else {  // c0[t] = w[t], c1[t] = w[t - sz + 1]
+      const int t = q < 200 ? q - 160 : q - 200;
+      const double2 v = t < sz ? W(q < 200 ? t : t - sz + 1) : make_double2(0.0, 0.0);
+      re = v.x;
+      im = v.y;
+    }
+    out[i] = make_float2((float)re, (float)im);
+  }
+}
+
 // S and W in the same in-plane layout Ls (planes of TH columns).
 static int launch_zconv(mm_deconv* d, float2* S, long long s_ts, const SpecLayout& Ls, const float2* Wz,
                         const mm_tile_dev* td, int ntiles, int tpb, hipStream_t st) {
@@ -1270,6 +1430,7 @@ extern "C" int mm_deconv_create(int sz, int tile_size, int overlap, int zpad, mm
       return MM_ENOMEM;
     }
   }
+  d->zfft = d->zconv && zpad == sz - 1 && sz <= 40;
   // Rader passes with the z-kernel pass: spectra column-blocked by the x-pass group
   // width, so that every x transform group is one contiguous run
   d->blk = d->rd && d->zconv ? d->pfG : 0;
@@ -1343,7 +1504,10 @@ extern "C" int mm_deconv_build_wiener(mm_deconv* d, const float* psf_centered, d
   MM_LAUNCH_CHECK();
   if (d->zconv)  // z-domain kernels w = unnormalised inverse z-DFT of W (the 1/Zp is already in W)
     if ((rc = launch_c2c<true>(Hs, (long long)T * H, 1, T * H, 0, 1, d->pZ64, d->gZ64, 1.0, st))) return rc;
-  if (d->blk) {
+  if (d->zfft) {
+    const SpecLayout Ls{T, H, d->blk, (long long)T * d->Hs};
+    hipLaunchKernelGGL(k_zfft_ops, dim3(4096), dim3(256), 0, st, Hs, Zp, d->sz, Ls, (float2*)W_out);
+  } else if (d->blk) {
     const SpecLayout Ls{T, H, d->blk, (long long)T * d->Hs};
     hipLaunchKernelGGL(k_d2f_blocked, dim3(4096), dim3(256), 0, st, Hs, (float2*)W_out, Zp, Ls);
   } else {
@@ -1353,10 +1517,11 @@ extern "C" int mm_deconv_build_wiener(mm_deconv* d, const float* psf_centered, d
   return MM_OK;
 }
 
-extern "C" int mm_deconv_spec_layout(const mm_deconv* d, int* blk, int* Hs) {
+extern "C" int mm_deconv_spec_layout(const mm_deconv* d, int* blk, int* Hs, int* wplanes) {
   if (!d) return MM_EINVAL;
   if (blk) *blk = d->blk;
   if (Hs) *Hs = d->Hs;
+  if (wplanes) *wplanes = d->zfft ? ZF_OPS : d->Zp;
   return MM_OK;
 }
 
@@ -1419,7 +1584,9 @@ static int deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles,
   if (rc) return rc;
   MM_MARK(2);
   const int tpb = tiles_per_block < 1 ? 1 : (tiles_per_block > ntiles ? ntiles : tiles_per_block);
-  if (d->zconv) {  // 3. z pass as the folded z-kernel matrix
+  if (d->zfft) {  // 3. z pass by length-80 DFTs of each column
+    if ((rc = launch_zfft(d, S, s_ts, Ls, (const float2*)W, td, ntiles, tpb, st))) return rc;
+  } else if (d->zconv) {  // 3. z pass as the folded z-kernel matrix
     if ((rc = launch_zconv(d, S, s_ts, Ls, (const float2*)W, td, ntiles, tpb, st))) return rc;
   } else {  // 3. z pass with z FFTs and the Wiener multiply
     const int G = d->gZ;

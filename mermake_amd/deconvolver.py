@@ -8,8 +8,9 @@
   already divided by ``Zp*T*T`` (the ``ifftn`` normalisation), instead of the
   full ``(n_psf, Zp, T, T)`` spectra.  The PSF FFT behind it runs in float64.
   Stacks of up to 48 planes hold the z-domain kernels ``ifft_z`` of those
-  spectra, and tiles T = 113*M hold them column-blocked,
-  ``(n_psf, Zp, Hs/blk, T, blk)`` (``mm_deconv_spec_layout``).
+  spectra (with ``zpad = sz - 1``, sz <= 40: the 240 planes of the length-80
+  z-pass operator), and tiles T = 113*M hold them column-blocked,
+  ``(n_psf, planes, Hs/blk, T, blk)`` (``mm_deconv_spec_layout``).
 * ``tile_wise`` computes tiles in batches; the yielded views stay valid until
   the generator computes the next batch (the reference overwrote them at every
   step).  ``tile_pad`` / ``tile_res`` are shape-only (``meta``) tensors.
@@ -62,17 +63,17 @@ class _Plan:
         T, Zp, H = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         L.call("mm_deconv_dims", h, ctypes.byref(T), ctypes.byref(Zp), ctypes.byref(H))
         self.T, self.Zp, self.H = T.value, Zp.value, H.value
-        blk, Hs = ctypes.c_int(), ctypes.c_int()
-        L.call("mm_deconv_spec_layout", h, ctypes.byref(blk), ctypes.byref(Hs))
-        self.blk, self.Hs = blk.value, Hs.value
+        blk, Hs, wp = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        L.call("mm_deconv_spec_layout", h, ctypes.byref(blk), ctypes.byref(Hs), ctypes.byref(wp))
+        self.blk, self.Hs, self.wplanes = blk.value, Hs.value, wp.value
         self.spec_bytes = L.lib().mm_deconv_spec_bytes(h)
 
     def spectrum_shape(self, n):
-        """Shape of ``n`` Wiener operators in the plan's layout: (n, Zp, T, H) natural, or
-        (n, Zp, Hs // blk, T, blk) column-blocked."""
+        """Shape of ``n`` Wiener operators in the plan's layout: (n, planes, T, H) natural, or
+        (n, planes, Hs // blk, T, blk) column-blocked (planes: Zp, or 240 for the length-80 z pass)."""
         if self.blk:
-            return (n, self.Zp, self.Hs // self.blk, self.T, self.blk)
-        return (n, self.Zp, self.T, self.H)
+            return (n, self.wplanes, self.Hs // self.blk, self.T, self.blk)
+        return (n, self.wplanes, self.T, self.H)
 
     def __del__(self):
         try:
