@@ -271,6 +271,9 @@ def _rel_l2(a, b):
     ((3, 300, 300), 200, 13, 2, 1e-3, True, None),  # T = 226 = 113 * 2 (Rader, radix-2 final stage)
     ((3, 340, 340), 301, 19, 2, 1e-3, False, 5),    # T = 339 = 113 * 3
     ((2, 500, 500), 400, 26, 2, 1e-3, True, None),  # T = 452 = 113 * 4
+    ((40, 120, 120), 80, 10, 39, 1e-3, True, None),  # 40 planes, zpad = sz - 1: the length-80 z pass
+    ((40, 260, 260), 200, 13, 39, 1e-3, False, 15),  # ... with T = 226 (Rader, blocked spectra)
+    ((17, 100, 100), 60, 8, 16, 1e-3, True, None),   # length-80 z pass with a run-time depth
 ])
 def test_deconv_vs_oracle(dev, shape, ts, ov, zpad, beta, flat, blur):
     from mermake_amd.deconvolver import Deconvolver
