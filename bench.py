@@ -42,19 +42,35 @@ def m1_bytes_per_stack(geom, sz, T, Zp, flat=True):
     return tot
 
 
-def pass_bytes(geom, sz, T, Zp, single_psf=True, flat=True):
-    """Compulsory HBM bytes of each pass of mm_deconv_tiles for one batch (the kernels' algorithmic traffic)."""
+def pass_bytes(geom, sz, T, wplanes, window=(None, None), single_psf=True, flat=True):
+    """Compulsory HBM bytes of each pass of mm_deconv_tiles for one batch (the kernels' algorithmic traffic).
+
+    wplanes: planes of the z-pass operator (240 for the length-80 z pass); window: the rows and columns
+    [lo, hi) of the deconvolved tiles that the C2R produces (None: whole tiles)."""
     H = T // 2 + 1
     n = len(geom)
     spec = 8 * sz * T * H
     inp = sum(2 * sz * g["xd"] * g["yd"] + (4 * g["xd"] * g["yd"] if flat else 0) for g in geom)
-    W = 8 * Zp * T * H * (1 if single_psf else n)
+    W = 8 * wplanes * T * H * (1 if single_psf else n)
+    lo, hi = window
+    rows = T if lo is None else hi - lo
     return {"r2c": inp + n * spec, "x_fwd": 2 * n * spec, "z_wiener": 2 * n * spec + W, "x_inv": 2 * n * spec,
-            "c2r": n * spec + n * 4 * sz * T * T}
+            "c2r": n * spec * rows // T + n * 4 * sz * rows * rows}
 
 
-PASS_KERNEL = {"r2c": "k_pf_r2c", "x_fwd": "k_pf_c2c", "z_wiener": "k_tile_zconv", "x_inv": "k_pf_c2c",
-               "c2r": "k_pf_c2r"}
+def deconv_window(pipe, T):
+    """Rows and columns [lo, hi) of the deconvolved tiles the hyb pipeline computes (the blur window grown by the
+    blur half width, in whole C2R row pairs), or (None, None)."""
+    win = pipe.windows()
+    if win is None:
+        return None, None
+    d = pipe.blur_radius // 2
+    lo, hi = max(0, win[0][0] - d) & ~1, min(T, (win[0][1] + d + 1) & ~1)
+    return lo, hi
+
+
+PASS_KERNEL = {"r2c": "k_rd_r2c", "x_fwd": "k_rd_c2c<false", "z_wiener": "k_tile_zfft", "x_inv": "k_rd_c2c<true",
+               "c2r": "k_rd_c2r"}
 
 
 def measured_traffic(pass_name):
@@ -207,7 +223,7 @@ def main():
     L.call("mm_deconv_set_profiling", d._plan.h, 0)
     names = ["r2c", "x_fwd", "z_wiener", "x_inv", "c2r"]
     pt = dict(zip(names, [float(v) for v in passes]))
-    pb = pass_bytes(geom, cfg["sz"], T, Zp)
+    pb = pass_bytes(geom, cfg["sz"], T, d._plan.wplanes, deconv_window(pipe, T))
     ev[0].record(st)
     pipe(stacks[0], flat)
     ev[1].record(st)

@@ -1233,32 +1233,30 @@ static int launch_zfft(mm_deconv* d, float2* S, long long s_ts, const SpecLayout
 }
 
 // Operator planes of the FFT-80 z pass from the z-domain kernels w (Zp planes
-// of T x H natural-layout columns, float64), written in layout Ls.
-__global__ void k_zfft_ops(const double2* __restrict__ w, int Zp, int sz, SpecLayout Ls, float2* __restrict__ out) {
-  __shared__ double2 tw[80];  // exp(-2 pi i m / 80)
+// of T x H natural-layout columns, float64), written in layout Ls.  A block
+// takes ZF_G natural columns, stages their w in LDS and forms all ZF_OPS
+// planes of them.  The padding columns of a blocked layout are not written.
+__global__ __launch_bounds__(256) void k_zfft_ops(const double2* __restrict__ w, int Zp, int sz, SpecLayout Ls,
+                                                  float2* __restrict__ out) {
+  extern __shared__ __align__(16) double2 zw[];  // [Zp][ZF_G]
+  __shared__ double2 tw[80];                      // exp(-2 pi i m / 80)
   for (int m = threadIdx.x; m < 80; m += blockDim.x) {
     double sn, cs;
     sincospi(-2.0 * m / 80.0, &sn, &cs);
     tw[m] = make_double2(cs, sn);
   }
-  __syncthreads();
   const long long ncol = (long long)Ls.T * Ls.H;
-  const long long n = Ls.plane * ZF_OPS;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    const int q = (int)(i / Ls.plane);
-    const int o = (int)(i - (long long)q * Ls.plane);  // column in layout Ls
-    int x = o / Ls.H, ky = o - x * Ls.H;
-    if (Ls.blk) {
-      const int bw = Ls.T * Ls.blk, kb = o / bw;
-      x = (o - kb * bw) / Ls.blk;
-      ky = kb * Ls.blk + (o - kb * bw - x * Ls.blk);
-    }
-    if (ky >= Ls.H) {  // padding of the last column block
-      out[i] = make_float2(0.f, 0.f);
-      continue;
-    }
-    const long long c = (long long)x * Ls.H + ky;  // natural column
-    auto W = [&](int m) { m %= Zp; if (m < 0) m += Zp; return w[(long long)m * ncol + c]; };
+  const long long c0 = (long long)blockIdx.x * ZF_G;
+  const int gmax = (int)min((long long)ZF_G, ncol - c0);
+  for (int i = threadIdx.x; i < Zp * ZF_G; i += blockDim.x) {
+    const int p = i / ZF_G, g = i - (i / ZF_G) * ZF_G;
+    zw[i] = g < gmax ? w[(long long)p * ncol + c0 + g] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < ZF_OPS * ZF_G; i += blockDim.x) {
+    const int q = i / ZF_G, g = i - (i / ZF_G) * ZF_G;
+    if (g >= gmax) continue;
+    auto W = [&](int m) { m %= Zp; if (m < 0) m += Zp; return zw[m * ZF_G + g]; };
     double re = 0, im = 0;
     if (q < 160) {  // A[k] or G[k]
       const int k = q % 80;
@@ -1283,7 +1281,10 @@ __global__ void k_zfft_ops(const double2* __restrict__ w, int Zp, int sz, SpecLa
       re = v.x;
       im = v.y;
     }
-    out[i] = make_float2((float)re, (float)im);
+    const long long c = c0 + g;
+    const int x = (int)(c / Ls.H), ky = (int)(c - (long long)x * Ls.H);
+    const long long o = Ls.blk ? (long long)(ky / Ls.blk) * Ls.T * Ls.blk + (long long)x * Ls.blk + ky % Ls.blk : c;
+    out[(long long)q * Ls.plane + o] = make_float2((float)re, (float)im);
   }
 }
 
@@ -1506,7 +1507,10 @@ extern "C" int mm_deconv_build_wiener(mm_deconv* d, const float* psf_centered, d
     if ((rc = launch_c2c<true>(Hs, (long long)T * H, 1, T * H, 0, 1, d->pZ64, d->gZ64, 1.0, st))) return rc;
   if (d->zfft) {
     const SpecLayout Ls{T, H, d->blk, (long long)T * d->Hs};
-    hipLaunchKernelGGL(k_zfft_ops, dim3(4096), dim3(256), 0, st, Hs, Zp, d->sz, Ls, (float2*)W_out);
+    if (d->blk) MM_CHECK(hipMemsetAsync(W_out, 0, sizeof(float2) * ZF_OPS * Ls.plane, st));  // padding columns
+    const long long ncol = (long long)T * H;
+    hipLaunchKernelGGL(k_zfft_ops, dim3((unsigned)((ncol + ZF_G - 1) / ZF_G)), dim3(256),
+                       sizeof(double2) * Zp * ZF_G, st, Hs, Zp, d->sz, Ls, (float2*)W_out);
   } else if (d->blk) {
     const SpecLayout Ls{T, H, d->blk, (long long)T * d->Hs};
     hipLaunchKernelGGL(k_d2f_blocked, dim3(4096), dim3(256), 0, st, Hs, (float2*)W_out, Zp, Ls);
