@@ -711,6 +711,12 @@ struct RdR2cRaw {
   static_assert(NR <= 32 && NC <= 32, "validity masks are 32-bit");
 };
 
+__device__ __forceinline__ long long rd_readlane64(long long v, int lane) {
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)(unsigned long long)v, lane);
+  const unsigned hi = __builtin_amdgcn_readlane((unsigned)((unsigned long long)v >> 32), lane);
+  return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
 template <int M, int NR, int NC, bool F32, bool FLAT>
 __device__ __forceinline__ void rd_r2c_load(RdR2cRaw<NR, NC>& R, const mm_tile_dev& td, const RdCols<NC>& C, int T,
                                             int nrows, int q0) {
@@ -722,15 +728,29 @@ __device__ __forceinline__ void rd_r2c_load(RdR2cRaw<NR, NC>& R, const mm_tile_d
     R.colok |= (cc >= 0 ? 1u : 0u) << k;
     c[k] = cc < 0 ? 0u : (unsigned)cc;
   }
-  int z = q0 / T, x = q0 - z * T;
-  R.rowok = 0;
+  // Row i of the group is worked out by lane i (vector work in parallel instead
+  // of NR rounds of scalar 64-bit arithmetic), then broadcast with readlane.
+  const int lane = threadIdx.x & 63;
+  long long ro_l = 0, fo_l = 0;  // absent rows read the stack's first row
+  bool ok_l = false;
+  if (lane < NR) {
+    const int z0 = q0 / T;
+    int z = z0, x = q0 - z0 * T + lane;
+    if (x >= T) {  // NR < T: at most one plane boundary in a group
+      x -= T;
+      ++z;
+    }
+    const int rr = q0 + lane < nrows ? pad_map(x, td.xs, td.xd, td.xfirst, T) : -1;
+    ok_l = rr >= 0;
+    if (ok_l) {
+      ro_l = (long long)z * td.img_sz + (long long)(td.X0 + rr) * td.img_sx + td.Y0;
+      fo_l = (long long)(td.X0 + rr) * td.flat_sx + td.Y0;
+    }
+  }
+  R.rowok = (unsigned)__ballot(ok_l);  // lanes 0..NR-1 (NR <= 32)
 #pragma unroll
   for (int i = 0; i < NR; ++i) {
-    const int rr = q0 + i < nrows ? pad_map(x, td.xs, td.xd, td.xfirst, T) : -1;
-    R.rowok |= (rr >= 0 ? 1u : 0u) << i;
-    // absent rows read the stack's first row
-    const long long ro = rr >= 0 ? (long long)z * td.img_sz + (long long)(td.X0 + rr) * td.img_sx + td.Y0 : 0;
-    const long long fo = rr >= 0 ? (long long)(td.X0 + rr) * td.flat_sx + td.Y0 : 0;
+    const long long ro = rd_readlane64(ro_l, i), fo = rd_readlane64(fo_l, i);
     // uniform row pointers in the global address space: global loads with a
     // scalar base and a 32-bit lane offset
     const gptr<const char> rb = (gptr<const char>)td.img + ro * (F32 ? 4 : 2);
@@ -739,10 +759,6 @@ __device__ __forceinline__ void rd_r2c_load(RdR2cRaw<NR, NC>& R, const mm_tile_d
     for (int k = 0; k < NC; ++k) {
       R.P[i][k] = F32 ? *(gptr<const unsigned>)(rb + c[k] * 4u) : (unsigned)*(gptr<const uint16_t>)(rb + c[k] * 2u);
       if (FLAT) R.F[i][k] = *(gptr<const float>)(fb + c[k] * 4u);
-    }
-    if (++x == T) {
-      x = 0;
-      ++z;
     }
   }
 }
@@ -815,7 +831,13 @@ __global__ __launch_bounds__(256, 2) void k_rd_r2c(const mm_tile_dev* __restrict
       pa = q < nrows ? out + za * Ls.plane + Ls.row(xa) : nullptr;
       pb = q + 1 < nrows ? out + zb * Ls.plane + Ls.row(xb) : nullptr;
     };
-    if (Ls.blk) {
+    if (Ls.blk == 6) {  // the M = 6 plan: constant block width, the division is a multiply-shift
+      const int bstr = T * 6;
+      pf_final_r2c<M>(PfShape(N, RD_P), (const float2*)L.bufB, L.tw, G, rows, [&](int n) {
+        const int b = (int)(((unsigned)n * 43691u) >> 18);  // n / 6 for n < 2^15
+        return b * bstr + (n - 6 * b);
+      });
+    } else if (Ls.blk) {
       const FastDiv fb(Ls.blk);
       const int bstr = T * Ls.blk;
       pf_final_r2c<M>(PfShape(N, RD_P), (const float2*)L.bufB, L.tw, G, rows, [&](int n) {
