@@ -872,16 +872,30 @@ __global__ __launch_bounds__(256, 2) void k_rd_c2r(const float2* Sin, long long 
   const int nrb = (nrows + NR - 1) / NR;
   const long long ngrp = (long long)nrb * ntiles;
   float2 RA[G][NC], RB[G][NC];  // bins kk of the two rows of each transform
+  // row q of the window is (z, x) = (q / nx, xlo + q % nx); a group's rows cross
+  // at most one plane boundary (NR < nx), so one division per group
+  auto rowzx = [&](int q0, int z0, int i, int& z, int& x) {
+    x = q0 - z0 * nx + i;
+    z = z0;
+    if (x >= nx) {
+      x -= nx;
+      ++z;
+    }
+    x += xlo;
+  };
   auto fetch = [&](long long gi) {  // absent rows read the last row
     const int tile = (int)(gi / nrb);
     const int q0 = (int)(gi - (long long)tile * nrb) * NR;
+    const int z0 = q0 / nx;
     const float2* in = Sin + (long long)tile * s_tstride;
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-      const int qa = min(q0 + 2 * g, nrows - 1), qb = min(q0 + 2 * g + 1, nrows - 1);
-      const int za = qa / nx, zb = qb / nx;
-      const float2* ra = in + za * Ls.plane + Ls.row(xlo + qa - za * nx);
-      const float2* rb = in + zb * Ls.plane + Ls.row(xlo + qb - zb * nx);
+      const int ia = min(2 * g, nrows - 1 - q0), ib = min(2 * g + 1, nrows - 1 - q0);
+      int za, xa, zb, xb;
+      rowzx(q0, z0, ia, za, xa);
+      rowzx(q0, z0, ib, zb, xb);
+      const float2* ra = in + za * Ls.plane + Ls.row(xa);
+      const float2* rb = in + zb * Ls.plane + Ls.row(xb);
 #pragma unroll
       for (int k = 0; k < NC; ++k) {
         RA[g][k] = ra[kk[k]];
@@ -912,11 +926,14 @@ __global__ __launch_bounds__(256, 2) void k_rd_c2r(const float2* Sin, long long 
     if (grp + gridDim.x < ngrp) fetch(grp + gridDim.x);
     rd_stages<M, false, false>(L, G, RdMid{});
     float* out = res + (long long)tile * r_tstride;
+    const int z0 = q0 / nx;
     pf_final<M, true>(PfShape(N, RD_P), (const float2*)L.bufB, L.tw, G, [&](int g, int j, const float2* v) {
       const int q = q0 + 2 * g;
-      const int za = q / nx, zb = (q + 1) / nx;
-      float* oa = out + ((long long)za * N + xlo + q - za * nx) * N + j;
-      float* ob = out + ((long long)zb * N + xlo + q + 1 - zb * nx) * N + j;
+      int za, xa, zb, xb;
+      rowzx(q0, z0, 2 * g, za, xa);
+      rowzx(q0, z0, 2 * g + 1, zb, xb);
+      float* oa = out + ((long long)za * N + xa) * N + j;
+      float* ob = out + ((long long)zb * N + xb) * N + j;
 #pragma unroll
       for (int r = 0; r < M; ++r) {
         const bool cin = j + r * RD_P >= clo && j + r * RD_P < chi;
@@ -1566,7 +1583,9 @@ static int deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles,
   // The C2R packs rows 2m and 2m + 1 (of z * T + x) into one complex transform,
   // and a row's rounding depends on its partner: the window keeps whole pairs
   // (T even: lo and hi even; T odd: pairs straddle planes, whole planes).
-  if (d->T & 1) {
+  // (the C2R also assumes a group of rows crosses at most one plane boundary:
+  // windows narrower than 64 rows take whole planes)
+  if ((d->T & 1) || hi - lo < 64) {
     lo = 0;
     hi = d->T;
   } else {
