@@ -717,9 +717,10 @@ __device__ __forceinline__ long long rd_readlane64(long long v, int lane) {
   return (long long)(((unsigned long long)hi << 32) | lo);
 }
 
+// Rows x0 .. x0 + NR - 1 of plane z (those < T exist).
 template <int M, int NR, int NC, bool F32, bool FLAT>
 __device__ __forceinline__ void rd_r2c_load(RdR2cRaw<NR, NC>& R, const mm_tile_dev& td, const RdCols<NC>& C, int T,
-                                            int nrows, int q0) {
+                                            int z, int x0) {
   unsigned c[NC];  // column offsets in elements (absent columns read data column 0 of the row)
   R.colok = 0;
 #pragma unroll
@@ -734,13 +735,8 @@ __device__ __forceinline__ void rd_r2c_load(RdR2cRaw<NR, NC>& R, const mm_tile_d
   long long ro_l = 0, fo_l = 0;  // absent rows read the stack's first row
   bool ok_l = false;
   if (lane < NR) {
-    const int z0 = q0 / T;
-    int z = z0, x = q0 - z0 * T + lane;
-    if (x >= T) {  // NR < T: at most one plane boundary in a group
-      x -= T;
-      ++z;
-    }
-    const int rr = q0 + lane < nrows ? pad_map(x, td.xs, td.xd, td.xfirst, T) : -1;
+    const int x = x0 + lane;
+    const int rr = x < T ? pad_map(x, td.xs, td.xd, td.xfirst, T) : -1;
     ok_l = rr >= 0;
     if (ok_l) {
       ro_l = (long long)z * td.img_sz + (long long)(td.X0 + rr) * td.img_sx + td.Y0;
@@ -788,23 +784,33 @@ __global__ __launch_bounds__(256, 2) void k_rd_r2c(const mm_tile_dev* __restrict
   const RdLds L = rd_carve(smem, N, M, G, tw, tab);
   __syncthreads();  // slot_in
   const RdCols<NC> C = rd_cols<M, NC>(L.slot_in);
-  const int nrb = (nrows + NR - 1) / NR;
-  const long long ngrp = (long long)nrb * ntiles;
+  // Group (tile, band of NR rows, plane) with the plane fastest: consecutive
+  // groups read the same flat-field rows, which then stay in L2.
+  const int sz = nrows / T, nband = (T + NR - 1) / NR;
+  const long long ngrp = (long long)sz * nband * ntiles;
+  auto where = [&](long long gi, int& tile, int& z, int& x0) {
+    tile = (int)(gi / ((long long)sz * nband));
+    const int rem = (int)(gi - (long long)tile * sz * nband);
+    const int band = rem / sz;
+    z = rem - band * sz;
+    x0 = band * NR;
+  };
   RdR2cRaw<NR, NC> R;
   auto fetch = [&](long long gi) {
-    const int tile = (int)(gi / nrb);
-    const int q0 = (int)(gi - (long long)tile * nrb) * NR;
+    int tile, z, x0;
+    where(gi, tile, z, x0);
     const mm_tile_dev& td = tiles[tile];
     if (td.img_f32) {
-      if (td.flat) rd_r2c_load<M, NR, NC, true, true>(R, td, C, T, nrows, q0);
-      else rd_r2c_load<M, NR, NC, true, false>(R, td, C, T, nrows, q0);
+      if (td.flat) rd_r2c_load<M, NR, NC, true, true>(R, td, C, T, z, x0);
+      else rd_r2c_load<M, NR, NC, true, false>(R, td, C, T, z, x0);
     } else {
-      if (td.flat) rd_r2c_load<M, NR, NC, false, true>(R, td, C, T, nrows, q0);
-      else rd_r2c_load<M, NR, NC, false, false>(R, td, C, T, nrows, q0);
+      if (td.flat) rd_r2c_load<M, NR, NC, false, true>(R, td, C, T, z, x0);
+      else rd_r2c_load<M, NR, NC, false, false>(R, td, C, T, z, x0);
     }
   };
+  auto tile_of = [&](long long gi) { return (int)(gi / ((long long)sz * nband)); };
   auto put = [&](long long gi) {
-    const mm_tile_dev& td = tiles[(int)(gi / nrb)];
+    const mm_tile_dev& td = tiles[tile_of(gi)];
     if (td.img_f32) {
       if (td.flat) rd_r2c_store<M, NR, NC, true, true>(L.bufA, R, C);
       else rd_r2c_store<M, NR, NC, true, false>(L.bufA, R, C);
@@ -821,15 +827,13 @@ __global__ __launch_bounds__(256, 2) void k_rd_r2c(const mm_tile_dev* __restrict
     __syncthreads();
     if (grp + gridDim.x < ngrp) fetch(grp + gridDim.x);
     rd_stages<M, false, false>(L, G, RdMid{});
-    const int tile = (int)(grp / nrb);
-    const int q0 = (int)(grp - (long long)tile * nrb) * NR;
-    float2* out = Sout + (long long)tile * s_tstride;
+    int tile, z, x0;
+    where(grp, tile, z, x0);
+    float2* out = Sout + (long long)tile * s_tstride + z * Ls.plane;
     auto rows = [&](int g, float2*& pa, float2*& pb) {
-      const int q = q0 + 2 * g;
-      const int za = q / T, xa = q - za * T;
-      const int zb = (q + 1) / T, xb = q + 1 - zb * T;
-      pa = q < nrows ? out + za * Ls.plane + Ls.row(xa) : nullptr;
-      pb = q + 1 < nrows ? out + zb * Ls.plane + Ls.row(xb) : nullptr;
+      const int xa = x0 + 2 * g;
+      pa = xa < T ? out + Ls.row(xa) : nullptr;
+      pb = xa + 1 < T ? out + Ls.row(xa + 1) : nullptr;
     };
     if (Ls.blk == 6) {  // the M = 6 plan: constant block width, the division is a multiply-shift
       const int bstr = T * 6;
@@ -997,7 +1001,7 @@ static void launch_rd_r2c(const mm_tile_dev* td, int ntiles, int T, int nrows, f
                           const SpecLayout& Ls, const RdTab& tab, const float2* tw, hipStream_t st) {
   constexpr int G = rd_row_g(M);
   const size_t lds = rd_lds_bytes(T, M, G, 0);
-  const long long ngrp = (long long)((nrows + 2 * G - 1) / (2 * G)) * ntiles;
+  const long long ngrp = (long long)(nrows / T) * ((T + 2 * G - 1) / (2 * G)) * ntiles;
   hipLaunchKernelGGL(k_rd_r2c<M>, dim3(pf_grid(ngrp, lds)), dim3(256), lds, st, td, ntiles, T, nrows, S, s_ts, Ls,
                      tab, tw);
 }
