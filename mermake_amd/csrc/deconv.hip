@@ -620,11 +620,12 @@ __global__ __launch_bounds__(256) void k_pf_c2r(const float2* Sin, long long s_t
 // ------------------------------------------------ Rader (p = 113) tile passes
 // The prime-first passes with the prime stage of fft_rader.h (T = 113 * M).
 // Same persistent structure: the next group's raw elements are prefetched
-// into registers during the current group's Rader and final stages.
+// into registers during the current group's Rader and final stages.  Only
+// outputs k in [klo, khi) are stored (the rows a windowed C2R reads).
 template <bool INV, int M>
 __global__ __launch_bounds__(256, 2) void k_rd_c2c(float2* data, long long estride, long long cstride, int ncols,
                                                 long long bstride, int nbatch, long long gstride, RdTab tab,
-                                                const float2* __restrict__ tw, int G) {
+                                                const float2* __restrict__ tw, int G, int klo, int khi) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int N = RD_P * M;
   const RdLds L = rd_carve(smem, N, M, G, tw, tab);
@@ -667,10 +668,12 @@ __global__ __launch_bounds__(256, 2) void k_rd_c2c(float2* data, long long estri
       if (g >= gmax) return;
       float2* col = base + (g * cs + j * es);
 #pragma unroll
-      for (int r = 0; r < M; ++r) col[r * RD_P * es] = v[r];
+      for (int r = 0; r < M; ++r)
+        if (j + r * RD_P >= klo && j + r * RD_P < khi) col[r * RD_P * es] = v[r];
     });
     base = nbase;
     gmax = ngmax;
+    if (RD_NBUF == 1) __syncthreads();  // the final stage's reads before the next group's stores
   }
 }
 
@@ -679,9 +682,8 @@ __global__ __launch_bounds__(256, 2) void k_rd_c2c(float2* data, long long estri
 // uniform row pointer plus a per-lane column offset fixed for the kernel.
 // Groups of G transforms (at most 36 row samples per thread; the 2G rows'
 // validity bits fit one 32-bit mask).  For M = 6 small groups (6 rows in the
-// R2C, 4 in the C2R) keep the LDS at 42 KB and 31 KB per block, so 3 and 4
-// blocks share a CU: about 7% and 8% less time than groups of 12 rows, whose
-// 77 KB allow only 2.
+// R2C, 4 in the C2R: 25 KB and 19 KB of LDS) let 4 blocks share a CU; they
+// measured 5 to 10% faster than groups of 12 rows.
 constexpr int rd_row_g(int M, bool c2r) { return M == 2 ? 16 : (M == 6 ? (c2r ? 2 : 3) : 9); }
 constexpr int rd_row_nc(int M) { return (RD_P * M + 255) / 256; }
 
@@ -782,6 +784,7 @@ __global__ __launch_bounds__(256, 2) void k_rd_r2c(const mm_tile_dev* __restrict
                                                 float2* Sout, long long s_tstride, SpecLayout Ls, RdTab tab,
                                                 const float2* __restrict__ tw) {
   constexpr int G = rd_row_g(M, false), NC = rd_row_nc(M), NR = 2 * G;
+  static_assert(RD_NBUF == 2 || G * M * 7 <= 256, "in-place Rader stages: one item per thread");
   extern __shared__ __align__(16) unsigned char smem[];
   const int N = RD_P * M;
   const RdLds L = rd_carve(smem, N, M, G, tw, tab);
@@ -855,6 +858,7 @@ __global__ __launch_bounds__(256, 2) void k_rd_r2c(const mm_tile_dev* __restrict
     } else {
       pf_final_r2c<M>(PfShape(N, RD_P), (const float2*)L.bufB, L.tw, G, rows);
     }
+    if (RD_NBUF == 1) __syncthreads();  // the final stage's reads before the next group's stores
   }
 }
 
@@ -865,6 +869,7 @@ __global__ __launch_bounds__(256, 2) void k_rd_c2r(const float2* Sin, long long 
                                                 long long r_tstride, int ntiles, int nrows, int xlo, int nx, int clo,
                                                 int chi, RdTab tab, const float2* __restrict__ tw) {
   constexpr int G = rd_row_g(M, true), NC = rd_row_nc(M), NR = 2 * G;
+  static_assert(RD_NBUF == 2 || G * M * 7 <= 256, "in-place Rader stages: one item per thread");
   extern __shared__ __align__(16) unsigned char smem[];
   const int N = RD_P * M, H = N / 2 + 1;
   const RdLds L = rd_carve(smem, N, M, G, tw, tab);
@@ -948,6 +953,7 @@ __global__ __launch_bounds__(256, 2) void k_rd_c2r(const float2* Sin, long long 
         if (cin && q + 1 < nrows) ob[r * RD_P] = v[r].y;
       }
     });
+    if (RD_NBUF == 1) __syncthreads();  // the final stage's reads before the next group's stores
   }
 }
 
@@ -987,14 +993,16 @@ static int launch_pf_c2c(const mm_deconv* d, float2* data, long long estride, lo
 template <bool INV>
 // x transforms of every (tile, z) plane of the spectra in layout Ls (the
 // groups of G columns are the layout's column blocks when it is blocked).
-static int launch_rd_c2c(const mm_deconv* d, float2* data, const SpecLayout& Ls, int nbatch, hipStream_t st) {
+static int launch_rd_c2c(const mm_deconv* d, float2* data, const SpecLayout& Ls, int nbatch, int klo, int khi,
+                         hipStream_t st) {
   const int G = d->pfG;
   const long long ngrp = (long long)((Ls.H + G - 1) / G) * nbatch;
   const size_t lds = rd_lds_bytes(d->T, d->pfM, G, 0);
   const long long es = Ls.blk ? Ls.blk : Ls.H;
   const long long gs = Ls.blk ? (long long)Ls.T * Ls.blk : G;
   PF_SWITCH(d->pfM, hipLaunchKernelGGL((k_rd_c2c<INV, MM>), dim3(pf_grid(ngrp, lds)), dim3(256), lds, st, data, es,
-                                       1LL, Ls.H, Ls.plane, nbatch, gs, INV ? d->rdI : d->rdF, d->pT.tw, G));
+                                       1LL, Ls.H, Ls.plane, nbatch, gs, INV ? d->rdI : d->rdF, d->pT.tw, G, klo,
+                                       khi));
   MM_LAUNCH_CHECK();
   return MM_OK;
 }
@@ -1445,7 +1453,7 @@ extern "C" int mm_deconv_create(int sz, int tile_size, int overlap, int zpad, mm
   d->rd = 0;
   if (d->pfM && d->pfS.p == RD_P) {
     const int M = d->pfM;
-    int G = 256 / (7 * M);
+    int G = 256 / (7 * M);  // G M 7 <= 256: the in-place stages take one item per thread
     while (G > 0 && (rd_lds_bytes(d->T, M, G, 0) > 81920 || G * M * RD_P > RD_NI * 256)) --G;
     if (G > 0 && (rc = mm_get_rader113(&d->rdF, &d->rdI)) == MM_OK) {
       d->rd = 1;
@@ -1630,7 +1638,7 @@ static int deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles,
   MM_MARK(1);
   int rc;
   // 2. C2C along x, all tiles and planes at once
-  if (d->rd) rc = launch_rd_c2c<false>(d, S, Ls, ntiles * sz, st);
+  if (d->rd) rc = launch_rd_c2c<false>(d, S, Ls, ntiles * sz, 0, T, st);
   else if (d->pfM) rc = launch_pf_c2c<false>(d, S, H, 1, H, (long long)T * H, ntiles * sz, st);
   else rc = launch_c2c<false>(S, H, 1, H, (long long)T * H, ntiles * sz, d->pT, d->gT_x, 1.0, st);
   if (rc) return rc;
@@ -1651,8 +1659,8 @@ static int deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles,
     MM_LAUNCH_CHECK();
   }
   MM_MARK(3);
-  // 4. inverse C2C along x
-  if (d->rd) rc = launch_rd_c2c<true>(d, S, Ls, ntiles * sz, st);
+  // 4. inverse C2C along x (the Rader form stores only the rows [lo, hi) the C2R reads)
+  if (d->rd) rc = launch_rd_c2c<true>(d, S, Ls, ntiles * sz, lo, hi, st);
   else if (d->pfM) rc = launch_pf_c2c<true>(d, S, H, 1, H, (long long)T * H, ntiles * sz, st);
   else rc = launch_c2c<true>(S, H, 1, H, (long long)T * H, ntiles * sz, d->pT, d->gT_x, 1.0, st);
   if (rc) return rc;

@@ -18,11 +18,11 @@
 // two radix-7 passes act on the same 7 elements, so they run back to back in
 // registers.  About 2.6 times fewer operations than the direct symmetric DFT.
 //
-// LDS per transform: two buffers of M * RD_S complex (slot 112 holds x_0,
-// slot 113 holds X_0), ping-ponged; the second one finally holds the
-// transforms X of the M sub-sequences, in natural order for the radix-M final
-// stage of fft_pf.h (row passes) or in Rader's output order for rd_final
-// (strided pass: no scattered LDS stores).
+// LDS per transform: M * RD_S complex (slot 112 holds x_0, slot 113 holds
+// X_0) in one buffer worked in place (or two ping-ponged, RD_NBUF = 2); the
+// last stage leaves the transforms X of the M sub-sequences at stride p, in
+// natural order for the radix-M final stage of fft_pf.h (row passes) or in
+// Rader's output order for rd_final (strided pass: no scattered LDS stores).
 #pragma once
 #include "fft_pf.h"
 
@@ -45,10 +45,17 @@ struct RdTab {
   const int* slot_in;
 };
 
-// LDS bytes: twiddles of N, ghat, perm_out, slot_in, two data buffers.
+// One data buffer per block, the stages in place (the third stage reads
+// everything before a barrier, then writes): half the LDS of ping-pong buffers,
+// so 3 (x passes) to 4 (row passes) blocks share a CU.  Needs G M 7 <= blockDim.x.
+#ifndef RD_NBUF
+#define RD_NBUF 1
+#endif
+
+// LDS bytes: twiddles of N, ghat, perm_out, slot_in, RD_NBUF data buffers.
 __host__ __device__ __forceinline__ size_t rd_lds_bytes(int N, int M, int G, size_t extra) {
   return mm_align16((size_t)N * 8) + (size_t)RD_L * 8 + mm_align16((size_t)(RD_L + RD_P) * 4) +
-         (size_t)2 * G * M * RD_S * 8 + extra;
+         (size_t)RD_NBUF * G * M * RD_S * 8 + extra;
 }
 
 struct RdLds {
@@ -74,8 +81,12 @@ __device__ __forceinline__ RdLds rd_carve(unsigned char* smem, int N, int M, int
   off += mm_align16((size_t)(RD_L + RD_P) * 4);
   L.bufA = (float2*)(smem + off);
   off += (size_t)G * M * RD_S * 8;
-  L.bufB = (float2*)(smem + off);
-  off += (size_t)G * M * RD_S * 8;
+  if (RD_NBUF == 2) {
+    L.bufB = (float2*)(smem + off);
+    off += (size_t)G * M * RD_S * 8;
+  } else {
+    L.bufB = L.bufA;
+  }
   L.extra = smem + off;
   for (int i = threadIdx.x; i < N; i += blockDim.x) L.tw[i] = tw[i];
   for (int i = threadIdx.x; i < RD_L; i += blockDim.x) {
@@ -242,6 +253,34 @@ __device__ __forceinline__ void rd_stages(const RdLds& L, int G, const RdMid& mi
   __syncthreads();
   // inverse DFT-16 over k1: butterfly q = m2 < 7 -> C_b, b = 7 m1 + 16 m2;
   // X_{gamma^b} = x_0 + C_b
+  if (RD_NBUF == 1) {  // Y overlaps other transforms' slots: every read, a barrier, then the writes
+    const int it = threadIdx.x;   // ndft * 7 <= blockDim.x
+    const bool act = it < ndft * 7;
+    const int d = it / 7, q = it - (it / 7) * 7;
+    float2 v[16], x0, X0;
+    if (act) {
+      const float2* in = L.bufA + d * RD_S + q;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) v[r] = in[7 * r];
+      x0 = L.bufA[d * RD_S + 112];
+      X0 = L.bufA[d * RD_S + 113];
+      bfly16<true>(v);
+    }
+    __syncthreads();
+    if (act) {
+      float2* Y = L.bufB + d * RD_P;
+      if (BORD) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) Y[1 + q + 7 * r] = vadd(x0, v[r]);
+      } else {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) Y[L.perm_out[q + 7 * r]] = vadd(x0, v[r]);
+      }
+      if (q == 0) Y[0] = X0;
+    }
+    __syncthreads();
+    return;
+  }
   for (int it = threadIdx.x; it < ndft * 7; it += blockDim.x) {
     const int d = it / 7, q = it - (it / 7) * 7;
     const float2* in = L.bufA + d * RD_S + q;
