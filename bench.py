@@ -149,6 +149,71 @@ def serve_leg(pipe, stacks, flat, cfg, args, dev):
     return out
 
 
+def feed_leg(dev, cfg):
+    """Host feed rates: bz2 zarr decode (``read_im`` on a thread pool) of a sample of microscope
+    frames, and the pinned host-to-device copy of one whole FOV of uint16 stacks."""
+    import shutil
+    import tempfile
+    import torch
+    from mermake_amd import io as mio
+    side, nch, nz = cfg["sx"], cfg["channels"], 5  # 16 chunks: one per decode thread
+    rng = np.random.default_rng(3)
+    frames = rng.integers(18, 31, (1 + nch * nz, side, side), dtype=np.uint8)  # ~sqrt of 600-count Poisson frames
+    root = tempfile.mkdtemp(prefix="mm_feed_")
+    try:
+        d = os.path.join(root, "H1_set1", "001", "data")
+        os.makedirs(d)
+        meta = {"zarr_format": 2, "shape": list(frames.shape), "chunks": [1, side, side], "dtype": "|u1",
+                "compressor": {"id": "bz2", "level": 9}, "fill_value": 0, "filters": None, "order": "C"}
+        json.dump(meta, open(os.path.join(d, ".zarray"), "w"))
+        import bz2
+        import concurrent.futures
+
+        def put(i):
+            open(os.path.join(d, f"{i}.0.0"), "wb").write(bz2.compress(frames[i].tobytes(), 9))
+        with concurrent.futures.ThreadPoolExecutor(16) as ex:
+            list(ex.map(put, range(frames.shape[0])))
+        path = os.path.join(root, "H1_set1", "Conv_zscan__001.zarr")
+        os.makedirs(path)
+        open(os.path.splitext(path)[0] + ".xml", "w").write(
+            f'<settings><stage_position type="custom">0,0</stage_position>'
+            f'<z_offsets type="string">0:-15:-0.4:{nch}</z_offsets></settings>')
+        mio.read_im(path)
+        a = time.perf_counter()
+        im = mio.read_im(path)
+        dec_s = time.perf_counter() - a
+        assert im.shape == (nch, nz, side, side)
+        raw_mb = frames[1:].nbytes / 1e6
+        fov_frames = 1 + nch * cfg["sz"]
+        host = torch.empty((nch, cfg["sz"], side, side), dtype=torch.uint16, pin_memory=True)
+        cs = torch.cuda.Stream(device=dev)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        with torch.cuda.stream(cs):
+            host.to(dev, non_blocking=True)
+            ev[0].record(cs)
+            devt = host.to(dev, non_blocking=True)
+            ev[1].record(cs)
+        torch.cuda.synchronize()
+        h2d_ms = ev[0].elapsed_time(ev[1])
+        del host, devt
+        workers = mio._workers()
+        rate = raw_mb / dec_s
+        out = {"sample": f"{frames.shape[0] - 1} bz2 (level 9) frames of {side}^2 uint8 ({nz} z x {nch} channels), "
+                         f"read_im on {workers} threads",
+               "decode_MB_per_s": rate, "threads": workers,
+               "fov_decode_s_est": (fov_frames * side * side / 1e6) / rate,
+               "h2d_GB_per_s": host_bytes(cfg) / (h2d_ms * 1e-3) / 1e9, "fov_h2d_ms": h2d_ms}
+        log(f"feed: bz2 decode {rate:.0f} MB/s on {workers} threads (a {fov_frames}-frame FOV: "
+            f"{out['fov_decode_s_est']:.2f} s), H2D {out['h2d_GB_per_s']:.1f} GB/s ({h2d_ms:.1f} ms per FOV)")
+        return out
+    finally:
+        shutil.rmtree(root, ignore_errors=True)
+
+
+def host_bytes(cfg):
+    return cfg["channels"] * cfg["sz"] * cfg["sx"] * cfg["sy"] * 2
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -159,6 +224,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the served (request loop) leg")
     ap.add_argument("--serve-requests", type=int, default=12)
     ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate of the served leg (FOV/s)")
+    ap.add_argument("--no-feed", action="store_true", help="skip the host feed leg (zarr decode, pinned H2D)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="end with torch.neg over 1 GiB of float32 (reads 1 GiB, writes 1 GiB): the reference "
                          "kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -250,6 +316,10 @@ def main():
     if not args.no_serve:
         served = serve_leg(pipe, stacks, flat, cfg, args, dev)
 
+    feed = None
+    if not args.no_feed:
+        feed = feed_leg(dev, cfg)
+
     cpu = None
     if not args.no_cpu:
         log("CPU baseline (oracle, bounded sample)")
@@ -279,6 +349,7 @@ def main():
         "latency_ms": {"p50": 1e3 * lat_sorted[len(lat) // 2], "p99": 1e3 * lat_sorted[-1]},
         "plain_gpu": plain,
         "served": served,
+        "feed": feed,
         "cpu_baseline": cpu,
     }
     print(json.dumps(rec), flush=True)

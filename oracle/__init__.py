@@ -22,4 +22,4 @@ Pins (what each function is checked against) are listed in every module's
 header; a function that nothing pins says "parity unpinned" there.
 """
 
-from . import fill, blur, maxima, deconv, align  # noqa: F401
+from . import fill, blur, maxima, deconv, align, io  # noqa: F401

@@ -1,0 +1,66 @@
+"""GPU tests of the host feed: ``load_flats`` (HIP box blur, median, division) against the
+oracle, and the pinned double-buffered ``FovFeeder``."""
+import numpy as np
+import pytest
+
+from mermake_amd import io as mio
+from oracle import io as oio
+
+from fixtures import make_fov
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("shape", [(256, 290), (255, 291)])  # even and odd pixel counts (median of 2 / of 1)
+def test_load_flats_bitexact(dev, tmp_path, shape):
+    rng = np.random.default_rng(5)
+    ims = [rng.random((300, 330), dtype=np.float32) * 1000 + 200,
+           rng.integers(100, 60000, (280, 300)).astype(np.uint16)]
+    for i, im in enumerate(ims):
+        np.savez(tmp_path / f"Scope0_{i}.npz", im=im)
+    got = mio.load_flats(str(tmp_path / "Scope0"), shape, device=dev)
+    assert got.shape == (2,) + shape and got.is_cuda
+    want = np.stack([oio.load_flat(im, shape) for im in ims])
+    np.testing.assert_array_equal(got.cpu().numpy(), want)
+
+
+def test_load_flats_constant_is_ones(dev, tmp_path):
+    np.savez(tmp_path / "flat_a.npz", im=np.full((64, 70), 3.5, np.float32))
+    got = mio.load_flats(str(tmp_path / "flat_"), (50, 60), device=dev).cpu().numpy()
+    np.testing.assert_array_equal(got, np.ones((1, 50, 60), np.float32))
+    with pytest.raises(FileNotFoundError):
+        mio.load_flats(str(tmp_path / "none_"), (50, 60), device=dev)
+
+
+def test_fov_feeder_stages_fovs_to_device(dev, tmp_path):
+    """Three FOVs of different depths through the two pinned staging buffers (the third reuses
+    the first buffer, grown): every channel on the device equals ``read_im``'s."""
+    rng = np.random.default_rng(6)
+    paths, want = [], []
+    for i, n in enumerate((10, 13, 7)):
+        frames = rng.integers(0, 256, (n, 40, 48), dtype=np.uint8)
+        p = make_fov(str(tmp_path / "H1_set2"), f"{i:03d}", frames, pos=f"{i}.5,-{i}.25")
+        paths.append(p)
+        want.append(mio.read_im(p))
+    print("feeder FOVs", [w.shape for w in want], flush=True)
+    with mio.FovFeeder(paths, device=dev) as feed:
+        got = [(p, [c.cpu().numpy() for c in chans], x, y) for p, chans, x, y in feed]
+    assert [g[0] for g in got] == paths
+    for i, (p, chans, x, y) in enumerate(got):
+        assert (x, y) == (i + 0.5, -(i + 0.25))
+        assert len(chans) == 3
+        for c in range(3):
+            np.testing.assert_array_equal(chans[c], want[i][c])
+
+
+def test_fov_feeder_reports_unreadable_fov(dev, tmp_path):
+    frames = np.zeros((4, 8, 8), np.uint8)
+    good = make_fov(str(tmp_path / "H1_set1"), "001", frames)
+    feed = mio.FovFeeder([good, str(tmp_path / "H1_set1" / "Conv_zscan__009.zarr")], device=dev)
+    try:
+        p, chans, _, _ = next(feed)
+        assert p == good and chans[0].shape == (1, 8, 8)
+        with pytest.raises(RuntimeError):
+            next(feed)
+    finally:
+        feed.close()
