@@ -83,16 +83,35 @@ class _Plan:
             pass
 
 
-def tile_geometry(shape, tile_size, overlap, n_psf=1):
+def nearest_psf(cx, cy, psf_keys):
+    """Index of the PSF whose key ``(..., x, y)`` is nearest to the point (cx, cy); the
+    first such key on ties."""
+    best, bd = 0, None
+    for i, key in enumerate(psf_keys):
+        d = (float(key[-2]) - cx) ** 2 + (float(key[-1]) - cy) ** 2
+        if bd is None or d < bd:
+            best, bd = i, d
+    return best
+
+
+def tile_geometry(shape, tile_size, overlap, n_psf=1, psf_keys=None, psf_mode="zip"):
     """Tiles of ``Deconvolver.tiled`` (``deconvolver.py:186-206``) with their padding
-    parameters (``:117-144``), truncated as ``zip`` does when several PSFs are given."""
+    parameters (``:117-144``).  With several PSFs, ``psf_mode`` pairs them with the tiles:
+    ``"zip"`` as the reference (``deconvolver.py:110,116``: x-major order, truncated to the
+    shorter list), ``"keyed"`` every tile with the PSF whose key ``(z, x, y)`` is nearest the
+    centre of the tile's own region."""
+    if psf_mode not in ("zip", "keyed"):
+        raise ValueError(f"psf_mode must be 'zip' or 'keyed', not {psf_mode!r}")
+    keyed = psf_mode == "keyed" and n_psf > 1
+    if keyed and (psf_keys is None or len(psf_keys) != n_psf):
+        raise ValueError("keyed PSF pairing needs one (z, x, y) key per PSF")
     sz, sx, sy = shape
     T = tile_size + 2 * overlap
     out = []
     k = 0
     for x in range(0, sx, tile_size):
         for y in range(0, sy, tile_size):
-            if n_psf > 1 and k >= n_psf:
+            if n_psf > 1 and not keyed and k >= n_psf:
                 return out
             x0, x1 = max(x - overlap, 0), min(x + tile_size + overlap, sx)
             y0, y1 = max(y - overlap, 0), min(y + tile_size + overlap, sy)
@@ -102,6 +121,9 @@ def tile_geometry(shape, tile_size, overlap, n_psf=1):
             out.append(dict(x=x, y=y, X0=x0, Y0=y0, xd=xd, yd=yd, xs=overlap if x == 0 else 0,
                             ys=overlap if y == 0 else 0, xfirst=int(x == 0), yfirst=int(y == 0),
                             xe=min(xend, T), ye=min(yend, T), psf=0 if n_psf == 1 else k))
+            if keyed:
+                out[-1]["psf"] = nearest_psf(x + (min(x + tile_size, sx) - x) / 2,
+                                             y + (min(y + tile_size, sy) - y) / 2, psf_keys)
             k += 1
     return out
 
@@ -158,8 +180,9 @@ class Deconvolver:
     batch_bytes = int(os.environ.get("MERMAKE_BATCH_BYTES", str(24 << 30)))
 
     def __init__(self, psfs, channel_shape, tile_size=300, zpad=0, overlap=89, beta=0.001, xp=None, device=None,
-                 **kwargs):
+                 psf_mode="zip", **kwargs):
         self.tile_size = tile_size
+        self.psf_mode = psf_mode  # "zip" (the reference's pairing) or "keyed" (nearest PSF key per tile)
         self.tile_height = channel_shape[0]
         self.overlap = overlap
         self.zpad = zpad
@@ -171,6 +194,8 @@ class Deconvolver:
         else:
             self.psf_keys = None
             centred = [self.center_psf(np.asarray(psfs))]
+        if psf_mode not in ("zip", "keyed"):
+            raise ValueError(f"psf_mode must be 'zip' or 'keyed', not {psf_mode!r}")
         self._plan = _Plan(self.tile_height, tile_size, overlap, zpad)
         self.psf_fft = torch.empty(self._plan.spectrum_shape(len(centred)), dtype=torch.complex64,
                                    device=self.device)
@@ -184,6 +209,11 @@ class Deconvolver:
         torch.cuda.current_stream(self.device).synchronize()
         del ws
         self._ws = {}
+
+    def geometry(self, shape):
+        """Tiles of a (z, x, y) channel shape with their PSF indices (``psf_mode``)."""
+        return tile_geometry(tuple(shape), self.tile_size, self.overlap, self.psf_fft.shape[0], self.psf_keys,
+                             self.psf_mode)
 
     # ------------------------------------------------------------ reference API
     @property
@@ -277,7 +307,7 @@ class Deconvolver:
         self._check(image)
         img = prepare_stack(image)
         flat = prepare_flat(flat_field, img)
-        geom = tile_geometry(img.shape, self.tile_size, self.overlap, self.psf_fft.shape[0])
+        geom = self.geometry(img.shape)
         T, sz = self._plan.T, self.tile_height
         single = self.psf_fft.shape[0] == 1
         B = self.batch_size(len(geom), 4 * sz * T * T if want_raw else 0)

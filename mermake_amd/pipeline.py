@@ -25,7 +25,7 @@ import numpy as np
 import torch
 
 from . import _lib as L
-from .deconvolver import Deconvolver, prepare_flat, prepare_stack, tile_geometry, tile_table
+from .deconvolver import Deconvolver, prepare_flat, prepare_stack, tile_table
 from .maxima import find_local_maxima_signed
 
 
@@ -48,11 +48,14 @@ class HybPipeline:
         self._ws = {}
 
     @classmethod
-    def from_settings(cls, psfs, channel_shape, tile_size=500, overlap=89, beta=1e-4, zpad=None, **hybs):
-        """Deconvolver with ``zpad = sz - 1`` as in ``__main__.py:184,195``."""
+    def from_settings(cls, psfs, channel_shape, tile_size=500, overlap=89, beta=1e-4, zpad=None, psf_mode="zip",
+                      **hybs):
+        """Deconvolver with ``zpad = sz - 1`` as in ``__main__.py:184,195``; ``psf_mode`` pairs
+        several PSFs with the tiles (``Deconvolver``)."""
         if zpad is None:
             zpad = channel_shape[0] - 1
-        d = Deconvolver(psfs, channel_shape, tile_size=tile_size, zpad=zpad, overlap=overlap, beta=beta)
+        d = Deconvolver(psfs, channel_shape, tile_size=tile_size, zpad=zpad, overlap=overlap, beta=beta,
+                        psf_mode=psf_mode)
         return cls(d, **hybs)
 
     def _buf(self, name, shape, dtype):
@@ -70,7 +73,7 @@ class HybPipeline:
                None if flat is None else flat.data_ptr())
         t = self._tables.get(key)
         if t is None:
-            geom = tile_geometry(img.shape, self.d.tile_size, self.d.overlap, self.d.psf_fft.shape[0])
+            geom = self.d.geometry(img.shape)
             t = (geom, tile_table(geom, img, flat))
             if len(self._tables) > 8:
                 self._tables.clear()

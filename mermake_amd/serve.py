@@ -26,7 +26,7 @@ import numpy as np
 import torch
 
 from . import _lib as L
-from .deconvolver import prepare_flat, prepare_stack, tile_geometry, tile_records
+from .deconvolver import prepare_flat, prepare_stack, tile_records
 
 _REC = L.TILE_DTYPE.itemsize
 
@@ -198,7 +198,7 @@ class HybServer:
             img = prepare_stack(stack)
             f = prepare_flat(flat, img)
             req._dev[ch] = img
-            geom = tile_geometry(tuple(img.shape), d.tile_size, d.overlap, d.psf_fft.shape[0])
+            geom = d.geometry(img.shape)
             recs = tile_records(geom, img, f)
             if f is not None:
                 req._dev.append(f)

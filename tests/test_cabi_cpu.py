@@ -71,6 +71,32 @@ def test_tile_geometry_matches_reference_loop():
             assert t["xs"] == (ov if x == 0 else 0)
 
 
+def test_keyed_psf_assignment_on_the_psf_grid():
+    """A 10 x 10 grid of PSFs keyed at the centres of 300-px cells of a 3000^2 FOV, used on a
+    2304^2 FOV (8 x 8 tiles of 300, the last 204 wide): every tile takes the key nearest its
+    centre, by brute force; the reference's zip pairing keeps only the first 64 PSFs in x-major
+    order, so tile (1, 0) gets the PSF of cell (0, 8)."""
+    import pytest
+    from mermake_amd.deconvolver import tile_geometry
+    keys = [(0, 150 + 300 * i, 150 + 300 * j) for i in range(10) for j in range(10)]
+    shape, ts, ov = (40, 2304, 2304), 300, 89
+    g = tile_geometry(shape, ts, ov, len(keys), keys, "keyed")
+    assert len(g) == 64
+    for t in g:
+        cx = t["x"] + (min(t["x"] + ts, shape[1]) - t["x"]) / 2
+        cy = t["y"] + (min(t["y"] + ts, shape[2]) - t["y"]) / 2
+        d2 = [(k[1] - cx) ** 2 + (k[2] - cy) ** 2 for k in keys]
+        assert t["psf"] == int(np.argmin(d2))
+        assert t["psf"] == 10 * (t["x"] // 300) + t["y"] // 300
+    z = tile_geometry(shape, ts, ov, len(keys), keys, "zip")
+    assert [t["psf"] for t in z] == list(range(64))
+    assert (z[8]["x"], z[8]["y"]) == (300, 0) and keys[z[8]["psf"]] == (0, 150, 2550)
+    with pytest.raises(ValueError):
+        tile_geometry(shape, ts, ov, len(keys), None, "keyed")
+    with pytest.raises(ValueError):
+        tile_geometry(shape, ts, ov, len(keys), keys, "nearest")
+
+
 def test_smooth_length():
     from mermake_amd.utils import smooth_length
     for n in [1, 2, 11, 13, 97, 1101, 1119]:

@@ -345,6 +345,27 @@ def test_multi_psf_pairing(dev):
         assert _rel_l2(gt.cpu().numpy(), wt) < 1e-5
 
 
+def test_keyed_psf_pairing_equals_zip_of_assigned_psfs(dev):
+    """Keyed pairing (every tile takes the PSF whose (z, x, y) key is nearest its centre) gives
+    the zip result of the per-tile PSF list it assigns: tiles centred at (10, 10), (10, 30),
+    (30, 10), (30, 30) take keys 2, 0, 1 and 0 (a tie between 0 and 1 goes to the first)."""
+    from mermake_amd.deconvolver import Deconvolver
+    rng = np.random.default_rng(8)
+    shape = (6, 40, 40)
+    img = rng.integers(0, 3000, size=shape).astype(np.uint16)
+    p = [rng.random((3, 5, 5)).astype(np.float32) for _ in range(3)]
+    keyed = {(0, 10, 30): p[0], (0, 30, 10): p[1], (0, 10, 10): p[2]}
+    d = Deconvolver(keyed, shape, tile_size=20, zpad=2, overlap=4, beta=1e-3, psf_mode="keyed")
+    assert [t["psf"] for t in d.geometry(shape)] == [2, 0, 1, 0]
+    got = list(d.tile_wise(T(img, dev)))
+    assigned = {f"t{k}": p[i] for k, i in enumerate((2, 0, 1, 0))}
+    W = odec.wiener(assigned, shape[0], 20, 2, 4, 1e-3, mode="fp64")
+    want = list(odec.tile_wise(img, W, None, None, 20, 4, 2))
+    assert len(got) == len(want) == 4
+    for (_, _, gt, _), (_, _, wt, _) in zip(got, want):
+        assert _rel_l2(gt.cpu().numpy(), wt) < 1e-5
+
+
 def test_wiener_closed_form(dev):
     """Delta PSF: W = 1/(1 + beta |L|^2) / (Zp T T); L(0) = 1 so the DC gain is 1/(1+beta)."""
     from mermake_amd.deconvolver import Deconvolver

@@ -104,3 +104,28 @@ def test_server_rejects_oversized_request(pipe, fovs):
     srv = HybServer(pipe, batch_tiles=4, slots=1, hbm_budget=1024)
     with pytest.raises(ValueError):
         srv.run([FovRequest(0, [torch.from_numpy(c) for c in chans], torch.from_numpy(flat))])
+
+
+def test_server_keyed_multi_psf_equals_alone(fovs, dev):
+    """Several PSFs keyed by position (the served stream's pairing): every tile of every FOV
+    size takes its nearest PSF, in the mix as alone."""
+    from mermake_amd.pipeline import HybPipeline
+    from mermake_amd.serve import FovRequest, HybServer
+    from mermake_amd.synth import gaussian_psf
+    data, _ = fovs
+    psfs = {(0, x, y): gaussian_psf((11, 31, 31), 1.2 + 0.2 * i, 1.0 + 0.15 * i)
+            for i, (x, y) in enumerate(((75, 75), (75, 225), (225, 75), (225, 225)))}
+    pk = HybPipeline.from_settings(psfs, SHAPES[0], tile_size=100, overlap=20, beta=1e-4, psf_mode="keyed", **HYB)
+    used = {t["psf"] for shape in SHAPES for t in pk.d.geometry(shape)}
+    assert used == {0, 1, 2, 3}
+    alone = []
+    for chans, flat in data:
+        fl = torch.from_numpy(flat).to(dev)
+        alone.append([pk(torch.from_numpy(c).to(dev), fl).cpu().numpy() for c in chans])
+    srv = HybServer(pk, batch_tiles=7, slots=2)
+    reqs = [FovRequest(i, [torch.from_numpy(c).to(dev) for c in chans], torch.from_numpy(flat).to(dev),
+                       arrival=0.0) for i, (chans, flat) in enumerate(data)]
+    done = srv.run(reqs)
+    for r in done:
+        for c, sp in enumerate(r.spots):
+            assert np.array_equal(sp, alone[r.rid][c]), (r.rid, c)
