@@ -844,8 +844,8 @@ __global__ __launch_bounds__(256, 2) void k_rd_r2c(const mm_tile_dev* __restrict
     if (Ls.blk == 6) {  // the M = 6 plan: constant block width, the division is a multiply-shift
       const int bstr = T * 6;
       pf_final_r2c<M>(PfShape(N, RD_P), (const float2*)L.bufB, L.tw, G, rows, [&](int n) {
-        const int b = (int)(((unsigned)n * 43691u) >> 18);  // n / 6 for n < 2^15
-        return b * bstr + (n - 6 * b);
+        const int b = (int)(__umul24((unsigned)n, 43691u) >> 18);  // n / 6 for n < 2^15
+        return __mul24(b, bstr) + (n - 6 * b);
       });
     } else if (Ls.blk) {
       const FastDiv fb(Ls.blk);

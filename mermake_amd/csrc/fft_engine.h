@@ -67,16 +67,17 @@ struct FastDiv {
   int d;
   float inv;
   __device__ __forceinline__ explicit FastDiv(int dd) : d(dd), inv(1.0f / (float)dd) {}
+  // q * d by the full-rate 24-bit multiply (both factors and the product are below 2^23)
   __device__ __forceinline__ int div(int n) const {
     int q = (int)((float)n * inv);
-    const int r = n - q * d;
+    const int r = n - __mul24(q, d);
     if (r >= d) ++q;
     else if (r < 0) --q;
     return q;
   }
   __device__ __forceinline__ void divmod(int n, int& q, int& r) const {
     q = (int)((float)n * inv);
-    r = n - q * d;
+    r = n - __mul24(q, d);
     if (r >= d) { ++q; r -= d; }
     else if (r < 0) { --q; r += d; }
   }
