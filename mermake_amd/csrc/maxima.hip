@@ -80,6 +80,68 @@ __device__ __forceinline__ bool is_max_at(const VolDesc& V, int z, int x, int y,
   return true;
 }
 
+// Sphere offsets 1 <= dz^2 + dx^2 + dy^2 <= 25 ordered by distance, built at compile
+// time: the sphere of delta <= 5 is the prefix of the kSphN[delta] nearest points.  A
+// voxel is a maximum iff no neighbour of its sphere exceeds it, in whatever order they
+// are tested; testing the nearest ones first rejects the non-maxima (whose larger
+// neighbour is usually adjacent) after a few loads instead of scanning from the far
+// side of the sphere.
+#define MX_SPH_DMAX 5
+#define MX_SPH_N 514
+struct SphTab {
+  signed char d[MX_SPH_N][4];  // dz, dx, dy, unused
+  int n[MX_SPH_DMAX + 1];      // points with d^2 <= delta^2 (centre excluded)
+};
+
+constexpr SphTab make_sph_tab() {
+  SphTab t{};
+  int k = 0;
+  for (int s = 1; s <= MX_SPH_DMAX * MX_SPH_DMAX; ++s) {
+    for (int dz = -MX_SPH_DMAX; dz <= MX_SPH_DMAX; ++dz)
+      for (int dx = -MX_SPH_DMAX; dx <= MX_SPH_DMAX; ++dx)
+        for (int dy = -MX_SPH_DMAX; dy <= MX_SPH_DMAX; ++dy)
+          if (dz * dz + dx * dx + dy * dy == s) {
+            t.d[k][0] = (signed char)dz;
+            t.d[k][1] = (signed char)dx;
+            t.d[k][2] = (signed char)dy;
+            ++k;
+          }
+    for (int r = 0; r <= MX_SPH_DMAX; ++r)
+      if (s == r * r) t.n[r] = k;
+  }
+  return t;
+}
+
+__constant__ SphTab kSph = make_sph_tab();
+static_assert(make_sph_tab().n[1] == 6 && make_sph_tab().n[3] == 122 && make_sph_tab().n[5] == MX_SPH_N,
+              "sphere sizes of maxima.cu (6, 122, 514 neighbours at delta 1, 3, 5)");
+
+// Interior voxel (the whole sphere inside the volume, no reflection): the sphere's
+// element offsets are in LDS (soff, built per block), tested nearest first, four loads
+// at a time.
+__device__ __forceinline__ bool is_max_sph_in(const float* __restrict__ c0, const int* __restrict__ soff, int n,
+                                              float c, float sign) {
+  int i = 0;
+  for (; i + 4 <= n; i += 4) {
+    const float a0 = c0[soff[i]], a1 = c0[soff[i + 1]], a2 = c0[soff[i + 2]], a3 = c0[soff[i + 3]];
+    if (c < sign * a0 || c < sign * a1 || c < sign * a2 || c < sign * a3) return false;
+  }
+  for (; i < n; ++i)
+    if (c < sign * c0[soff[i]]) return false;
+  return true;
+}
+
+// Voxel near a border: the same sphere, nearest first, with the single reflection of
+// maxima.cu:44-52 (neighbours still outside are skipped, as in is_max_at).
+__device__ __forceinline__ bool is_max_sph_border(const VolDesc& V, int z, int x, int y, float c, float sign, int n) {
+  for (int i = 0; i < n; ++i) {
+    const int nz = refl1(z + kSph.d[i][0], V.D), nx = refl1(x + kSph.d[i][1], V.H), ny = refl1(y + kSph.d[i][2], V.W);
+    if (nz < 0 || nz >= V.D || nx < 0 || nx >= V.H || ny < 0 || ny >= V.W) continue;
+    if (c < sign * V.p[nz * V.sZ + nx * V.sX + ny]) return false;
+  }
+  return true;
+}
+
 // delta = 1 (the hyb setting): the 6 face neighbours, all loads issued at once.
 // An axis too short for the single reflection leaves the neighbour out, as above.
 __device__ __forceinline__ bool is_max_d1(const VolDesc& V, int z, int x, int y, float c, float sign) {
@@ -116,6 +178,13 @@ __global__ __launch_bounds__(256) void k_max_mark(VS vs, float sign, float thr, 
   const int v = blockIdx.y;
   const VolDesc V = vs(v);
   const int lane = threadIdx.x & 63;
+  // element offsets of the sorted sphere (2 <= delta <= 5), for the interior voxels
+  __shared__ int soff[MX_SPH_N];
+  const bool sph = delta >= 2 && delta <= MX_SPH_DMAX;
+  const int nsph = sph ? kSph.n[delta] : 0;
+  for (int i = threadIdx.x; i < nsph; i += blockDim.x)
+    soff[i] = (int)(kSph.d[i][0] * V.sZ + kSph.d[i][1] * V.sX + kSph.d[i][2]);
+  if (sph) __syncthreads();
   const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (row >= nrow_max) return;
   const long long ridx = (long long)v * nrow_max + row;
@@ -123,6 +192,7 @@ __global__ __launch_bounds__(256) void k_max_mark(VS vs, float sign, float thr, 
   const int z = row / V.H, x = row - (row / V.H) * V.H;
   if (row < V.D * V.H && x >= V.w0 && x < V.w1) {  // rows outside the window keep a zero count
     const float* rp = V.p + z * V.sZ + x * V.sX;
+    const bool zx_in = z >= delta && z < V.D - delta && x >= delta && x < V.H - delta;  // uniform
     unsigned long long* mw = masks + ridx * ww;
     const int y0 = max(V.w0, 0), y1 = min(V.W, V.w1);
     for (int k0 = 0; k0 < ww; k0 += MX_CHUNK) {
@@ -137,8 +207,17 @@ __global__ __launch_bounds__(256) void k_max_mark(VS vs, float sign, float thr, 
         if (k0 + k >= ww) break;
         const int y = (k0 + k) * 64 + lane;
         // maxima.cu:19: v <= threshold is rejected (a NaN threshold rejects nothing)
-        const bool m = y >= y0 && y < y1 && !(c[k] <= thr) &&
-                       (delta == 1 ? is_max_d1(V, z, x, y, c[k], sign) : is_max_at(V, z, x, y, c[k], sign, delta));
+        bool m = y >= y0 && y < y1 && !(c[k] <= thr);
+        if (m) {
+          if (delta == 1) {
+            m = is_max_d1(V, z, x, y, c[k], sign);
+          } else if (sph) {
+            if (zx_in && y >= delta && y < V.W - delta) m = is_max_sph_in(rp + y, soff, nsph, c[k], sign);
+            else m = is_max_sph_border(V, z, x, y, c[k], sign, nsph);
+          } else {
+            m = is_max_at(V, z, x, y, c[k], sign, delta);
+          }
+        }
         const unsigned long long b = __ballot(m);
         if (lane == 0) mw[k0 + k] = b;
         cnt += __popcll(b);

@@ -174,7 +174,8 @@ def _peaky(rng, shape, n=30):
 
 
 @pytest.mark.parametrize("shape,delta", [((12, 20, 24), 1), ((10, 17, 13), 2), ((8, 30, 31), 3), ((12, 12, 12), 5),
-                                         ((2, 2, 2), 1), ((1, 1, 1), 1), ((10, 5, 3), 1)])
+                                         ((2, 2, 2), 1), ((1, 1, 1), 1), ((10, 5, 3), 1), ((16, 40, 70), 4),
+                                         ((14, 33, 90), 5), ((20, 64, 130), 3)])
 def test_maxima_sets_exact(dev, shape, delta):
     from mermake_amd.maxima import maxima_coords
     rng = np.random.default_rng(4)
@@ -186,6 +187,20 @@ def test_maxima_sets_exact(dev, shape, delta):
     assert np.array_equal(got, np.stack([z, x, y], 1).astype(np.int32))
     if np.prod(shape) <= 4000:
         assert [tuple(r) for r in got] == omax.local_maxima_bruteforce(img, thr, delta)
+
+
+@pytest.mark.parametrize("delta,sign", [(2, 1.0), (3, 1.0), (5, 1.0), (5, -1.0)])
+def test_maxima_plateaus_exact(dev, delta, sign):
+    """Values on a few levels: plateaus of equal neighbours (ties count as maxima) inside the
+    volume and on its borders, for the distance-ordered sphere test and its reflecting form."""
+    from mermake_amd.maxima import maxima_coords
+    rng = np.random.default_rng(delta)
+    img = rng.integers(0, 4, (14, 37, 75)).astype(np.float32)
+    thr = 0.5 if sign > 0 else -2.5
+    got, _ = maxima_coords(T(img, dev), thr, delta, sign=sign)
+    z, x, y = omax.local_maxima(sign * img, np.float32(thr), delta)
+    assert len(z) > 0
+    assert np.array_equal(got.cpu().numpy(), np.stack([z, x, y], 1).astype(np.int32))
 
 
 def test_maxima_ties_and_views(dev):
