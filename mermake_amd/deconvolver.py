@@ -285,10 +285,11 @@ class Deconvolver:
         delta = None if blur_radius is None else int(blur_radius) // 2
         args = (self._plan.h, L.ptr(table), ntiles, L.ptr(self.psf_fft), self.psf_fft.shape[0],
                 int(tiles_per_block), L.ptr(spec), L.ptr(res))
-        if window is None or delta is None:
+        if window is None:
             L.call("mm_deconv_tiles", *args, st)
         else:
-            L.call("mm_deconv_tiles_window", *args, max(0, window[0] - delta), min(T, window[1] + delta), st)
+            d = 0 if delta is None else delta
+            L.call("mm_deconv_tiles_window", *args, max(0, window[0] - d), min(T, window[1] + d), st)
         if blur_radius is None:
             return res
         # deconvolver.py:160-163: res - box_y(box_x(res)); the spectrum buffer is free now
@@ -303,7 +304,7 @@ class Deconvolver:
                    int(window[0]), int(window[1]), st)
         return out
 
-    def _batches(self, image, flat_field, blur_radius, want_raw):
+    def _batches(self, image, flat_field, blur_radius, want_raw, window=None):
         self._check(image)
         img = prepare_stack(image)
         flat = prepare_flat(flat_field, img)
@@ -314,7 +315,7 @@ class Deconvolver:
         for b0 in range(0, len(geom), B):
             g = geom[b0:b0 + B]
             table = tile_table(g, img, flat)
-            res = self.run_tiles(table, len(g), len(g) if single else 1, blur_radius)
+            res = self.run_tiles(table, len(g), len(g) if single else 1, blur_radius, window=window)
             raw = None
             if want_raw:
                 raw = torch.empty((len(g), sz, T, T), dtype=torch.float32, device=self.device)
@@ -335,7 +336,9 @@ class Deconvolver:
         if output.dtype != torch.float32 or output.stride(-1) != 1:
             raise ValueError("output must be float32 with unit stride along y")
         sz, sx, sy = image.shape
-        for g, table, res, _, _ in self._batches(image, flat_field, blur_radius, False):
+        # the stitch reads rows and columns [ov, T - ov) of every tile: only those are computed
+        T, ov = self._plan.T, self.overlap
+        for g, table, res, _, _ in self._batches(image, flat_field, blur_radius, False, window=(ov, T - ov)):
             L.call("mm_deconv_stitch", self._plan.h, L.ptr(table), len(g), L.ptr(res), L.ptr(output),
                    output.stride(0), output.stride(1), sx, sy, L.stream_ptr(self.device))
         return output

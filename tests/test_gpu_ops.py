@@ -381,6 +381,28 @@ def test_keyed_psf_pairing_equals_zip_of_assigned_psfs(dev):
         assert _rel_l2(gt.cpu().numpy(), wt) < 1e-5
 
 
+@pytest.mark.parametrize("blur_radius,shape,ts,ov", [(50, (12, 300, 260), 100, 40), (None, (10, 230, 230), 100, 20),
+                                                    (30, (8, 160, 150), 60, 21), (50, (8, 300, 290), 146, 40)])
+def test_apply_window_equals_stitched_tiles(dev, blur_radius, shape, ts, ov):
+    """``apply`` computes each tile only at the rows and columns [ov, T - ov) its stitch reads;
+    the result equals the whole tiles of ``tile_wise`` stitched as ``deconvolver.py:179-183``,
+    bit for bit."""
+    from mermake_amd.deconvolver import Deconvolver
+    from mermake_amd.synth import gaussian_psf
+    rng = np.random.default_rng(9)
+    img = rng.integers(100, 4000, size=shape).astype(np.uint16)
+    flat = (0.8 + 0.4 * rng.random(shape[1:])).astype(np.float32)
+    d = Deconvolver(gaussian_psf((9, 21, 21), 1.5, 1.2), shape, tile_size=ts, zpad=(shape[0] - 1) // 2, overlap=ov,
+                    beta=0.01)
+    got = d.apply(T(img, dev), flat_field=T(flat, dev), blur_radius=blur_radius).cpu().numpy()
+    want = np.zeros(shape, np.float32)
+    for x, y, t, _ in d.tile_wise(T(img, dev), flat_field=T(flat, dev), blur_radius=blur_radius):
+        t = t.cpu().numpy()
+        xe, ye = t.shape[1], t.shape[2]
+        want[:, x:x + xe - 2 * ov, y:y + ye - 2 * ov] = t[:, ov:xe - ov, ov:ye - ov]
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
 def test_wiener_closed_form(dev):
     """Delta PSF: W = 1/(1 + beta |L|^2) / (Zp T T); L(0) = 1 so the DC gain is 1/(1+beta)."""
     from mermake_amd.deconvolver import Deconvolver
