@@ -68,7 +68,8 @@ def read_zarr(path, first=0, workers=None):
     """The zarr v2 array at directory ``path`` as a NumPy array, from index ``first`` of axis 0
     on (only the chunks that hold those frames are decoded).  Missing chunks read as
     ``fill_value``; filters are not supported."""
-    meta = json.load(open(os.path.join(path, ".zarray")))
+    with open(os.path.join(path, ".zarray")) as f:
+        meta = json.load(f)
     if meta.get("zarr_format") != 2:
         raise ValueError(f"{path}: not a zarr v2 array")
     if meta.get("filters"):
@@ -133,15 +134,17 @@ def read_im(path, return_pos=False):
     xml_file = os.path.splitext(path)[0] + ".xml"
     x = y = None
     if os.path.exists(xml_file):
-        txt = open(xml_file, "r", encoding="ISO-8859-1").read()
+        with open(xml_file, "r", encoding="ISO-8859-1") as f:
+            txt = f.read()
         zstack = _xml_tag(txt, '<z_offsets type="string">')
         # the reference evaluates "x,y"; the tag holds two numbers
         x, y = (float(v) for v in _xml_tag(txt, '<stage_position type="custom">').split(","))
         nch = int(zstack.split(":")[-1])
         nzs = (shape[0] // nch) * nch
         image = image[:nzs].reshape([shape[0] // nch, nch, shape[-2], shape[-1]]).swapaxes(0, 1)
-    if image.dtype == np.uint8:
-        image = image.astype(np.uint16) ** 2
+    if image.dtype == np.uint8:  # squared in place of the widened copy
+        image = image.astype(np.uint16)
+        np.multiply(image, image, out=image)
     if return_pos:
         if x is None:
             raise ValueError(f"{path}: no XML with the stage position")
