@@ -664,7 +664,7 @@ __global__ __launch_bounds__(256, 2) void k_rd_c2c(float2* data, long long estri
       fetch(nbase, ngmax);
     }
     rd_stages<M, true, true>(L, G, mid);
-    rd_final<M, INV>((const float2*)L.bufB, L.tw, L.perm_out, G, [&](int g, int j, const float2* v) {
+    rd_final<M, INV>((const float2*)L.bufB, L.tw, L.pos, G, [&](int g, int j, const float2* v) {
       if (g >= gmax) return;
       float2* col = base + (g * cs + j * es);
 #pragma unroll

@@ -52,9 +52,9 @@ struct RdTab {
 #define RD_NBUF 1
 #endif
 
-// LDS bytes: twiddles of N, ghat, perm_out, slot_in, RD_NBUF data buffers.
+// LDS bytes: twiddles of N, ghat, perm_out, slot_in, pos, RD_NBUF data buffers.
 __host__ __device__ __forceinline__ size_t rd_lds_bytes(int N, int M, int G, size_t extra) {
-  return mm_align16((size_t)N * 8) + (size_t)RD_L * 8 + mm_align16((size_t)(RD_L + RD_P) * 4) +
+  return mm_align16((size_t)N * 8) + (size_t)RD_L * 8 + mm_align16((size_t)(RD_L + 2 * RD_P) * 4) +
          (size_t)RD_NBUF * G * M * RD_S * 8 + extra;
 }
 
@@ -63,6 +63,7 @@ struct RdLds {
   float2* ghat;
   int* perm_out;
   int* slot_in;
+  int* pos;  // [p]: slot of X_k in Rader's output order (0 for k = 0, 1 + s for k = perm_out[s])
   float2* bufA;
   float2* bufB;
   unsigned char* extra;
@@ -78,7 +79,8 @@ __device__ __forceinline__ RdLds rd_carve(unsigned char* smem, int N, int M, int
   off += RD_L * 8;
   L.perm_out = (int*)(smem + off);
   L.slot_in = L.perm_out + RD_L;
-  off += mm_align16((size_t)(RD_L + RD_P) * 4);
+  L.pos = L.slot_in + RD_P;
+  off += mm_align16((size_t)(RD_L + 2 * RD_P) * 4);
   L.bufA = (float2*)(smem + off);
   off += (size_t)G * M * RD_S * 8;
   if (RD_NBUF == 2) {
@@ -92,7 +94,9 @@ __device__ __forceinline__ RdLds rd_carve(unsigned char* smem, int N, int M, int
   for (int i = threadIdx.x; i < RD_L; i += blockDim.x) {
     L.ghat[i] = T.ghat[i];
     L.perm_out[i] = T.perm_out[i];
+    L.pos[T.perm_out[i]] = i + 1;  // perm_out is a permutation of 1 .. p - 1
   }
+  if (threadIdx.x == 0) L.pos[0] = 0;
   for (int i = threadIdx.x; i < RD_P; i += blockDim.x) L.slot_in[i] = T.slot_in[i];
   return L;
 }
@@ -304,21 +308,20 @@ __device__ __forceinline__ void rd_stages(const RdLds& L, int G, const RdMid& mi
 }
 
 // Radix-M final stage over Y in the order BORD of rd_stages, consecutive
-// lanes on consecutive transforms (strided pass): the item kk reads its
-// contiguous slot kk of the M sub-transforms and takes k = 0 (kk = 0) or
-// perm_out[kk - 1] for the twiddles and the output rows; st(g, k, v) stores
+// lanes on consecutive transforms (strided pass), items in natural output order:
+// item k (transform g) reads slot pos[k] of the M sub-transforms, so that
+// consecutive lanes store consecutive output rows; st(g, k, v) stores
 // v[r] = Z[k + r p].
 template <int M, bool INV, class St>
 __device__ __forceinline__ void rd_final(const float2* __restrict__ Y, const float2* __restrict__ tw,
-                                         const int* __restrict__ perm_out, int G, St st) {
+                                         const int* __restrict__ pos, int G, St st) {
   const int N = RD_P * M;
   const int tot = G * RD_P;
   const FastDiv fG(G);
   for (int t = threadIdx.x; t < tot; t += blockDim.x) {
-    int g, kk;
-    fG.divmod(t, kk, g);
-    const int k = kk == 0 ? 0 : perm_out[kk - 1];
-    const float2* y = Y + (size_t)g * N + kk;
+    int g, k;
+    fG.divmod(t, k, g);
+    const float2* y = Y + (size_t)g * N + pos[k];
     float2 v[M];
     v[0] = y[0];
 #pragma unroll
