@@ -102,19 +102,35 @@ class _Slot:
         self.table_h[:n * _REC].copy_(torch.from_numpy(recs.view(np.uint8)))
         st = ctypes.c_void_p(self.stream.cuda_stream)
         counts, kept, total = self.stats[:n], self.stats[self.B:self.B + n], self.stats[2 * self.B:]
+        # the windows of the keep filter, as in HybPipeline: the deconvolution, the blur and the
+        # candidates only where the kept spots can come from
+        win = pipe.windows()
+        delta = None if pipe.blur_radius is None else int(pipe.blur_radius) // 2
         with torch.cuda.stream(self.stream):
             self.table[:n * _REC].copy_(self.table_h[:n * _REC], non_blocking=True)
             tab = L.ptr(self.table)
-            L.call("mm_deconv_tiles", d._plan.h, tab, n, L.ptr(d.psf_fft), d.psf_fft.shape[0], n if single else 1,
-                   L.ptr(self.spec), L.ptr(self.res), st)
+            dargs = (d._plan.h, tab, n, L.ptr(d.psf_fft), d.psf_fft.shape[0], n if single else 1, L.ptr(self.spec),
+                     L.ptr(self.res))
+            if win is None or delta is None:
+                L.call("mm_deconv_tiles", *dargs, st)
+            else:
+                L.call("mm_deconv_tiles_window", *dargs, max(0, win[0][0] - delta), min(T, win[0][1] + delta), st)
             out = self.res
-            if pipe.blur_radius is not None:
+            if delta is not None:
                 # blur subtraction into the spectrum buffer, which is free again (deconvolver.py:160-163)
                 out = self.spec[:n * sz * T * T * 4].view(torch.float32)
-                L.call("mm_blur_sub_xy", L.ptr(self.res), L.ptr(out), L.ptr(self.tmp), n * sz, T,
-                       int(pipe.blur_radius) // 2, st)
-            L.call("mm_tiles_maxima", d._plan.h, tab, n, L.ptr(out), pipe.threshold, pipe.delta, L.ptr(self.coords),
-                   self.cap, L.ptr(counts), L.ptr(self.mws), self.mws_bytes, st)
+                if win is None:
+                    L.call("mm_blur_sub_xy", L.ptr(self.res), L.ptr(out), L.ptr(self.tmp), n * sz, T, delta, st)
+                else:
+                    L.call("mm_blur_sub_xy_window", L.ptr(self.res), L.ptr(out), L.ptr(self.tmp), n * sz, T, delta,
+                           win[0][0], win[0][1], st)
+            if win is None:
+                L.call("mm_tiles_maxima", d._plan.h, tab, n, L.ptr(out), pipe.threshold, pipe.delta,
+                       L.ptr(self.coords), self.cap, L.ptr(counts), L.ptr(self.mws), self.mws_bytes, st)
+            else:
+                L.call("mm_tiles_maxima_window", d._plan.h, tab, n, L.ptr(out), pipe.threshold, pipe.delta,
+                       win[1][0], win[1][1], L.ptr(self.coords), self.cap, L.ptr(counts), L.ptr(self.mws),
+                       self.mws_bytes, st)
             L.call("mm_tiles_fit", d._plan.h, tab, n, L.ptr(out), L.ptr(self.coords), L.ptr(counts), self.cap,
                    pipe.delta_fit, pipe.sigmaZ, pipe.sigmaXY, L.ptr(self.fit), st)
             L.call("mm_tiles_keep", tab, n, d.overlap, L.ptr(self.fit), L.ptr(counts), self.cap, L.ptr(kept),
@@ -257,6 +273,7 @@ class HybServer:
         done = []
         launches = 0
         tiles = 0
+        overflows = 0
         while len(done) < len(reqs):
             now = time.perf_counter() - t0
             progressed = False
@@ -286,6 +303,7 @@ class HybServer:
                     progressed = True
                 elif slot.state == "stats" and slot.ev_stats.query():
                     if not slot.fetch_rows():
+                        overflows += 1
                         self._overflow(slot, time.perf_counter() - t0, done)
                         slot.state = "free"
                     progressed = True
@@ -305,7 +323,8 @@ class HybServer:
             "requests": len(reqs), "wall_s": wall, "fovs_per_s": len(reqs) / wall if wall > 0 else None,
             "p50_latency_s": lat[len(lat) // 2] if lat else None,
             "p99_latency_s": lat[min(len(lat) - 1, int(np.ceil(0.99 * len(lat))) - 1)] if lat else None,
-            "launches": launches, "tiles": tiles, "batch_tiles": self.B, "slots": len(self.slots),
+            "launches": launches, "tiles": tiles, "overflow_batches": overflows, "batch_tiles": self.B,
+            "slots": len(self.slots),
             "hbm_budget_bytes": self.budget,
         }
         return reqs
