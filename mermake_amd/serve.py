@@ -45,7 +45,8 @@ class FovRequest:
         self.t_done = None
         self.spots = None
         self._dev = None
-        self._ev = None
+        self._evs = None   # copy-done event of each channel (the flat is copied before channel 0)
+        self._next = 0     # channels whose tiles are queued
         self._rows = None
         self._left = 0
         self._bytes = 0
@@ -185,58 +186,62 @@ class HybServer:
         return n
 
     def _admit(self, req, now):
+        """Start the request's host-to-device copies on the copy stream, the flat first, with an
+        event after each channel: a channel's tiles are queued as soon as its own copy is done."""
         req._bytes = self._need(req)
         self.inflight_bytes += req._bytes
         req.t_admit = now
-        dev = []
+        d = self.pipe.d
+        req._rows = [[None] * len(d.geometry(tuple(c.shape))) for c in req.channels]
+        req._left = sum(len(r) for r in req._rows)
+        req._next = 0
+        dev, evs = [], []
         with torch.cuda.stream(self.copy_stream):
-            for c in req.channels + ([req.flat] if req.flat is not None else []):
+            for c in ([req.flat] if req.flat is not None else []) + req.channels:
                 if isinstance(c, torch.Tensor) and c.is_cuda:
                     dev.append(c)
                 else:
                     h = c if isinstance(c, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(c))
                     dev.append(h.to(self.device, non_blocking=True))
-            req._ev = torch.cuda.Event()
-            req._ev.record(self.copy_stream)
-        req._dev = dev
+                if c is not req.flat:
+                    ev = torch.cuda.Event()
+                    ev.record(self.copy_stream)
+                    evs.append(ev)
+        req._evs = evs
+        req._dev = dev  # [flat], channels...
 
-    def _enqueue(self, req, jobs):
+    def _enqueue_channel(self, req, ch, jobs):
         d = self.pipe.d
-        nch = len(req.channels)
-        flat = None
-        chans = req._dev[:nch]
-        if req.flat is not None:
-            flat = req._dev[nch]
-        req._rows = []
-        req._left = 0
-        for ch, stack in enumerate(chans):
-            d._check(stack)
-            img = prepare_stack(stack)
-            f = prepare_flat(flat, img)
-            req._dev[ch] = img
-            geom = d.geometry(img.shape)
-            recs = tile_records(geom, img, f)
-            if f is not None:
-                req._dev.append(f)
-            req._rows.append([None] * len(geom))
-            req._left += len(geom)
-            for ti in range(len(geom)):
-                jobs.append(_Job(req, ch, ti, recs[ti]))
-        if req._left == 0:
-            req.spots = [np.zeros((0, 8), np.float32) for _ in chans]
+        k = 1 if req.flat is not None else 0
+        flat = req._dev[0] if k else None
+        stack = req._dev[k + ch]
+        d._check(stack)
+        img = prepare_stack(stack)
+        f = prepare_flat(flat, img)
+        req._dev[k + ch] = img
+        if f is not None and f is not flat:
+            req._dev.append(f)
+        recs = tile_records(d.geometry(img.shape), img, f)
+        for ti in range(len(recs)):
+            jobs.append(_Job(req, ch, ti, recs[ti]))
 
     # ------------------------------------------------------------------ completion
+    def _complete(self, req, now, done):
+        """A request is done when every channel's tiles are queued and finished."""
+        if req.t_done is not None or req._left or req._next < len(req.channels):
+            return
+        req.spots = [np.concatenate(r) if r else np.zeros((0, 8), np.float32) for r in req._rows]
+        req.t_done = now
+        req._rows = None
+        req._dev = None
+        self.inflight_bytes -= req._bytes
+        done.append(req)
+
     def _finish_tile(self, job, rows, now, done):
         req = job.req
         req._rows[job.ch][job.ti] = rows
         req._left -= 1
-        if req._left == 0:
-            req.spots = [np.concatenate(r) if r else np.zeros((0, 8), np.float32) for r in req._rows]
-            req.t_done = now
-            req._rows = None
-            req._dev = None
-            self.inflight_bytes -= req._bytes
-            done.append(req)
+        self._complete(req, now, done)
 
     def _overflow(self, slot, now, done):
         """A tile had more candidates than the slot's cap: redo the batch's tiles one stack at a time."""
@@ -286,13 +291,13 @@ class HybServer:
                 copying.append(r)
                 progressed = True
             for r in list(copying):
-                if r._ev.query():
+                while r._next < len(r.channels) and r._evs[r._next].query():
+                    self._enqueue_channel(r, r._next, jobs)
+                    r._next += 1
+                    progressed = True
+                if r._next == len(r.channels):
                     copying.remove(r)
-                    self._enqueue(r, jobs)
-                    if r._left == 0:
-                        r.t_done = time.perf_counter() - t0
-                        self.inflight_bytes -= r._bytes
-                        done.append(r)
+                    self._complete(r, time.perf_counter() - t0, done)  # requests without tiles
                     progressed = True
             for slot in self.slots:
                 if slot.state == "free" and jobs:
