@@ -97,7 +97,7 @@ struct mm_deconv {
   int zsrc_h[4 * 48];       // host copy (kernel argument of the z-kernel pass)
   int pfM, pfG;             // prime-first T passes: T = p * pfM, pfG transforms per block (pfM = 0: off)
   PfShape pfS;
-  int rd;                   // the prime p = 113 runs by Rader's algorithm (tables rdF / rdI)
+  int rd, rdP;              // the prime rdP (113 or 239) runs by Rader's algorithm (tables rdF / rdI)
   int zfft;                 // z pass by length-80 DFTs (zpad = sz - 1, sz <= 40): operator planes, not w
   int blk;                  // spectra column-blocked by blk columns (0: natural layout)
   int Hs;                   // columns of one spectrum row in memory (H rounded up to blk)

@@ -617,18 +617,18 @@ __global__ __launch_bounds__(256) void k_pf_c2r(const float2* Sin, long long s_t
   }
 }
 
-// ------------------------------------------------ Rader (p = 113) tile passes
-// The prime-first passes with the prime stage of fft_rader.h (T = 113 * M).
+// ------------------------------------------------ Rader (p = 113, 239) tile passes
+// The prime-first passes with the prime stage of fft_rader.h (T = p * M).
 // Same persistent structure: the next group's raw elements are prefetched
 // into registers during the current group's Rader and final stages.  Only
 // outputs k in [klo, khi) are stored (the rows a windowed C2R reads).
-template <bool INV, int M>
+template <int PR, bool INV, int M>
 __global__ __launch_bounds__(256, 2) void k_rd_c2c(float2* data, long long estride, long long cstride, int ncols,
                                                 long long bstride, int nbatch, long long gstride, RdTab tab,
                                                 const float2* __restrict__ tw, int G, int klo, int khi) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int N = RD_P * M;
-  const RdLds L = rd_carve(smem, N, M, G, tw, tab);
+  const int N = PR * M;
+  const RdLds L = rd_carve<PR>(smem, N, M, G, tw, tab);
   __syncthreads();  // slot_in is read by rd_items
   const int ncb = (ncols + G - 1) / G;
   const long long ngrp = (long long)ncb * nbatch;
@@ -641,8 +641,8 @@ __global__ __launch_bounds__(256, 2) void k_rd_c2c(float2* data, long long estri
     gmax = min(G, ncols - cb * G);
   };
   RdItems I;
-  rd_items<M, true>(G, L.slot_in, I);
-  const RdMid mid = rd_mid(L);
+  rd_items<PR, M, true>(G, L.slot_in, I);
+  const RdMid mid = PR == 113 ? rd_mid(L) : RdMid{};
   float2 A[RD_NI];
   auto fetch = [&](const float2* base, int gmax) {  // absent columns read the last present one
     rd_fetch(I, [&](int u, int g, int n) { A[u] = base[min(g, gmax - 1) * cs + n * es]; });
@@ -663,17 +663,17 @@ __global__ __launch_bounds__(256, 2) void k_rd_c2c(float2* data, long long estri
       setup(grp + gridDim.x, nbase, ngmax);
       fetch(nbase, ngmax);
     }
-    rd_stages<M, true, true>(L, G, mid);
-    rd_final<M, INV>((const float2*)L.bufB, L.tw, L.pos, G, [&](int g, int j, const float2* v) {
+    rd_stages<PR, M, true, PR == 113>(L, G, mid);
+    rd_final<PR, M, INV>((const float2*)L.bufB, L.tw, L.pos, G, [&](int g, int j, const float2* v) {
       if (g >= gmax) return;
       float2* col = base + (g * cs + j * es);
 #pragma unroll
       for (int r = 0; r < M; ++r)
-        if (j + r * RD_P >= klo && j + r * RD_P < khi) col[r * RD_P * es] = v[r];
+        if (j + r * PR >= klo && j + r * PR < khi) col[r * PR * es] = v[r];
     });
     base = nbase;
     gmax = ngmax;
-    if (RD_NBUF == 1) __syncthreads();  // the final stage's reads before the next group's stores
+    __syncthreads();  // the final stage's reads before the next group's stores
   }
 }
 
@@ -684,23 +684,28 @@ __global__ __launch_bounds__(256, 2) void k_rd_c2c(float2* data, long long estri
 // validity bits fit one 32-bit mask).  For M = 6 small groups (6 rows in the
 // R2C, 4 in the C2R: 25 KB and 19 KB of LDS) let 4 blocks share a CU; they
 // measured 5 to 10% faster than groups of 12 rows.
-constexpr int rd_row_g(int M, bool c2r) { return M == 2 ? 16 : (M == 6 ? (c2r ? 2 : 3) : 9); }
-constexpr int rd_row_nc(int M) { return (RD_P * M + 255) / 256; }
+// For p = 239 (M = 2) G M B <= 256 allows up to 9 transforms.
+template <int PR>
+constexpr int rd_row_g(int M, bool c2r) {
+  return PR == 239 ? (c2r ? 3 : 4) : (M == 2 ? 16 : (M == 6 ? (c2r ? 2 : 3) : 9));
+}
+template <int PR>
+constexpr int rd_row_nc(int M) { return (PR * M + 255) / 256; }
 
 template <int NC>
 struct RdCols {
   int n[NC];   // column
-  int cs[NC];  // LDS slot of the column within one transform's M * RD_S block, -1 for n >= N
+  int cs[NC];  // LDS slot of the column within one transform's M * S block, -1 for n >= N
 };
 
-template <int M, int NC>
+template <int PR, int M, int NC>
 __device__ __forceinline__ RdCols<NC> rd_cols(const int* __restrict__ slot_in) {
   RdCols<NC> c;
 #pragma unroll
   for (int k = 0; k < NC; ++k) {
     const int n = threadIdx.x + k * 256;
     c.n[k] = n;
-    c.cs[k] = n < RD_P * M ? (n % M) * RD_S + slot_in[n / M] : -1;
+    c.cs[k] = n < PR * M ? (n % M) * RdSpec<PR>::S + slot_in[n / M] : -1;
   }
   return c;
 }
@@ -723,14 +728,14 @@ __device__ __forceinline__ long long rd_readlane64(long long v, int lane) {
 }
 
 // Rows x0 .. x0 + NR - 1 of plane z (those < T exist).
-template <int M, int NR, int NC, bool F32, bool FLAT>
+template <int PR, int M, int NR, int NC, bool F32, bool FLAT>
 __device__ __forceinline__ void rd_r2c_load(RdR2cRaw<NR, NC>& R, const mm_tile_dev& td, const RdCols<NC>& C, int T,
                                             int z, int x0) {
   unsigned c[NC];  // column offsets in elements (absent columns read data column 0 of the row)
   R.colok = 0;
 #pragma unroll
   for (int k = 0; k < NC; ++k) {
-    const int cc = C.n[k] < RD_P * M ? pad_map(C.n[k], td.ys, td.yd, td.yfirst, T) : -1;
+    const int cc = C.n[k] < PR * M ? pad_map(C.n[k], td.ys, td.yd, td.yfirst, T) : -1;
     R.colok |= (cc >= 0 ? 1u : 0u) << k;
     c[k] = cc < 0 ? 0u : (unsigned)cc;
   }
@@ -764,7 +769,7 @@ __device__ __forceinline__ void rd_r2c_load(RdR2cRaw<NR, NC>& R, const mm_tile_d
   }
 }
 
-template <int M, int NR, int NC, bool F32, bool FLAT>
+template <int PR, int M, int NR, int NC, bool F32, bool FLAT>
 __device__ __forceinline__ void rd_r2c_store(float2* __restrict__ bufA, const RdR2cRaw<NR, NC>& R,
                                              const RdCols<NC>& C) {
 #pragma unroll
@@ -773,23 +778,23 @@ __device__ __forceinline__ void rd_r2c_store(float2* __restrict__ bufA, const Rd
     for (int k = 0; k < NC; ++k) {
       if (C.cs[k] < 0) continue;
       const unsigned ok = (R.colok >> k) & 1u;
-      bufA[g * M * RD_S + C.cs[k]] =
+      bufA[g * M * RdSpec<PR>::S + C.cs[k]] =
           make_float2(r2c_val<F32, FLAT>(R.P[2 * g][k], R.F[2 * g][k], ok & (R.rowok >> (2 * g))),
                       r2c_val<F32, FLAT>(R.P[2 * g + 1][k], R.F[2 * g + 1][k], ok & (R.rowok >> (2 * g + 1))));
     }
 }
 
-template <int M>
+template <int PR, int M>
 __global__ __launch_bounds__(256, 2) void k_rd_r2c(const mm_tile_dev* __restrict__ tiles, int ntiles, int T, int nrows,
                                                 float2* Sout, long long s_tstride, SpecLayout Ls, RdTab tab,
                                                 const float2* __restrict__ tw) {
-  constexpr int G = rd_row_g(M, false), NC = rd_row_nc(M), NR = 2 * G;
-  static_assert(RD_NBUF == 2 || G * M * 7 <= 256, "in-place Rader stages: one item per thread");
+  constexpr int G = rd_row_g<PR>(M, false), NC = rd_row_nc<PR>(M), NR = 2 * G;
+  static_assert(G * M * RdSpec<PR>::B <= 256, "in-place Rader stages: one item per thread");
   extern __shared__ __align__(16) unsigned char smem[];
-  const int N = RD_P * M;
-  const RdLds L = rd_carve(smem, N, M, G, tw, tab);
+  const int N = PR * M;
+  const RdLds L = rd_carve<PR>(smem, N, M, G, tw, tab);
   __syncthreads();  // slot_in
-  const RdCols<NC> C = rd_cols<M, NC>(L.slot_in);
+  const RdCols<NC> C = rd_cols<PR, M, NC>(L.slot_in);
   // Group (tile, band of NR rows, plane) with the plane fastest: consecutive
   // groups read the same flat-field rows, which then stay in L2.
   const int sz = nrows / T, nband = (T + NR - 1) / NR;
@@ -807,22 +812,22 @@ __global__ __launch_bounds__(256, 2) void k_rd_r2c(const mm_tile_dev* __restrict
     where(gi, tile, z, x0);
     const mm_tile_dev& td = tiles[tile];
     if (td.img_f32) {
-      if (td.flat) rd_r2c_load<M, NR, NC, true, true>(R, td, C, T, z, x0);
-      else rd_r2c_load<M, NR, NC, true, false>(R, td, C, T, z, x0);
+      if (td.flat) rd_r2c_load<PR, M, NR, NC, true, true>(R, td, C, T, z, x0);
+      else rd_r2c_load<PR, M, NR, NC, true, false>(R, td, C, T, z, x0);
     } else {
-      if (td.flat) rd_r2c_load<M, NR, NC, false, true>(R, td, C, T, z, x0);
-      else rd_r2c_load<M, NR, NC, false, false>(R, td, C, T, z, x0);
+      if (td.flat) rd_r2c_load<PR, M, NR, NC, false, true>(R, td, C, T, z, x0);
+      else rd_r2c_load<PR, M, NR, NC, false, false>(R, td, C, T, z, x0);
     }
   };
   auto tile_of = [&](long long gi) { return (int)(gi / ((long long)sz * nband)); };
   auto put = [&](long long gi) {
     const mm_tile_dev& td = tiles[tile_of(gi)];
     if (td.img_f32) {
-      if (td.flat) rd_r2c_store<M, NR, NC, true, true>(L.bufA, R, C);
-      else rd_r2c_store<M, NR, NC, true, false>(L.bufA, R, C);
+      if (td.flat) rd_r2c_store<PR, M, NR, NC, true, true>(L.bufA, R, C);
+      else rd_r2c_store<PR, M, NR, NC, true, false>(L.bufA, R, C);
     } else {
-      if (td.flat) rd_r2c_store<M, NR, NC, false, true>(L.bufA, R, C);
-      else rd_r2c_store<M, NR, NC, false, false>(L.bufA, R, C);
+      if (td.flat) rd_r2c_store<PR, M, NR, NC, false, true>(L.bufA, R, C);
+      else rd_r2c_store<PR, M, NR, NC, false, false>(L.bufA, R, C);
     }
   };
   const int H = N / 2 + 1;
@@ -832,7 +837,7 @@ __global__ __launch_bounds__(256, 2) void k_rd_r2c(const mm_tile_dev* __restrict
     put(grp);
     __syncthreads();
     if (grp + gridDim.x < ngrp) fetch(grp + gridDim.x);
-    rd_stages<M, false, false>(L, G, RdMid{});
+    rd_stages<PR, M, false, false>(L, G, RdMid{});
     int tile, z, x0;
     where(grp, tile, z, x0);
     float2* out = Sout + (long long)tile * s_tstride + z * Ls.plane;
@@ -843,38 +848,38 @@ __global__ __launch_bounds__(256, 2) void k_rd_r2c(const mm_tile_dev* __restrict
     };
     if (Ls.blk == 6) {  // the M = 6 plan: constant block width, the division is a multiply-shift
       const int bstr = T * 6;
-      pf_final_r2c<M>(PfShape(N, RD_P), (const float2*)L.bufB, L.tw, G, rows, [&](int n) {
+      pf_final_r2c<M>(PfShape(N, PR), (const float2*)L.bufB, L.tw, G, rows, [&](int n) {
         const int b = (int)(__umul24((unsigned)n, 43691u) >> 18);  // n / 6 for n < 2^15
         return __mul24(b, bstr) + (n - 6 * b);
       });
     } else if (Ls.blk) {
       const FastDiv fb(Ls.blk);
       const int bstr = T * Ls.blk;
-      pf_final_r2c<M>(PfShape(N, RD_P), (const float2*)L.bufB, L.tw, G, rows, [&](int n) {
+      pf_final_r2c<M>(PfShape(N, PR), (const float2*)L.bufB, L.tw, G, rows, [&](int n) {
         int b, r;
         fb.divmod(n, b, r);
         return b * bstr + r;
       });
     } else {
-      pf_final_r2c<M>(PfShape(N, RD_P), (const float2*)L.bufB, L.tw, G, rows);
+      pf_final_r2c<M>(PfShape(N, PR), (const float2*)L.bufB, L.tw, G, rows);
     }
-    if (RD_NBUF == 1) __syncthreads();  // the final stage's reads before the next group's stores
+    __syncthreads();  // the final stage's reads before the next group's stores
   }
 }
 
 // Rows (z, x) with x in [xlo, xlo + nx) only (nrows = sz * nx of them, taken
 // in that order), output columns in [clo, chi) only.
-template <int M>
+template <int PR, int M>
 __global__ __launch_bounds__(256, 2) void k_rd_c2r(const float2* Sin, long long s_tstride, SpecLayout Ls, float* res,
                                                 long long r_tstride, int ntiles, int nrows, int xlo, int nx, int clo,
                                                 int chi, RdTab tab, const float2* __restrict__ tw) {
-  constexpr int G = rd_row_g(M, true), NC = rd_row_nc(M), NR = 2 * G;
-  static_assert(RD_NBUF == 2 || G * M * 7 <= 256, "in-place Rader stages: one item per thread");
+  constexpr int G = rd_row_g<PR>(M, true), NC = rd_row_nc<PR>(M), NR = 2 * G;
+  static_assert(G * M * RdSpec<PR>::B <= 256, "in-place Rader stages: one item per thread");
   extern __shared__ __align__(16) unsigned char smem[];
-  const int N = RD_P * M, H = N / 2 + 1;
-  const RdLds L = rd_carve(smem, N, M, G, tw, tab);
+  const int N = PR * M, H = N / 2 + 1;
+  const RdLds L = rd_carve<PR>(smem, N, M, G, tw, tab);
   __syncthreads();
-  const RdCols<NC> C = rd_cols<M, NC>(L.slot_in);
+  const RdCols<NC> C = rd_cols<PR, M, NC>(L.slot_in);
   int kk[NC];  // in-row offset of the half-spectrum bin of column n (bin 0 for the absent columns n >= N)
 #pragma unroll
   for (int k = 0; k < NC; ++k) {
@@ -931,15 +936,15 @@ __global__ __launch_bounds__(256, 2) void k_rd_c2r(const float2* Sin, long long 
         float2 b = q + 1 < nrows ? RB[g][k] : make_float2(0.f, 0.f);
         if (n >= H) { a.y = -a.y; b.y = -b.y; }
         if (n == 0 || 2 * n == N) { a.y = 0.f; b.y = 0.f; }  // real part semantics of ifftn(...).real
-        L.bufA[g * M * RD_S + C.cs[k]] = make_float2(a.x - b.y, a.y + b.x);
+        L.bufA[g * M * RdSpec<PR>::S + C.cs[k]] = make_float2(a.x - b.y, a.y + b.x);
       }
     }
     __syncthreads();
     if (grp + gridDim.x < ngrp) fetch(grp + gridDim.x);
-    rd_stages<M, false, false>(L, G, RdMid{});
+    rd_stages<PR, M, false, false>(L, G, RdMid{});
     float* out = res + (long long)tile * r_tstride;
     const int z0 = q0 / nx;
-    pf_final<M, true>(PfShape(N, RD_P), (const float2*)L.bufB, L.tw, G, [&](int g, int j, const float2* v) {
+    pf_final<M, true>(PfShape(N, PR), (const float2*)L.bufB, L.tw, G, [&](int g, int j, const float2* v) {
       const int q = q0 + 2 * g;
       int za, xa, zb, xb;
       rowzx(q0, z0, 2 * g, za, xa);
@@ -948,21 +953,21 @@ __global__ __launch_bounds__(256, 2) void k_rd_c2r(const float2* Sin, long long 
       float* ob = out + ((long long)zb * N + xb) * N + j;
 #pragma unroll
       for (int r = 0; r < M; ++r) {
-        const bool cin = j + r * RD_P >= clo && j + r * RD_P < chi;
-        if (cin && q < nrows) oa[r * RD_P] = v[r].x;
-        if (cin && q + 1 < nrows) ob[r * RD_P] = v[r].y;
+        const bool cin = j + r * PR >= clo && j + r * PR < chi;
+        if (cin && q < nrows) oa[r * PR] = v[r].x;
+        if (cin && q + 1 < nrows) ob[r * PR] = v[r].y;
       }
     });
-    if (RD_NBUF == 1) __syncthreads();  // the final stage's reads before the next group's stores
+    __syncthreads();  // the final stage's reads before the next group's stores
   }
 }
 
-#define PF_SWITCH(M, CALL)                    \
+#define PF_SWITCH(M, ...)                     \
   switch (M) {                                \
-    case 2: { constexpr int MM = 2; CALL; } break; \
-    case 3: { constexpr int MM = 3; CALL; } break; \
-    case 4: { constexpr int MM = 4; CALL; } break; \
-    case 6: { constexpr int MM = 6; CALL; } break; \
+    case 2: { constexpr int MM = 2; __VA_ARGS__; } break; \
+    case 3: { constexpr int MM = 3; __VA_ARGS__; } break; \
+    case 4: { constexpr int MM = 4; __VA_ARGS__; } break; \
+    case 6: { constexpr int MM = 6; __VA_ARGS__; } break; \
     default: mm_set_error("prime-first FFT: radix %d", M); return MM_EINVAL; \
   }
 
@@ -990,6 +995,19 @@ static int launch_pf_c2c(const mm_deconv* d, float2* data, long long estride, lo
   return MM_OK;
 }
 
+// Rader plans: p = 113 with M in {2, 3, 4, 6}, p = 239 with M = 2 (PP, MM constant in CALL).
+#define RD_SWITCH(P, M, ...)                                                         \
+  if ((P) == 113) {                                                                  \
+    constexpr int PP = 113;                                                          \
+    PF_SWITCH(M, __VA_ARGS__);                                                       \
+  } else if ((P) == 239 && (M) == 2) {                                               \
+    constexpr int PP = 239, MM = 2;                                                  \
+    __VA_ARGS__;                                                                     \
+  } else {                                                                           \
+    mm_set_error("Rader FFT: prime %d with radix %d", (int)(P), (int)(M));           \
+    return MM_EINVAL;                                                                \
+  }
+
 template <bool INV>
 // x transforms of every (tile, z) plane of the spectra in layout Ls (the
 // groups of G columns are the layout's column blocks when it is blocked).
@@ -997,34 +1015,35 @@ static int launch_rd_c2c(const mm_deconv* d, float2* data, const SpecLayout& Ls,
                          hipStream_t st) {
   const int G = d->pfG;
   const long long ngrp = (long long)((Ls.H + G - 1) / G) * nbatch;
-  const size_t lds = rd_lds_bytes(d->T, d->pfM, G, 0);
   const long long es = Ls.blk ? Ls.blk : Ls.H;
   const long long gs = Ls.blk ? (long long)Ls.T * Ls.blk : G;
-  PF_SWITCH(d->pfM, hipLaunchKernelGGL((k_rd_c2c<INV, MM>), dim3(pf_grid(ngrp, lds)), dim3(256), lds, st, data, es,
-                                       1LL, Ls.H, Ls.plane, nbatch, gs, INV ? d->rdI : d->rdF, d->pT.tw, G, klo,
-                                       khi));
+  RD_SWITCH(d->rdP, d->pfM, {
+    const size_t lds = rd_lds_bytes<PP>(d->T, MM, G, 0);
+    hipLaunchKernelGGL((k_rd_c2c<PP, INV, MM>), dim3(pf_grid(ngrp, lds)), dim3(256), lds, st, data, es, 1LL, Ls.H,
+                       Ls.plane, nbatch, gs, INV ? d->rdI : d->rdF, d->pT.tw, G, klo, khi);
+  });
   MM_LAUNCH_CHECK();
   return MM_OK;
 }
 
-template <int M>
+template <int PR, int M>
 static void launch_rd_r2c(const mm_tile_dev* td, int ntiles, int T, int nrows, float2* S, long long s_ts,
                           const SpecLayout& Ls, const RdTab& tab, const float2* tw, hipStream_t st) {
-  constexpr int G = rd_row_g(M, false);
-  const size_t lds = rd_lds_bytes(T, M, G, 0);
+  constexpr int G = rd_row_g<PR>(M, false);
+  const size_t lds = rd_lds_bytes<PR>(T, M, G, 0);
   const long long ngrp = (long long)(nrows / T) * ((T + 2 * G - 1) / (2 * G)) * ntiles;
-  hipLaunchKernelGGL(k_rd_r2c<M>, dim3(pf_grid(ngrp, lds)), dim3(256), lds, st, td, ntiles, T, nrows, S, s_ts, Ls,
-                     tab, tw);
+  hipLaunchKernelGGL((k_rd_r2c<PR, M>), dim3(pf_grid(ngrp, lds)), dim3(256), lds, st, td, ntiles, T, nrows, S, s_ts,
+                     Ls, tab, tw);
 }
 
-template <int M>
+template <int PR, int M>
 static void launch_rd_c2r(const float2* S, long long s_ts, const SpecLayout& Ls, float* res, long long r_ts,
                           int ntiles, int sz, int lo, int hi, const RdTab& tab, const float2* tw, hipStream_t st) {
-  constexpr int G = rd_row_g(M, true);
-  const size_t lds = rd_lds_bytes(RD_P * M, M, G, 0);
+  constexpr int G = rd_row_g<PR>(M, true);
+  const size_t lds = rd_lds_bytes<PR>(PR * M, M, G, 0);
   const int nrows = sz * (hi - lo);
   const long long ngrp = (long long)((nrows + 2 * G - 1) / (2 * G)) * ntiles;
-  hipLaunchKernelGGL(k_rd_c2r<M>, dim3(pf_grid(ngrp, lds)), dim3(256), lds, st, S, s_ts, Ls, res, r_ts, ntiles,
+  hipLaunchKernelGGL((k_rd_c2r<PR, M>), dim3(pf_grid(ngrp, lds)), dim3(256), lds, st, S, s_ts, Ls, res, r_ts, ntiles,
                      nrows, lo, hi - lo, lo, hi, tab, tw);
 }
 
@@ -1451,12 +1470,16 @@ extern "C" int mm_deconv_create(int sz, int tile_size, int overlap, int zpad, mm
   }
   // Rader form of the prime 113 (the full-FOV tile T = 678 = 113 * 6)
   d->rd = 0;
-  if (d->pfM && d->pfS.p == RD_P) {
-    const int M = d->pfM;
-    int G = 256 / (7 * M);  // G M 7 <= 256: the in-place stages take one item per thread
-    while (G > 0 && (rd_lds_bytes(d->T, M, G, 0) > 81920 || G * M * RD_P > RD_NI * 256)) --G;
-    if (G > 0 && (rc = mm_get_rader113(&d->rdF, &d->rdI)) == MM_OK) {
+  d->rdP = 0;
+  if (d->pfM && (d->pfS.p == 113 || (d->pfS.p == 239 && d->pfM == 2))) {
+    const int M = d->pfM, p = d->pfS.p;
+    const int B = p == 113 ? RdSpec<113>::B : RdSpec<239>::B;
+    int G = 256 / (B * M);  // G M B <= 256: the in-place stages take one item per thread
+    auto lds = [&](int g) { return p == 113 ? rd_lds_bytes<113>(d->T, M, g, 0) : rd_lds_bytes<239>(d->T, M, g, 0); };
+    while (G > 0 && (lds(G) > 81920 || G * M * p > RD_NI * 256)) --G;
+    if (G > 0 && (rc = mm_get_rader(p, &d->rdF, &d->rdI)) == MM_OK) {
       d->rd = 1;
+      d->rdP = p;
       d->pfG = G;
     } else if (rc) {
       delete d;
@@ -1619,7 +1642,7 @@ static int deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles,
   const long long s_ts = (long long)sz * Ls.plane;
   const int nrows = sz * T;
   if (d->rd) {  // 1. R2C along y of the padded rows (prime-first form, Rader prime stage)
-    PF_SWITCH(d->pfM, launch_rd_r2c<MM>(td, ntiles, T, nrows, S, s_ts, Ls, d->rdF, d->pT.tw, st));
+    RD_SWITCH(d->rdP, d->pfM, (launch_rd_r2c<PP, MM>(td, ntiles, T, nrows, S, s_ts, Ls, d->rdF, d->pT.tw, st)));
     MM_LAUNCH_CHECK();
   } else if (d->pfM) {  // 1. R2C along y of the padded rows (prime-first form)
     const int G = d->pfG;
@@ -1666,8 +1689,8 @@ static int deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles,
   if (rc) return rc;
   MM_MARK(4);
   if (d->rd) {  // 5. C2R along y into res (prime-first form, Rader prime stage)
-    PF_SWITCH(d->pfM,
-              launch_rd_c2r<MM>(S, s_ts, Ls, res, (long long)sz * T * T, ntiles, sz, lo, hi, d->rdI, d->pT.tw, st));
+    RD_SWITCH(d->rdP, d->pfM,
+              (launch_rd_c2r<PP, MM>(S, s_ts, Ls, res, (long long)sz * T * T, ntiles, sz, lo, hi, d->rdI, d->pT.tw, st)));
     MM_LAUNCH_CHECK();
   } else if (d->pfM) {  // 5. C2R along y into res (prime-first form)
     const int G = d->pfG;

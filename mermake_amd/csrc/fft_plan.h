@@ -9,7 +9,7 @@ int mm_get_plan_f64(int N, FftPlanDev<double2>* out);
 
 // Rader tables of p = 113 on the current device (created on first use, kept
 // until the process ends), for the forward and the inverse transform.
-int mm_get_rader113(RdTab* fwd, RdTab* inv);
+int mm_get_rader(int p, RdTab* fwd, RdTab* inv);  // p = 113 or 239
 
 // Largest number of transforms G (at most gmax, at least 1) whose LDS
 // (fft_lds_bytes with `extra_per_g` more bytes per transform) fits `budget`.

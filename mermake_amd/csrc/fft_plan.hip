@@ -3,6 +3,7 @@
 
 #include <cmath>
 #include <map>
+#include <utility>
 #include <mutex>
 #include <vector>
 
@@ -108,20 +109,38 @@ int get_plan(int N, FftPlanDev<C>* out, int tag) {
   return MM_OK;
 }
 
-std::map<int, void*> g_rader;  // device -> Rader tables of p = 113
+std::map<std::pair<int, int>, void*> g_rader;  // (device, p) -> Rader tables
+
+int inv_mod(int a, int m) {  // a^-1 mod m (gcd(a, m) = 1)
+  a %= m;
+  for (int x = 1; x < m; ++x)
+    if (a * x % m == 1) return x;
+  return 1;
+}
 
 }  // namespace
 
-int mm_get_rader113(RdTab* fwd, RdTab* inv) {
+int mm_get_rader(int p, RdTab* fwd, RdTab* inv) {
+  int A, B;
+  if (p == 113) {
+    A = RdSpec<113>::A;
+    B = RdSpec<113>::B;
+  } else if (p == 239) {
+    A = RdSpec<239>::A;
+    B = RdSpec<239>::B;
+  } else {
+    mm_set_error("Rader tables: no plan for the prime %d", p);
+    return MM_EINVAL;
+  }
+  const int L = p - 1;
   int dev = 0;
   MM_CHECK(hipGetDevice(&dev));
   std::lock_guard<std::mutex> lk(g_plan_mu);
   void* buf = nullptr;
-  auto it = g_rader.find(dev);
+  auto it = g_rader.find({dev, p});
   if (it != g_rader.end()) {
     buf = it->second;
   } else {
-    const int p = RD_P, L = RD_L;
     int gam = 0;  // smallest primitive root of p
     for (int c = 2; c < p && !gam; ++c) {
       int x = 1, ord = 0;
@@ -130,25 +149,27 @@ int mm_get_rader113(RdTab* fwd, RdTab* inv) {
     }
     int ginv = 1;
     for (int e = 0; e < p - 2; ++e) ginv = ginv * gam % p;  // gam^(p-2) = gam^-1
-    // Good-Thomas maps of 112 = 16 * 7: n = 7 n1 + 16 n2, k = 49 k1 + 64 k2 (mod 112)
+    // Good-Thomas maps of L = A * B: n = B n1 + A n2, k = cA k1 + cB k2 (mod L)
+    const int iBA = inv_mod(B, A), iAB = inv_mod(A, B);
+    const int cA = B * iBA % L, cB = A * iAB % L;
     std::vector<int> pw(L), perm_out(L), slot_in(p, 0);
     int x = 1, y = 1;
     for (int a = 0; a < L; ++a) {
-      pw[a] = x;                                   // gam^a
-      slot_in[y] = (4 * a) % 7 + 7 * ((7 * a) % 16);  // gam^-a -> slot n2 + 7 n1 of f_a
+      pw[a] = x;                                              // gam^a
+      slot_in[y] = (a * iAB) % B + B * ((a * iBA) % A);       // gam^-a -> slot n2 + B n1 of f_a
       x = x * gam % p;
       y = y * ginv % p;
     }
     slot_in[0] = L;  // x_0
-    for (int m1 = 0; m1 < 16; ++m1)
-      for (int m2 = 0; m2 < 7; ++m2) perm_out[m2 + 7 * m1] = pw[(7 * m1 + 16 * m2) % L];
+    for (int m1 = 0; m1 < A; ++m1)
+      for (int m2 = 0; m2 < B; ++m2) perm_out[m2 + B * m1] = pw[(B * m1 + A * m2) % L];
     const long double PI = 3.141592653589793238462643383279502884L;
     std::vector<float2> h(2 * L);
-    for (int dir = 0; dir < 2; ++dir) {  // ghat = DFT_112(g) / 112, g_n = exp(-/+ 2 pi i gam^n / p)
+    for (int dir = 0; dir < 2; ++dir) {  // ghat = DFT_L(g) / L, g_n = exp(-/+ 2 pi i gam^n / p)
       const long double sg = dir ? 1.0L : -1.0L;
-      for (int k1 = 0; k1 < 16; ++k1)
-        for (int k2 = 0; k2 < 7; ++k2) {
-          const int k = (49 * k1 + 64 * k2) % L;
+      for (int k1 = 0; k1 < A; ++k1)
+        for (int k2 = 0; k2 < B; ++k2) {
+          const int k = (cA * k1 + cB * k2) % L;
           long double re = 0, im = 0;
           for (int n = 0; n < L; ++n) {
             const long double ag = sg * 2.0L * PI * pw[n] / p;
@@ -156,7 +177,7 @@ int mm_get_rader113(RdTab* fwd, RdTab* inv) {
             re += cosl(ag + aw);
             im += sinl(ag + aw);
           }
-          h[dir * L + k1 + 16 * k2] = make_float2((float)(re / L), (float)(im / L));
+          h[dir * L + k1 + A * k2] = make_float2((float)(re / L), (float)(im / L));
         }
     }
     const size_t fbytes = sizeof(float2) * 2 * L, ibytes = sizeof(int) * (L + p);
@@ -165,12 +186,12 @@ int mm_get_rader113(RdTab* fwd, RdTab* inv) {
     ints.insert(ints.end(), slot_in.begin(), slot_in.end());
     MM_CHECK(hipMemcpy(buf, h.data(), fbytes, hipMemcpyHostToDevice));
     MM_CHECK(hipMemcpy((char*)buf + fbytes, ints.data(), ibytes, hipMemcpyHostToDevice));
-    g_rader[dev] = buf;
+    g_rader[{dev, p}] = buf;
   }
   const float2* f = (const float2*)buf;
-  const int* ip = (const int*)(f + 2 * RD_L);
-  *fwd = RdTab{f, ip, ip + RD_L};
-  *inv = RdTab{f + RD_L, ip, ip + RD_L};
+  const int* ip = (const int*)(f + 2 * L);
+  *fwd = RdTab{f, ip, ip + L};
+  *inv = RdTab{f + L, ip, ip + L};
   return MM_OK;
 }
 

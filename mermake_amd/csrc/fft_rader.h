@@ -1,44 +1,40 @@
-// Prime-first FFT with the prime radix p = 113 run by Rader's algorithm, for
-// T = 113 * M (678 = 113 * 6, the full-FOV tile).
+// Prime-first FFT with the prime radix p run by Rader's algorithm, for
+// T = p * M: p = 113 (678 = 113 * 6, the full-FOV tile) and p = 239 (478 =
+// 239 * 2, the tile of tile_size 300 and overlap 89).
 //
 // For a butterfly j of the prime-first order (stride 1, no twiddles) the DFT
 //   X_k = sum_{r=0}^{p-1} x_r w^{r k},  w = exp(-/+ 2 pi i / p)
-// is, with a primitive root gamma (3 for 113), r = gamma^-a and k = gamma^b,
+// is, with a primitive root gamma, r = gamma^-a and k = gamma^b,
 //   X_{gamma^b} = x_0 + (f (*) g)_b,  f_a = x_{gamma^-a},  g_n = w^{gamma^n},
 //   X_0 = x_0 + sum_a f_a,
-// a cyclic convolution of length L = p - 1 = 112 = 16 * 7.  16 and 7 are
-// coprime, so the FFT-112 runs in the Good-Thomas (prime-factor) form without
-// twiddles: with n = 7 n1 + 16 n2 and k = 49 k1 + 64 k2 (mod 112),
-//   W112^{n k} = W16^{n1 k1} W7^{n2 k2}.
-// Forward: DFT-16 over n1, then DFT-7 over n2; the product with
+// a cyclic convolution of length L = p - 1 = A * B with A and B coprime
+// (112 = 16 * 7, 238 = 17 * 14), so the FFT-L runs in the Good-Thomas
+// (prime-factor) form without twiddles: with n = B n1 + A n2 and
+// k = cA k1 + cB k2 (mod L; cA = 1 mod A, 0 mod B and cB the other way round),
+//   WL^{n k} = WA^{n1 k1} WB^{n2 k2}.
+// Forward: DFT-A over n1, then DFT-B over n2; the product with
 // G = FFT(g) / L (precomputed in float64, stored in the (k1, k2) order); the
-// inverse: DFT-7 over k2, then DFT-16 over k1, whose outputs (m1, m2) are
-// C_b at b = 7 m1 + 16 m2.  The index maps live in the tables: slot_in places
+// inverse: DFT-B over k2, then DFT-A over k1, whose outputs (m1, m2) are
+// C_b at b = B m1 + A m2.  The index maps live in the tables: slot_in places
 // f_a at its (n1, n2) slot, perm_out maps (m1, m2) straight to gamma^b.  The
-// two radix-7 passes act on the same 7 elements, so they run back to back in
-// registers.  About 2.6 times fewer operations than the direct symmetric DFT.
+// two radix-B passes act on the same B elements, so they run back to back in
+// registers.  About 2.6 times fewer operations than the direct symmetric DFT
+// for p = 113.
 //
-// LDS per transform: M * RD_S complex (slot 112 holds x_0, slot 113 holds
-// X_0) in one buffer worked in place (or two ping-ponged, RD_NBUF = 2); the
-// last stage leaves the transforms X of the M sub-sequences at stride p, in
-// natural order for the radix-M final stage of fft_pf.h (row passes) or in
-// Rader's output order for rd_final (strided pass: no scattered LDS stores).
+// LDS per transform: M * S complex (slot L holds x_0, slot L + 1 holds X_0)
+// in one buffer worked in place; the last stage leaves the transforms X of
+// the M sub-sequences at stride p, in natural order for the radix-M final
+// stage of fft_pf.h (row passes) or in Rader's output order for rd_final
+// (strided pass: no scattered LDS stores).
 #pragma once
 #include "fft_pf.h"
 
-#define RD_P 113
-#define RD_L 112
-// LDS slots per transform: 112 for f, then x_0 and X_0; the stride is padded
-// from 114 to 122 so that the transforms of one group start on different LDS
-// banks (about 0.5 ms less per 40 x 2800^2 stack across the four Rader passes)
-#define RD_S 122
 #define RD_NI 16  // fetch items (single elements) per thread: G * M * p <= RD_NI * blockDim.x
 
 // Device tables (float64-accurate, float32 storage): ghat forward / inverse
-// [112] each (already divided by 112) at k1 + 16 k2; perm_out[m2 + 7 m1] =
-// gamma^b mod p (b = 7 m1 + 16 m2 mod 112); slot_in[113]: the slot
-// n2 + 7 n1 of element r = gamma^-a (n1 = 7 a mod 16, n2 = 4 a mod 7), 112
-// for r = 0.
+// [L] each (already divided by L) at k1 + A k2; perm_out[m2 + B m1] =
+// gamma^b mod p (b = B m1 + A m2 mod L); slot_in[p]: the slot n2 + B n1 of
+// element r = gamma^-a (a = B n1 + A n2 mod L), L for r = 0.
 struct RdTab {
   const float2* ghat;  // of the transform direction
   const int* perm_out;
@@ -46,17 +42,11 @@ struct RdTab {
 };
 
 // One data buffer per block, the stages in place (the third stage reads
-// everything before a barrier, then writes): half the LDS of ping-pong buffers,
-// so 3 (x passes) to 4 (row passes) blocks share a CU.  Needs G M 7 <= blockDim.x.
-#ifndef RD_NBUF
-#define RD_NBUF 1
-#endif
+// everything before a barrier, then writes).  Needs G M B <= blockDim.x.
 
-// LDS bytes: twiddles of N, ghat, perm_out, slot_in, pos, RD_NBUF data buffers.
-__host__ __device__ __forceinline__ size_t rd_lds_bytes(int N, int M, int G, size_t extra) {
-  return mm_align16((size_t)N * 8) + (size_t)RD_L * 8 + mm_align16((size_t)(RD_L + 2 * RD_P) * 4) +
-         (size_t)RD_NBUF * G * M * RD_S * 8 + extra;
-}
+// LDS bytes: twiddles of N, ghat, perm_out, slot_in, pos, the data buffer.
+template <int P>
+__host__ __device__ __forceinline__ size_t rd_lds_bytes(int N, int M, int G, size_t extra);
 
 struct RdLds {
   float2* tw;
@@ -65,41 +55,17 @@ struct RdLds {
   int* slot_in;
   int* pos;  // [p]: slot of X_k in Rader's output order (0 for k = 0, 1 + s for k = perm_out[s])
   float2* bufA;
-  float2* bufB;
+  float2* bufB;  // the same buffer (the stages run in place)
   unsigned char* extra;
 };
 
-__device__ __forceinline__ RdLds rd_carve(unsigned char* smem, int N, int M, int G, const float2* __restrict__ tw,
-                                          const RdTab& T) {
-  RdLds L;
-  size_t off = 0;
-  L.tw = (float2*)smem;
-  off += mm_align16((size_t)N * 8);
-  L.ghat = (float2*)(smem + off);
-  off += RD_L * 8;
-  L.perm_out = (int*)(smem + off);
-  L.slot_in = L.perm_out + RD_L;
-  L.pos = L.slot_in + RD_P;
-  off += mm_align16((size_t)(RD_L + 2 * RD_P) * 4);
-  L.bufA = (float2*)(smem + off);
-  off += (size_t)G * M * RD_S * 8;
-  if (RD_NBUF == 2) {
-    L.bufB = (float2*)(smem + off);
-    off += (size_t)G * M * RD_S * 8;
-  } else {
-    L.bufB = L.bufA;
-  }
-  L.extra = smem + off;
-  for (int i = threadIdx.x; i < N; i += blockDim.x) L.tw[i] = tw[i];
-  for (int i = threadIdx.x; i < RD_L; i += blockDim.x) {
-    L.ghat[i] = T.ghat[i];
-    L.perm_out[i] = T.perm_out[i];
-    L.pos[T.perm_out[i]] = i + 1;  // perm_out is a permutation of 1 .. p - 1
-  }
-  if (threadIdx.x == 0) L.pos[0] = 0;
-  for (int i = threadIdx.x; i < RD_P; i += blockDim.x) L.slot_in[i] = T.slot_in[i];
-  return L;
-}
+// cos / sin (2 pi m / 17), m = 1..8
+__device__ __constant__ static const double kCos17[8] = {
+    0.9324722294043558, 0.7390089172206591, 0.4457383557765383, 0.09226835946330202,
+    -0.2736629900720829, -0.6026346363792563, -0.850217135729614, -0.9829730996839018};
+__device__ __constant__ static const double kSin17[8] = {
+    0.3612416661871529, 0.6736956436465572, 0.8951632913550623, 0.9957341762950345,
+    0.961825643172819, 0.7980172272802396, 0.5264321628773561, 0.18374951781657037};
 
 // ------------------------------------------------------------ radix 16
 // w16^m (forward: exp(-2 pi i m / 16)); m is a compile-time constant after unrolling.
@@ -141,6 +107,79 @@ __device__ __forceinline__ void bfly16(float2* v) {
   }
 }
 
+// ------------------------------------------------------------ radix 14
+// DFT-14 in the Good-Thomas form 2 x 7: n = 7 a + 2 b, k = 7 k1 + 8 k2 (mod 14),
+// W14^{n k} = W2^{a k1} W7^{b k2}; natural order in and out.
+template <bool INV>
+__device__ __forceinline__ void bfly14(float2* v) {
+  float2 u[2][7];
+#pragma unroll
+  for (int b = 0; b < 7; ++b) {
+    float2 t[2] = {v[(2 * b) % 14], v[(7 + 2 * b) % 14]};
+    bfly2<INV>(t);
+    u[0][b] = t[0];
+    u[1][b] = t[1];
+  }
+  bfly_odd<7, INV>(u[0], kCos7, kSin7);
+  bfly_odd<7, INV>(u[1], kCos7, kSin7);
+#pragma unroll
+  for (int k1 = 0; k1 < 2; ++k1)
+#pragma unroll
+    for (int k2 = 0; k2 < 7; ++k2) v[(7 * k1 + 8 * k2) % 14] = u[k1][k2];
+}
+
+// ------------------------------------------------------------ primes
+// L = p - 1 = A * B (coprime); S: LDS slots per transform (L for f, then x_0
+// and X_0, padded so that the transforms of a group start on different banks:
+// 122 measured 0.5 ms faster than 114 per 40 x 2800^2 stack for p = 113).
+template <int P> struct RdSpec;
+template <> struct RdSpec<113> {
+  static constexpr int L = 112, A = 16, B = 7, S = 122;
+  template <bool INV> __device__ static __forceinline__ void bflyA(float2* v) { bfly16<INV>(v); }
+  template <bool INV> __device__ static __forceinline__ void bflyB(float2* v) { bfly_odd<7, INV>(v, kCos7, kSin7); }
+};
+template <> struct RdSpec<239> {
+  static constexpr int L = 238, A = 17, B = 14, S = 244;
+  template <bool INV> __device__ static __forceinline__ void bflyA(float2* v) { bfly_odd<17, INV>(v, kCos17, kSin17); }
+  template <bool INV> __device__ static __forceinline__ void bflyB(float2* v) { bfly14<INV>(v); }
+};
+
+template <int P>
+__host__ __device__ __forceinline__ size_t rd_lds_bytes(int N, int M, int G, size_t extra) {
+  using R = RdSpec<P>;
+  return mm_align16((size_t)N * 8) + (size_t)R::L * 8 + mm_align16((size_t)(R::L + 2 * P) * 4) +
+         (size_t)G * M * R::S * 8 + extra;
+}
+
+template <int P>
+__device__ __forceinline__ RdLds rd_carve(unsigned char* smem, int N, int M, int G, const float2* __restrict__ tw,
+                                          const RdTab& T) {
+  using R = RdSpec<P>;
+  RdLds L;
+  size_t off = 0;
+  L.tw = (float2*)smem;
+  off += mm_align16((size_t)N * 8);
+  L.ghat = (float2*)(smem + off);
+  off += R::L * 8;
+  L.perm_out = (int*)(smem + off);
+  L.slot_in = L.perm_out + R::L;
+  L.pos = L.slot_in + P;
+  off += mm_align16((size_t)(R::L + 2 * P) * 4);
+  L.bufA = (float2*)(smem + off);
+  L.bufB = L.bufA;
+  off += (size_t)G * M * R::S * 8;
+  L.extra = smem + off;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) L.tw[i] = tw[i];
+  for (int i = threadIdx.x; i < R::L; i += blockDim.x) {
+    L.ghat[i] = T.ghat[i];
+    L.perm_out[i] = T.perm_out[i];
+    L.pos[T.perm_out[i]] = i + 1;  // perm_out is a permutation of 1 .. p - 1
+  }
+  if (threadIdx.x == 0) L.pos[0] = 0;
+  for (int i = threadIdx.x; i < P; i += blockDim.x) L.slot_in[i] = T.slot_in[i];
+  return L;
+}
+
 // ------------------------------------------------------------ fetch / put
 // Item (g, j, r), r < p: element n = j + r M of transform g goes to slot
 // slot_in[r] of DFT (g, j).  nn = n | g << 16; sl = LDS slot.
@@ -150,10 +189,10 @@ struct RdItems {
 };
 
 // GFAST: g fastest (strided pass); otherwise j, then r, then g (rows).
-template <int M, bool GFAST>
+template <int P, int M, bool GFAST>
 __device__ __forceinline__ void rd_items(int G, const int* __restrict__ slot_in, RdItems& I) {
-  const int tot = G * M * RD_P;
-  const FastDiv fG(G), fp(RD_P);
+  const int tot = G * M * P;
+  const FastDiv fG(G), fp(P);
   I.nval = 0;
 #pragma unroll
   for (int u = 0; u < RD_NI; ++u) {
@@ -173,7 +212,7 @@ __device__ __forceinline__ void rd_items(int G, const int* __restrict__ slot_in,
         fp.divmod(q, g, r);
       }
       I.nn[u] = (unsigned)(j + r * M) | ((unsigned)g << 16);
-      I.sl[u] = (unsigned)((g * M + j) * RD_S + slot_in[r]);
+      I.sl[u] = (unsigned)((g * M + j) * RdSpec<P>::S + slot_in[r]);
       I.nval = u + 1;
     }
   }
@@ -197,10 +236,9 @@ __device__ __forceinline__ void rd_put(float2* __restrict__ bufA, const RdItems&
 }
 
 // ------------------------------------------------------------ Rader stages
-// Per-thread constants of the middle stage.  Its butterfly q = it & 15 equals
-// threadIdx.x & 15 in every round (blockDim.x is a multiple of 16), so its
-// G_{q + 16 r} (r = 0..6) can stay in registers (REGC; kernels short of
-// registers read them from LDS).
+// Per-thread constants of the middle stage.  When A divides blockDim.x its
+// butterfly q = it % A equals threadIdx.x % A in every round, so its
+// G_{q + A r} (r < B) can stay in registers (REGC; p = 113 only).
 struct RdMid {
   float2 gh[7];
 };
@@ -213,96 +251,81 @@ __device__ __forceinline__ RdMid rd_mid(const RdLds& L) {
   return c;
 }
 
-// After rd_put and a barrier: bufA holds f (slots 0..111) and x_0 (slot 112) of
+// After rd_put and a barrier: bufA holds f (slots 0..L-1) and x_0 (slot L) of
 // the G*M DFTs.  Ends (after its last barrier) with Y in bufB, at d = g M + j
 // (N = p M):
 //   BORD false: Y[d p + k] = X_k (natural order),
-//   BORD true:  Y[d p] = X_0, Y[d p + 1 + s] = X_{perm_out[s]}, s = m2 + 7 m1
+//   BORD true:  Y[d p] = X_0, Y[d p + 1 + s] = X_{perm_out[s]}, s = m2 + B m1
 //               (rd_final reads it).
-// The first stage writes its output (n2, k1) at k1 * 7 + n2, so that both its
-// stores and the middle stage's loads run across the banks.
-template <int M, bool BORD, bool REGC>
+// Every stage works in place: stage 1 item q = n2 < B reads and writes slots
+// q + B n1 (its output (n2, k1) at k1 * B + n2, so that both its stores and
+// the middle stage's loads run across the banks), stage 2 item q = k1 < A
+// slots q B + r; stage 3's outputs overlap other transforms' slots, so it reads
+// everything, waits for the block, then writes (G M B <= blockDim.x).
+template <int P, int M, bool BORD, bool REGC>
 __device__ __forceinline__ void rd_stages(const RdLds& L, int G, const RdMid& mid) {
+  using R = RdSpec<P>;
+  constexpr int A = R::A, B = R::B, S = R::S, LL = R::L;
+  static_assert(!REGC || (P == 113 && 256 % A == 0), "middle-stage constants in registers: A | blockDim.x");
   const int ndft = G * M;
-  // forward DFT-16 over n1: butterfly q = n2 < 7 reads slots q + 7 n1
-  for (int it = threadIdx.x; it < ndft * 7; it += blockDim.x) {
-    const int d = it / 7, q = it - (it / 7) * 7;
-    const float2* in = L.bufA + d * RD_S + q;
-    float2 v[16];
+  // forward DFT-A over n1: butterfly q = n2 < B reads slots q + B n1
+  for (int it = threadIdx.x; it < ndft * B; it += blockDim.x) {
+    const int d = it / B, q = it - (it / B) * B;
+    float2* io = L.bufA + d * S + q;
+    float2 v[A];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) v[r] = in[7 * r];
-    bfly16<false>(v);
-    float2* out = L.bufB + d * RD_S + q;
+    for (int r = 0; r < A; ++r) v[r] = io[B * r];
+    R::template bflyA<false>(v);
 #pragma unroll
-    for (int r = 0; r < 16; ++r) out[7 * r] = v[r];
+    for (int r = 0; r < A; ++r) io[B * r] = v[r];
   }
   __syncthreads();
-  // DFT-7 over n2 -> F(k1 = q, k2), times ghat, inverse DFT-7 over k2
-  for (int it = threadIdx.x; it < ndft * 16; it += blockDim.x) {
-    const int d = it >> 4, q = it & 15;
-    const float2* in = L.bufB + d * RD_S + q * 7;
-    float2 v[7];
-    v[0] = in[0];
+  // DFT-B over n2 -> F(k1 = q, k2), times ghat, inverse DFT-B over k2
+  for (int it = threadIdx.x; it < ndft * A; it += blockDim.x) {
+    const int d = it / A, q = it - (it / A) * A;
+    float2* io = L.bufA + d * S + q * B;
+    float2 v[B];
 #pragma unroll
-    for (int r = 1; r < 7; ++r) v[r] = in[r];
-    bfly_odd<7, false>(v, kCos7, kSin7);
-    if (q == 0) L.bufA[d * RD_S + 113] = vadd(L.bufA[d * RD_S + 112], v[0]);  // X_0 = x_0 + F_0
+    for (int r = 0; r < B; ++r) v[r] = io[r];
+    R::template bflyB<false>(v);
+    if (q == 0) L.bufA[d * S + LL + 1] = vadd(L.bufA[d * S + LL], v[0]);  // X_0 = x_0 + F_0
 #pragma unroll
-    for (int r = 0; r < 7; ++r) v[r] = vmul(v[r], REGC ? mid.gh[r] : L.ghat[q + 16 * r]);
-    bfly_odd<7, true>(v, kCos7, kSin7);
-    float2* out = L.bufA + d * RD_S + q * 7;
+    for (int r = 0; r < B; ++r) {
+      float2 gh;
+      if constexpr (REGC) gh = mid.gh[r];
+      else gh = L.ghat[q + A * r];
+      v[r] = vmul(v[r], gh);
+    }
+    R::template bflyB<true>(v);
 #pragma unroll
-    for (int r = 0; r < 7; ++r) out[r] = v[r];
+    for (int r = 0; r < B; ++r) io[r] = v[r];
   }
   __syncthreads();
-  // inverse DFT-16 over k1: butterfly q = m2 < 7 -> C_b, b = 7 m1 + 16 m2;
+  // inverse DFT-A over k1: butterfly q = m2 < B -> C_b, b = B m1 + A m2;
   // X_{gamma^b} = x_0 + C_b
-  if (RD_NBUF == 1) {  // Y overlaps other transforms' slots: every read, a barrier, then the writes
-    const int it = threadIdx.x;   // ndft * 7 <= blockDim.x
-    const bool act = it < ndft * 7;
-    const int d = it / 7, q = it - (it / 7) * 7;
-    float2 v[16], x0, X0;
-    if (act) {
-      const float2* in = L.bufA + d * RD_S + q;
+  const int it = threadIdx.x;
+  const bool act = it < ndft * B;
+  const int d = it / B, q = it - (it / B) * B;
+  float2 v[A], x0, X0;
+  if (act) {
+    const float2* in = L.bufA + d * S + q;
 #pragma unroll
-      for (int r = 0; r < 16; ++r) v[r] = in[7 * r];
-      x0 = L.bufA[d * RD_S + 112];
-      X0 = L.bufA[d * RD_S + 113];
-      bfly16<true>(v);
-    }
-    __syncthreads();
-    if (act) {
-      float2* Y = L.bufB + d * RD_P;
-      if (BORD) {
-#pragma unroll
-        for (int r = 0; r < 16; ++r) Y[1 + q + 7 * r] = vadd(x0, v[r]);
-      } else {
-#pragma unroll
-        for (int r = 0; r < 16; ++r) Y[L.perm_out[q + 7 * r]] = vadd(x0, v[r]);
-      }
-      if (q == 0) Y[0] = X0;
-    }
-    __syncthreads();
-    return;
+    for (int r = 0; r < A; ++r) v[r] = in[B * r];
+    x0 = L.bufA[d * S + LL];
+    X0 = L.bufA[d * S + LL + 1];
+    R::template bflyA<true>(v);
   }
-  for (int it = threadIdx.x; it < ndft * 7; it += blockDim.x) {
-    const int d = it / 7, q = it - (it / 7) * 7;
-    const float2* in = L.bufA + d * RD_S + q;
-    float2 v[16];
-    v[0] = in[0];
-#pragma unroll
-    for (int r = 1; r < 16; ++r) v[r] = in[7 * r];
-    bfly16<true>(v);
-    const float2 x0 = L.bufA[d * RD_S + 112];
-    float2* Y = L.bufB + d * RD_P;
+  __syncthreads();
+  if (act) {
+    float2* Y = L.bufB + d * P;
     if (BORD) {
 #pragma unroll
-      for (int r = 0; r < 16; ++r) Y[1 + q + 7 * r] = vadd(x0, v[r]);
+      for (int r = 0; r < A; ++r) Y[1 + q + B * r] = vadd(x0, v[r]);
     } else {
 #pragma unroll
-      for (int r = 0; r < 16; ++r) Y[L.perm_out[q + 7 * r]] = vadd(x0, v[r]);
+      for (int r = 0; r < A; ++r) Y[L.perm_out[q + B * r]] = vadd(x0, v[r]);
     }
-    if (q == 0) Y[0] = L.bufA[d * RD_S + 113];
+    if (q == 0) Y[0] = X0;
   }
   __syncthreads();
 }
@@ -312,11 +335,11 @@ __device__ __forceinline__ void rd_stages(const RdLds& L, int G, const RdMid& mi
 // item k (transform g) reads slot pos[k] of the M sub-transforms, so that
 // consecutive lanes store consecutive output rows; st(g, k, v) stores
 // v[r] = Z[k + r p].
-template <int M, bool INV, class St>
+template <int P, int M, bool INV, class St>
 __device__ __forceinline__ void rd_final(const float2* __restrict__ Y, const float2* __restrict__ tw,
                                          const int* __restrict__ pos, int G, St st) {
-  const int N = RD_P * M;
-  const int tot = G * RD_P;
+  const int N = P * M;
+  const int tot = G * P;
   const FastDiv fG(G);
   for (int t = threadIdx.x; t < tot; t += blockDim.x) {
     int g, k;
@@ -325,7 +348,7 @@ __device__ __forceinline__ void rd_final(const float2* __restrict__ Y, const flo
     float2 v[M];
     v[0] = y[0];
 #pragma unroll
-    for (int r = 1; r < M; ++r) v[r] = vmul(y[r * RD_P], twl<INV>(tw, k * r));
+    for (int r = 1; r < M; ++r) v[r] = vmul(y[r * P], twl<INV>(tw, k * r));
     pf_bfly<M, INV>(v);
     st(g, k, v);
   }

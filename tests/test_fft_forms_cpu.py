@@ -1,8 +1,9 @@
 """CPU checks of the algebra behind two kernel forms (no GPU):
 
-* the Rader stage of p = 113 with its FFT-112 in the Good-Thomas form, using the
-  index maps and the (k1, k2) order of G that mm_get_rader113 builds
-  (csrc/fft_plan.hip, csrc/fft_rader.h), against numpy.fft;
+* the Rader stage of p = 113 (FFT-112 as 16 x 7) and p = 239 (FFT-238 as 17 x 14)
+  in the Good-Thomas form, using the index maps and the (k1, k2) order of G that
+  mm_get_rader builds (csrc/fft_plan.hip, csrc/fft_rader.h), against numpy.fft,
+  and the DFT-14 of the p = 239 stages as 2 x 7;
 * the length-80 z pass (csrc/zfft80.h, k_tile_zfft): reflection padding with
   zpad = sz - 1 as Toeplitz + Hankel circular convolutions, against the z-kernel
   matrix built from the padding copies as k_tile_zconv builds it.
@@ -10,67 +11,88 @@
 import numpy as np
 import pytest
 
-P, L = 113, 112
+SPECS = {113: (16, 7), 239: (17, 14)}  # p: (A, B), p - 1 = A * B
 
 
-def _rader_tables():
+def _rader_tables(P):
+    A, B = SPECS[P]
+    L = P - 1
     gam = next(c for c in range(2, P) if len({pow(c, e, P) for e in range(1, P)}) == P - 1)
     ginv = pow(gam, P - 2, P)
     pw = [pow(gam, a, P) for a in range(L)]
+    iBA, iAB = pow(B, -1, A), pow(A, -1, B)
     slot_in = [0] * P
     for a in range(L):
-        slot_in[pow(ginv, a, P)] = (4 * a) % 7 + 7 * ((7 * a) % 16)
+        slot_in[pow(ginv, a, P)] = (a * iAB) % B + B * ((a * iBA) % A)
     slot_in[0] = L
     perm_out = [0] * L
-    for m1 in range(16):
-        for m2 in range(7):
-            perm_out[m2 + 7 * m1] = pw[(7 * m1 + 16 * m2) % L]
+    for m1 in range(A):
+        for m2 in range(B):
+            perm_out[m2 + B * m1] = pw[(B * m1 + A * m2) % L]
     return pw, slot_in, perm_out
 
 
-def _ghat(pw, sign):
+def _ghat(P, pw, sign):
+    A, B = SPECS[P]
+    L = P - 1
+    cA, cB = B * pow(B, -1, A) % L, A * pow(A, -1, B) % L
     G = np.fft.fft(np.exp(sign * 2j * np.pi * np.array(pw) / P)) / L
     h = np.zeros(L, complex)
-    for k1 in range(16):
-        for k2 in range(7):
-            h[k1 + 16 * k2] = G[(49 * k1 + 64 * k2) % L]
+    for k1 in range(A):
+        for k2 in range(B):
+            h[k1 + A * k2] = G[(cA * k1 + cB * k2) % L]
     return h
 
 
-def _rader(x, inv):
-    """The three LDS stages of rd_stages on one transform (slots of 114 values)."""
-    pw, slot_in, perm_out = _rader_tables()
-    gh = _ghat(pw, 1.0 if inv else -1.0)
+def _rader(P, x, inv):
+    """The three in-place LDS stages of rd_stages on one transform (slots of L + 2 values)."""
+    A, B = SPECS[P]
+    L = P - 1
+    pw, slot_in, perm_out = _rader_tables(P)
+    gh = _ghat(P, pw, 1.0 if inv else -1.0)
     dft = np.fft.fft
     idft = lambda v: np.fft.ifft(v) * len(v)
-    A = np.zeros(114, complex)
+    S = np.zeros(L + 2, complex)
     for r in range(P):
-        A[slot_in[r]] = x[r]
-    B = np.zeros(114, complex)
-    for q in range(7):  # DFT-16 over n1, written at k1 * 7 + n2
-        v = dft(A[[q + 7 * r for r in range(16)]])
-        B[[7 * k1 + q for k1 in range(16)]] = v
-    A2 = A.copy()
-    for q in range(16):  # DFT-7 over n2, times G, inverse DFT-7
-        v = dft(B[[q * 7 + r for r in range(7)]])
+        S[slot_in[r]] = x[r]
+    for q in range(B):  # DFT-A over n1 at slots q + B n1, written back in place as k1
+        S[[q + B * r for r in range(A)]] = dft(S[[q + B * r for r in range(A)]])
+    for q in range(A):  # DFT-B over n2, times G, inverse DFT-B
+        v = dft(S[[q * B + r for r in range(B)]])
         if q == 0:
-            A2[113] = A[112] + v[0]
-        A2[[q * 7 + r for r in range(7)]] = idft(v * gh[[q + 16 * r for r in range(7)]])
+            S[L + 1] = S[L] + v[0]
+        S[[q * B + r for r in range(B)]] = idft(v * gh[[q + A * r for r in range(B)]])
     Y = np.zeros(P, complex)
-    for q in range(7):  # inverse DFT-16 over k1 -> C_b at perm_out
-        v = idft(A2[[q + 7 * r for r in range(16)]])
-        Y[[perm_out[q + 7 * r] for r in range(16)]] = A2[112] + v
-    Y[0] = A2[113]
+    for q in range(B):  # inverse DFT-A over k1 -> C_b at perm_out
+        v = idft(S[[q + B * r for r in range(A)]])
+        Y[[perm_out[q + B * r] for r in range(A)]] = S[L] + v
+    Y[0] = S[L + 1]
     return Y
 
 
+@pytest.mark.parametrize("P", [113, 239])
 @pytest.mark.parametrize("inv", [False, True])
-def test_rader_good_thomas_tables(inv):
+def test_rader_good_thomas_tables(P, inv):
     rng = np.random.default_rng(3)
     x = rng.standard_normal(P) + 1j * rng.standard_normal(P)
     want = np.fft.ifft(x) * P if inv else np.fft.fft(x)
-    got = _rader(x, inv)
+    got = _rader(P, x, inv)
     assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
+
+
+def test_dft14_good_thomas():
+    """bfly14: n = 7 a + 2 b, k = 7 k1 + 8 k2 (mod 14); DFT-2 over a, then DFT-7 over b."""
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal(14) + 1j * rng.standard_normal(14)
+    u = np.zeros((2, 7), complex)
+    for b in range(7):
+        u[:, b] = np.fft.fft([x[(2 * b) % 14], x[(7 + 2 * b) % 14]])
+    u = np.fft.fft(u, axis=1)
+    got = np.zeros(14, complex)
+    for k1 in range(2):
+        for k2 in range(7):
+            got[(7 * k1 + 8 * k2) % 14] = u[k1, k2]
+    np.testing.assert_allclose(got, np.fft.fft(x), atol=1e-12)
 
 
 def _zmap(sz, zpad):

@@ -289,6 +289,8 @@ def _rel_l2(a, b):
     ((40, 120, 120), 80, 10, 39, 1e-3, True, None),  # 40 planes, zpad = sz - 1: the length-80 z pass
     ((40, 260, 260), 200, 13, 39, 1e-3, False, 15),  # ... with T = 226 (Rader, blocked spectra)
     ((17, 100, 100), 60, 8, 16, 1e-3, True, None),   # length-80 z pass with a run-time depth
+    ((3, 400, 400), 300, 89, 2, 1e-3, True, 15),    # T = 478 = 239 * 2 (Rader 239, FFT-238 as 17 x 14)
+    ((10, 330, 330), 300, 89, 9, 1e-3, False, 15),  # ... with the length-80 z pass
 ])
 def test_deconv_vs_oracle(dev, shape, ts, ov, zpad, beta, flat, blur):
     from mermake_amd.deconvolver import Deconvolver
@@ -317,6 +319,7 @@ def test_deconv_vs_oracle(dev, shape, ts, ov, zpad, beta, flat, blur):
 
 @pytest.mark.parametrize("shape,ts,ov,lo,hi", [((3, 300, 300), 200, 13, 5, 215),   # T = 226 (Rader, blocked spectra)
                                                  ((3, 760, 760), 500, 89, 67, 611),  # T = 678
+                                                 ((3, 420, 420), 300, 89, 67, 411),  # T = 478 (Rader 239)
                                                  ((4, 120, 120), 60, 20, 9, 91)])    # T = 100
 def test_deconv_window_bitexact(dev, shape, ts, ov, lo, hi):
     """mm_deconv_tiles_window equals mm_deconv_tiles at rows and columns [lo, hi) of every plane, bit for bit."""

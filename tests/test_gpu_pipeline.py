@@ -79,7 +79,8 @@ def test_hyb_pipeline_vs_oracle(dev, small_hyb):
 
 
 @pytest.mark.parametrize("delta,delta_fit,ts,ov", [(1, 3, 100, 30), (2, 1, 100, 30), (1, 0, 100, 30),
-                                                   (1, 3, 200, 13)])  # T = 226: Rader passes, blocked spectra
+                                                   (1, 3, 200, 13),   # T = 226: Rader passes, blocked spectra
+                                                   (1, 3, 300, 89)])  # T = 478: Rader 239
 def test_hyb_window_equals_whole_tiles(dev, small_hyb, delta, delta_fit, ts, ov):
     """Deconvolution, blur and maxima restricted to the margins the keep filter can use give the same kept rows,
     bit for bit, as whole tiles; the windowed candidate lists are the whole-tile lists restricted to the window."""
