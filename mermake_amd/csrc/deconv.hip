@@ -1288,9 +1288,77 @@ __global__ __launch_bounds__(256, SZ ? 2 : 1) void k_tile_zfft(float2* S, long l
   }
 }
 
+// The same z pass when every tile has its own PSF (tiles_per_block = 1: a grid
+// of PSFs): nothing to share, so no LDS staging; a block takes 256 columns of
+// one tile and every thread reads its column's operator straight from memory
+// (coalesced across the block), so all 256 threads have work.
+template <int SZ>
+__global__ __launch_bounds__(256, SZ ? 2 : 1) void k_tile_zfft_own(float2* S, long long s_tstride, int TH, int sz_rt,
+                                                       const float2* __restrict__ Ops, long long ops_stride,
+                                                       const mm_tile_dev* __restrict__ tiles, int ntiles) {
+  const int sz = SZ ? SZ : sz_rt;
+  const int ncb = (TH + 255) / 256;
+  const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int cb = bid % ncb, j = bid / ncb;
+  const int c = cb * 256 + threadIdx.x;
+  if (j >= ntiles || c >= TH) return;
+  const float2* Op = Ops + (long long)tiles[j].psf * ops_stride + c;
+  float2* col = S + (long long)j * s_tstride + c;
+  float2 x[40];
+  {
+    const float2* p = col;
+#pragma unroll
+    for (int s = 0; s < 40; ++s) {
+      x[s] = s < sz ? *p : make_float2(0.f, 0.f);
+      p += TH;
+      asm volatile("" : "+v"(p));
+    }
+  }
+  const float2 xf = x[0];
+  float2 xl = x[SZ ? SZ - 1 : 0];  // x[sz - 1]
+  if (!SZ) {
+#pragma unroll
+    for (int s = 1; s < 40; ++s) xl = s == sz - 1 ? x[s] : xl;
+  }
+  float2 U[5][16];
+  zf_fwd_stage1(x, U);
+  auto AGk = [&](int k) {
+    const float2 a = Op[(long long)k * TH], b = Op[(long long)(80 + k) * TH];
+    return make_float4(a.x, a.y, b.x, b.y);
+  };
+  zf_middle<0>(U, AGk);
+  zf_middle<1>(U, AGk);
+  zf_middle<2>(U, AGk);
+  float2 y[40];
+  zf_inv_stage3(U, y);
+  asm volatile("" : "+v"(col));
+#pragma unroll
+  for (int t = 0; t < 40; ++t) {
+    if (t < sz) {
+      const float2 a = Op[(long long)(160 + t) * TH], b = Op[(long long)(200 + t) * TH];  // c0[t], c1[t]
+      *col = make_float2(y[t].x - (a.x * xf.x - a.y * xf.y) - (b.x * xl.x - b.y * xl.y),
+                         y[t].y - (a.x * xf.y + a.y * xf.x) - (b.x * xl.y + b.y * xl.x));
+    }
+    col += TH;
+    asm volatile("" : "+v"(col));
+  }
+}
+
 static int launch_zfft(mm_deconv* d, float2* S, long long s_ts, const SpecLayout& Ls, const float2* Ops,
                        const mm_tile_dev* td, int ntiles, int tpb, hipStream_t st) {
   const int TH = (int)Ls.plane;
+  if (tpb == 1) {  // one PSF per tile
+    const long long nblk = (long long)((TH + 255) / 256) * ntiles;
+    if (nblk > 0x7fffffffLL) { mm_set_error("z pass: too many blocks"); return MM_EINVAL; }
+    if (d->sz == 40)
+      hipLaunchKernelGGL(k_tile_zfft_own<40>, dim3((unsigned)nblk), dim3(256), 0, st, S, s_ts, TH, d->sz, Ops,
+                         (long long)ZF_OPS * TH, td, ntiles);
+    else
+      hipLaunchKernelGGL(k_tile_zfft_own<0>, dim3((unsigned)nblk), dim3(256), 0, st, S, s_ts, TH, d->sz, Ops,
+                         (long long)ZF_OPS * TH, td, ntiles);
+    MM_LAUNCH_CHECK();
+    return MM_OK;
+  }
   const int ngroups = (ntiles + tpb - 1) / tpb;
   const long long nblk = (long long)((TH + ZF_G - 1) / ZF_G) * ngroups;
   if (nblk > 0x7fffffffLL) { mm_set_error("z pass: too many blocks"); return MM_EINVAL; }

@@ -363,7 +363,8 @@ def test_multi_psf_pairing(dev):
         assert _rel_l2(gt.cpu().numpy(), wt) < 1e-5
 
 
-def test_keyed_psf_pairing_equals_zip_of_assigned_psfs(dev):
+@pytest.mark.parametrize("zpad", [2, 5])  # 5 = sz - 1: the length-80 z pass with one PSF per tile
+def test_keyed_psf_pairing_equals_zip_of_assigned_psfs(dev, zpad):
     """Keyed pairing (every tile takes the PSF whose (z, x, y) key is nearest its centre) gives
     the zip result of the per-tile PSF list it assigns: tiles centred at (10, 10), (10, 30),
     (30, 10), (30, 30) take keys 2, 0, 1 and 0 (a tie between 0 and 1 goes to the first)."""
@@ -373,12 +374,12 @@ def test_keyed_psf_pairing_equals_zip_of_assigned_psfs(dev):
     img = rng.integers(0, 3000, size=shape).astype(np.uint16)
     p = [rng.random((3, 5, 5)).astype(np.float32) for _ in range(3)]
     keyed = {(0, 10, 30): p[0], (0, 30, 10): p[1], (0, 10, 10): p[2]}
-    d = Deconvolver(keyed, shape, tile_size=20, zpad=2, overlap=4, beta=1e-3, psf_mode="keyed")
+    d = Deconvolver(keyed, shape, tile_size=20, zpad=zpad, overlap=4, beta=1e-3, psf_mode="keyed")
     assert [t["psf"] for t in d.geometry(shape)] == [2, 0, 1, 0]
     got = list(d.tile_wise(T(img, dev)))
     assigned = {f"t{k}": p[i] for k, i in enumerate((2, 0, 1, 0))}
-    W = odec.wiener(assigned, shape[0], 20, 2, 4, 1e-3, mode="fp64")
-    want = list(odec.tile_wise(img, W, None, None, 20, 4, 2))
+    W = odec.wiener(assigned, shape[0], 20, zpad, 4, 1e-3, mode="fp64")
+    want = list(odec.tile_wise(img, W, None, None, 20, 4, zpad))
     assert len(got) == len(want) == 4
     for (_, _, gt, _), (_, _, wt, _) in zip(got, want):
         assert _rel_l2(gt.cpu().numpy(), wt) < 1e-5
@@ -404,6 +405,22 @@ def test_apply_window_equals_stitched_tiles(dev, blur_radius, shape, ts, ov):
         xe, ye = t.shape[1], t.shape[2]
         want[:, x:x + xe - 2 * ov, y:y + ye - 2 * ov] = t[:, ov:xe - ov, ov:ye - ov]
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_multi_psf_length80_zpass_40_planes(dev):
+    """40 planes, zpad = 39, one PSF per tile (the grid form of the length-80 z pass, k_tile_zfft_own<40>)."""
+    from mermake_amd.deconvolver import Deconvolver
+    rng = np.random.default_rng(10)
+    shape = (40, 60, 60)
+    img = rng.integers(100, 4000, size=shape).astype(np.uint16)
+    psfs = {k: rng.random((5, 7, 7)).astype(np.float32) + np.float32(0.1) for k in ("a", "b", "c", "d")}
+    d = Deconvolver(psfs, shape, tile_size=30, zpad=39, overlap=6, beta=1e-3)
+    got = list(d.tile_wise(T(img, dev)))
+    W = odec.wiener(psfs, shape[0], 30, 39, 6, 1e-3, mode="fp64")
+    want = list(odec.tile_wise(img, W, None, None, 30, 6, 39))
+    assert len(got) == len(want) == 4
+    for (_, _, gt, _), (_, _, wt, _) in zip(got, want):
+        assert _rel_l2(gt.cpu().numpy(), wt) < 1e-5
 
 
 def test_wiener_closed_form(dev):
