@@ -187,6 +187,16 @@ int mm_tiles_maxima(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, con
 int mm_tiles_maxima_window(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const float* res,
                            float threshold, int delta, int lo, int hi, int* coords, int cap, int* counts, void* ws,
                            size_t ws_bytes, void* stream);
+/* mm_blur_sub_xy_window(res -> out, rows and columns [blo, bhi), blur_delta 15
+ * or 25) followed by mm_tiles_maxima_window(out, [lo, hi) inside [blo, bhi)),
+ * with the same outputs as those two calls.  The blur kernel also records the
+ * threshold test of every output it writes, as bits in the maxima workspace,
+ * so the maxima pass reads only candidates and their neighbours instead of the
+ * whole window.  Replaces the pair deconvolver.py:160-163 (blur) +
+ * maxima.py:17-98 (find_local_maxima) of the hyb caller (__main__.py:254). */
+int mm_tiles_blur_maxima_window(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const float* res, float* out,
+                                int blur_delta, int blo, int bhi, float threshold, int delta, int lo, int hi,
+                                int* coords, int cap, int* counts, void* ws, size_t ws_bytes, void* stream);
 int mm_tiles_fit(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const float* res, const int* coords,
                  const int* counts, int cap, int delta_fit, float sigmaZ, float sigmaXY, float* out8, void* stream);
 /* keep-filter on the fitted x,y (ov <= . < view extent - ov), offset to stack

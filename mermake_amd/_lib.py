@@ -54,6 +54,8 @@ _SIGS = {
     "mm_delta_fit": (_i, [_vp, _i, _i, _i, _ll, _ll, _f, _vp, _i, _ll, _ll, _vp, _vp, _i, _i, _f, _f, _vp, _vp]),
     "mm_tiles_maxima": (_i, [_vp, _vp, _i, _vp, _f, _i, _vp, _i, _vp, _vp, _sz, _vp]),
     "mm_tiles_maxima_window": (_i, [_vp, _vp, _i, _vp, _f, _i, _i, _i, _vp, _i, _vp, _vp, _sz, _vp]),
+    "mm_tiles_blur_maxima_window": (_i, [_vp, _vp, _i, _vp, _vp, _i, _i, _i, _f, _i, _i, _i, _vp, _i, _vp, _vp, _sz,
+                                         _vp]),
     "mm_tiles_fit": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _i, _i, _f, _f, _vp, _vp]),
     "mm_tiles_keep": (_i, [_vp, _i, _i, _vp, _vp, _i, _vp, _vp, _i, _vp, _vp]),
     "mm_points_round_bounds": (_i, [_vp, _i, _i, _f, _vp, _vp, _vp]),

@@ -159,11 +159,12 @@ __device__ __forceinline__ void blur_x_col(const float* __restrict__ rp, const f
 }
 
 // y pass of one item: outputs y0 .. y0 + YB - 1 (< hi) of row gx.  IN: every
-// window inside the row (counts 2D + 1).
-template <int D, bool IN>
-__device__ __forceinline__ void blur_y_item(const float* __restrict__ trow, const float* __restrict__ rrow,
-                                            float* __restrict__ orow, int T, int y0, int hi,
-                                            const float* __restrict__ rcp) {
+// window inside the row (counts 2D + 1).  MARK: returns bit yo set for each
+// output o with !(sign o <= thr), the maxima threshold test.
+template <int D, bool IN, bool MARK>
+__device__ __forceinline__ unsigned blur_y_item(const float* __restrict__ trow, const float* __restrict__ rrow,
+                                                float* __restrict__ orow, int T, int y0, int hi,
+                                                const float* __restrict__ rcp, float sign, float thr) {
   constexpr int YB = BLUR_YB, W = YB + 2 * D;
   float u[W];
 #pragma unroll
@@ -188,22 +189,34 @@ __device__ __forceinline__ void blur_y_item(const float* __restrict__ trow, cons
     }
   }
   div_counts<YB>(sum, n, y, q);
+  unsigned bits = 0;
 #pragma unroll
   for (int yo = 0; yo < YB; ++yo)
-    if (y0 + yo < hi) orow[y0 + yo] = rv[yo] - q[yo];
+    if (y0 + yo < hi) {
+      const float o = rv[yo] - q[yo];
+      orow[y0 + yo] = o;
+      if (MARK && !(sign * o <= thr)) bits |= 1u << yo;
+    }
+  return bits;
 }
 
-template <int D>
+// MARK: also the maxima threshold bits of every computed output, as ww 64-bit
+// words per row of the (nplanes, T) rows (word k bit b: column 64 k + b; zero
+// outside [lo, hi)), for rows [lo, hi).
+template <int D, bool MARK>
 __global__ __launch_bounds__(256) void k_blur_sub_xy(const float* __restrict__ res, float* __restrict__ out, int T,
-                                                     int nplanes, int npatch, int lo, int hi) {
+                                                     int nplanes, int npatch, int lo, int hi, float sign, float thr,
+                                                     unsigned long long* __restrict__ marks, int ww) {
   constexpr int BX = BLUR_BX, YB = BLUR_YB;
   const int nyb = (hi - lo + YB - 1) / YB;
   const int pitch = blur_pitch(T, D, lo, hi);
   extern __shared__ __align__(16) float smem_f[];
-  float* rcp = smem_f;                 // [64]: RN(1/n)
-  float* rown = smem_f + 64;           // [BX]: window counts of the patch rows
-  float* rowy = smem_f + 64 + BX;      // [BX]: their RN(1/n)
-  float* tb = smem_f + 64 + 2 * BX;    // [BX][pitch]: box_x of the patch rows, D zero columns each side
+  unsigned long long* aw = (unsigned long long*)smem_f;  // [BX][ww] threshold words (MARK)
+  float* const sf = smem_f + (MARK ? 2 * BX * ww : 0);
+  float* rcp = sf;                 // [64]: RN(1/n)
+  float* rown = sf + 64;           // [BX]: window counts of the patch rows
+  float* rowy = sf + 64 + BX;      // [BX]: their RN(1/n)
+  float* tb = sf + 64 + 2 * BX;    // [BX][pitch]: box_x of the patch rows, D zero columns each side
   // neighbouring patches of a plane run on one XCD and share its L2 for the halo rows
   const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
   const int plane = bid / npatch, patch = bid - (bid / npatch) * npatch;
@@ -222,6 +235,8 @@ __global__ __launch_bounds__(256) void k_blur_sub_xy(const float* __restrict__ r
     const int r = i / nz, c = i - r * nz;
     tb[r * pitch + (c < D ? c : T + c)] = 0.0f;
   }
+  if (MARK)
+    for (int i = threadIdx.x; i < BX * ww; i += blockDim.x) aw[i] = 0ull;
   __syncthreads();
   // x pass over the columns the y windows of [lo, hi) read
   const int chi = min(T, hi + D);
@@ -244,23 +259,54 @@ __global__ __launch_bounds__(256) void k_blur_sub_xy(const float* __restrict__ r
     const float* trow = tb + xo * pitch + y0;  // trow[w] = box_x at column y0 - D + w
     const float* rrow = rp + (long long)gx * T;
     float* orow = out + (long long)plane * T * T + (long long)gx * T;
-    if (y0 - D >= 0 && y0 + YB - 1 + D <= T - 1) blur_y_item<D, true>(trow, rrow, orow, T, y0, hi, rcp);
-    else blur_y_item<D, false>(trow, rrow, orow, T, y0, hi, rcp);
+    const unsigned bits = y0 - D >= 0 && y0 + YB - 1 + D <= T - 1
+                              ? blur_y_item<D, true, MARK>(trow, rrow, orow, T, y0, hi, rcp, sign, thr)
+                              : blur_y_item<D, false, MARK>(trow, rrow, orow, T, y0, hi, rcp, sign, thr);
+    if (MARK && bits) {  // rare: LDS OR into the row's word(s); bits past hi are zero
+      const int w = y0 >> 6, sh = y0 & 63;
+      atomicOr(aw + xo * ww + w, (unsigned long long)bits << sh);
+      const unsigned long long up = sh ? (unsigned long long)bits >> (64 - sh) : 0ull;
+      if (up) atomicOr(aw + xo * ww + w + 1, up);
+    }
+  }
+  if (MARK) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < BX * ww; i += blockDim.x) {
+      const int xo = i / ww;
+      if (x0 + xo < hi) marks[((long long)plane * T + x0 + xo) * ww + (i - xo * ww)] = aw[i];
+    }
   }
 }
 
 template <int D>
-static int launch_blur_sub(const float* res, float* out, int nplanes, int T, int lo, int hi, hipStream_t st) {
+static int launch_blur_sub(const float* res, float* out, int nplanes, int T, int lo, int hi, hipStream_t st,
+                           unsigned long long* marks = nullptr, float sign = 1.0f, float thr = 0.0f) {
   if (2 * D + 1 > 63) { mm_set_error("mm_blur_sub_xy: window too wide"); return MM_EINVAL; }
-  const size_t lds = sizeof(float) * (64 + 2 * BLUR_BX + (size_t)BLUR_BX * blur_pitch(T, D, lo, hi));
+  const int ww = (T + 63) / 64;
+  const size_t lds = sizeof(float) * (64 + 2 * BLUR_BX + (size_t)BLUR_BX * blur_pitch(T, D, lo, hi)) +
+                     (marks ? sizeof(unsigned long long) * BLUR_BX * ww : 0);
   if (lds > 163840) { mm_set_error("mm_blur_sub_xy: plane of %d columns too wide", T); return MM_EINVAL; }
   const int npatch = (hi - lo + BLUR_BX - 1) / BLUR_BX;
   const long long nblk = (long long)npatch * nplanes;
   if (nblk > 0x7fffffffLL) { mm_set_error("mm_blur_sub_xy: too many planes"); return MM_EINVAL; }
-  hipLaunchKernelGGL(k_blur_sub_xy<D>, dim3((unsigned)nblk), dim3(256), lds, st, res, out, T, nplanes, npatch, lo,
-                     hi);
+  if (marks)
+    hipLaunchKernelGGL((k_blur_sub_xy<D, true>), dim3((unsigned)nblk), dim3(256), lds, st, res, out, T, nplanes,
+                       npatch, lo, hi, sign, thr, marks, ww);
+  else
+    hipLaunchKernelGGL((k_blur_sub_xy<D, false>), dim3((unsigned)nblk), dim3(256), lds, st, res, out, T, nplanes,
+                       npatch, lo, hi, 0.0f, 0.0f, nullptr, 0);
   MM_LAUNCH_CHECK();
   return MM_OK;
+}
+
+// The windowed fused blur that also writes the maxima threshold bits (see
+// k_blur_sub_xy): the first stage of mm_tiles_blur_maxima_window.
+int mm_blur_sub_mark(const float* res, float* out, int nplanes, int T, int delta, int lo, int hi, float sign,
+                     float thr, unsigned long long* marks, hipStream_t st) {
+  if (delta == 15) return launch_blur_sub<15>(res, out, nplanes, T, lo, hi, st, marks, sign, thr);
+  if (delta == 25) return launch_blur_sub<25>(res, out, nplanes, T, lo, hi, st, marks, sign, thr);
+  mm_set_error("mm_tiles_blur_maxima_window: no fused blur for delta %d (15 and 25 only)", delta);
+  return MM_EINVAL;
 }
 
 // out = res - box_y(box_x(res)) over nplanes (T, T) planes; `out` must not overlap

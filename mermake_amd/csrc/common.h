@@ -30,6 +30,11 @@ void mm_set_error(const char* fmt, ...);
 
 enum { MM_OK = 0, MM_EINVAL = 1000, MM_ECAP = 1001, MM_ENOMEM = 1002 };
 
+// fused windowed blur subtraction (delta 15 or 25) that also writes the maxima
+// threshold bits, ceil(T / 64) words per row of the nplanes x T rows (blur_fill.hip)
+int mm_blur_sub_mark(const float* res, float* out, int nplanes, int T, int delta, int lo, int hi, float sign,
+                     float thr, unsigned long long* marks, hipStream_t st);
+
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {

@@ -169,12 +169,13 @@ __device__ __forceinline__ bool is_max_d1(const VolDesc& V, int z, int x, int y,
   return m;
 }
 
-// Rows r = z H + x of volume blockIdx.y, four per block (one per wavefront);
-// ww words of 64 voxels per row.  Wavefront-only work: no block barrier.
+// Rows r = z hl + x of volume blockIdx.y (hl >= H: the workspace's rows per
+// plane), four per block (one per wavefront); ww words of 64 voxels per row.
+// Wavefront-only work: no block barrier.
 template <class VS>
 __global__ __launch_bounds__(256) void k_max_mark(VS vs, float sign, float thr, int delta,
                                                   unsigned long long* __restrict__ masks, int* __restrict__ rcnt,
-                                                  int nrow_max, int ww) {
+                                                  int nrow_max, int hl, int ww) {
   const int v = blockIdx.y;
   const VolDesc V = vs(v);
   const int lane = threadIdx.x & 63;
@@ -189,8 +190,8 @@ __global__ __launch_bounds__(256) void k_max_mark(VS vs, float sign, float thr, 
   if (row >= nrow_max) return;
   const long long ridx = (long long)v * nrow_max + row;
   int cnt = 0;
-  const int z = row / V.H, x = row - (row / V.H) * V.H;
-  if (row < V.D * V.H && x >= V.w0 && x < V.w1) {  // rows outside the window keep a zero count
+  const int z = row / hl, x = row - (row / hl) * hl;
+  if (z < V.D && x < V.H && x >= V.w0 && x < V.w1) {  // rows outside the window keep a zero count
     const float* rp = V.p + z * V.sZ + x * V.sX;
     const bool zx_in = z >= delta && z < V.D - delta && x >= delta && x < V.H - delta;  // uniform
     unsigned long long* mw = masks + ridx * ww;
@@ -227,6 +228,56 @@ __global__ __launch_bounds__(256) void k_max_mark(VS vs, float sign, float thr, 
   if (lane == 0) rcnt[ridx] = cnt;
 }
 
+// The same marks from threshold bits already in the masks (written by the blur
+// that produced the volume: bit y of row r set iff !(sign v <= thr)): one thread
+// per row turns every non-zero word into its maxima word in place, so only the
+// candidates' neighbourhoods are read.  Zero words stay zero.
+template <class VS>
+__global__ __launch_bounds__(256) void k_max_mark_above(VS vs, float sign, int delta,
+                                                        unsigned long long* __restrict__ masks,
+                                                        int* __restrict__ rcnt, int nrow_max, int hl, int ww) {
+  const int v = blockIdx.y;
+  const VolDesc V = vs(v);
+  __shared__ int soff[MX_SPH_N];
+  const bool sph = delta >= 2 && delta <= MX_SPH_DMAX;
+  const int nsph = sph ? kSph.n[delta] : 0;
+  for (int i = threadIdx.x; i < nsph; i += blockDim.x)
+    soff[i] = (int)(kSph.d[i][0] * V.sZ + kSph.d[i][1] * V.sX + kSph.d[i][2]);
+  if (sph) __syncthreads();
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= nrow_max) return;
+  const long long ridx = (long long)v * nrow_max + row;
+  const int z = row / hl, x = row - (row / hl) * hl;
+  int cnt = 0;
+  if (z < V.D && x < V.H && x >= V.w0 && x < V.w1) {
+    const float* rp = V.p + z * V.sZ + x * V.sX;
+    const bool zx_in = z >= delta && z < V.D - delta && x >= delta && x < V.H - delta;
+    unsigned long long* mw = masks + ridx * ww;
+    const int y0 = max(V.w0, 0), y1 = min(V.W, V.w1);
+    for (int k = 0; k < ww; ++k) {
+      unsigned long long a = mw[k];
+      if (!a) continue;
+      unsigned long long m = 0;
+      while (a) {
+        const int b = __ffsll((long long)a) - 1;
+        a &= a - 1;
+        const int y = k * 64 + b;
+        if (y < y0 || y >= y1) continue;
+        const float c = sign * rp[y];
+        bool mx;
+        if (delta == 1) mx = is_max_d1(V, z, x, y, c, sign);
+        else if (sph) mx = zx_in && y >= delta && y < V.W - delta ? is_max_sph_in(rp + y, soff, nsph, c, sign)
+                                                                   : is_max_sph_border(V, z, x, y, c, sign, nsph);
+        else mx = is_max_at(V, z, x, y, c, sign, delta);
+        if (mx) m |= 1ull << b;
+      }
+      mw[k] = m;
+      cnt += __popcll(m);
+    }
+  }
+  rcnt[ridx] = cnt;
+}
+
 // exclusive scan of the block counts of volume blockIdx.x; total -> counts[v]
 __global__ __launch_bounds__(1024) void k_max_scan(int* __restrict__ bcnt, int nblk, int* __restrict__ counts) {
   const int v = blockIdx.x;
@@ -260,18 +311,18 @@ __global__ __launch_bounds__(1024) void k_max_scan(int* __restrict__ bcnt, int n
 template <class VS>
 __global__ __launch_bounds__(256) void k_max_compact(VS vs, const unsigned long long* __restrict__ masks,
                                                      const int* __restrict__ roff, const int* __restrict__ counts,
-                                                     int nrow_max, int ww, int* __restrict__ coords, int cap) {
+                                                     int nrow_max, int hl, int ww, int* __restrict__ coords,
+                                                     int cap) {
   const int v = blockIdx.y;
   const VolDesc V = vs(v);
   const int lane = threadIdx.x & 63;
   const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int nrow = V.D * V.H;
-  if (row >= nrow) return;
+  if (row >= nrow_max) return;
   const long long ridx = (long long)v * nrow_max + row;
   const int base = roff[ridx];
   const int next = row + 1 < nrow_max ? roff[ridx + 1] : counts[v];
   if (next == base || base >= cap) return;  // no maxima in this row, or all beyond the cap
-  const int z = row / V.H, x = row - (row / V.H) * V.H;
+  const int z = row / hl, x = row - (row / hl) * hl;
   const unsigned long long* mw = masks + ridx * ww;
   int pre = 0;
   for (int k = 0; k < ww; ++k) {
@@ -576,9 +627,11 @@ static int sphere_table(int dfit, int** dev_out, int* P_out) {
 }
 
 // D, H, W: bounds of every volume's extents (the workspace layout).
+// marked: the workspace's masks already hold the threshold bits of every row in
+// the candidate window (k_blur_sub_xy with a mark output), rows z H + x.
 template <class VS>
 static int run_maxima(VS vs, int nvol, int D, int H, int W, float sign, float thr, int delta, int* coords, int cap,
-                      int* counts, void* ws, size_t ws_bytes, hipStream_t st) {
+                      int* counts, void* ws, size_t ws_bytes, hipStream_t st, bool marked = false) {
   if (delta < 0 || delta > 31) { mm_set_error("delta %d outside [0, 31]", delta); return MM_EINVAL; }
   if (nvol == 0) return MM_OK;
   if (ws_bytes < mm_maxima_ws_bytes(D, H, W, nvol)) { mm_set_error("maxima workspace too small"); return MM_EINVAL; }
@@ -592,12 +645,17 @@ static int run_maxima(VS vs, int nvol, int D, int H, int W, float sign, float th
     MM_LAUNCH_CHECK();
     return MM_OK;
   }
-  hipLaunchKernelGGL((k_max_mark<VS>), dim3(nbx, nvol), dim3(256), 0, st, vs, sign, thr, delta, masks, rcnt, nrow,
-                     ww);
+  if (marked) {
+    hipLaunchKernelGGL((k_max_mark_above<VS>), dim3((nrow + 255) / 256, nvol), dim3(256), 0, st, vs, sign, delta,
+                       masks, rcnt, nrow, H, ww);
+  } else {
+    hipLaunchKernelGGL((k_max_mark<VS>), dim3(nbx, nvol), dim3(256), 0, st, vs, sign, thr, delta, masks, rcnt, nrow,
+                       H, ww);
+  }
   MM_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_max_scan, dim3(nvol), dim3(1024), 0, st, rcnt, nrow, counts);
   MM_LAUNCH_CHECK();
-  hipLaunchKernelGGL((k_max_compact<VS>), dim3(nbx, nvol), dim3(256), 0, st, vs, masks, rcnt, counts, nrow, ww,
+  hipLaunchKernelGGL((k_max_compact<VS>), dim3(nbx, nvol), dim3(256), 0, st, vs, masks, rcnt, counts, nrow, H, ww,
                      coords, cap);
   MM_LAUNCH_CHECK();
   return MM_OK;
@@ -651,6 +709,28 @@ extern "C" int mm_tiles_maxima_window(mm_deconv* d, const mm_tile_desc* tiles_de
   TileVols vs{(const mm_tile_dev*)tiles_dev, res, d->sz, d->T, lo, hi};
   return run_maxima(vs, ntiles, d->sz, d->T, d->T, 1.0f, threshold, delta, coords, cap, counts, ws, ws_bytes,
                     (hipStream_t)stream);
+}
+
+extern "C" int mm_tiles_blur_maxima_window(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles,
+                                           const float* res, float* out, int blur_delta, int blo, int bhi,
+                                           float threshold, int delta, int lo, int hi, int* coords, int cap,
+                                           int* counts, void* ws, size_t ws_bytes, void* stream) {
+  if (!d || !tiles_dev || !res || !out || blo < 0 || bhi > d->T || blo >= bhi || lo < blo || hi > bhi || hi <= lo) {
+    mm_set_error("mm_tiles_blur_maxima_window: bad argument (blur [%d, %d) maxima [%d, %d) T %d)", blo, bhi, lo, hi,
+                 d ? d->T : 0);
+    return MM_EINVAL;
+  }
+  if (delta < 0 || delta > 31) { mm_set_error("delta %d outside [0, 31]", delta); return MM_EINVAL; }
+  if (ntiles == 0) return MM_OK;
+  const int T = d->T, sz = d->sz;
+  if (ws_bytes < mm_maxima_ws_bytes(sz, T, T, ntiles)) { mm_set_error("maxima workspace too small"); return MM_EINVAL; }
+  hipStream_t st = (hipStream_t)stream;
+  // the masks of run_maxima's workspace, rows z T + x of each tile = rows of the blur's planes
+  int rc = mm_blur_sub_mark(res, out, ntiles * sz, T, blur_delta, blo, bhi, 1.0f, threshold,
+                            (unsigned long long*)ws, st);
+  if (rc) return rc;
+  TileVols vs{(const mm_tile_dev*)tiles_dev, out, sz, T, lo, hi};
+  return run_maxima(vs, ntiles, sz, T, T, 1.0f, threshold, delta, coords, cap, counts, ws, ws_bytes, st, true);
 }
 
 extern "C" int mm_tiles_fit(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const float* res,

@@ -270,13 +270,25 @@ class Deconvolver:
         per = self._plan.spec_bytes + 4 * self.tile_height * T * T + extra_per_tile
         return max(1, min(ntiles, self.batch_bytes // per))
 
-    def run_tiles(self, table, ntiles, tiles_per_block, blur_radius, res_out=None, window=None):
+    @staticmethod
+    def fused_blur_maxima(blur_radius):
+        """Whether ``run_tiles(..., maxima=...)`` can run the maxima inside the blur entry
+        (the fused blur widths, box half-widths 15 and 25)."""
+        return blur_radius is not None and int(blur_radius) // 2 in (15, 25)
+
+    def run_tiles(self, table, ntiles, tiles_per_block, blur_radius, res_out=None, window=None, maxima=None):
         """Deconvolution (+ blur subtraction) of ``ntiles`` descriptors: ``(ntiles, sz, T, T)``.
 
         With a blur the result lives in the spectrum workspace (valid until the next call).
         ``window=(lo, hi)`` computes the blur subtraction only at rows and columns [lo, hi)
         (``mm_blur_sub_xy_window``), and the deconvolution only where those boxes read
-        (``mm_deconv_tiles_window``); the rest of the result is left undefined."""
+        (``mm_deconv_tiles_window``); the rest of the result is left undefined.
+        ``maxima=(threshold, delta, (lo, hi), coords, cap, counts, ws, ws_bytes)`` with a window
+        and a fused blur width also finds the maxima of the result at rows and columns [lo, hi)
+        in the same entry (``mm_tiles_blur_maxima_window``, the outputs of
+        ``mm_tiles_maxima_window``)."""
+        if maxima is not None and (window is None or not self.fused_blur_maxima(blur_radius)):
+            raise ValueError("maxima in run_tiles need a window and a blur half-width of 15 or 25")
         T, sz = self._plan.T, self.tile_height
         spec = self._workspace("spec", ntiles * self._plan.spec_bytes)
         res = res_out if res_out is not None else torch.empty((ntiles, sz, T, T), dtype=torch.float32,
@@ -299,6 +311,11 @@ class Deconvolver:
             tmp = self._workspace("blur_tmp", ntiles * sz * T * T * 4)
         if window is None:
             L.call("mm_blur_sub_xy", L.ptr(res), L.ptr(out), L.ptr(tmp), ntiles * sz, T, delta, st)
+        elif maxima is not None:
+            thr, mdelta, (lo, hi), coords, cap, counts, ws, ws_bytes = maxima
+            L.call("mm_tiles_blur_maxima_window", self._plan.h, L.ptr(table), ntiles, L.ptr(res), L.ptr(out), delta,
+                   int(window[0]), int(window[1]), float(thr), int(mdelta), int(lo), int(hi), L.ptr(coords), int(cap),
+                   L.ptr(counts), L.ptr(ws), int(ws_bytes), st)
         else:
             L.call("mm_blur_sub_xy_window", L.ptr(res), L.ptr(out), L.ptr(tmp), ntiles * sz, T, delta,
                    int(window[0]), int(window[1]), st)

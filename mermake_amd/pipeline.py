@@ -105,18 +105,21 @@ class HybPipeline:
         T, sz = d._plan.T, d.tile_height
         st = L.stream_ptr(d.device)
         win = self.windows()
-        res = self._buf("res", (ntiles, sz, T, T), torch.float32)
-        res = d.run_tiles(table, ntiles, ntiles if single else 1, self.blur_radius, res_out=res,
-                          window=None if win is None else win[0])
         ws_bytes = L.lib().mm_maxima_ws_bytes(sz, T, T, ntiles)
         ws = self._buf("mws", (ws_bytes,), torch.uint8)
         coords = self._buf("coords", (ntiles, cap, 3), torch.int32)
         counts = self._buf("counts", (ntiles,), torch.int32)
         fit = self._buf("fit", (ntiles, cap, 8), torch.float32)
+        # the threshold test of the maxima rides on the blur (one entry) where the blur width allows
+        fused = win is not None and d.fused_blur_maxima(self.blur_radius)
+        mx = (self.threshold, self.delta, win[1], coords, cap, counts, ws, ws_bytes) if fused else None
+        res = self._buf("res", (ntiles, sz, T, T), torch.float32)
+        res = d.run_tiles(table, ntiles, ntiles if single else 1, self.blur_radius, res_out=res,
+                          window=None if win is None else win[0], maxima=mx)
         if win is None:
             L.call("mm_tiles_maxima", d._plan.h, L.ptr(table), ntiles, L.ptr(res), self.threshold, self.delta,
                    L.ptr(coords), cap, L.ptr(counts), L.ptr(ws), ws_bytes, st)
-        else:
+        elif not fused:
             L.call("mm_tiles_maxima_window", d._plan.h, L.ptr(table), ntiles, L.ptr(res), self.threshold,
                    self.delta, win[1][0], win[1][1], L.ptr(coords), cap, L.ptr(counts), L.ptr(ws), ws_bytes, st)
         L.call("mm_tiles_fit", d._plan.h, L.ptr(table), ntiles, L.ptr(res), L.ptr(coords), L.ptr(counts), cap,

@@ -117,18 +117,23 @@ class _Slot:
             else:
                 L.call("mm_deconv_tiles_window", *dargs, max(0, win[0][0] - delta), min(T, win[0][1] + delta), st)
             out = self.res
+            fused = win is not None and d.fused_blur_maxima(pipe.blur_radius)
             if delta is not None:
                 # blur subtraction into the spectrum buffer, which is free again (deconvolver.py:160-163)
                 out = self.spec[:n * sz * T * T * 4].view(torch.float32)
                 if win is None:
                     L.call("mm_blur_sub_xy", L.ptr(self.res), L.ptr(out), L.ptr(self.tmp), n * sz, T, delta, st)
+                elif fused:  # blur + maxima threshold bits, then the maxima of the candidates
+                    L.call("mm_tiles_blur_maxima_window", d._plan.h, tab, n, L.ptr(self.res), L.ptr(out), delta,
+                           win[0][0], win[0][1], pipe.threshold, pipe.delta, win[1][0], win[1][1],
+                           L.ptr(self.coords), self.cap, L.ptr(counts), L.ptr(self.mws), self.mws_bytes, st)
                 else:
                     L.call("mm_blur_sub_xy_window", L.ptr(self.res), L.ptr(out), L.ptr(self.tmp), n * sz, T, delta,
                            win[0][0], win[0][1], st)
             if win is None:
                 L.call("mm_tiles_maxima", d._plan.h, tab, n, L.ptr(out), pipe.threshold, pipe.delta,
                        L.ptr(self.coords), self.cap, L.ptr(counts), L.ptr(self.mws), self.mws_bytes, st)
-            else:
+            elif not fused:
                 L.call("mm_tiles_maxima_window", d._plan.h, tab, n, L.ptr(out), pipe.threshold, pipe.delta,
                        win[1][0], win[1][1], L.ptr(self.coords), self.cap, L.ptr(counts), L.ptr(self.mws),
                        self.mws_bytes, st)

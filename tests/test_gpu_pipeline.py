@@ -118,6 +118,57 @@ def test_hyb_window_equals_whole_tiles(dev, small_hyb, delta, delta_fit, ts, ov)
         assert np.array_equal(cw[t, :nw[t]], f[keep]), t
 
 
+@pytest.mark.parametrize("delta,blur_radius,thr,ts,ov,same_window", [
+    (1, 30, 2000.0, 100, 30, False), (1, 30, -50.0, 100, 30, False),  # almost every voxel a candidate
+    (3, 50, 500.0, 200, 13, False),   # T = 226, sorted-sphere test, blur half-width 25
+    (5, 30, 1000.0, 100, 30, True),   # largest sphere; maxima window = blur window
+    (2, 50, 800.0, 100, 30, False),
+    (1, 30, float("nan"), 300, 89, False)])  # NaN threshold rejects nothing; T = 478
+def test_blur_maxima_entry_equals_separate_calls(dev, small_hyb, delta, blur_radius, thr, ts, ov, same_window):
+    """``mm_tiles_blur_maxima_window`` (threshold bits written by the blur, maxima from the bits) gives the
+    blurred window and the candidate lists of ``mm_blur_sub_xy_window`` + ``mm_tiles_maxima_window``,
+    bit for bit, edge tiles included."""
+    from mermake_amd import _lib as L
+    from mermake_amd.pipeline import HybPipeline
+    raw, flat, psf = small_hyb
+    hyb = dict(HYB, delta=delta, delta_fit=3, threshold=thr, blur_radius=blur_radius)
+    win = HybPipeline.from_settings(psf, raw.shape, tile_size=ts, overlap=ov, beta=1e-4, **hyb)
+    d = win.d
+    geom, table = win._table(*_prepared(d, raw, flat, dev))
+    n = len(geom)
+    (b0, b1), (c0, c1) = win.windows()
+    if same_window:
+        c0, c1 = b0, b1
+    Tt, sz = d._plan.T, d.tile_height
+    bd = blur_radius // 2
+    print("blur+maxima entry", delta, blur_radius, thr, Tt, n, (b0, b1), (c0, c1), flush=True)
+    res = d.run_tiles(table, n, n, None).clone()
+    ws_bytes = L.lib().mm_maxima_ws_bytes(sz, Tt, Tt, n)
+    cap = 1 << 15
+    st = L.stream_ptr(dev)
+    got = []
+    for fused in (False, True):
+        ws = torch.full((ws_bytes,), 0xA5, dtype=torch.uint8, device=dev)  # stale workspace contents
+        out = torch.full_like(res, 7.0)
+        coords = torch.zeros((n, cap, 3), dtype=torch.int32, device=dev)
+        counts = torch.zeros(n, dtype=torch.int32, device=dev)
+        if fused:
+            L.call("mm_tiles_blur_maxima_window", d._plan.h, L.ptr(table), n, L.ptr(res), L.ptr(out), bd, b0, b1,
+                   thr, delta, c0, c1, L.ptr(coords), cap, L.ptr(counts), L.ptr(ws), ws_bytes, st)
+        else:
+            L.call("mm_blur_sub_xy_window", L.ptr(res), L.ptr(out), None, n * sz, Tt, bd, b0, b1, st)
+            L.call("mm_tiles_maxima_window", d._plan.h, L.ptr(table), n, L.ptr(out), thr, delta, c0, c1,
+                   L.ptr(coords), cap, L.ptr(counts), L.ptr(ws), ws_bytes, st)
+        torch.cuda.synchronize(dev)
+        got.append((out[:, :, b0:b1, b0:b1].cpu().numpy(), coords.cpu().numpy(), counts.cpu().numpy()))
+    (o1, k1, n1), (o2, k2, n2) = got
+    assert np.array_equal(o1.view(np.uint32), o2.view(np.uint32))
+    assert np.array_equal(n1, n2) and n1.sum() > 0
+    for t in range(n):
+        m = min(int(n1[t]), cap)
+        assert np.array_equal(k1[t, :m], k2[t, :m]), t
+
+
 def _prepared(d, raw, flat, dev):
     from mermake_amd.deconvolver import prepare_flat, prepare_stack
     img = prepare_stack(T(raw, dev))
