@@ -113,9 +113,10 @@ def serve_leg(pipe, stacks, flat, cfg, args, dev):
     """The request loop on mixed FOV sizes (40 x {2048, 2304, 2800}^2, 3 channels each, host inputs in
     pinned memory): Poisson arrivals at ``--serve-rate``, then all requests at once (saturation)."""
     import torch
+    from mermake_amd.deconvolver import SquaredU8
     from mermake_amd.serve import FovRequest, HybServer, poisson_arrivals
     from mermake_amd.synth import stack_torch
-    fovs = {}
+    fovs, fovs_u8 = {}, {}
     for side in (2048, 2304, 2800):
         if side == cfg["sx"]:
             chans, fl = list(stacks), flat
@@ -126,16 +127,21 @@ def serve_leg(pipe, stacks, flat, cfg, args, dev):
                 chans.append(s_)
                 fl = f_
         fovs[side] = ([c.cpu().pin_memory() for c in chans], fl.cpu().pin_memory())
+        # the microscope's uint8 frames of the same shapes (read_im squares them): half the PCIe bytes
+        fovs_u8[side] = ([SquaredU8.of(torch.sqrt(c.float()).clamp_(max=255).to(torch.uint8).cpu()).pin_memory()
+                          for c in chans], fovs[side][1])
         torch.cuda.synchronize()
     sides = [2048, 2304, 2800]
     rng = np.random.default_rng(7)
     order = [sides[i] for i in rng.integers(0, 3, size=args.serve_requests)]
     srv = HybServer(pipe, batch_tiles=36, slots=2)
-    out = {"fov_sides": order, "inputs": "host pinned uint16 stacks + float32 flat (H2D inside the loop)"}
-    for name, arr in (("poisson", poisson_arrivals(len(order), args.serve_rate, seed=11)),
-                      ("saturated", np.zeros(len(order)))):
-        srv.run([FovRequest(-1 - i, *fovs[side]) for i, side in enumerate(order[:2])])  # warm the slots
-        reqs = [FovRequest(i, *fovs[side], arrival=float(a)) for i, (side, a) in enumerate(zip(order, arr))]
+    out = {"fov_sides": order, "inputs": "host pinned uint16 stacks + float32 flat (H2D inside the loop); "
+                                         "saturated_u8: the uint8 frames whose squares are the pixels (SquaredU8)"}
+    for name, arr, src in (("poisson", poisson_arrivals(len(order), args.serve_rate, seed=11), fovs),
+                           ("saturated", np.zeros(len(order)), fovs),
+                           ("saturated_u8", np.zeros(len(order)), fovs_u8)):
+        srv.run([FovRequest(-1 - i, *src[side]) for i, side in enumerate(order[:2])])  # warm the slots
+        reqs = [FovRequest(i, *src[side], arrival=float(a)) for i, (side, a) in enumerate(zip(order, arr))]
         srv.run(reqs)
         st = dict(srv.stats)
         vox = sum(cfg["channels"] * cfg["sz"] * s_ * s_ for s_ in order)

@@ -76,7 +76,7 @@ typedef struct mm_deconv mm_deconv;
 /* Tile descriptor, one per overlapped tile (deconvolver.py:186-206 window,
  * :118-148 placement/reflection).  Layout is part of the ABI (104 bytes). */
 typedef struct mm_tile_desc {
-  const void* img;   /* stack base (uint16 or float32), element (0,0,0) */
+  const void* img;   /* stack base (see img_type), element (0,0,0) */
   const float* flat; /* 2D flat field base, or NULL */
   long long img_sz;  /* stack stride between z planes */
   long long img_sx;  /* stack stride between x rows */
@@ -88,7 +88,7 @@ typedef struct mm_tile_desc {
   int xe, ye;        /* extents of the yielded views, clipped to T */
   int psf;           /* Wiener spectrum index */
   int ox, oy;        /* tile origin (x, y) */
-  int img_f32;       /* 1: float32 stack, 0: uint16 */
+  int img_type;  /* 0: uint16, 1: float32, 2: uint8 whose squares are the pixels (read_im, io.py:140-142) */
   int pad0, pad1;
 } mm_tile_desc;
 

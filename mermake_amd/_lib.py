@@ -115,6 +115,6 @@ TILE_DTYPE = np.dtype([
     ("img", "<u8"), ("flat", "<u8"), ("img_sz", "<i8"), ("img_sx", "<i8"), ("flat_sx", "<i8"),
     ("X0", "<i4"), ("Y0", "<i4"), ("xd", "<i4"), ("yd", "<i4"), ("xs", "<i4"), ("ys", "<i4"),
     ("xfirst", "<i4"), ("yfirst", "<i4"), ("xe", "<i4"), ("ye", "<i4"), ("psf", "<i4"),
-    ("ox", "<i4"), ("oy", "<i4"), ("img_f32", "<i4"), ("pad0", "<i4"), ("pad1", "<i4"),
+    ("ox", "<i4"), ("oy", "<i4"), ("img_type", "<i4"), ("pad0", "<i4"), ("pad1", "<i4"),
 ])
 assert TILE_DTYPE.itemsize == 104

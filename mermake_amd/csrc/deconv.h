@@ -22,8 +22,13 @@ struct SpecLayout {
 
 // One overlapped tile of a z-stack (reference: deconvolver.py:186-206 for the
 // window, :118-148 for the placement and reflections).  Filled on the host.
+// Stack pixel types: uint16, float32, or the microscope's uint8 values whose
+// squares are the pixels (read_im's astype(uint16) ** 2, io.py:140-142; exact in
+// float32, at most 255^2), squared on load.
+enum { MM_PIX_U16 = 0, MM_PIX_F32 = 1, MM_PIX_U8SQ = 2 };
+
 struct mm_tile_dev {
-  const void* img;          // stack base (uint16 or float32), element (0,0,0)
+  const void* img;          // stack base (see img_type), element (0,0,0)
   const float* flat;        // flat-field base (2D, element (0,0)) or nullptr
   long long img_sz;         // stack stride between z planes (elements)
   long long img_sx;         // stack stride between x rows (elements); y stride is 1
@@ -35,7 +40,7 @@ struct mm_tile_dev {
   int xe, ye;               // extents of the yielded view (clipped to T)
   int psf;                  // index of the Wiener spectrum
   int ox, oy;               // tile origin in the stack (for spot offsets)
-  int img_f32;              // 1: float32 stack, 0: uint16 stack
+  int img_type;              // MM_PIX_U16, MM_PIX_F32 or MM_PIX_U8SQ
   int pad0, pad1;
 };
 
@@ -58,10 +63,19 @@ __device__ __forceinline__ int pad_map(int j, int s, int d, int first, int T) {
   return -1;
 }
 
+__device__ __forceinline__ float pix_at(const void* p, int type, long long o) {
+  if (type == MM_PIX_F32) return ((const float*)p)[o];
+  if (type == MM_PIX_U8SQ) {
+    const unsigned u = ((const uint8_t*)p)[o];
+    return (float)(u * u);
+  }
+  return (float)((const uint16_t*)p)[o];
+}
+
 // Value of the flat-divided padded tile at data position (z, r, c) of the window.
 __device__ __forceinline__ float tile_src(const mm_tile_dev& t, int z, int r, int c) {
   const long long o = (long long)z * t.img_sz + (long long)(t.X0 + r) * t.img_sx + (t.Y0 + c);
-  float v = t.img_f32 ? ((const float*)t.img)[o] : (float)((const uint16_t*)t.img)[o];
+  float v = pix_at(t.img, t.img_type, o);
   if (t.flat) v = v / t.flat[(long long)(t.X0 + r) * t.flat_sx + (t.Y0 + c)];
   return v;
 }
@@ -74,7 +88,7 @@ __device__ __forceinline__ float tile_raw(const mm_tile_dev& t, int T, int z, in
   const int c = pad_map(y, t.ys, t.yd, t.yfirst, T);
   if (r < 0 || c < 0) return 0.0f;
   const long long o = (long long)z * t.img_sz + (long long)(t.X0 + r) * t.img_sx + (t.Y0 + c);
-  float v = t.img_f32 ? ((const float*)t.img)[o] : (float)((const uint16_t*)t.img)[o];
+  float v = pix_at(t.img, t.img_type, o);
   if (t.flat) {
     const float f = t.flat[(long long)(t.X0 + r) * t.flat_sx + (t.Y0 + c)];
     v = v / f;

@@ -145,7 +145,7 @@ __global__ __launch_bounds__(256) void k_tile_r2c(const mm_tile_dev* __restrict_
         const int cc = ymap[y];
         if (rr >= 0 && cc >= 0) {
           const long long o = (long long)z * td.img_sz + (long long)(td.X0 + rr) * td.img_sx + (td.Y0 + cc);
-          v[u] = td.img_f32 ? ((const float*)td.img)[o] : (float)((const uint16_t*)td.img)[o];
+          v[u] = pix_at(td.img, td.img_type, o);
           if (td.flat) f[u] = td.flat[(long long)(td.X0 + rr) * td.flat_sx + (td.Y0 + cc)];
         }
       }
@@ -369,13 +369,22 @@ struct R2cRaw {
   unsigned m0;
 };
 
-template <int M, bool F32, bool FLAT>
+// Raw bits of a stack element: float32 bits, or the integer pixel value (u8 squared).
+template <int PIX>
+__device__ __forceinline__ unsigned pix_bits(const void* img, long long o) {
+  if (PIX == MM_PIX_F32) return __float_as_uint(((const float*)img)[o]);
+  if (PIX == MM_PIX_U8SQ) {
+    const unsigned u = ((const uint8_t*)img)[o];
+    return u * u;
+  }
+  return (unsigned)((const uint16_t*)img)[o];
+}
+
+template <int M, int PIX, bool FLAT>
 __device__ __forceinline__ void r2c_fetch(int G, const PfItems& I, R2cRaw& R, const mm_tile_dev& td,
                                           const R2cTab& tb) {
   const float* fl = td.flat;
-  auto raw = [&](long long o) -> unsigned {
-    return F32 ? __float_as_uint(((const float*)td.img)[o]) : (unsigned)((const uint16_t*)td.img)[o];
-  };
+  auto raw = [&](long long o) -> unsigned { return pix_bits<PIX>(td.img, o); };
   R.msk = 0;
   R.m0 = 0;
   pf_fetch<M>(
@@ -486,17 +495,20 @@ __global__ __launch_bounds__(256) void k_pf_r2c(const mm_tile_dev* __restrict__ 
   // pixel type and flat presence are uniform over a group (one tile): one code path each
   auto fetch = [&](long long gi, const R2cTab& tb) {
     const mm_tile_dev& td = tiles[(int)(gi / nrb)];
-    if (td.img_f32) {
-      if (td.flat) r2c_fetch<M, true, true>(G, I, R, td, tb);
-      else r2c_fetch<M, true, false>(G, I, R, td, tb);
+    if (td.img_type == MM_PIX_F32) {
+      if (td.flat) r2c_fetch<M, MM_PIX_F32, true>(G, I, R, td, tb);
+      else r2c_fetch<M, MM_PIX_F32, false>(G, I, R, td, tb);
+    } else if (td.img_type == MM_PIX_U8SQ) {
+      if (td.flat) r2c_fetch<M, MM_PIX_U8SQ, true>(G, I, R, td, tb);
+      else r2c_fetch<M, MM_PIX_U8SQ, false>(G, I, R, td, tb);
     } else {
-      if (td.flat) r2c_fetch<M, false, true>(G, I, R, td, tb);
-      else r2c_fetch<M, false, false>(G, I, R, td, tb);
+      if (td.flat) r2c_fetch<M, MM_PIX_U16, true>(G, I, R, td, tb);
+      else r2c_fetch<M, MM_PIX_U16, false>(G, I, R, td, tb);
     }
   };
   auto put = [&](long long gi) {
     const mm_tile_dev& td = tiles[(int)(gi / nrb)];
-    if (td.img_f32) {
+    if (td.img_type == MM_PIX_F32) {
       if (td.flat) r2c_put<M, true, true>(S, G, L.sd, I, R);
       else r2c_put<M, true, false>(S, G, L.sd, I, R);
     } else {
@@ -728,7 +740,7 @@ __device__ __forceinline__ long long rd_readlane64(long long v, int lane) {
 }
 
 // Rows x0 .. x0 + NR - 1 of plane z (those < T exist).
-template <int PR, int M, int NR, int NC, bool F32, bool FLAT>
+template <int PR, int M, int NR, int NC, int PIX, bool FLAT>
 __device__ __forceinline__ void rd_r2c_load(RdR2cRaw<NR, NC>& R, const mm_tile_dev& td, const RdCols<NC>& C, int T,
                                             int z, int x0) {
   unsigned c[NC];  // column offsets in elements (absent columns read data column 0 of the row)
@@ -759,11 +771,17 @@ __device__ __forceinline__ void rd_r2c_load(RdR2cRaw<NR, NC>& R, const mm_tile_d
     const long long ro = rd_readlane64(ro_l, i), fo = rd_readlane64(fo_l, i);
     // uniform row pointers in the global address space: global loads with a
     // scalar base and a 32-bit lane offset
-    const gptr<const char> rb = (gptr<const char>)td.img + ro * (F32 ? 4 : 2);
+    constexpr int ES = PIX == MM_PIX_F32 ? 4 : PIX == MM_PIX_U16 ? 2 : 1;  // element bytes
+    const gptr<const char> rb = (gptr<const char>)td.img + ro * ES;
     const gptr<const char> fb = (gptr<const char>)td.flat + fo * 4;
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
-      R.P[i][k] = F32 ? *(gptr<const unsigned>)(rb + c[k] * 4u) : (unsigned)*(gptr<const uint16_t>)(rb + c[k] * 2u);
+      if (PIX == MM_PIX_F32) R.P[i][k] = *(gptr<const unsigned>)(rb + c[k] * 4u);
+      else if (PIX == MM_PIX_U16) R.P[i][k] = (unsigned)*(gptr<const uint16_t>)(rb + c[k] * 2u);
+      else {
+        const unsigned u = *(gptr<const uint8_t>)(rb + c[k]);
+        R.P[i][k] = u * u;
+      }
       if (FLAT) R.F[i][k] = *(gptr<const float>)(fb + c[k] * 4u);
     }
   }
@@ -811,18 +829,21 @@ __global__ __launch_bounds__(256, 2) void k_rd_r2c(const mm_tile_dev* __restrict
     int tile, z, x0;
     where(gi, tile, z, x0);
     const mm_tile_dev& td = tiles[tile];
-    if (td.img_f32) {
-      if (td.flat) rd_r2c_load<PR, M, NR, NC, true, true>(R, td, C, T, z, x0);
-      else rd_r2c_load<PR, M, NR, NC, true, false>(R, td, C, T, z, x0);
+    if (td.img_type == MM_PIX_F32) {
+      if (td.flat) rd_r2c_load<PR, M, NR, NC, MM_PIX_F32, true>(R, td, C, T, z, x0);
+      else rd_r2c_load<PR, M, NR, NC, MM_PIX_F32, false>(R, td, C, T, z, x0);
+    } else if (td.img_type == MM_PIX_U8SQ) {
+      if (td.flat) rd_r2c_load<PR, M, NR, NC, MM_PIX_U8SQ, true>(R, td, C, T, z, x0);
+      else rd_r2c_load<PR, M, NR, NC, MM_PIX_U8SQ, false>(R, td, C, T, z, x0);
     } else {
-      if (td.flat) rd_r2c_load<PR, M, NR, NC, false, true>(R, td, C, T, z, x0);
-      else rd_r2c_load<PR, M, NR, NC, false, false>(R, td, C, T, z, x0);
+      if (td.flat) rd_r2c_load<PR, M, NR, NC, MM_PIX_U16, true>(R, td, C, T, z, x0);
+      else rd_r2c_load<PR, M, NR, NC, MM_PIX_U16, false>(R, td, C, T, z, x0);
     }
   };
   auto tile_of = [&](long long gi) { return (int)(gi / ((long long)sz * nband)); };
   auto put = [&](long long gi) {
     const mm_tile_dev& td = tiles[tile_of(gi)];
-    if (td.img_f32) {
+    if (td.img_type == MM_PIX_F32) {
       if (td.flat) rd_r2c_store<PR, M, NR, NC, true, true>(L.bufA, R, C);
       else rd_r2c_store<PR, M, NR, NC, true, false>(L.bufA, R, C);
     } else {

@@ -9,7 +9,7 @@
 // (z, x) of the volume, handled by one wavefront with the lanes along y:
 //   k_max_mark     one bit per voxel (64-voxel ballot words per row) + row counts
 //   k_max_scan     exclusive scan of the row counts of each volume
-//   k_max_compact  popcount ranks -> (z, x, y) at row offset + rank
+//   k_max_compact  (z, x, y) of each set bit at its row offset + rank, one thread per row
 // The fit runs one wavefront per candidate: the lanes sample the delta_fit
 // sphere, and the statistics are wave reductions (two-pass mean/variance).
 #include "deconv.h"
@@ -307,36 +307,32 @@ __global__ __launch_bounds__(1024) void k_max_scan(int* __restrict__ bcnt, int n
   if (threadIdx.x == 0) counts[v] = carry;
 }
 
-// Row r of volume blockIdx.y writes its maxima at roff[r] + rank (raster order).
-template <class VS>
-__global__ __launch_bounds__(256) void k_max_compact(VS vs, const unsigned long long* __restrict__ masks,
+// Row r of volume blockIdx.y writes its maxima at roff[r] + rank (raster order):
+// one thread per row, since most rows hold none.
+__global__ __launch_bounds__(256) void k_max_compact(const unsigned long long* __restrict__ masks,
                                                      const int* __restrict__ roff, const int* __restrict__ counts,
                                                      int nrow_max, int hl, int ww, int* __restrict__ coords,
                                                      int cap) {
   const int v = blockIdx.y;
-  const VolDesc V = vs(v);
-  const int lane = threadIdx.x & 63;
-  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= nrow_max) return;
   const long long ridx = (long long)v * nrow_max + row;
-  const int base = roff[ridx];
+  int pos = roff[ridx];
   const int next = row + 1 < nrow_max ? roff[ridx + 1] : counts[v];
-  if (next == base || base >= cap) return;  // no maxima in this row, or all beyond the cap
+  if (next == pos || pos >= cap) return;  // no maxima in this row, or all beyond the cap
   const int z = row / hl, x = row - (row / hl) * hl;
   const unsigned long long* mw = masks + ridx * ww;
-  int pre = 0;
+  int* o = coords + (long long)v * cap * 3;
   for (int k = 0; k < ww; ++k) {
-    const unsigned long long m = mw[k];
-    if ((m >> lane) & 1ull) {
-      const int pos = base + pre + __popcll(m & ((1ull << lane) - 1ull));
-      if (pos < cap) {
-        int* o = coords + ((long long)v * cap + pos) * 3;
-        o[0] = z;
-        o[1] = x;
-        o[2] = k * 64 + lane;
-      }
+    unsigned long long m = mw[k];
+    while (m) {
+      const int b = __ffsll((long long)m) - 1;
+      m &= m - 1;
+      o[pos * 3] = z;
+      o[pos * 3 + 1] = x;
+      o[pos * 3 + 2] = k * 64 + b;
+      if (++pos >= cap) return;
     }
-    pre += __popcll(m);
   }
 }
 
@@ -655,8 +651,8 @@ static int run_maxima(VS vs, int nvol, int D, int H, int W, float sign, float th
   MM_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_max_scan, dim3(nvol), dim3(1024), 0, st, rcnt, nrow, counts);
   MM_LAUNCH_CHECK();
-  hipLaunchKernelGGL((k_max_compact<VS>), dim3(nbx, nvol), dim3(256), 0, st, vs, masks, rcnt, counts, nrow, H, ww,
-                     coords, cap);
+  hipLaunchKernelGGL(k_max_compact, dim3((nrow + 255) / 256, nvol), dim3(256), 0, st, masks, rcnt, counts, nrow, H,
+                     ww, coords, cap);
   MM_LAUNCH_CHECK();
   return MM_OK;
 }

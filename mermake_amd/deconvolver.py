@@ -142,18 +142,44 @@ def tile_records(geom, image, flat):
         arr[i]["ox"], arr[i]["oy"] = g["x"], g["y"]
         arr[i]["img"] = image.data_ptr()
         arr[i]["img_sz"], arr[i]["img_sx"] = image.stride(0), image.stride(1)
-        arr[i]["img_f32"] = 1 if image.dtype == torch.float32 else 0
+        arr[i]["img_type"] = pixel_type(image)
         if flat is not None:
             arr[i]["flat"] = flat.data_ptr()
             arr[i]["flat_sx"] = flat.stride(0)
     return arr
 
 
+class SquaredU8(torch.Tensor):
+    """A uint8 stack of the microscope's raw values whose pixels are their squares, the
+    ``astype(uint16) ** 2`` of ``read_im`` (io.py:140-142).  The tile loader squares them on the
+    device, so such a stack is half the bytes of its uint16 form to copy and to read, with the same
+    results.  Make one with ``SquaredU8.of(frames)``; ``.to(device)`` and views keep the type."""
+
+    @staticmethod
+    def of(frames):
+        t = frames if isinstance(frames, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(frames))
+        if t.dtype != torch.uint8:
+            raise TypeError(f"SquaredU8 holds uint8 values, not {t.dtype}")
+        return t.as_subclass(SquaredU8)
+
+
+def pixel_type(image):
+    """``mm_tile_desc.img_type`` of a prepared stack: 0 uint16, 1 float32, 2 squared uint8."""
+    if isinstance(image, SquaredU8):
+        return 2
+    return 1 if image.dtype == torch.float32 else 0
+
+
 def prepare_stack(image):
-    """The stack as a device tensor the tile loader reads (uint16 or float32, unit y stride)."""
+    """The stack as a device tensor the tile loader reads (uint16, float32 or ``SquaredU8``, unit
+    y stride)."""
     image = as_device(image)
     if image.ndim != 3:
         raise ValueError("expected a (z, x, y) stack")
+    if isinstance(image, SquaredU8):
+        if image.dtype != torch.uint8:
+            raise TypeError("a SquaredU8 stack must hold uint8 values")
+        return unit_last_stride(image)
     if image.dtype not in (torch.uint16, torch.float32):
         image = image.to(torch.float32)   # tile_pad[...] = tile casts to float32 (deconvolver.py:124)
     return unit_last_stride(image)

@@ -115,3 +115,17 @@ def test_synthetic_generator_seeded():
     assert a.dtype == np.uint16 and np.array_equal(a, b) and f.shape == (40, 40)
     p = gaussian_psf((21, 41, 41), 1.5, 1.2)
     assert abs(p.sum() - 1) < 1e-5 and np.unravel_index(p.argmax(), p.shape) == (10, 20, 20)
+
+
+def test_squared_u8_stack_type():
+    """``SquaredU8`` keeps its type through views and copies, maps to pixel type 2 and holds
+    uint8 only."""
+    import pytest
+    import torch
+    from mermake_amd.deconvolver import SquaredU8, pixel_type
+    a = SquaredU8.of(np.arange(60, dtype=np.uint8).reshape(3, 4, 5))
+    for t in (a, a[:, 1:], a.contiguous(), a.clone()):
+        assert isinstance(t, SquaredU8) and pixel_type(t) == 2
+    assert pixel_type(torch.zeros(2, dtype=torch.uint16)) == 0 and pixel_type(torch.zeros(2)) == 1
+    with pytest.raises(TypeError):
+        SquaredU8.of(np.zeros((2, 2, 2), np.uint16))

@@ -169,6 +169,31 @@ def test_blur_maxima_entry_equals_separate_calls(dev, small_hyb, delta, blur_rad
         assert np.array_equal(k1[t, :m], k2[t, :m]), t
 
 
+@pytest.mark.parametrize("ts,ov", [(100, 30), (200, 13), (300, 89)])  # prime-first, Rader 113, Rader 239 loaders
+def test_squared_u8_stack_equals_uint16_squares(dev, small_hyb, ts, ov):
+    """A ``SquaredU8`` stack (uint8 values squared by the tile loader) gives the spots, the deconvolved
+    tiles and the raw views of its uint16 squares, bit for bit."""
+    from mermake_amd.deconvolver import SquaredU8
+    from mermake_amd.pipeline import HybPipeline
+    raw, flat, psf = small_hyb
+    u8 = np.minimum(np.sqrt(raw), 255).astype(np.uint8)
+    u8[0, :3, :5] = 255  # the largest square
+    sq = u8.astype(np.uint16) ** 2
+    hyb = dict(HYB, threshold=500)
+    pipe = HybPipeline.from_settings(psf, raw.shape, tile_size=ts, overlap=ov, beta=1e-4, **hyb)
+    a = pipe(SquaredU8.of(T(u8, dev)), T(flat, dev)).cpu().numpy()
+    b = pipe(T(sq, dev), T(flat, dev)).cpu().numpy()
+    assert len(a) > 20 and np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    d = pipe.d
+    # each generator's views live in buffers the next batch reuses: copy one side first
+    ga = [(x, y, t.clone(), r.clone()) for x, y, t, r in d.tile_wise(SquaredU8.of(T(u8, dev)), flat_field=T(flat, dev))]
+    gb = list(d.tile_wise(T(sq, dev), flat_field=T(flat, dev)))
+    assert len(ga) == len(gb) > 0
+    for (xa, ya, ta, ra), (xb, yb, tb, rb) in zip(ga, gb):
+        assert (xa, ya) == (xb, yb)
+        assert torch.equal(ta, tb) and torch.equal(ra, rb)
+
+
 def _prepared(d, raw, flat, dev):
     from mermake_amd.deconvolver import prepare_flat, prepare_stack
     img = prepare_stack(T(raw, dev))

@@ -129,3 +129,25 @@ def test_server_keyed_multi_psf_equals_alone(fovs, dev):
     for r in done:
         for c, sp in enumerate(r.spots):
             assert np.array_equal(sp, alone[r.rid][c]), (r.rid, c)
+
+
+def test_server_squared_u8_requests_equal_uint16(fovs, pipe, dev):
+    """FOVs sent as pinned ``SquaredU8`` frames (half the bytes) give the spots of their uint16
+    squares, bit for bit, mixed with uint16 requests."""
+    from mermake_amd.deconvolver import SquaredU8
+    from mermake_amd.serve import FovRequest, HybServer
+    data, _ = fovs
+    reqs, want = [], []
+    for i, (chans, flat) in enumerate(data):
+        u8 = [np.minimum(np.sqrt(c), 255).astype(np.uint8) for c in chans]
+        sq = [u.astype(np.uint16) ** 2 for u in u8]
+        fl = torch.from_numpy(flat)
+        want.append([pipe(torch.from_numpy(c).to(dev), fl.to(dev)).cpu().numpy() for c in sq])
+        ch = [SquaredU8.of(torch.from_numpy(u)).pin_memory() if i % 2 == 0 else torch.from_numpy(c).pin_memory()
+              for u, c in zip(u8, sq)]
+        reqs.append(FovRequest(i, ch, fl.pin_memory(), arrival=0.0))
+    assert sum(len(x) for w in want for x in w) > 20
+    done = HybServer(pipe, batch_tiles=7, slots=2).run(reqs)
+    for r in done:
+        for c, s in enumerate(r.spots):
+            assert np.array_equal(s, want[r.rid][c]), (r.rid, c)
