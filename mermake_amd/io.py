@@ -123,10 +123,12 @@ def _xml_tag(txt, tag):
     return txt.split(tag)[-1].split("</")[0]
 
 
-def read_im(path, return_pos=False):
+def read_im(path, return_pos=False, square=True):
     """``io.py:116-146``.  ``path`` names ``<dir>/<prefix>_<fov>.zarr``; the array is
     ``<dir>/<fov>/data``.  Returns (channels, z, x, y) when the XML beside it gives the channel
-    count, else (frames, x, y); with ``return_pos`` also the stage position (x, y)."""
+    count, else (frames, x, y); with ``return_pos`` also the stage position (x, y).
+    ``square=False`` leaves uint8 frames unsquared (for ``SquaredU8`` stacks, squared on the
+    device)."""
     dirname = os.path.dirname(path)
     fov = os.path.basename(path).split("_")[-1].split(".")[0]
     image = read_zarr(os.path.join(dirname, fov, "data"), first=1)
@@ -142,7 +144,7 @@ def read_im(path, return_pos=False):
         nch = int(zstack.split(":")[-1])
         nzs = (shape[0] // nch) * nch
         image = image[:nzs].reshape([shape[0] // nch, nch, shape[-2], shape[-1]]).swapaxes(0, 1)
-    if image.dtype == np.uint8:  # squared in place of the widened copy
+    if image.dtype == np.uint8 and square:  # squared in place of the widened copy
         image = image.astype(np.uint16)
         np.multiply(image, image, out=image)
     if return_pos:
@@ -201,10 +203,12 @@ class FovFeeder:
     are handed out, so they can go straight into ``HybPipeline`` / ``HybServer``.
 
     The reference's ``ImageQueue`` prefetches decoded images on a thread (``io.py:448-463``);
-    its host-to-device copy then runs on the compute stream."""
+    its host-to-device copy then runs on the compute stream.  ``u8=True`` hands uint8 frames out
+    as ``SquaredU8`` channels (squared by the tile loaders): half the bytes to stage and copy."""
 
-    def __init__(self, paths, device=None, prefetch=2):
+    def __init__(self, paths, device=None, prefetch=2, u8=False):
         self.paths = list(paths)
+        self.u8 = u8
         self.device = device or torch.device("cuda", torch.cuda.current_device())
         self.copy_stream = torch.cuda.Stream(device=self.device)
         self._q = queue.Queue(maxsize=max(1, prefetch))
@@ -224,7 +228,7 @@ class FovFeeder:
                 break
             try:
                 t0 = time.perf_counter()
-                im, x, y = read_im(p, return_pos=True)
+                im, x, y = read_im(p, return_pos=True, square=not self.u8)
                 self.read_seconds += time.perf_counter() - t0
                 self._q.put((p, im, x, y, None))
             except Exception as e:  # reported to the consumer, as the reference warns and goes on
@@ -247,7 +251,11 @@ class FovFeeder:
             raw = buf[:nbytes].to(self.device, non_blocking=True)
             ev = torch.cuda.Event()
             ev.record(self.copy_stream)
-        dev = raw.view(torch.uint16 if im.dtype == np.uint16 else torch.float32).view(im.shape)
+        if im.dtype == np.uint8:
+            from .deconvolver import SquaredU8
+            dev = SquaredU8.of(raw.view(im.shape))
+        else:
+            dev = raw.view(torch.uint16 if im.dtype == np.uint16 else torch.float32).view(im.shape)
         self._busy[k] = ev
         self.h2d_bytes += nbytes
         return dev, ev
@@ -262,7 +270,7 @@ class FovFeeder:
         p, im, x, y, err = item
         if err is not None:
             raise RuntimeError(f"failed to read {p}: {err}") from err
-        if im.dtype not in (np.uint16, np.float32):
+        if im.dtype not in (np.uint16, np.float32) and not (self.u8 and im.dtype == np.uint8):
             im = im.astype(np.float32)
         dev, ev = self._stage(im)
         torch.cuda.current_stream(self.device).wait_event(ev)

@@ -2,6 +2,7 @@
 oracle, and the pinned double-buffered ``FovFeeder``."""
 import numpy as np
 import pytest
+import torch
 
 from mermake_amd import io as mio
 from oracle import io as oio
@@ -51,6 +52,22 @@ def test_fov_feeder_stages_fovs_to_device(dev, tmp_path):
         assert len(chans) == 3
         for c in range(3):
             np.testing.assert_array_equal(chans[c], want[i][c])
+
+
+def test_fov_feeder_u8_hands_out_squared_u8(dev, tmp_path):
+    """``u8=True``: the channels are ``SquaredU8`` views of the uint8 frames, whose squares are
+    ``read_im``'s channels."""
+    from mermake_amd.deconvolver import SquaredU8
+    rng = np.random.default_rng(8)
+    frames = rng.integers(0, 256, (1 + 3 * 5, 24, 40), dtype=np.uint8)
+    p = make_fov(str(tmp_path / "H2_set1"), "004", frames)
+    want = mio.read_im(p)
+    with mio.FovFeeder([p], device=dev, u8=True) as feed:
+        (_, chans, _, _), = list(feed)
+    assert feed.h2d_bytes == want.size  # one byte a pixel
+    for c in range(3):
+        assert isinstance(chans[c], SquaredU8) and chans[c].dtype == torch.uint8
+        np.testing.assert_array_equal(chans[c].cpu().numpy().astype(np.uint16) ** 2, want[c])
 
 
 def test_fov_feeder_reports_unreadable_fov(dev, tmp_path):

@@ -60,6 +60,15 @@ def test_read_im_deinterleaves_and_squares(tmp_path):
     assert mio.get_ifov(path) == 31 and mio.get_iset(path) == 7
 
 
+def test_read_im_unsquared_uint8(tmp_path):
+    rng = np.random.default_rng(9)
+    frames = rng.integers(0, 256, (1 + 3 * 2, 6, 7), dtype=np.uint8)
+    path = make_fov(str(tmp_path / "H3_set1"), "005", frames)
+    raw = mio.read_im(path, square=False)
+    assert raw.dtype == np.uint8 and raw.shape == (3, 2, 6, 7)
+    np.testing.assert_array_equal(raw.astype(np.uint16) ** 2, mio.read_im(path))
+
+
 def test_read_im_without_xml(tmp_path):
     frames = np.arange(5 * 4 * 4, dtype=np.uint8).reshape(5, 4, 4)
     path = make_fov(str(tmp_path / "d"), "002", frames)
