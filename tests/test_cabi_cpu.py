@@ -42,6 +42,20 @@ def test_version_and_error_string():
     assert isinstance(L.mm_last_error(), bytes)
 
 
+def test_blur_maxima_entry_rejects_bad_windows_without_a_gpu():
+    """Argument checks of ``mm_tiles_blur_maxima_window`` run before any device call: a missing
+    plan is MM_EINVAL with a message."""
+    L = ctypes.CDLL(_lib_path())
+    L.mm_last_error.restype = ctypes.c_char_p
+    f = L.mm_tiles_blur_maxima_window
+    f.restype = ctypes.c_int
+    vp, i, fl = ctypes.c_void_p, ctypes.c_int, ctypes.c_float
+    f.argtypes = [vp, vp, i, vp, vp, i, i, i, fl, i, i, i, vp, i, vp, vp, ctypes.c_size_t, vp]
+    one = ctypes.c_void_p(1)
+    assert f(None, one, 1, one, one, 15, 10, 50, 100.0, 1, 12, 40, one, 8, one, one, 1 << 20, None) == 1000
+    assert b"bad argument" in L.mm_last_error()
+
+
 def test_tile_descriptor_layout():
     from mermake_amd import _lib
     assert _lib.TILE_DTYPE.itemsize == 104
