@@ -168,9 +168,10 @@ int mm_local_maxima(const float* img, int D, int H, int W, long long sZ, long lo
                     int delta, int* coords, int cap, int* count, void* ws, size_t ws_bytes, void* stream);
 /* Replaces delta_fit_cross_corr (mermake/maxima.cu:148-341): out8 rows
  * [zc, xc, yc, background, a, habs, hn, h] for the *count candidates.
- * raw: float32 (raw_f32=1) or uint16 volume with strides rZ, rX. */
+ * raw: volume with strides rZ, rX whose pixels are raw_type 0 uint16,
+ * 1 float32 or 2 uint8 values standing for their squares (as img_type). */
 int mm_delta_fit(const float* img, int D, int H, int W, long long sZ, long long sX, float sign, const void* raw,
-                 int raw_f32, long long rZ, long long rX, const int* coords, const int* count, int cap, int delta_fit,
+                 int raw_type, long long rZ, long long rX, const int* coords, const int* count, int cap, int delta_fit,
                  float sigmaZ, float sigmaXY, float* out8, void* stream);
 
 /* Batched forms over the tile views res[t][:, :xe, :ye] of mm_deconv_tiles:
@@ -192,7 +193,9 @@ int mm_tiles_maxima_window(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntil
  * with the same outputs as those two calls.  The blur kernel also records the
  * threshold test of every output it writes, as bits in the maxima workspace,
  * so the maxima pass reads only candidates and their neighbours instead of the
- * whole window.  Replaces the pair deconvolver.py:160-163 (blur) +
+ * whole window.  The neighbour tests read out at [lo - delta, hi + delta),
+ * which must lie inside [blo, bhi) except where blo = 0 or bhi = T (MM_EINVAL
+ * otherwise).  Replaces the pair deconvolver.py:160-163 (blur) +
  * maxima.py:17-98 (find_local_maxima) of the hyb caller (__main__.py:254). */
 int mm_tiles_blur_maxima_window(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const float* res, float* out,
                                 int blur_delta, int blo, int bhi, float threshold, int delta, int lo, int hi,
@@ -201,9 +204,11 @@ int mm_tiles_fit(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const 
                  const int* counts, int cap, int delta_fit, float sigmaZ, float sigmaXY, float* out8, void* stream);
 /* keep-filter on the fitted x,y (ov <= . < view extent - ov), offset to stack
  * coordinates, stacked in tile order into Xh (out_cap rows); *total (device)
- * = rows kept.  kept_ws: ntiles ints. (mermake/__main__.py:256-265) */
+ * = rows kept; *max_count (device, may be NULL) = the largest of the ntiles
+ * candidate counts (above cap, candidates were dropped).  kept_ws: ntiles
+ * ints. (mermake/__main__.py:256-265) */
 int mm_tiles_keep(const mm_tile_desc* tiles_dev, int ntiles, int overlap, const float* fit, const int* counts,
-                  int cap, int* kept_ws, float* Xh, int out_cap, int* total, void* stream);
+                  int cap, int* kept_ws, float* Xh, int out_cap, int* total, int* max_count, void* stream);
 
 /* ------------------------------------------------------------------ drift
  * Building blocks of Aligner (mermake/align.py:37-118) and utils.fftconvolve

@@ -57,7 +57,7 @@ _SIGS = {
     "mm_tiles_blur_maxima_window": (_i, [_vp, _vp, _i, _vp, _vp, _i, _i, _i, _f, _i, _i, _i, _vp, _i, _vp, _vp, _sz,
                                          _vp]),
     "mm_tiles_fit": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _i, _i, _f, _f, _vp, _vp]),
-    "mm_tiles_keep": (_i, [_vp, _i, _i, _vp, _vp, _i, _vp, _vp, _i, _vp, _vp]),
+    "mm_tiles_keep": (_i, [_vp, _i, _i, _vp, _vp, _i, _vp, _vp, _i, _vp, _vp, _vp]),
     "mm_points_round_bounds": (_i, [_vp, _i, _i, _f, _vp, _vp, _vp]),
     "mm_points_kept_max": (_i, [_vp, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), _vp, _vp]),
     "mm_points_raster": (_i, [_vp, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), _vp, _i, _i, _i, _i, _i, _i, _i, _vp]),

@@ -337,15 +337,14 @@ __global__ __launch_bounds__(256) void k_max_compact(const unsigned long long* _
 }
 
 // ------------------------------------------------------------------- fit
-struct RawPlain {
+struct RawPlain {  // pixel type as mm_tile_desc.img_type: uint16, float32 or squared uint8
   const void* p;
-  int f32;
+  int type;
   long long sZ, sX;
   struct Ctx {};
   __device__ __forceinline__ Ctx ctx(int) const { return Ctx{}; }
   __device__ __forceinline__ float at(const Ctx&, int z, int x, int y) const {
-    const long long o = z * sZ + x * sX + y;
-    return f32 ? ((const float*)p)[o] : (float)((const uint16_t*)p)[o];
+    return pix_at(p, type, z * sZ + x * sX + y);
   }
 };
 
@@ -545,7 +544,8 @@ __global__ __launch_bounds__(256) void k_keep_count(const float* __restrict__ fi
 __global__ __launch_bounds__(256) void k_keep_write(const float* __restrict__ fit, const int* __restrict__ counts,
                                                     int cap, const mm_tile_dev* __restrict__ tiles, int ov,
                                                     const int* __restrict__ kept, int ntiles, float* __restrict__ Xh,
-                                                    int out_cap, int* __restrict__ total) {
+                                                    int out_cap, int* __restrict__ total,
+                                                    int* __restrict__ max_count) {
   const int v = blockIdx.x;
   __shared__ int base_s;
   __shared__ int wc[4];
@@ -553,7 +553,14 @@ __global__ __launch_bounds__(256) void k_keep_write(const float* __restrict__ fi
     int b = 0;
     for (int u = 0; u < v; ++u) b += kept[u];
     base_s = b;
-    if (v == ntiles - 1) *total = b + kept[v];
+    if (v == ntiles - 1) {
+      *total = b + kept[v];
+      if (max_count) {  // the largest candidate count: above cap, the batch is redone with a larger cap
+        int m = 0;
+        for (int u = 0; u < ntiles; ++u) m = max(m, counts[u]);
+        *max_count = m;
+      }
+    }
   }
   __syncthreads();
   int base = base_s;
@@ -667,16 +674,20 @@ extern "C" int mm_local_maxima(const float* img, int D, int H, int W, long long 
 }
 
 extern "C" int mm_delta_fit(const float* img, int D, int H, int W, long long sZ, long long sX, float sign,
-                            const void* raw, int raw_f32, long long rZ, long long rX, const int* coords,
+                            const void* raw, int raw_type, long long rZ, long long rX, const int* coords,
                             const int* count, int cap, int delta_fit, float sigmaZ, float sigmaXY, float* out8,
                             void* stream) {
   if (!img || !raw || !coords || !count || !out8) { mm_set_error("mm_delta_fit: null argument"); return MM_EINVAL; }
+  if (raw_type != MM_PIX_U16 && raw_type != MM_PIX_F32 && raw_type != MM_PIX_U8SQ) {
+    mm_set_error("mm_delta_fit: raw_type %d is not 0 (uint16), 1 (float32) or 2 (squared uint8)", raw_type);
+    return MM_EINVAL;
+  }
   int* offs = nullptr;
   int P = 0;
   int rc = sphere_table(delta_fit, &offs, &P);
   if (rc) return rc;
   OneVol vs{VolDesc{img, D, H, W, sZ, sX, 0, 0x7fffffff}};
-  RawPlain rs{raw, raw_f32, rZ, rX};
+  RawPlain rs{raw, raw_type, rZ, rX};
   int blocks = (cap + 3) / 4;
   if (blocks > 2048) blocks = 2048;
   if (blocks < 1) blocks = 1;
@@ -717,6 +728,13 @@ extern "C" int mm_tiles_blur_maxima_window(mm_deconv* d, const mm_tile_desc* til
     return MM_EINVAL;
   }
   if (delta < 0 || delta > 31) { mm_set_error("delta %d outside [0, 31]", delta); return MM_EINVAL; }
+  // the neighbour tests read out at [lo - delta, hi + delta): inside the blurred window, or up to
+  // the tile edge where the window reaches it (the tests reflect there)
+  if ((blo > 0 && lo - delta < blo) || (bhi < d->T && hi + delta > bhi)) {
+    mm_set_error("mm_tiles_blur_maxima_window: maxima [%d, %d) with delta %d reads outside the blur window [%d, %d)",
+                 lo, hi, delta, blo, bhi);
+    return MM_EINVAL;
+  }
   if (ntiles == 0) return MM_OK;
   const int T = d->T, sz = d->sz;
   if (ws_bytes < mm_maxima_ws_bytes(sz, T, T, ntiles)) { mm_set_error("maxima workspace too small"); return MM_EINVAL; }
@@ -751,14 +769,18 @@ extern "C" int mm_tiles_fit(mm_deconv* d, const mm_tile_desc* tiles_dev, int nti
 
 extern "C" int mm_tiles_keep(const mm_tile_desc* tiles_dev, int ntiles, int overlap, const float* fit,
                              const int* counts, int cap, int* kept_ws, float* Xh, int out_cap, int* total,
-                             void* stream) {
+                             int* max_count, void* stream) {
+  if (ntiles < 0 || (ntiles > 0 && (!tiles_dev || !fit || !counts || !kept_ws || !Xh || !total))) {
+    mm_set_error("mm_tiles_keep: bad argument");
+    return MM_EINVAL;
+  }
   if (ntiles == 0) return MM_OK;
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_keep_count, dim3(ntiles), dim3(256), 0, st, fit, counts, cap, (const mm_tile_dev*)tiles_dev,
                      overlap, kept_ws);
   MM_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_keep_write, dim3(ntiles), dim3(256), 0, st, fit, counts, cap, (const mm_tile_dev*)tiles_dev,
-                     overlap, kept_ws, ntiles, Xh, out_cap, total);
+                     overlap, kept_ws, ntiles, Xh, out_cap, total, max_count);
   MM_LAUNCH_CHECK();
   return MM_OK;
 }

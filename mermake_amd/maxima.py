@@ -24,8 +24,24 @@ def _workspace(device, nbytes):
     return ws
 
 
+def raw_type(raw):
+    """``raw_type`` of ``mm_delta_fit`` for a prepared raw volume: 0 uint16, 1 float32, 2 a
+    ``SquaredU8`` stack (uint8 values whose squares are the pixels)."""
+    from .deconvolver import SquaredU8
+    if isinstance(raw, SquaredU8):
+        if raw.dtype != torch.uint8:
+            raise TypeError("a SquaredU8 volume must hold uint8 values")
+        return 2
+    if raw.dtype == torch.float32:
+        return 1
+    if raw.dtype == torch.uint16:
+        return 0
+    raise TypeError(f"raw volumes are uint16, float32 or SquaredU8, not {raw.dtype}")
+
+
 def _raw_view(raw, shape):
-    if raw.dtype not in (torch.float32, torch.uint16):
+    from .deconvolver import SquaredU8
+    if not isinstance(raw, SquaredU8) and raw.dtype not in (torch.float32, torch.uint16):
         raw = raw.to(torch.float32)   # maxima.py:83-84: raw is cast to float32
     if tuple(raw.shape) != tuple(shape):
         raise ValueError(f"raw shape {tuple(raw.shape)} does not match image shape {tuple(shape)}")
@@ -74,14 +90,16 @@ def find_local_maxima(image, threshold=None, delta=1, delta_fit=0, raw=None, sig
     out = torch.zeros((n, 8), dtype=torch.float32, device=image.device)
     D, H, W = image.shape
     L.call("mm_delta_fit", L.ptr(image), D, H, W, image.stride(0), image.stride(1), 1.0, L.ptr(raw),
-           1 if raw.dtype == torch.float32 else 0, raw.stride(0), raw.stride(1), L.ptr(coords), L.ptr(count), n,
+           raw_type(raw), raw.stride(0), raw.stride(1), L.ptr(coords), L.ptr(count), n,
            int(delta_fit), float(sigmaZ), float(sigmaXY), L.ptr(out), L.stream_ptr(image.device))
     return out
 
 
 def find_local_maxima_signed(image, raw, threshold, delta, delta_fit, sigmaZ, sigmaXY, sign):
     """``find_local_maxima`` of ``sign*image`` without materialising the negated volume
-    (the DAPI minus pass of ``__main__.py:224-225``)."""
+    (the DAPI minus pass of ``__main__.py:224-225``).  ``raw``: a prepared stack (uint16, float32 or
+    ``SquaredU8``) of the image's shape."""
+    raw = _raw_view(raw, image.shape)
     coords, count = maxima_coords(image, threshold, delta, sign=sign)
     n = coords.shape[0]
     if n == 0:
@@ -89,6 +107,6 @@ def find_local_maxima_signed(image, raw, threshold, delta, delta_fit, sigmaZ, si
     out = torch.zeros((n, 8), dtype=torch.float32, device=image.device)
     D, H, W = image.shape
     L.call("mm_delta_fit", L.ptr(image), D, H, W, image.stride(0), image.stride(1), float(sign), L.ptr(raw),
-           1 if raw.dtype == torch.float32 else 0, raw.stride(0), raw.stride(1), L.ptr(coords), L.ptr(count), n,
+           raw_type(raw), raw.stride(0), raw.stride(1), L.ptr(coords), L.ptr(count), n,
            int(delta_fit), float(sigmaZ), float(sigmaXY), L.ptr(out), L.stream_ptr(image.device))
     return out

@@ -126,25 +126,29 @@ class HybPipeline:
                self.delta_fit, self.sigmaZ, self.sigmaXY, L.ptr(fit), st)
         return counts, fit
 
+    def _keep(self, table, n, counts, fit, cap):
+        """Keep-filter of a batch: the kept rows (device) and, on the host, ``[largest candidate
+        count, rows kept, kept per tile...]`` (one synchronisation)."""
+        d = self.d
+        stats = self._buf("stats", (n + 2,), torch.int32)
+        xh = torch.empty((n * cap, 8), dtype=torch.float32, device=d.device)
+        L.call("mm_tiles_keep", L.ptr(table), n, d.overlap, L.ptr(fit), L.ptr(counts), cap, L.ptr(stats[2:]),
+               L.ptr(xh), n * cap, L.ptr(stats[1:2]), L.ptr(stats[0:1]), L.stream_ptr(d.device))
+        return xh, stats.cpu()
+
     def tile_rows(self, table, n):
         """Kept spot rows of each of ``n`` tile descriptors (a device table), as host arrays;
         the candidate cap grows until no tile overflows it."""
         d = self.d
         single = d.psf_fft.shape[0] == 1
-        st = L.stream_ptr(d.device)
         cap = self.cap
         while True:
             counts, fit = self._spots_batch(table, n, single, cap)
-            kept = self._buf("kept", (n,), torch.int32)
-            total = self._buf("total", (1,), torch.int32)
-            xh = torch.empty((n * cap, 8), dtype=torch.float32, device=d.device)
-            L.call("mm_tiles_keep", L.ptr(table), n, d.overlap, L.ptr(fit), L.ptr(counts), cap, L.ptr(kept),
-                   L.ptr(xh), n * cap, L.ptr(total), st)
-            stats = torch.cat([counts.max().view(1), kept]).cpu()
+            xh, stats = self._keep(table, n, counts, fit, cap)
             if int(stats[0]) > cap:
                 cap = int(stats[0])
                 continue
-            k = stats[1:].numpy()
+            k = stats[2:].numpy()
             rows = xh[:int(k.sum())].cpu().numpy()
             offs = np.concatenate([[0], np.cumsum(k)])
             return [rows[offs[i]:offs[i + 1]].copy() for i in range(n)]
@@ -160,17 +164,20 @@ class HybPipeline:
         if flat_fields is None:
             flat_fields = [None] * len(stacks)
         tables, owner = [], []
+        # the prepared stacks and flats (device copies of host inputs, dtype casts) stay referenced
+        # until every batch that reads them through the tile tables has finished
+        alive = []
         for si, (stack, flat_field) in enumerate(zip(stacks, flat_fields)):
             d._check(stack)
             img = prepare_stack(stack)
             flat = prepare_flat(flat_field, img)
+            alive.append((img, flat))
             geom, tab = self._table(img, flat)
             tables.append(tab)
             owner += [si] * len(geom)
         n = len(owner)
         table_all = tables[0] if len(tables) == 1 else torch.cat(tables)
         single = d.psf_fft.shape[0] == 1
-        st = L.stream_ptr(d.device)
         parts = [[] for _ in stacks]
         B = 1 if self.plain else d.batch_size(n, self.cap * 48 + d.tile_height * d._plan.T ** 2 // 8)
         rec = L.TILE_DTYPE.itemsize
@@ -181,23 +188,18 @@ class HybPipeline:
             while True:
                 counts, fit = self._spots_batch(table, nb, single, cap)
                 if self.plain:
-                    mx = int(counts.max().item())       # sync after the maxima, as maxima.py:46-50
+                    mx = int(counts.cpu().max())        # sync after the maxima, as maxima.py:46-50
                 else:
                     mx = None
                 if mx is not None and mx > cap:
                     cap = mx
                     continue
-                kept = self._buf("kept", (nb,), torch.int32)
-                total = self._buf("total", (1,), torch.int32)
-                xh = torch.empty((nb * cap, 8), dtype=torch.float32, device=d.device)
-                L.call("mm_tiles_keep", L.ptr(table), nb, d.overlap, L.ptr(fit), L.ptr(counts), cap, L.ptr(kept),
-                       L.ptr(xh), nb * cap, L.ptr(total), st)
-                stats = torch.cat([counts.max().view(1), kept]).cpu()   # one sync per batch
+                xh, stats = self._keep(table, nb, counts, fit, cap)   # one sync per batch
                 mx = int(stats[0])
                 if mx > cap:
                     cap = mx
                     continue
-                kept_h = stats[1:].tolist()
+                kept_h = stats[2:].tolist()
                 off = 0
                 for i in range(nb):                    # tiles of one stack are contiguous: split the rows by owner
                     si = owner[b0 + i]
@@ -208,6 +210,7 @@ class HybPipeline:
                             parts[si].append([off, off + kept_h[i], xh])
                     off += kept_h[i]
                 break
+        del alive   # every batch has synchronised on its keep-filter statistics
         out = []
         for si in range(len(stacks)):
             views = [x[a:b] for a, b, x in parts[si]]

@@ -140,7 +140,7 @@ class _Slot:
             L.call("mm_tiles_fit", d._plan.h, tab, n, L.ptr(out), L.ptr(self.coords), L.ptr(counts), self.cap,
                    pipe.delta_fit, pipe.sigmaZ, pipe.sigmaXY, L.ptr(self.fit), st)
             L.call("mm_tiles_keep", tab, n, d.overlap, L.ptr(self.fit), L.ptr(counts), self.cap, L.ptr(kept),
-                   L.ptr(self.xh), n * self.cap, L.ptr(total), st)
+                   L.ptr(self.xh), n * self.cap, L.ptr(total), None, st)
             self.stats_h.copy_(self.stats, non_blocking=True)
             self.ev_stats.record(self.stream)
         self.jobs, self.n, self.state = jobs, n, "stats"

@@ -14,7 +14,8 @@ Follows ``mermake/align.py:12-177`` and ``mermake/utils.py:10-36`` (``fftconvolv
   (rounds with no pair are skipped), ``Npts`` from the last round.
 * ``DualAligner`` combination (``align.py:138-156``).
 
-Pins: the drift known answer of the reference fixture
+Pins: ``fftconvolve`` in all three modes against ``scipy.signal.fftconvolve`` (ranks 1
+to 3); the drift known answer of the reference fixture
 ``tests/examples/single/out/drift_Conv_zscan__020--_set1.pkl`` (N+ = 621, N- = 261,
 zero shift; committed as ``tests/golden/drift_known_answer.json``); the branch logic of
 ``tests/test_align.py:181-233``; synthetic integer shifts that must be recovered exactly.

@@ -20,7 +20,9 @@ Pins: ``tests/examples/single/out/deconv.300.*.npz`` (the reference's golden
 float16 deconvolution of the real fixture, ``tests/test_deconvolver.py:422-434``)
 through committed tile windows under ``tests/golden/``; closed forms for the
 Wiener spectrum (delta PSF -> ``W = 1/(1+beta|L|^2)``, ``L(0) = 1``);
-``tests/test_deconvolver.py:158-218`` (``center_psf`` and ``tiled``).
+``tests/test_deconvolver.py:158-218`` (``center_psf`` and ``tiled``); the flat field
+by linearity (deconvolving v with flat f equals deconvolving v/f without one, with or
+without blur; the raw views hold (v/f)*f on the data mask and v/f in the pads).
 """
 import os
 from itertools import cycle

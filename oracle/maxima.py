@@ -30,9 +30,12 @@ unit population std (no guard: std 0 gives NaN as in the reference);
 Pins: the reference's own tests ``tests/test_maxima.py:26-166`` (counts, shapes,
 threshold/delta behaviour) replayed in ``tests/test_oracle_cpu.py``; a brute-force
 triple loop over every voxel and every sphere neighbour on small volumes; for
-the fit, closed forms (symmetric peak -> integer centroid, constant image ->
-``hn = a = 0``, background = min) and an independent per-candidate scalar loop
-that follows the CUDA kernel line by line.
+the fit, closed forms (symmetric peak -> integer centroid, linear ramp -> centroid
+shifted by mean(offset^2)/delta_fit, constant image -> ``hn = a = 0``, background =
+min), the correlation columns as Pearson correlations (an image or raw equal to
+c*G + d on the sphere gives exactly +-1 with population statistics; arbitrary
+volumes give ``numpy.corrcoef`` of the sphere samples), and an independent
+per-candidate scalar loop that follows the CUDA kernel line by line.
 """
 import numpy as np
 

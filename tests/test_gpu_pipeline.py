@@ -121,7 +121,7 @@ def test_hyb_window_equals_whole_tiles(dev, small_hyb, delta, delta_fit, ts, ov)
 @pytest.mark.parametrize("delta,blur_radius,thr,ts,ov,same_window", [
     (1, 30, 2000.0, 100, 30, False), (1, 30, -50.0, 100, 30, False),  # almost every voxel a candidate
     (3, 50, 500.0, 200, 13, False),   # T = 226, sorted-sphere test, blur half-width 25
-    (5, 30, 1000.0, 100, 30, True),   # largest sphere; maxima window = blur window
+    (5, 30, 1000.0, 100, 30, True),   # largest sphere; maxima window as wide as the blur window allows
     (2, 50, 800.0, 100, 30, False),
     (1, 30, float("nan"), 300, 89, False)])  # NaN threshold rejects nothing; T = 478
 def test_blur_maxima_entry_equals_separate_calls(dev, small_hyb, delta, blur_radius, thr, ts, ov, same_window):
@@ -138,7 +138,7 @@ def test_blur_maxima_entry_equals_separate_calls(dev, small_hyb, delta, blur_rad
     n = len(geom)
     (b0, b1), (c0, c1) = win.windows()
     if same_window:
-        c0, c1 = b0, b1
+        c0, c1 = b0 + delta, b1 - delta
     Tt, sz = d._plan.T, d.tile_height
     bd = blur_radius // 2
     print("blur+maxima entry", delta, blur_radius, thr, Tt, n, (b0, b1), (c0, c1), flush=True)
@@ -215,6 +215,32 @@ def test_dapi_pipeline_vs_oracle(dev):
         assert abs(len(g) - len(w)) <= max(2, 0.02 * len(w)), (len(g), len(w))
         err, rate = _match(g, w, 0.5)
         assert rate > 0.98 and err < 1e-3, (err, rate)
+
+
+def test_dapi_pipeline_squared_u8_equals_uint16_squares(dev):
+    """A ``SquaredU8`` DAPI stack gives the spots of its uint16 squares bit for bit: the deconvolution
+    loaders and the fit's raw reads (``habs``, ``a``) both square the uint8 values."""
+    from mermake_amd.deconvolver import Deconvolver, SquaredU8
+    from mermake_amd.pipeline import DapiPipeline
+    from mermake_amd.synth import gaussian_psf, stack_np
+    raw, flat = stack_np(13, (12, 160, 160), 60, amp=(3000, 20000), flat=True)
+    u8 = np.minimum(np.sqrt(raw), 255).astype(np.uint8)
+    sq = u8.astype(np.uint16) ** 2
+    psf = gaussian_psf((12, 31, 31), 1.5, 1.2)
+    kw = dict(blur_radius=50, threshold=3.0, delta=5, delta_fit=5, sigmaZ=1, sigmaXY=1.5)
+    d = Deconvolver(psf, raw.shape, tile_size=100, zpad=(raw.shape[0] - 1) // 2, overlap=20, beta=0.01)
+    dp = DapiPipeline(d, **kw)
+    pa, ma = [t.cpu().numpy() for t in dp(SquaredU8.of(T(u8, dev)), T(flat, dev))]
+    pb, mb = [t.cpu().numpy() for t in dp(T(sq, dev), T(flat, dev))]
+    assert len(pa) > 5 and len(ma) > 5
+    assert np.array_equal(pa.view(np.uint32), pb.view(np.uint32))
+    assert np.array_equal(ma.view(np.uint32), mb.view(np.uint32))
+    # find_local_maxima with a SquaredU8 raw reads the squares too
+    from mermake_amd.maxima import find_local_maxima
+    img = torch.from_numpy(np.random.default_rng(3).random(u8.shape, dtype=np.float32)).to(dev)
+    fa = find_local_maxima(img, threshold=0.9, delta=1, delta_fit=3, raw=SquaredU8.of(T(u8, dev))).cpu().numpy()
+    fb = find_local_maxima(img, threshold=0.9, delta=1, delta_fit=3, raw=T(sq, dev)).cpu().numpy()
+    assert len(fa) > 5 and np.array_equal(fa.view(np.uint32), fb.view(np.uint32))
 
 
 def test_fftconvolve_vs_oracle(dev):

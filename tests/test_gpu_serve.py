@@ -59,6 +59,22 @@ def test_run_many_uneven_fovs_equals_alone(fovs, pipe, alone, dev):
         assert np.array_equal(g.cpu().numpy(), alone[i][c]), (i, c)
 
 
+def test_run_many_host_stacks_equal_alone(fovs, pipe, alone, dev):
+    """Host NumPy stacks (each copied to the device inside ``run_many``, with the flats cast from
+    float64) give the spots of each stack alone: the copies stay alive until every batch reading them
+    has finished, while later copies are allocated."""
+    data, _ = fovs
+    stacks, flats, where = [], [], []
+    for i, (chans, flat) in enumerate(data):
+        for c, ch in enumerate(chans):
+            stacks.append(np.ascontiguousarray(ch))
+            flats.append(flat.astype(np.float64))
+            where.append((i, c))
+    got = pipe.run_many(stacks, flats)
+    for (i, c), g in zip(where, got):
+        assert np.array_equal(g.cpu().numpy(), alone[i][c]), (i, c)
+
+
 @pytest.mark.parametrize("host_inputs,batch_tiles,budget_fovs", [(False, 5, None), (True, 7, 1.5), (True, 36, None)])
 def test_server_mix_equals_alone(fovs, pipe, alone, dev, host_inputs, batch_tiles, budget_fovs):
     """Requests arriving while others run are admitted as memory frees up and batched together;

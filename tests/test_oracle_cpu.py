@@ -226,3 +226,116 @@ def test_drift_integer_shift_oracle():
 def test_empty_reference_raises():
     with pytest.raises(ValueError):
         oalign.Aligner(np.ones((0, 4)), resc=1, trim=1)
+
+
+# ------------------------------------------------------------------ pins of the fit's correlations
+def _sphere_mask(d):
+    """Offsets (dz, dx, dy) of the delta_fit sphere from np.indices (order-free)."""
+    g = np.indices((2 * d + 1,) * 3) - d
+    m = (g ** 2).sum(0) <= d * d
+    return g[0][m], g[1][m], g[2][m]
+
+
+def _gauss_field(shape, c, sz, sxy):
+    z, x, y = np.indices(shape)
+    return np.exp(-((z - c[0]) ** 2 / (2 * sz * sz) + (x - c[1]) ** 2 / (2 * sxy * sxy)
+                    + (y - c[2]) ** 2 / (2 * sxy * sxy)))
+
+
+@pytest.mark.parametrize("dfit,sz,sxy", [(3, 1.0, 1.5), (5, 1.0, 1.5), (2, 1.3, 0.8)])
+def test_fit_correlations_closed_forms(dfit, sz, sxy):
+    """``a`` and ``hn`` (maxima.cu:249-298) are Pearson correlations with the sampled Gaussian, computed
+    with population statistics.  Closed forms: an image equal to c*G + d on the sphere gives hn = 1
+    (c > 0) or -1 (c < 0), a raw equal to c*G + d gives a = +-1 exactly; a ddof=1 std gives
+    (n-1)/n instead, swapped columns flip the signs.  For arbitrary volumes the correlations equal
+    numpy.corrcoef of the sphere samples (interior candidates: no reflection)."""
+    shape = (2 * dfit + 5, 2 * dfit + 7, 2 * dfit + 6)
+    c = (dfit + 2, dfit + 3, dfit + 2)
+    G = _gauss_field(shape, c, sz, sxy)
+    img = 3.0 * G + 100.0
+    raw = -2.0 * G + 50.0
+    r = omax.delta_fit(img, raw, np.array([c[0]]), np.array([c[1]]), np.array([c[2]]), dfit, sz, sxy)[0]
+    assert abs(r[6] - 1.0) < 1e-12 and abs(r[4] + 1.0) < 1e-12, r
+    r = omax.delta_fit(-img, -raw, np.array([c[0]]), np.array([c[1]]), np.array([c[2]]), dfit, sz, sxy)[0]
+    assert abs(r[6] + 1.0) < 1e-12 and abs(r[4] - 1.0) < 1e-12, r
+    # arbitrary volumes: numpy.corrcoef of the samples
+    rng = np.random.default_rng(dfit)
+    img = rng.random(shape) * 50 + 3 * G
+    raw = rng.random(shape) * 900 - G * 200
+    dz, dx, dy = _sphere_mask(dfit)
+    gs = np.exp(-(dz ** 2 / (2 * sz * sz) + dx ** 2 / (2 * sxy * sxy) + dy ** 2 / (2 * sxy * sxy)))
+    iv = img[c[0] + dz, c[1] + dx, c[2] + dy]
+    rv = raw[c[0] + dz, c[1] + dx, c[2] + dy]
+    r = omax.delta_fit(img, raw, np.array([c[0]]), np.array([c[1]]), np.array([c[2]]), dfit, sz, sxy)[0]
+    np.testing.assert_allclose(r[6], np.corrcoef(iv, gs)[0, 1], rtol=1e-12)
+    np.testing.assert_allclose(r[4], np.corrcoef(rv, gs)[0, 1], rtol=1e-12)
+    assert r[3] == iv.min() and r[5] == raw[c] and r[7] == img[c]
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_fit_centroid_linear_ramp(axis):
+    """On a linear ramp alpha*coord + beta the background is the sample at offset -d, so the weights are
+    alpha*(off + d) and the centroid moves by mean(off^2)/d along the ramp's axis, 0 along the others."""
+    d = 3
+    shape = (13, 14, 15)
+    c = (6, 7, 7)
+    img = 2.5 * np.indices(shape)[axis] + 40.0
+    r = omax.delta_fit(img, img, np.array([c[0]]), np.array([c[1]]), np.array([c[2]]), d, 1.0, 1.5)[0]
+    off = _sphere_mask(d)[axis]
+    want = np.array(c, dtype=np.float64)
+    want[axis] += (off.astype(np.float64) ** 2).mean() / d
+    np.testing.assert_allclose(r[:3], want, rtol=0, atol=1e-12)
+
+
+# ------------------------------------------------------------------ fftconvolve vs SciPy
+@pytest.mark.parametrize("sa,sb", [((17,), (5,)), ((9, 12), (4, 3)), ((5, 17, 23), (3, 9, 4)), ((6, 8, 7), (6, 8, 7)),
+                                   ((11, 4, 9), (2, 4, 1))])
+def test_fftconvolve_matches_scipy(sa, sb):
+    """``utils.py:10-36`` in all three modes against ``scipy.signal.fftconvolve`` (rank 1 to 3; for 'valid'
+    the first input is at least as large as the second in every axis, where SciPy does not swap them)."""
+    from scipy.signal import fftconvolve as sfc
+    rng = np.random.default_rng(len(sa) * 10 + sa[0])
+    a = rng.standard_normal(sa)
+    b = rng.standard_normal(sb)
+    for mode in ("full", "same", "valid"):
+        got = oalign.fftconvolve(a, b, mode=mode)
+        want = sfc(a, b, mode=mode)
+        assert got.shape == want.shape, (mode, got.shape, want.shape)
+        np.testing.assert_allclose(got, want, rtol=1e-10, atol=1e-10 * np.abs(want).max())
+    with pytest.raises(ValueError):
+        oalign.fftconvolve(a, b, mode="nope")
+
+
+# ------------------------------------------------------------------ flat field by linearity
+def test_flat_field_divide_and_multiply_back_by_linearity():
+    """deconvolver.py:127-129,166-168: the data mask of the padded tile is divided by the flat before the
+    reflections, and only the padded tile (the raw view) is multiplied back after the blur.  So
+
+    * with or without blur, deconvolving v with flat f equals deconvolving the float image v/f without a
+      flat (the deconvolved tile stays flat-divided);
+    * the raw views hold (v/f)*f on the data mask and the reflected v/f in the pads."""
+    from mermake_amd.synth import gaussian_psf
+    rng = np.random.default_rng(5)
+    sz, sx, sy, ts, ov, zpad = 6, 70, 64, 40, 9, 5
+    v = rng.poisson(300.0, size=(sz, sx, sy)).astype(np.uint16)
+    f = (0.6 + 0.4 * rng.random((sx, sy))).astype(np.float64)
+    psf = gaussian_psf((sz, 9, 9), 1.2, 1.1)
+    W = odec.wiener(psf, sz, ts, zpad, ov, 1e-3, mode="fp64")
+    vf = v.astype(np.float64) / f[None]
+    for blur in (None, 6):
+        a = list(odec.tile_wise(v, W, f, blur, ts, ov, zpad))
+        b = list(odec.tile_wise(vf, W, None, blur, ts, ov, zpad))
+        assert len(a) == len(b) == 4
+        for (x, y, ra, pa), (_, _, rb, pb) in zip(a, b):
+            xs, ys = (ov if x == 0 else 0), (ov if y == 0 else 0)
+            x0, y0 = max(x - ov, 0), max(y - ov, 0)
+            xd, yd = min(x + ts + ov, sx) - x0, min(y + ts + ov, sy) - y0
+            m = np.zeros(ra.shape[1:], bool)
+            m[xs:xs + xd, ys:ys + yd] = True
+            fm = np.ones(ra.shape[1:])
+            fm[xs:xs + xd, ys:ys + yd] = f[x0:x0 + xd, y0:y0 + yd]
+            np.testing.assert_allclose(ra, rb, rtol=1e-9, atol=1e-9 * np.abs(rb).max())
+            # raw views: (v/f)*f on the data mask, v/f (reflected) elsewhere
+            np.testing.assert_allclose(pa[:, m], (pb * fm[None])[:, m], rtol=1e-14)
+            np.testing.assert_allclose(pa[:, ~m], pb[:, ~m], rtol=1e-14)
+            assert np.allclose(pa[:, m], v[:, x0:x0 + xd, y0:y0 + yd].reshape(sz, -1), rtol=1e-13)
