@@ -166,6 +166,18 @@ int mm_fft3d_c2c(void* data, int n0, int n1, int n2, int axes_mask, int inverse,
 size_t mm_maxima_ws_bytes(int D, int H, int W, int nvol);
 int mm_local_maxima(const float* img, int D, int H, int W, long long sZ, long long sX, float sign, float threshold,
                     int delta, int* coords, int cap, int* count, void* ws, size_t ws_bytes, void* stream);
+/* mm_local_maxima without a workspace argument and without any allocation: the
+ * row masks and counts live in a library-owned device array of
+ * MM_MAXIMA_SCRATCH_BYTES bytes (one per GPU, held until the process ends).
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Volumes whose mm_maxima_ws_bytes(D, H, W, 1) exceeds the array
+ * (about 40 x 2800 x 3300 voxels) fail with MM_ECAP.  Every call on one GPU
+ * shares that GPU's array, so calls that overlap on one GPU (other streams,
+ * other threads) race; calls in stream order do not. */
+#define MM_MAXIMA_SCRATCH_BYTES (48u << 20)
+int mm_local_maxima_noalloc(const float* img, int D, int H, int W, long long sZ, long long sX, float sign,
+                            float threshold, int delta, int* coords, int cap, int* count, void* stream);
 /* Replaces delta_fit_cross_corr (mermake/maxima.cu:148-341): out8 rows
  * [zc, xc, yc, background, a, habs, hn, h] for the *count candidates.
  * raw: volume with strides rZ, rX whose pixels are raw_type 0 uint16,

@@ -784,3 +784,57 @@ extern "C" int mm_tiles_keep(const mm_tile_desc* tiles_dev, int ntiles, int over
   MM_LAUNCH_CHECK();
   return MM_OK;
 }
+
+// ------------------------------------------------------- allocation-free maxima
+// The workspace of mm_local_maxima_noalloc: a library-owned array that every GPU
+// the library runs on holds until the process ends.  Host code never passes the
+// array's name to a kernel (on the host it is only a stand-in for the symbol):
+// it passes the device address hipGetSymbolAddress returns, kept per device.
+static __device__ __align__(256) unsigned char scratch[MM_MAXIMA_SCRATCH_BYTES];
+
+#define MM_SCRATCH_MAX_DEVICES 64
+
+// Device address of `scratch` on the current device, looked up once per device.
+static int scratch_address(void** out) {
+  static std::mutex mu;
+  static void* addr[MM_SCRATCH_MAX_DEVICES] = {nullptr};
+  int dev = 0;
+  MM_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MM_SCRATCH_MAX_DEVICES) {
+    mm_set_error("mm_local_maxima_noalloc: device %d beyond the %d the library tracks", dev, MM_SCRATCH_MAX_DEVICES);
+    return MM_EINVAL;
+  }
+  std::lock_guard<std::mutex> lk(mu);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    MM_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) {
+      mm_set_error("mm_local_maxima_noalloc: no device address for the scratch array");
+      return MM_EINVAL;
+    }
+    addr[dev] = p;
+  }
+  *out = addr[dev];
+  return MM_OK;
+}
+
+extern "C" int mm_local_maxima_noalloc(const float* img, int D, int H, int W, long long sZ, long long sX, float sign,
+                                       float threshold, int delta, int* coords, int cap, int* count, void* stream) {
+  if (!img || !coords || !count || D < 0 || H < 0 || W < 0) {
+    mm_set_error("mm_local_maxima_noalloc: bad argument");
+    return MM_EINVAL;
+  }
+  const size_t need = mm_maxima_ws_bytes(D, H, W, 1);
+  if (need > (size_t)MM_MAXIMA_SCRATCH_BYTES) {
+    mm_set_error("mm_local_maxima_noalloc: a %d x %d x %d volume needs %zu workspace bytes, over the %zu of the "
+                 "library's scratch array (use mm_local_maxima with a workspace)",
+                 D, H, W, need, (size_t)MM_MAXIMA_SCRATCH_BYTES);
+    return MM_ECAP;
+  }
+  void* ws = nullptr;
+  int rc = scratch_address(&ws);
+  if (rc) return rc;
+  // every pass writes the workspace words and row counts it later reads: nothing has to start at zero
+  OneVol vs{VolDesc{img, D, H, W, sZ, sX, 0, 0x7fffffff}};
+  return run_maxima(vs, 1, D, H, W, sign, threshold, delta, coords, cap, count, ws, need, (hipStream_t)stream);
+}

@@ -143,3 +143,18 @@ def test_squared_u8_stack_type():
     assert pixel_type(torch.zeros(2, dtype=torch.uint16)) == 0 and pixel_type(torch.zeros(2)) == 1
     with pytest.raises(TypeError):
         SquaredU8.of(np.zeros((2, 2, 2), np.uint16))
+
+
+def test_noalloc_maxima_rejects_oversized_volume_without_a_gpu():
+    """``mm_local_maxima_noalloc`` checks the workspace a volume needs against its scratch array before any
+    device call: too large is MM_ECAP (1001) with a message."""
+    L = ctypes.CDLL(_lib_path())
+    L.mm_last_error.restype = ctypes.c_char_p
+    f = L.mm_local_maxima_noalloc
+    f.restype = ctypes.c_int
+    vp, i, ll, fl = ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong, ctypes.c_float
+    f.argtypes = [vp, i, i, i, ll, ll, fl, fl, i, vp, i, vp, vp]
+    one = ctypes.c_void_p(1)
+    assert f(one, 48, 3000, 3000, 9000000, 3000, 1.0, 0.5, 1, one, 8, one, None) == 1001
+    assert b"scratch" in L.mm_last_error()
+    assert f(None, 4, 4, 4, 16, 4, 1.0, 0.5, 1, one, 8, one, None) == 1000
