@@ -5,7 +5,8 @@ continuous positions, convolved analytically with a Gaussian PSF (variances
 add, the peak shrinks by sigma_punctum/sigma_total per axis), on a Poisson
 background.  An optional smooth illumination ``g(x,y) = 1 - 0.3 r^2`` (r = 1 at
 the corner) multiplies background + signal before the Poisson draw and is
-returned as the float32 flat field.
+returned as the float32 flat field.  DAPI-style stacks (``nuclei_torch``) hold ellipsoidal nuclei
+textured with chromatin spots, optionally shifted by an integer offset for drift tests.
 """
 import math
 
@@ -77,6 +78,87 @@ def stack_np(seed, shape, n_puncta, psf_sigma=(1.5, 1.2), spot_sigma=(1.0, 1.5),
     return raw, g
 
 
+def _add_gaussians(vol, shape, pos, a, sz_tot, sxy_tot, radius=(6, 7), chunk=2048):
+    """vol (flat float32 device tensor of ``shape``) += a * exp(-d^2 / 2 sigma^2) around each position."""
+    import torch
+    sz, sx, sy = shape
+    device = vol.device
+    rz, rxy = radius
+    dz = torch.arange(-rz, rz + 1, device=device)
+    dxy = torch.arange(-rxy, rxy + 1, device=device)
+    for c0 in range(0, pos.shape[0], chunk):
+        p = pos[c0:c0 + chunk]
+        aa = a[c0:c0 + chunk]
+        c = torch.round(p).long()
+        z = c[:, 0:1] + dz[None]
+        x = c[:, 1:2] + dxy[None]
+        y = c[:, 2:3] + dxy[None]
+        gz = torch.exp(-(z - p[:, 0:1]) ** 2 / (2 * sz_tot ** 2)) * ((z >= 0) & (z < sz))
+        gx = torch.exp(-(x - p[:, 1:2]) ** 2 / (2 * sxy_tot ** 2)) * ((x >= 0) & (x < sx))
+        gy = torch.exp(-(y - p[:, 2:3]) ** 2 / (2 * sxy_tot ** 2)) * ((y >= 0) & (y < sy))
+        w = aa[:, None, None, None] * gz[:, :, None, None] * gx[:, None, :, None] * gy[:, None, None, :]
+        lin = (z.clamp(0, sz - 1)[:, :, None, None] * sx + x.clamp(0, sx - 1)[:, None, :, None]) * sy \
+            + y.clamp(0, sy - 1)[:, None, None, :]
+        vol.index_add_(0, lin.reshape(-1), w.reshape(-1).to(torch.float32))
+
+
+def nuclei_torch(seed, shape, n_blobs, device, shift=(0, 0, 0), noise_seed=None, radius=(20.0, 40.0),
+                 zradius=(6.0, 12.0), amp=(1500.0, 3000.0), spots=12, spot_amp=(3000.0, 8000.0),
+                 spot_sigma=(1.5, 2.0), background=500.0, chunk=8):
+    """uint16 DAPI-style device stack: ``n_blobs`` ellipsoidal nuclei (semi-axes U(radius) in x and y,
+    U(zradius) in z, smooth edges, amplitude U(amp)), each textured with ``spots`` Gaussian chromatin spots
+    (sigma ``spot_sigma`` in z and xy, amplitude U(spot_amp)) at uniform points inside it, on a Poisson
+    background.  The scene (blob and spot positions) is fixed by ``seed``; ``shift`` (dz, dx, dy) moves all of
+    it, and ``noise_seed`` (default ``seed``) draws the Poisson noise, so a shifted pair of stacks is
+    ``nuclei_torch(s, ..., noise_seed=a)`` and ``nuclei_torch(s, ..., shift=d, noise_seed=b)``."""
+    import torch
+    sz, sx, sy = shape
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed))
+    f64 = dict(generator=g, device=device, dtype=torch.float64)
+    lo = torch.tensor([0.0, 0.0, 0.0], dtype=torch.float64, device=device)
+    ext = torch.tensor([sz, sx, sy], dtype=torch.float64, device=device)
+    cen = lo + torch.rand((n_blobs, 3), **f64) * ext
+    rxy = radius[0] + (radius[1] - radius[0]) * torch.rand((n_blobs, 2), **f64)
+    rz = zradius[0] + (zradius[1] - zradius[0]) * torch.rand((n_blobs, 1), **f64)
+    rad = torch.cat([rz, rxy], 1)
+    a = amp[0] + (amp[1] - amp[0]) * torch.rand(n_blobs, **f64)
+    # chromatin spots at uniform points of each ellipsoid (direction, radius u^(1/3) * 0.85)
+    d = torch.randn((n_blobs, spots, 3), **f64)
+    d = d / d.norm(dim=2, keepdim=True).clamp_min(1e-9)
+    u = torch.rand((n_blobs, spots, 1), **f64) ** (1.0 / 3.0) * 0.85
+    spos = (cen[:, None, :] + d * u * rad[:, None, :]).reshape(-1, 3)
+    sa = spot_amp[0] + (spot_amp[1] - spot_amp[0]) * torch.rand(n_blobs * spots, **f64)
+    sh = torch.tensor([float(v) for v in shift], dtype=torch.float64, device=device)
+    cen = cen + sh
+    spos = spos + sh
+    vol = torch.full((sz * sx * sy,), float(background), dtype=torch.float32, device=device)
+    # blobs: a * sigmoid(6 (1 - q^(1/2))), q the ellipsoid's normalised radius squared, over a box of 1.3 r
+    bz, bxy = int(math.ceil(zradius[1] * 1.3)), int(math.ceil(radius[1] * 1.3))
+    oz = torch.arange(-bz, bz + 1, device=device)
+    oxy = torch.arange(-bxy, bxy + 1, device=device)
+    for c0 in range(0, n_blobs, chunk):
+        p, r, aa = cen[c0:c0 + chunk], rad[c0:c0 + chunk], a[c0:c0 + chunk]
+        c = torch.round(p).long()
+        z = c[:, 0:1] + oz[None]
+        x = c[:, 1:2] + oxy[None]
+        y = c[:, 2:3] + oxy[None]
+        qz = ((z - p[:, 0:1]) / r[:, 0:1]) ** 2
+        qx = ((x - p[:, 1:2]) / r[:, 1:2]) ** 2
+        qy = ((y - p[:, 2:3]) / r[:, 2:3]) ** 2
+        q = qz[:, :, None, None] + qx[:, None, :, None] + qy[:, None, None, :]
+        ok = (((z >= 0) & (z < sz))[:, :, None, None] & ((x >= 0) & (x < sx))[:, None, :, None]
+              & ((y >= 0) & (y < sy))[:, None, None, :])
+        w = aa[:, None, None, None] * torch.sigmoid(6.0 * (1.0 - torch.sqrt(q))) * ok
+        lin = (z.clamp(0, sz - 1)[:, :, None, None] * sx + x.clamp(0, sx - 1)[:, None, :, None]) * sy \
+            + y.clamp(0, sy - 1)[:, None, None, :]
+        vol.index_add_(0, lin.reshape(-1), w.reshape(-1).to(torch.float32))
+    _add_gaussians(vol, shape, spos, sa, spot_sigma[0], spot_sigma[1], radius=(6, 8))
+    gn = torch.Generator(device=device)
+    gn.manual_seed(int(seed if noise_seed is None else noise_seed))
+    return torch.poisson(vol.view(sz, sx, sy), generator=gn).clamp_(0, 65535).to(torch.int32).to(torch.uint16)
+
+
 def stack_torch(seed, shape, density, device, psf_sigma=(1.5, 1.2), spot_sigma=(1.0, 1.5), background=500.0,
                 amp=(2000.0, 10000.0), flat=False, chunk=2048):
     """uint16 device stack (and float32 device flat or None) for large FOVs.
@@ -94,23 +176,7 @@ def stack_torch(seed, shape, density, device, psf_sigma=(1.5, 1.2), spot_sigma=(
         [sz, sx, sy], dtype=torch.float64, device=device)
     a = amp[0] + (amp[1] - amp[0]) * torch.rand(n, generator=g, device=device, dtype=torch.float64)
     vol = torch.full((sz * sx * sy,), float(background), dtype=torch.float32, device=device)
-    rz, rxy = 6, 7
-    dz = torch.arange(-rz, rz + 1, device=device)
-    dxy = torch.arange(-rxy, rxy + 1, device=device)
-    for c0 in range(0, n, chunk):
-        p = pos[c0:c0 + chunk]
-        aa = a[c0:c0 + chunk]
-        c = torch.round(p).long()
-        z = c[:, 0:1] + dz[None]
-        x = c[:, 1:2] + dxy[None]
-        y = c[:, 2:3] + dxy[None]
-        gz = torch.exp(-(z - p[:, 0:1]) ** 2 / (2 * sz_tot ** 2)) * ((z >= 0) & (z < sz))
-        gx = torch.exp(-(x - p[:, 1:2]) ** 2 / (2 * sxy_tot ** 2)) * ((x >= 0) & (x < sx))
-        gy = torch.exp(-(y - p[:, 2:3]) ** 2 / (2 * sxy_tot ** 2)) * ((y >= 0) & (y < sy))
-        w = (aa * peak)[:, None, None, None] * gz[:, :, None, None] * gx[:, None, :, None] * gy[:, None, None, :]
-        lin = (z.clamp(0, sz - 1)[:, :, None, None] * sx + x.clamp(0, sx - 1)[:, None, :, None]) * sy \
-            + y.clamp(0, sy - 1)[:, None, None, :]
-        vol.index_add_(0, lin.reshape(-1), w.reshape(-1).to(torch.float32))
+    _add_gaussians(vol, shape, pos, a * peak, sz_tot, sxy_tot, chunk=chunk)
     vol = vol.view(sz, sx, sy)
     gflat = None
     if flat:

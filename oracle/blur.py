@@ -32,12 +32,11 @@ def box_1d(image, size, axis=0, out=None, dtype=np.float32):
     x = np.moveaxis(image, axis, 0)
     acc = np.zeros_like(x)
     num = np.zeros(n, dtype=np.int64)
-    idx = np.arange(n)
-    for k in range(-delta, delta + 1):
-        j = idx + k
-        ok = (j >= 0) & (j < n)
-        acc[idx[ok]] += x[j[ok]]
-        num[ok] += 1
+    for k in range(-delta, delta + 1):     # window offsets low to high: the sequential sum order
+        lo, hi = max(0, -k), min(n, n - k)  # outputs i whose input i + k lies on the axis
+        if lo < hi:
+            acc[lo:hi] += x[lo + k:hi + k]
+            num[lo:hi] += 1
     shape = [n] + [1] * (x.ndim - 1)
     res = (acc / num.reshape(shape).astype(dtype)).astype(dtype)
     res = np.moveaxis(res, 0, axis)
