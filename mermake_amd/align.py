@@ -6,7 +6,9 @@ leaves the linear correlation unchanged), a deterministic argmax (the
 correlation of two binary images is an integer, so ``rint`` makes the peak
 exact; ties go to the first index, as ``argmax`` does) and the five rounds of
 nearest-neighbour refinement (brute force in float64, lowest index on distance
-ties, replacing the cupyx KD-tree of ``align.py:8,32``).
+ties, replacing the cupyx KD-tree of ``align.py:8,32``).  The spectrum of the
+reference raster stays resident in each ``Aligner`` (the reference recomputes it
+in every ``fftconvolve`` call), so a FOV block's drifts share it.
 
 Results are returned as NumPy arrays (``drift``'s records are pickled).
 """
@@ -83,6 +85,8 @@ class Aligner:
         self.X_ref = X_ref[:, :3].contiguous()
         self._ref = _Raster(self.X_ref, resc, trim)
         self.Xm_ref = np.zeros(3, dtype=np.int32)
+        self._grid = None   # padded grid of the resident reference spectrum
+        self._fb = None     # FFT of the flipped reference raster on that grid
 
     @property
     def im_ref(self):
@@ -95,20 +99,33 @@ class Aligner:
         r.write(buf)
         return buf.real.contiguous(), np.zeros(3, dtype=np.int32)
 
+    def reference_spectrum(self, full):
+        """FFT of the flipped reference raster on a padded grid of at least ``full`` per axis.
+
+        The spectrum stays resident: it is computed once, on a grid with room for moving rasters a
+        little larger than the reference's own (2 Rb + 2 per axis), and recomputed on a larger grid only
+        when a correlation needs more.  A linear correlation computed on any grid of at least ``full`` is the
+        same, so every correlation reuses it."""
+        if self._grid is None or any(f > g for f, g in zip(full, self._grid)):
+            Rb = self._ref.shape
+            grid = tuple(smooth_length(max(f, 2 * b + 2)) for f, b in zip(full, Rb))
+            fb = torch.zeros(grid, dtype=torch.complex64, device=self.X_ref.device)
+            self._ref.write(fb, flip=True)
+            L.call("mm_fft3d_c2c", L.ptr(fb), grid[0], grid[1], grid[2], 7, 0, L.stream_ptr(fb.device))
+            self._grid, self._fb = grid, fb
+        return self._grid, self._fb
+
     def correlation_peak(self, X):
         """Integer lag of the maximum of ``fftconvolve(im, im_ref[::-1,::-1,::-1])`` (``align.py:104-110``)."""
         r = _Raster(X, self.resc, self.trim)
         Ra, Rb = r.shape, self._ref.shape
         full = [a + b - 1 for a, b in zip(Ra, Rb)]
-        grid = [smooth_length(f) for f in full]
+        grid, fb = self.reference_spectrum(full)
         dev = X.device
         st = L.stream_ptr(dev)
         fa = torch.zeros(grid, dtype=torch.complex64, device=dev)
-        fb = torch.zeros(grid, dtype=torch.complex64, device=dev)
         r.write(fa)
-        self._ref.write(fb, flip=True)
-        for f in (fa, fb):
-            L.call("mm_fft3d_c2c", L.ptr(f), grid[0], grid[1], grid[2], 7, 0, st)
+        L.call("mm_fft3d_c2c", L.ptr(fa), grid[0], grid[1], grid[2], 7, 0, st)
         L.call("mm_cmul", L.ptr(fa), L.ptr(fb), fa.numel(), st)
         L.call("mm_fft3d_c2c", L.ptr(fa), grid[0], grid[1], grid[2], 7, 1, st)
         ws = torch.empty(L.lib().mm_argmax_ws_bytes(), dtype=torch.uint8, device=dev)

@@ -1,26 +1,41 @@
-"""Served form of the hyb path: a continuous-batching request loop over FOV stacks.
+"""Served form of the FOV path: a continuous-batching request loop over FOV stacks.
 
-A request is one FOV: several channel stacks (uint16 or float32, host or device) that share
-a flat field.  Requests arrive over time, as from the microscope.  The loop
+A request is one FOV as the microscope delivers it (the per-image body of
+``mermake/__main__.py:211-284``): hyb channel stacks (uint16, float32 or ``SquaredU8``; host or
+device) with their flat fields, optionally the DAPI channel, a PSF mode and the FOV block it
+belongs to.  Requests arrive over time.  The loop
 
-* admits an arrived request when its inputs fit the HBM budget (device memory free when the
-  server starts, minus a reserve, minus the inputs of the requests still in flight), and
-  copies host inputs to the device on a copy stream of its own;
-* keeps one FIFO of tile jobs over every admitted request and fills each launch with up to
-  ``batch_tiles`` of them, so tiles of different requests and FOV sizes share launches;
+* admits an arrived request when its inputs fit the HBM budget (device memory free after the
+  server's own buffers are allocated, minus a reserve, minus the inputs of the requests still in
+  flight) and copies host inputs to the device on a copy stream of its own.  A request that cannot
+  be served (wrong rank or depth, flat of the wrong shape, unknown PSF mode, over the budget, a
+  failure while its tiles are prepared) fails alone: its ``error`` is set, its bytes are released
+  and the loop goes on, as the reference's image queue skips a file it cannot read
+  (``io.py:451-460``);
+* keeps one FIFO of tile jobs per PSF mode (each mode is a ``HybPipeline`` with its own resident
+  Wiener spectra: one PSF, or a keyed PSF grid) over every admitted request, and fills each launch
+  with up to ``batch_tiles`` jobs of the mode whose oldest job arrived first, so tiles of different
+  requests and FOV sizes share launches;
 * runs up to ``slots`` batches at once, each on its own stream with its own buffers: the
   deconvolution of one batch overlaps the maxima / fit of another and the host's readback;
-* reads results back asynchronously into pinned memory (counts first, then exactly the kept
-  rows) and completes a request when its last tile is back, recording the latency from
-  arrival to the last spot array on the host.
+* runs each request's DAPI channel through a ``DapiPipeline`` (``__main__.py:213-229``) on a stream
+  of its own as soon as its copy has landed, and the drift (``align.drift``, ``__main__.py:272-282``)
+  of a FOV block once the block is closed (a later request of another FOV, or ``last``) and its
+  members are done.  The block's ``DualAligner`` keeps the reference raster spectra resident for
+  all of its members;
+* reads results back asynchronously into pinned memory (counts first, then exactly the kept rows)
+  and completes a request when its last tile and its DAPI features are back, recording the latency
+  from arrival to the last spot array on the host.
 
-Each channel's spots equal ``HybPipeline(stack, flat)`` (the hyb loop of
-``mermake/__main__.py:242-269``): a tile is computed by the same kernels whatever batch it
-lands in, and rows are assembled in tile order.
+Each channel's spots equal ``HybPipeline(stack, flat)`` of the request's PSF mode (the hyb loop of
+``__main__.py:242-269``), its DAPI features equal ``DapiPipeline(dapi, flat)``, and its drift equals
+``DualAligner(block[len(block) // 2], th=4).get_best_translation_pointsV2(request)``: a tile is
+computed by the same kernels whatever batch it lands in, and rows are assembled in tile order.
 """
 import collections
 import ctypes
 import time
+import types
 
 import numpy as np
 import torch
@@ -32,24 +47,57 @@ _REC = L.TILE_DTYPE.itemsize
 
 
 class FovRequest:
-    """One FOV: ``channels`` (each a (z, x, y) stack), an optional 2D ``flat`` and the
-    ``arrival`` time in seconds after the server's start.  ``spots`` (one (N, 8) float32
-    array per channel) and the timestamps are filled by the server."""
+    """One FOV.
 
-    def __init__(self, rid, channels, flat=None, arrival=0.0):
+    ``channels``: hyb channel stacks (z, x, y); ``flat``: one 2D flat for all of them, a list with one
+    per channel (``None`` entries allowed), or ``None``; ``dapi`` / ``dapi_flat``: the DAPI channel and
+    its flat (optional); ``psf``: key of the server's PSF mode for the hyb channels (``None``: the
+    first); ``fov``: FOV block key for the drift (``None``: no drift); ``last``: closes the block;
+    ``arrival``: seconds after the server's start.
+
+    Filled by the server: ``spots`` (one (N, 8) float32 array per hyb channel), ``Xh_plus`` /
+    ``Xh_minus`` (DAPI features), ``drift`` (``(tzxyf, t_plus, t_minus, N_plus, N_minus)``) and
+    ``drift_ref`` (the rid of the block's reference), ``error`` (the reason a request failed, else
+    ``None``), ``t_admit`` / ``t_done``."""
+
+    def __init__(self, rid, channels, flat=None, arrival=0.0, dapi=None, dapi_flat=None, psf=None, fov=None,
+                 last=False):
         self.rid = rid
         self.channels = list(channels)
         self.flat = flat
+        self.dapi = dapi
+        self.dapi_flat = dapi_flat
+        self.psf = psf
+        self.fov = fov
+        self.last = bool(last)
         self.arrival = float(arrival)
         self.t_admit = None
         self.t_done = None
         self.spots = None
-        self._dev = None
-        self._evs = None   # copy-done event of each channel (the flat is copied before channel 0)
-        self._next = 0     # channels whose tiles are queued
+        self.Xh_plus = None
+        self.Xh_minus = None
+        self.drift = None
+        self.drift_ref = None
+        self.drift_error = None
+        self.error = None
+        self._mode = None
+        self._chan = None     # device channels
+        self._flats = None    # device flat of each channel (or None)
+        self._dapi = None     # (device DAPI stack, device flat)
+        self._evs = None      # copy-done event of each channel
+        self._dapi_ev = None
+        self._dapi_done = True
+        self._next = 0        # channels whose tiles are queued
         self._rows = None
         self._left = 0
         self._bytes = 0
+
+    def channel_flats(self):
+        if isinstance(self.flat, (list, tuple)):
+            if len(self.flat) != len(self.channels):
+                raise ValueError(f"{len(self.flat)} flats for {len(self.channels)} channels")
+            return list(self.flat)
+        return [self.flat] * len(self.channels)
 
     @property
     def latency(self):
@@ -57,27 +105,34 @@ class FovRequest:
 
 
 class _Job:
-    __slots__ = ("req", "ch", "ti", "rec")
+    __slots__ = ("req", "ch", "ti", "rec", "mode")
 
-    def __init__(self, req, ch, ti, rec):
-        self.req, self.ch, self.ti, self.rec = req, ch, ti, rec
+    def __init__(self, req, ch, ti, rec, mode):
+        self.req, self.ch, self.ti, self.rec, self.mode = req, ch, ti, rec, mode
+
+
+def _fused_blur(pipe):
+    return pipe.blur_radius is not None and int(pipe.blur_radius) // 2 in (15, 25)
 
 
 class _Slot:
-    """Device and pinned buffers of one in-flight batch of up to ``B`` tiles."""
+    """Device and pinned buffers of one in-flight batch, sized for the largest batch of any mode."""
 
-    def __init__(self, pipe, B, cap, device):
-        d = pipe.d
-        T, sz = d._plan.T, d.tile_height
-        self.B, self.cap = B, cap
+    def __init__(self, pipes, B_of, cap, device):
+        self.cap = cap
+        self.B = max(B_of.values())
         self.stream = torch.cuda.Stream(device)
-        self.spec = torch.empty(B * d._plan.spec_bytes, dtype=torch.uint8, device=device)
-        self.res = torch.empty((B, sz, T, T), dtype=torch.float32, device=device)
+        spec = max(B_of[k] * p.d._plan.spec_bytes for k, p in pipes.items())
+        res = max(B_of[k] * p.d.tile_height * p.d._plan.T ** 2 for k, p in pipes.items())
+        self.spec = torch.empty(spec, dtype=torch.uint8, device=device)
+        self.res = torch.empty(res, dtype=torch.float32, device=device)
         self.tmp = None
-        if pipe.blur_radius is not None and int(pipe.blur_radius) // 2 not in (15, 25):
-            self.tmp = torch.empty((B, sz, T, T), dtype=torch.float32, device=device)
-        self.mws_bytes = L.lib().mm_maxima_ws_bytes(sz, T, T, B)
+        if any(p.blur_radius is not None and not _fused_blur(p) for p in pipes.values()):
+            self.tmp = torch.empty(res, dtype=torch.float32, device=device)
+        self.mws_bytes = max(L.lib().mm_maxima_ws_bytes(p.d.tile_height, p.d._plan.T, p.d._plan.T, B_of[k])
+                             for k, p in pipes.items())
         self.mws = torch.empty(self.mws_bytes, dtype=torch.uint8, device=device)
+        B = self.B
         self.coords = torch.empty((B, cap, 3), dtype=torch.int32, device=device)
         self.fit = torch.empty((B, cap, 8), dtype=torch.float32, device=device)
         self.stats = torch.empty((2 * B + 1,), dtype=torch.int32, device=device)  # counts | kept | total
@@ -89,46 +144,51 @@ class _Slot:
         self.ev_stats = torch.cuda.Event()
         self.ev_rows = torch.cuda.Event()
         self.jobs = None
+        self.pipe = None
         self.state = "free"
         self.n = 0
         self.total = 0
 
-    def launch(self, pipe, jobs, single):
+    def launch(self, pipe, jobs):
         d = pipe.d
         n = len(jobs)
         T, sz = d._plan.T, d.tile_height
+        single = d.psf_fft.shape[0] == 1
         recs = np.empty(n, dtype=L.TILE_DTYPE)
         for i, j in enumerate(jobs):
             recs[i] = j.rec
         self.table_h[:n * _REC].copy_(torch.from_numpy(recs.view(np.uint8)))
         st = ctypes.c_void_p(self.stream.cuda_stream)
         counts, kept, total = self.stats[:n], self.stats[self.B:self.B + n], self.stats[2 * self.B:]
+        res = self.res[:n * sz * T * T]
+        tmp = None if self.tmp is None else self.tmp[:n * sz * T * T]
         # the windows of the keep filter, as in HybPipeline: the deconvolution, the blur and the
         # candidates only where the kept spots can come from
         win = pipe.windows()
         delta = None if pipe.blur_radius is None else int(pipe.blur_radius) // 2
+        self.jobs, self.n, self.pipe, self.state = jobs, n, pipe, "stats"
         with torch.cuda.stream(self.stream):
             self.table[:n * _REC].copy_(self.table_h[:n * _REC], non_blocking=True)
             tab = L.ptr(self.table)
             dargs = (d._plan.h, tab, n, L.ptr(d.psf_fft), d.psf_fft.shape[0], n if single else 1, L.ptr(self.spec),
-                     L.ptr(self.res))
+                     L.ptr(res))
             if win is None or delta is None:
                 L.call("mm_deconv_tiles", *dargs, st)
             else:
                 L.call("mm_deconv_tiles_window", *dargs, max(0, win[0][0] - delta), min(T, win[0][1] + delta), st)
-            out = self.res
+            out = res
             fused = win is not None and d.fused_blur_maxima(pipe.blur_radius)
             if delta is not None:
                 # blur subtraction into the spectrum buffer, which is free again (deconvolver.py:160-163)
                 out = self.spec[:n * sz * T * T * 4].view(torch.float32)
                 if win is None:
-                    L.call("mm_blur_sub_xy", L.ptr(self.res), L.ptr(out), L.ptr(self.tmp), n * sz, T, delta, st)
+                    L.call("mm_blur_sub_xy", L.ptr(res), L.ptr(out), L.ptr(tmp), n * sz, T, delta, st)
                 elif fused:  # blur + maxima threshold bits, then the maxima of the candidates
-                    L.call("mm_tiles_blur_maxima_window", d._plan.h, tab, n, L.ptr(self.res), L.ptr(out), delta,
+                    L.call("mm_tiles_blur_maxima_window", d._plan.h, tab, n, L.ptr(res), L.ptr(out), delta,
                            win[0][0], win[0][1], pipe.threshold, pipe.delta, win[1][0], win[1][1],
                            L.ptr(self.coords), self.cap, L.ptr(counts), L.ptr(self.mws), self.mws_bytes, st)
                 else:
-                    L.call("mm_blur_sub_xy_window", L.ptr(self.res), L.ptr(out), L.ptr(self.tmp), n * sz, T, delta,
+                    L.call("mm_blur_sub_xy_window", L.ptr(res), L.ptr(out), L.ptr(tmp), n * sz, T, delta,
                            win[0][0], win[0][1], st)
             if win is None:
                 L.call("mm_tiles_maxima", d._plan.h, tab, n, L.ptr(out), pipe.threshold, pipe.delta,
@@ -143,7 +203,6 @@ class _Slot:
                    L.ptr(self.xh), n * self.cap, L.ptr(total), None, st)
             self.stats_h.copy_(self.stats, non_blocking=True)
             self.ev_stats.record(self.stream)
-        self.jobs, self.n, self.state = jobs, n, "stats"
 
     def fetch_rows(self):
         """After the counts are back: copy exactly the kept rows (or report an overflowing tile)."""
@@ -159,102 +218,201 @@ class _Slot:
         return True
 
 
-class HybServer:
-    """Continuous-batching request loop over a :class:`~mermake_amd.pipeline.HybPipeline`."""
+def _pctl(v, q):
+    v = sorted(v)
+    return v[min(len(v) - 1, int(np.ceil(q * len(v))) - 1)] if v else None
 
-    def __init__(self, pipe, batch_tiles=36, slots=2, cap=None, hbm_reserve=4 << 30, hbm_budget=None):
-        self.pipe = pipe
-        d = pipe.d
-        self.device = d.device
-        self.B = int(batch_tiles)
-        self.cap = int(cap or pipe.cap)
-        self.slots = [_Slot(pipe, self.B, self.cap, self.device) for _ in range(slots)]
+
+class HybServer:
+    """Continuous-batching request loop over one or several PSF modes (``HybPipeline`` each), with an
+    optional DAPI pipeline and the drift of FOV blocks.
+
+    ``pipe``: a ``HybPipeline``, or a dict {mode key: HybPipeline} (requests choose with ``psf=key``);
+    ``batch_tiles``: tiles per launch, an int or a dict per mode; ``dapi``: a ``DapiPipeline`` for the
+    requests' DAPI channels; ``drift_th``: the threshold of ``align.drift`` (``align.py:169``)."""
+
+    def __init__(self, pipe, batch_tiles=36, slots=2, cap=None, hbm_reserve=4 << 30, hbm_budget=None, dapi=None,
+                 drift_th=4):
+        self.pipes = dict(pipe) if isinstance(pipe, dict) else {None: pipe}
+        if not self.pipes:
+            raise ValueError("HybServer needs at least one pipeline")
+        self.default_mode = next(iter(self.pipes))
+        self.pipe = self.pipes[self.default_mode]
+        devs = {p.d.device for p in self.pipes.values()}
+        if len(devs) != 1:
+            raise ValueError("every pipeline of a server must live on one device")
+        self.device = devs.pop()
+        self.B_of = {k: int(batch_tiles[k] if isinstance(batch_tiles, dict) else batch_tiles) for k in self.pipes}
+        self.B = max(self.B_of.values())
+        self.cap = int(cap or max(p.cap for p in self.pipes.values()))
+        self.slots = [_Slot(self.pipes, self.B_of, self.cap, self.device) for _ in range(slots)]
         self.copy_stream = torch.cuda.Stream(self.device)
+        self.dapi = dapi
+        self.dapi_stream = torch.cuda.Stream(self.device) if dapi is not None else None
+        self.drift_th = drift_th
         torch.cuda.synchronize(self.device)
         free, _ = torch.cuda.mem_get_info(self.device)
         self.budget = int(hbm_budget) if hbm_budget is not None else max(0, free - int(hbm_reserve))
-        self.single = d.psf_fft.shape[0] == 1
         self.inflight_bytes = 0
         self.stats = {}
 
     # ------------------------------------------------------------------ admission
     @staticmethod
     def _host_bytes(t):
+        if t is None:
+            return 0
         if isinstance(t, torch.Tensor):
             return 0 if t.is_cuda else t.numel() * t.element_size()
         return np.asarray(t).nbytes
 
     def _need(self, req):
-        n = sum(self._host_bytes(c) for c in req.channels)
-        if req.flat is not None:
-            n += self._host_bytes(req.flat)
-        return n
+        flats = {id(f): f for f in (req.channel_flats() if isinstance(req.flat, (list, tuple)) else [req.flat])}
+        n = sum(self._host_bytes(c) for c in req.channels) + sum(self._host_bytes(f) for f in flats.values())
+        return n + self._host_bytes(req.dapi) + self._host_bytes(req.dapi_flat)
+
+    @staticmethod
+    def _shape(t):
+        return tuple(t.shape) if hasattr(t, "shape") else np.asarray(t).shape
+
+    def _validate(self, req):
+        """The checks a request must pass before any of its work is queued (ValueError otherwise)."""
+        if req.psf not in self.pipes and not (req.psf is None and self.default_mode in self.pipes):
+            raise ValueError(f"unknown PSF mode {req.psf!r} (the server has {list(self.pipes)})")
+        d = self.pipes[req.psf if req.psf in self.pipes else self.default_mode].d
+        for c, f in zip(req.channels, req.channel_flats()):
+            s = self._shape(c)
+            if len(s) != 3 or s[0] != d.tile_height:
+                raise ValueError(f"channel of shape {s}: the pipeline takes (z, x, y) stacks of depth {d.tile_height}")
+            if f is not None:
+                fs = self._shape(f)
+                if fs not in (s[1:], (1,) + s[1:]):
+                    raise ValueError(f"flat field shape {fs} does not match the stack's {s[1:]}")
+        if req.dapi is not None:
+            if self.dapi is None:
+                raise ValueError("a DAPI channel was sent to a server without a DAPI pipeline")
+            s = self._shape(req.dapi)
+            if len(s) != 3 or s[0] != self.dapi.d.tile_height:
+                raise ValueError(f"DAPI stack of shape {s}: the DAPI pipeline takes depth {self.dapi.d.tile_height}")
+            if req.dapi_flat is not None and self._shape(req.dapi_flat) not in (s[1:], (1,) + s[1:]):
+                raise ValueError(f"DAPI flat shape {self._shape(req.dapi_flat)} does not match {s[1:]}")
+        need = self._need(req)
+        if need > self.budget:
+            raise ValueError(f"request {req.rid} needs {need} bytes of HBM, over the budget {self.budget}")
+
+    def _to_device(self, t):
+        if t is None:
+            return None
+        if isinstance(t, torch.Tensor) and t.is_cuda:
+            return t
+        h = t if isinstance(t, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(t))
+        return h.to(self.device, non_blocking=True)
 
     def _admit(self, req, now):
-        """Start the request's host-to-device copies on the copy stream, the flat first, with an
-        event after each channel: a channel's tiles are queued as soon as its own copy is done."""
+        """Start the request's host-to-device copies on the copy stream, each channel's flat before the
+        channel, with an event after each channel: a channel's tiles are queued as soon as its own copy is
+        done."""
         req._bytes = self._need(req)
         self.inflight_bytes += req._bytes
         req.t_admit = now
-        d = self.pipe.d
-        req._rows = [[None] * len(d.geometry(tuple(c.shape))) for c in req.channels]
-        req._left = sum(len(r) for r in req._rows)
+        req._rows = [None] * len(req.channels)
+        req._left = 0
         req._next = 0
-        dev, evs = [], []
+        req._dapi_done = req.dapi is None
+        req._mode = req.psf if req.psf in self.pipes else self.default_mode
+        copied = {}
+        chans, flats, evs = [], [], []
         with torch.cuda.stream(self.copy_stream):
-            for c in ([req.flat] if req.flat is not None else []) + req.channels:
-                if isinstance(c, torch.Tensor) and c.is_cuda:
-                    dev.append(c)
-                else:
-                    h = c if isinstance(c, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(c))
-                    dev.append(h.to(self.device, non_blocking=True))
-                if c is not req.flat:
-                    ev = torch.cuda.Event()
-                    ev.record(self.copy_stream)
-                    evs.append(ev)
-        req._evs = evs
-        req._dev = dev  # [flat], channels...
+            for c, f in zip(req.channels, req.channel_flats()):
+                if f is not None and id(f) not in copied:
+                    copied[id(f)] = self._to_device(f)
+                flats.append(None if f is None else copied[id(f)])
+                chans.append(self._to_device(c))
+                ev = torch.cuda.Event()
+                ev.record(self.copy_stream)
+                evs.append(ev)
+            if req.dapi is not None:
+                req._dapi = (self._to_device(req.dapi), self._to_device(req.dapi_flat))
+                req._dapi_ev = torch.cuda.Event()
+                req._dapi_ev.record(self.copy_stream)
+        req._chan, req._flats, req._evs = chans, flats, evs
 
-    def _enqueue_channel(self, req, ch, jobs):
-        d = self.pipe.d
-        k = 1 if req.flat is not None else 0
-        flat = req._dev[0] if k else None
-        stack = req._dev[k + ch]
+    def _enqueue_channel(self, req, ch, fifo):
+        d = self.pipes[req._mode].d
+        stack, flat = req._chan[ch], req._flats[ch]
         d._check(stack)
         img = prepare_stack(stack)
         f = prepare_flat(flat, img)
-        req._dev[k + ch] = img
-        if f is not None and f is not flat:
-            req._dev.append(f)
+        req._chan[ch] = img
+        req._flats[ch] = f           # keeps a cast or reshaped flat alive with the request
         recs = tile_records(d.geometry(img.shape), img, f)
+        req._rows[ch] = [None] * len(recs)
+        req._left += len(recs)
         for ti in range(len(recs)):
-            jobs.append(_Job(req, ch, ti, recs[ti]))
+            fifo.append(_Job(req, ch, ti, recs[ti], req._mode))
+
+    def _run_dapi(self, req):
+        """The DAPI block of ``__main__.py:217-225`` for one request, on the DAPI stream."""
+        stack, flat = req._dapi
+        with torch.cuda.stream(self.dapi_stream):
+            self.dapi_stream.wait_event(req._dapi_ev)
+            plus, minus = self.dapi(stack, flat)
+            req.Xh_plus = plus.cpu().numpy()
+            req.Xh_minus = minus.cpu().numpy()
+        req._dapi = None
+        req._dapi_done = True
 
     # ------------------------------------------------------------------ completion
+    def _release(self, req):
+        if req._bytes:
+            self.inflight_bytes -= req._bytes
+            req._bytes = 0
+        req._chan = req._flats = req._dapi = req._evs = req._dapi_ev = None
+
+    def _fail(self, req, err, now, done):
+        if req.t_done is not None:
+            return
+        req.error = f"{type(err).__name__}: {err}" if isinstance(err, BaseException) else str(err)
+        req.spots = None
+        req._rows = None
+        req._left = 0
+        req._dapi_done = True
+        req.t_done = now
+        self._release(req)
+        done.append(req)
+
     def _complete(self, req, now, done):
-        """A request is done when every channel's tiles are queued and finished."""
-        if req.t_done is not None or req._left or req._next < len(req.channels):
+        """A request is done when every channel's tiles are queued and finished and its DAPI features
+        are back."""
+        if req.t_done is not None or req._left or req._next < len(req.channels) or not req._dapi_done:
             return
         req.spots = [np.concatenate(r) if r else np.zeros((0, 8), np.float32) for r in req._rows]
         req.t_done = now
         req._rows = None
-        req._dev = None
-        self.inflight_bytes -= req._bytes
+        self._release(req)
         done.append(req)
 
     def _finish_tile(self, job, rows, now, done):
         req = job.req
+        if req.error is not None:
+            return
         req._rows[job.ch][job.ti] = rows
         req._left -= 1
         self._complete(req, now, done)
 
-    def _overflow(self, slot, now, done):
-        """A tile had more candidates than the slot's cap: redo the batch's tiles one stack at a time."""
-        for job in slot.jobs:
+    def _redo_alone(self, jobs, now, done):
+        """Each job alone through its pipeline (a batch overflowed the candidate cap, or failed); a job that
+        fails again fails its request."""
+        for job in jobs:
+            if job.req.error is not None:
+                continue
             recs = np.empty(1, dtype=L.TILE_DTYPE)
             recs[0] = job.rec
-            table = torch.from_numpy(recs.view(np.uint8).copy()).to(self.device)
-            rows = self.pipe.tile_rows(table, 1)[0]
+            try:
+                table = torch.from_numpy(recs.view(np.uint8).copy()).to(self.device)
+                rows = self.pipes[job.mode].tile_rows(table, 1)[0]
+            except Exception as e:  # noqa: BLE001 - a request fails alone
+                self._fail(job.req, e, now, done)
+                continue
             self._finish_tile(job, rows, now, done)
 
     def _parse(self, slot, now, done):
@@ -267,74 +425,172 @@ class HybServer:
             self._finish_tile(job, rows[off:off + k].copy(), now, done)
             off += k
 
+    # ------------------------------------------------------------------ drift of FOV blocks
+    def _run_drift(self, block):
+        """``align.drift`` (``align.py:163-177``) over a closed block's served members: the reference is
+        the middle member, and its DualAligner (resident raster spectra) serves every member."""
+        from .align import DualAligner
+        members = [r for r in block if r.error is None and r.Xh_plus is not None]
+        if not members:
+            return
+        ref = members[len(members) // 2]
+        t0 = time.perf_counter()
+        with torch.cuda.stream(self.dapi_stream):
+            def feats(r):
+                return types.SimpleNamespace(Xh_plus=torch.from_numpy(r.Xh_plus).to(self.device),
+                                             Xh_minus=torch.from_numpy(r.Xh_minus).to(self.device))
+            try:
+                dual = DualAligner(feats(ref), th=self.drift_th)
+                for r in members:
+                    r.drift = dual.get_best_translation_pointsV2(feats(r))
+                    r.drift_ref = ref.rid
+            except Exception as e:  # noqa: BLE001 - the block's drift fails, its members' spots stand
+                for r in members:
+                    r.drift = None
+                    r.drift_error = f"{type(e).__name__}: {e}"
+        self._drift_s.append(time.perf_counter() - t0)
+
     # ------------------------------------------------------------------ loop
     def run(self, requests):
-        """Serve ``requests`` (arrival times relative to now); returns them completed, with
-        ``spots``, ``t_admit`` and ``t_done`` set, and fills ``self.stats``."""
+        """Serve ``requests`` (arrival times relative to now); returns them completed, with ``spots``,
+        ``Xh_plus`` / ``Xh_minus``, ``drift``, ``error``, ``t_admit`` and ``t_done`` set, and fills
+        ``self.stats``."""
         reqs = sorted(requests, key=lambda r: r.arrival)
-        for r in reqs:
-            if self._need(r) > self.budget:
-                raise ValueError(f"request {r.rid} needs {self._need(r)} bytes of HBM, over the budget {self.budget}")
         t0 = time.perf_counter()
         nxt = 0
         waiting = collections.deque()
         copying = []
-        jobs = collections.deque()
+        fifos = {k: collections.deque() for k in self.pipes}
         done = []
+        blocks = []          # [fov, members, closed]
+        self._drift_s = []
         launches = 0
         tiles = 0
         overflows = 0
-        while len(done) < len(reqs):
+        tiles_of = {k: 0 for k in self.pipes}
+
+        def close_blocks(keep_open=None):
+            for b in blocks:
+                if b[0] != keep_open:
+                    b[2] = True
+
+        while True:
             now = time.perf_counter() - t0
             progressed = False
             while nxt < len(reqs) and reqs[nxt].arrival <= now:
-                waiting.append(reqs[nxt])
+                r = reqs[nxt]
                 nxt += 1
+                if r.fov is not None:
+                    # a request of another FOV closes the open block (Block.add, io.py:302-310)
+                    close_blocks(keep_open=r.fov)
+                    if not blocks or blocks[-1][0] != r.fov or blocks[-1][2]:
+                        blocks.append([r.fov, [], False])
+                    blocks[-1][1].append(r)
+                    if r.last:
+                        blocks[-1][2] = True
+                try:
+                    self._validate(r)
+                except Exception as e:  # noqa: BLE001 - a request fails alone
+                    self._fail(r, e, now, done)
+                    progressed = True
+                    continue
+                waiting.append(r)
             while waiting and self.inflight_bytes + self._need(waiting[0]) <= self.budget:
                 r = waiting.popleft()
-                self._admit(r, now)
+                try:
+                    self._admit(r, now)
+                except Exception as e:  # noqa: BLE001
+                    self._fail(r, e, now, done)
+                    continue
                 copying.append(r)
                 progressed = True
             for r in list(copying):
-                while r._next < len(r.channels) and r._evs[r._next].query():
-                    self._enqueue_channel(r, r._next, jobs)
-                    r._next += 1
+                if r.t_done is not None:
+                    copying.remove(r)
+                    continue
+                try:
+                    while r._next < len(r.channels) and r._evs[r._next].query():
+                        self._enqueue_channel(r, r._next, fifos[r._mode])
+                        r._next += 1
+                        progressed = True
+                    if not r._dapi_done and r._dapi_ev.query():
+                        self._run_dapi(r)
+                        progressed = True
+                except Exception as e:  # noqa: BLE001 - a request fails alone
+                    self._fail(r, e, time.perf_counter() - t0, done)
+                    copying.remove(r)
                     progressed = True
-                if r._next == len(r.channels):
+                    continue
+                if r._next == len(r.channels) and r._dapi_done:
                     copying.remove(r)
                     self._complete(r, time.perf_counter() - t0, done)  # requests without tiles
                     progressed = True
             for slot in self.slots:
-                if slot.state == "free" and jobs:
-                    batch = [jobs.popleft() for _ in range(min(self.B, len(jobs)))]
-                    slot.launch(self.pipe, batch, self.single)
+                if slot.state == "free":
+                    for q in fifos.values():   # jobs of failed requests are dropped
+                        while q and q[0].req.error is not None:
+                            q.popleft()
+                    live = [k for k, q in fifos.items() if q]
+                    if not live:
+                        continue
+                    mode = min(live, key=lambda k: (fifos[k][0].req.arrival, fifos[k][0].req.t_admit))
+                    q = fifos[mode]
+                    batch = []
+                    while q and len(batch) < self.B_of[mode]:
+                        j = q.popleft()
+                        if j.req.error is None:
+                            batch.append(j)
+                    if not batch:
+                        continue
+                    try:
+                        slot.launch(self.pipes[mode], batch)
+                    except Exception:  # noqa: BLE001 - retried tile by tile; failing tiles fail their request
+                        torch.cuda.synchronize(self.device)
+                        slot.state = "free"
+                        self._redo_alone(batch, time.perf_counter() - t0, done)
+                        progressed = True
+                        continue
                     launches += 1
                     tiles += len(batch)
+                    tiles_of[mode] += len(batch)
                     progressed = True
                 elif slot.state == "stats" and slot.ev_stats.query():
                     if not slot.fetch_rows():
                         overflows += 1
-                        self._overflow(slot, time.perf_counter() - t0, done)
+                        self._redo_alone(slot.jobs, time.perf_counter() - t0, done)
                         slot.state = "free"
                     progressed = True
                 elif slot.state == "rows" and slot.ev_rows.query():
                     self._parse(slot, time.perf_counter() - t0, done)
                     slot.state = "free"
                     progressed = True
+            if nxt == len(reqs) and not waiting:
+                close_blocks()
+            for b in blocks:
+                if b[2] and all(r.t_done is not None for r in b[1]):
+                    self._run_drift(b[1])
+                    b[2] = None          # drifted
+                    progressed = True
+            blocks = [b for b in blocks if b[2] is not None]
+            if len(done) == len(reqs) and not blocks:
+                break
             if not progressed:
-                idle = not jobs and not copying and all(s.state == "free" for s in self.slots)
+                idle = not any(fifos.values()) and not copying and all(s.state == "free" for s in self.slots)
                 if idle and not waiting and nxt < len(reqs):
                     time.sleep(max(0.0, reqs[nxt].arrival - (time.perf_counter() - t0)))
                 else:
                     time.sleep(20e-6)
         wall = time.perf_counter() - t0
-        lat = sorted(r.latency for r in reqs)
+        ok = [r for r in reqs if r.error is None]
+        lat = [r.latency for r in ok]
         self.stats = {
-            "requests": len(reqs), "wall_s": wall, "fovs_per_s": len(reqs) / wall if wall > 0 else None,
-            "p50_latency_s": lat[len(lat) // 2] if lat else None,
-            "p99_latency_s": lat[min(len(lat) - 1, int(np.ceil(0.99 * len(lat))) - 1)] if lat else None,
-            "launches": launches, "tiles": tiles, "overflow_batches": overflows, "batch_tiles": self.B,
-            "slots": len(self.slots),
+            "requests": len(reqs), "failed": len(reqs) - len(ok), "wall_s": wall,
+            "fovs_per_s": len(ok) / wall if wall > 0 else None,
+            "p50_latency_s": _pctl(lat, 0.5), "p99_latency_s": _pctl(lat, 0.99),
+            "launches": launches, "tiles": tiles, "tiles_per_mode": {str(k): v for k, v in tiles_of.items()},
+            "overflow_batches": overflows, "batch_tiles": self.B, "slots": len(self.slots),
+            "dapi_stacks": sum(1 for r in ok if r.Xh_plus is not None),
+            "drift_blocks": len(self._drift_s), "drift_s_total": float(sum(self._drift_s)),
             "hbm_budget_bytes": self.budget,
         }
         return reqs

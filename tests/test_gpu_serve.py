@@ -113,13 +113,36 @@ def test_server_mix_equals_alone(fovs, pipe, alone, dev, host_inputs, batch_tile
             assert b.t_admit >= a.t_done - 1e-9
 
 
-def test_server_rejects_oversized_request(pipe, fovs):
+def test_server_fails_bad_requests_alone(pipe, fovs, alone, dev):
+    """Requests that cannot be served (over the HBM budget, wrong depth, a flat of the wrong shape, an
+    unknown PSF mode, a flat list of the wrong length) fail alone, with the reason in ``error``; the
+    others, arriving before, between and after them, give the spots of their channels alone."""
     from mermake_amd.serve import FovRequest, HybServer
     data, _ = fovs
-    chans, flat = data[0]
-    srv = HybServer(pipe, batch_tiles=4, slots=1, hbm_budget=1024)
-    with pytest.raises(ValueError):
-        srv.run([FovRequest(0, [torch.from_numpy(c) for c in chans], torch.from_numpy(flat))])
+    per = max(sum(c.nbytes for c in chans) + flat.nbytes for chans, flat in data)
+    good = [FovRequest(i, [torch.from_numpy(c).pin_memory() for c in chans], torch.from_numpy(flat).pin_memory(),
+                       arrival=0.002 * i) for i, (chans, flat) in enumerate(data)]
+    chans0, flat0 = data[0]
+    big = [torch.from_numpy(np.concatenate([c, c], axis=1)) for c in chans0 * 2]          # over the budget
+    bad = [FovRequest(100, big, None, arrival=0.0),
+           FovRequest(101, [torch.from_numpy(c[:5]) for c in chans0], torch.from_numpy(flat0), arrival=0.003),
+           FovRequest(102, [torch.from_numpy(c) for c in chans0], torch.from_numpy(flat0[:, :7]), arrival=0.003),
+           FovRequest(103, [torch.from_numpy(c) for c in chans0], torch.from_numpy(flat0), arrival=0.004,
+                      psf="nope"),
+           FovRequest(104, [torch.from_numpy(c) for c in chans0], [torch.from_numpy(flat0)], arrival=0.01)]
+    srv = HybServer(pipe, batch_tiles=5, slots=2, hbm_budget=int(2.5 * per))
+    done = srv.run(good + bad)
+    assert len(done) == len(good) + len(bad)
+    for r in bad:
+        assert r.error and r.spots is None and r.t_done is not None, (r.rid, r.error)
+    assert "budget" in bad[0].error and "depth" in bad[1].error and "flat" in bad[2].error
+    assert "PSF mode" in bad[3].error and "flats for" in bad[4].error
+    for r in good:
+        assert r.error is None
+        for c, sp in enumerate(r.spots):
+            assert np.array_equal(sp, alone[r.rid][c]), (r.rid, c)
+    assert srv.stats["failed"] == len(bad) and srv.stats["requests"] == len(good) + len(bad)
+    assert srv.inflight_bytes == 0
 
 
 def test_server_keyed_multi_psf_equals_alone(fovs, dev):
@@ -167,3 +190,71 @@ def test_server_squared_u8_requests_equal_uint16(fovs, pipe, dev):
     for r in done:
         for c, s in enumerate(r.spots):
             assert np.array_equal(s, want[r.rid][c]), (r.rid, c)
+
+
+def _dapi_pipe(depth, dev):
+    from mermake_amd.deconvolver import Deconvolver
+    from mermake_amd.pipeline import DapiPipeline
+    from mermake_amd.synth import gaussian_psf
+    psf = gaussian_psf((depth, 31, 31), 1.5, 1.2)
+    d = Deconvolver(psf, (depth, 300, 300), tile_size=100, zpad=(depth - 1) // 2, overlap=20, beta=0.01)
+    return DapiPipeline(d, blur_radius=50, threshold=1.0, delta=5, delta_fit=5, sigmaZ=1, sigmaXY=1.5)
+
+
+def test_server_mixed_psf_modes_dapi_and_drift_equal_alone(fovs, pipe, alone, dev):
+    """One server holds a single-PSF mode and a keyed multi-PSF mode, serves each request's DAPI channel and
+    the drift of its FOV block: every request's hyb spots equal its mode's pipeline alone, its DAPI features
+    equal DapiPipeline alone, and its drift equals ``DualAligner(block[len // 2], th)`` alone
+    (``__main__.py:213-229,272-282``)."""
+    import types
+    from mermake_amd.align import DualAligner
+    from mermake_amd.pipeline import HybPipeline
+    from mermake_amd.serve import FovRequest, HybServer
+    from mermake_amd.synth import gaussian_psf, nuclei_torch
+    data, _ = fovs
+    psfs = {(0, x, y): gaussian_psf((11, 31, 31), 1.2 + 0.2 * i, 1.0 + 0.15 * i)
+            for i, (x, y) in enumerate(((75, 75), (75, 225), (225, 75), (225, 225)))}
+    grid = HybPipeline.from_settings(psfs, SHAPES[0], tile_size=100, overlap=20, beta=1e-4, psf_mode="keyed", **HYB)
+    dapi = _dapi_pipe(12, dev)
+    modes = ["single", "grid", "grid", "single"]
+    fov_of = ["A", "A", "B", "B"]
+    shifts = [(0, 0, 0), (1, 4, -3), (0, 0, 0), (-1, -5, 6)]
+    dapis = [nuclei_torch(50 + (0 if f == "A" else 1), (12,) + SHAPES[0 if f == "A" else 2][1:], 6, dev, shift=sh,
+                          noise_seed=60 + i, radius=(12.0, 20.0), zradius=(3.0, 5.0), spots=20)
+             for i, (f, sh) in enumerate(zip(fov_of, shifts))]
+    want_spots, want_dapi = [], []
+    for i, (chans, flat) in enumerate(data):
+        p = pipe if modes[i] == "single" else grid
+        fl = torch.from_numpy(flat).to(dev)
+        want_spots.append([p(torch.from_numpy(c).to(dev), fl).cpu().numpy() for c in chans])
+        pl, mi = dapi(dapis[i])
+        want_dapi.append((pl.cpu().numpy(), mi.cpu().numpy()))
+    reqs = [FovRequest(i, [torch.from_numpy(c).pin_memory() for c in chans], torch.from_numpy(flat).pin_memory(),
+                       arrival=0.001 * i, dapi=dapis[i].cpu().pin_memory(), psf=modes[i], fov=fov_of[i],
+                       last=(i == 3)) for i, (chans, flat) in enumerate(data)]
+    print("DAPI features", [(len(p), len(m)) for p, m in want_dapi], flush=True)
+    # a small scene: the drift threshold sits just above the DAPI threshold so that both signs keep points
+    srv = HybServer({"single": pipe, "grid": grid}, batch_tiles={"single": 6, "grid": 5}, slots=2, dapi=dapi,
+                    drift_th=1.2)
+    done = srv.run(reqs)
+    for r in done:
+        assert r.error is None, r.error
+        for c, sp in enumerate(r.spots):
+            assert np.array_equal(sp, want_spots[r.rid][c]), (r.rid, c)
+        assert np.array_equal(r.Xh_plus, want_dapi[r.rid][0]) and np.array_equal(r.Xh_minus, want_dapi[r.rid][1])
+    st = srv.stats
+    assert st["dapi_stacks"] == 4 and st["drift_blocks"] == 2
+    assert st["tiles_per_mode"]["single"] > 0 and st["tiles_per_mode"]["grid"] > 0
+
+    def feats(i):
+        return types.SimpleNamespace(Xh_plus=torch.from_numpy(want_dapi[i][0]).to(dev),
+                                     Xh_minus=torch.from_numpy(want_dapi[i][1]).to(dev))
+    for members in ([0, 1], [2, 3]):
+        ref = members[len(members) // 2]
+        dual = DualAligner(feats(ref), th=1.2)
+        for i in members:
+            f, tp, tm, n_p, n_m = dual.get_best_translation_pointsV2(feats(i))
+            g = done[i].drift
+            assert done[i].drift_ref == ref
+            assert np.array_equal(f, g[0]) and np.array_equal(tp, g[1]) and np.array_equal(tm, g[2])
+            assert (n_p, n_m) == (g[3], g[4])

@@ -17,3 +17,15 @@ def test_request_latency():
     assert r.latency is None and r.rid == 7 and len(r.channels) == 1
     r.t_done = 2.0
     assert r.latency == 0.5
+
+
+def test_request_channel_flats():
+    import pytest
+    from mermake_amd.serve import FovRequest
+    a, b = np.zeros((3, 3), np.float32), np.ones((3, 3), np.float32)
+    chans = [np.zeros((2, 3, 3), np.uint16)] * 2
+    assert FovRequest(0, chans, a).channel_flats() == [a, a]
+    assert FovRequest(0, chans, [a, b]).channel_flats() == [a, b]
+    assert FovRequest(0, chans, None).channel_flats() == [None, None]
+    with pytest.raises(ValueError):
+        FovRequest(0, chans, [a]).channel_flats()
