@@ -327,3 +327,71 @@ def test_dapi_pair_drift_recovers_integer_shift(dev, shift):
     if n_m >= 50:      # the minus side (maxima of -buffer with h > 4) is weak on this scene
         assert np.array_equal(np.round(tm).astype(int), np.array(shift)), tm
     assert np.array_equal(np.round(f).astype(int), -np.array(shift)), f
+
+
+# ------------------------------------------------------------------ served stream
+def test_served_stream_full_sizes_equal_alone(dev):
+    """Served stream at full FOV sizes: six requests of 2 hyb channels + 1 DAPI channel, 40x{2048, 2304,
+    2800}^2, each FOV acquired twice (the second DAPI shifted by an integer offset), the single-PSF and the
+    keyed PSF-grid modes alternating, arriving as a Poisson stream.  Every request's hyb spots, DAPI features
+    and block drift equal those of the same pipelines alone, and each block's drift recovers its shift."""
+    from mermake_amd.align import DualAligner
+    from mermake_amd.deconvolver import Deconvolver
+    from mermake_amd.pipeline import DapiPipeline, HybPipeline
+    from mermake_amd.serve import FovRequest, HybServer, poisson_arrivals
+    from mermake_amd.synth import gaussian_psf, nuclei_torch, stack_torch
+    psf = gaussian_psf((40, 64, 64), 1.5, 1.2)
+    grid_psfs = {}
+    for i in range(4):
+        for j in range(4):
+            t = (i * 4 + j) / 15
+            grid_psfs[(0, 350 + 700 * i, 350 + 700 * j)] = gaussian_psf((40, 64, 64), 1.2 + 0.8 * t, 1.0 + 0.6 * t)
+    single = HybPipeline.from_settings(psf, (40, 2800, 2800), tile_size=500, overlap=89, beta=1e-4, **HYB)
+    grid = HybPipeline.from_settings(grid_psfs, (40, 2800, 2800), tile_size=300, overlap=89, beta=1e-4,
+                                     psf_mode="keyed", **HYB)
+    dapi = DapiPipeline(Deconvolver(psf, (40, 2800, 2800), tile_size=500, zpad=19, overlap=89, beta=0.01), **DAPI)
+    plan = [("A", 2048, "single", (0, 0, 0)), ("A", 2048, "grid", (1, -12, 9)),
+            ("B", 2304, "grid", (0, 0, 0)), ("B", 2304, "single", (-2, 21, 14)),
+            ("C", 2800, "single", (0, 0, 0)), ("C", 2800, "grid", (2, 8, -25))]
+    blobs = {2048: 930, 2304: 1180, 2800: 1740}
+    arrivals = poisson_arrivals(len(plan), 8.0, seed=5)
+    reqs, want = [], []
+    for i, (fov, side, mode, sh) in enumerate(plan):
+        chans, flat = [], None
+        for c in range(2):
+            s, flat = stack_torch(5000 + 10 * i + c, (40, side, side), 1.0 / 20 ** 3, dev, flat=True)
+            chans.append(s)
+        da = nuclei_torch(6000 + ord(fov), (40, side, side), blobs[side], dev, shift=sh, noise_seed=6100 + i)
+        p = single if mode == "single" else grid
+        spots = [p(s, flat).cpu().numpy() for s in chans]
+        pl, mi = dapi(da)
+        want.append((spots, pl.cpu().numpy(), mi.cpu().numpy()))
+        reqs.append(FovRequest(i, [s.cpu().pin_memory() for s in chans], flat.cpu().pin_memory(),
+                               arrival=float(arrivals[i]), dapi=da.cpu().pin_memory(), psf=mode, fov=fov))
+        del chans, da
+    srv = HybServer({"single": single, "grid": grid}, batch_tiles={"single": 36, "grid": 50}, slots=2, dapi=dapi)
+    done = srv.run(reqs)
+    st = srv.stats
+    log(f"served stream: {st['fovs_per_s']:.2f} FOV/s, p50 {st['p50_latency_s']:.3f} s, p99 {st['p99_latency_s']:.3f} s, "
+        f"tiles {st['tiles_per_mode']}, drift blocks {st['drift_blocks']}")
+    assert st["failed"] == 0 and st["drift_blocks"] == 3 and st["dapi_stacks"] == 6
+    for r in done:
+        spots, pl, mi = want[r.rid]
+        assert all(np.array_equal(a, b) for a, b in zip(r.spots, spots)) and len(spots[0]) > 1000, r.rid
+        assert np.array_equal(r.Xh_plus, pl) and np.array_equal(r.Xh_minus, mi), r.rid
+
+    def feats(i):
+        return types.SimpleNamespace(Xh_plus=torch.from_numpy(want[i][1]).to(dev),
+                                     Xh_minus=torch.from_numpy(want[i][2]).to(dev))
+    for members in ([0, 1], [2, 3], [4, 5]):
+        ref = members[len(members) // 2]
+        dual = DualAligner(feats(ref), th=4)
+        for i in members:
+            f, tp, tm, n_p, n_m = dual.get_best_translation_pointsV2(feats(i))
+            g = done[i].drift
+            assert done[i].drift_ref == ref and np.array_equal(f, g[0]) and (n_p, n_m) == (g[3], g[4]), i
+        # the reference member is the shifted one: the other member's drift is the negated shift's opposite
+        sh = np.array(plan[ref][3]) - np.array(plan[members[0]][3])
+        f0 = done[members[0]].drift[0]
+        log(f"block {plan[ref][0]}: drift of member {members[0]} {f0}, shift {sh}")
+        assert np.array_equal(np.round(f0).astype(int), sh), (f0, sh)
