@@ -11,7 +11,11 @@ delta_fit -> keep-filter, batched over its 36 tiles.  Synthetic seeded data
 Prints one JSON line: FOV voxels/s, the roofline of the dominant kernel
 (HIP-event timed on the stream the kernel runs on), the CPU baseline (the
 float32 NumPy/SciPy restatement on a bounded tile sample, extrapolated), and
-the plain one-tile-at-a-time GPU form for comparison.
+the plain one-tile-at-a-time GPU form for comparison.  Further legs (``legs``)
+measure the other workloads: the 100-PSF grid (40x3000^2 at tile 300) with its
+DAPI pass on nuclei, a batch of uneven FOVs (40 x {2048, 2304, 2800}^2) with
+DAPI-pair drift, and the served stream (2 hyb + 1 DAPI channels per request,
+single or multi-PSF, block drift) at Poisson rates against the plain form.
 """
 import argparse
 import ctypes
@@ -110,11 +114,12 @@ def cpu_baseline(stack_np, flat_np, psf, cfg, ntiles_per_stack):
 
 
 def serve_leg(pipe, stacks, flat, cfg, args, dev):
-    """The request loop on mixed FOV sizes (40 x {2048, 2304, 2800}^2, 3 channels each, host inputs in
-    pinned memory): Poisson arrivals at ``--serve-rate``, then all requests at once (saturation)."""
+    """The request loop saturated with hyb FOVs of mixed sizes (40 x {2048, 2304, 2800}^2, 3 channels each, host
+    inputs in pinned memory, all requests at t = 0): uint16 stacks, then the uint8 frames whose squares are the
+    pixels."""
     import torch
     from mermake_amd.deconvolver import SquaredU8
-    from mermake_amd.serve import FovRequest, HybServer, poisson_arrivals
+    from mermake_amd.serve import FovRequest, HybServer
     from mermake_amd.synth import stack_torch
     fovs, fovs_u8 = {}, {}
     for side in (2048, 2304, 2800):
@@ -137,8 +142,7 @@ def serve_leg(pipe, stacks, flat, cfg, args, dev):
     srv = HybServer(pipe, batch_tiles=36, slots=2)
     out = {"fov_sides": order, "inputs": "host pinned uint16 stacks + float32 flat (H2D inside the loop); "
                                          "saturated_u8: the uint8 frames whose squares are the pixels (SquaredU8)"}
-    for name, arr, src in (("poisson", poisson_arrivals(len(order), args.serve_rate, seed=11), fovs),
-                           ("saturated", np.zeros(len(order)), fovs),
+    for name, arr, src in (("saturated", np.zeros(len(order)), fovs),
                            ("saturated_u8", np.zeros(len(order)), fovs_u8)):
         srv.run([FovRequest(-1 - i, *src[side]) for i, side in enumerate(order[:2])])  # warm the slots
         reqs = [FovRequest(i, *src[side], arrival=float(a)) for i, (side, a) in enumerate(zip(order, arr))]
@@ -146,8 +150,6 @@ def serve_leg(pipe, stacks, flat, cfg, args, dev):
         st = dict(srv.stats)
         vox = sum(cfg["channels"] * cfg["sz"] * s_ * s_ for s_ in order)
         st["voxels_per_s"] = vox / st["wall_s"]
-        if name == "poisson":
-            st["rate_fov_per_s"] = args.serve_rate
         log(f"served {name}: {st['fovs_per_s']:.2f} FOV/s, p50 {st['p50_latency_s'] * 1e3:.1f} ms, "
             f"p99 {st['p99_latency_s'] * 1e3:.1f} ms")
         out[name] = st
@@ -244,6 +246,211 @@ def dapi_leg(stack, flat, psf, cfg):
     return out
 
 
+def grid_psfs(n, ts, shape=(40, 64, 64)):
+    """n x n Gaussian PSFs keyed (0, x, y) at the centres of the tiles of size ts, x-major, widths varying
+    smoothly across the field (sigma_xy 1.0 -> 1.6, sigma_z 1.2 -> 2.0)."""
+    from mermake_amd.synth import gaussian_psf
+    out = {}
+    for i in range(n):
+        for j in range(n):
+            t = (i + j) / max(1, 2 * (n - 1))
+            out[(0, i * ts + ts // 2, j * ts + ts // 2)] = gaussian_psf(shape, 1.2 + 0.8 * t, 1.0 + 0.6 * t)
+    return out
+
+
+def _timed(fn, reps):
+    """Mean wall ms of ``fn`` over ``reps`` runs after one warm-up (synchronised on both sides)."""
+    import torch
+    out = fn()
+    torch.cuda.synchronize()
+    a = time.perf_counter()
+    for _ in range(reps):
+        out = fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - a) / reps * 1e3, out
+
+
+def multi_psf_leg(dev, cfg, grid_pipe, dapi_grid, reps=3):
+    """The multi-PSF grid workload: one 40x3000^2 hyb stack at tile 300 / overlap 89 (100 tiles of T 478, Zp 118),
+    each tile with its own of 100 PSFs; and the DAPI pass of the same shape and PSF grid (zpad 19, Zp 78,
+    beta 0.01, blur 50, threshold 3, delta 5, delta_fit 5) on 2000 nuclei."""
+    import torch
+    from mermake_amd.deconvolver import tile_geometry
+    from mermake_amd.synth import nuclei_torch, stack_torch
+    sz, side = cfg["sz"], 3000
+    stack, flat = stack_torch(7000, (sz, side, side), cfg["density"], dev, flat=True)
+    d = grid_pipe.d
+    geom = d.geometry(stack.shape)
+    assert [g["psf"] for g in geom] == list(range(100)), "the 10x10 grid pairs each tile with its own PSF"
+    ms, out = _timed(lambda: grid_pipe(stack, flat), reps)
+    vox = stack.numel()
+    m1 = m1_bytes_per_stack(geom, sz, d._plan.T, d._plan.Zp)
+    hyb = {"workload": "40x3000^2 uint16 + flat, tile 300 / overlap 89, 100 tiles of 118x478x478 each with its own "
+                       "of 100 PSFs (keyed 10x10 grid)", "ms_per_stack": ms, "voxels_per_s": vox / (ms * 1e-3),
+           "m1_roofline_frac": m1 / (ms * 1e-3) / HBM_PEAK, "m1_bytes": m1, "spots": int(out.shape[0])}
+    log(f"multi-PSF grid: {ms:.1f} ms per 40x3000^2 stack ({vox / ms / 1e6:.1f} G voxels/s, M1 "
+        f"{hyb['m1_roofline_frac']:.3f}), {out.shape[0]} spots")
+    del stack, flat, out
+    nuc = nuclei_torch(7001, (sz, side, side), 2000, dev)
+    ms, (pl, mi) = _timed(lambda: dapi_grid(nuc), reps)
+    dd = dapi_grid.d
+    m1 = m1_bytes_per_stack(tile_geometry(nuc.shape, dd.tile_size, dd.overlap), sz, dd._plan.T, dd._plan.Zp,
+                            flat=False)
+    dapi = {"workload": "DAPI 40x3000^2 uint16, 2000 nuclei, the same 100-PSF grid, tile 300 / overlap 89, zpad 19 "
+                        "(Zp 78), beta 0.01, blur 50, threshold 3, delta 5, delta_fit 5", "ms_per_stack": ms,
+            "voxels_per_s": vox / (ms * 1e-3), "m1_roofline_frac": m1 / (ms * 1e-3) / HBM_PEAK, "m1_bytes": m1,
+            "spots_plus": int(pl.shape[0]), "spots_minus": int(mi.shape[0])}
+    log(f"DAPI nuclei (100-PSF grid): {ms:.1f} ms per stack ({vox / ms / 1e6:.1f} G voxels/s, M1 "
+        f"{dapi['m1_roofline_frac']:.3f}), {pl.shape[0]} + {mi.shape[0]} spots")
+    del nuc
+    torch.cuda.empty_cache()
+    return {"hyb": hyb, "dapi": dapi}
+
+
+def uneven_leg(dev, cfg, pipe, dapi_pipe, reps=3):
+    """Batched uneven tiles + drift: three hyb stacks of 40 x 2048^2, 2304^2 and 2800^2 (interior and edge tiles of
+    different sizes) in one run_many; and for each size a DAPI pair, the second shifted by a seeded integer offset
+    (dz in +-3, dy, dx in +-30) with independent noise, whose drift must come back exactly."""
+    import types
+    import torch
+    from mermake_amd.align import DualAligner
+    from mermake_amd.deconvolver import tile_geometry
+    from mermake_amd.synth import nuclei_torch, stack_torch
+    sz = cfg["sz"]
+    sides = (2048, 2304, 2800)
+    stacks, flats = [], []
+    for i, side in enumerate(sides):
+        s, f = stack_torch(7100 + i, (sz, side, side), cfg["density"], dev, flat=True)
+        stacks.append(s)
+        flats.append(f)
+    ms, out = _timed(lambda: pipe.run_many(stacks, flats), reps)
+    T, Zp = pipe.d._plan.T, pipe.d._plan.Zp
+    m1 = sum(m1_bytes_per_stack(tile_geometry(s.shape, cfg["tile_size"], cfg["overlap"]), sz, T, Zp) for s in stacks)
+    vox = sum(s.numel() for s in stacks)
+    batch = {"workload": "run_many over 40x2048^2, 40x2304^2 and 40x2800^2 uint16 + flats (25 + 25 + 36 tiles of "
+                         "T 678) in shared launches", "ms": ms, "voxels_per_s": vox / (ms * 1e-3),
+             "m1_roofline_frac": m1 / (ms * 1e-3) / HBM_PEAK, "spots": [int(o.shape[0]) for o in out]}
+    log(f"uneven batch: {ms:.1f} ms for 3 stacks ({vox / ms / 1e6:.1f} G voxels/s, M1 {batch['m1_roofline_frac']:.3f})")
+    del stacks, flats, out
+    rng = np.random.default_rng(31)
+    pairs = []
+    for i, side in enumerate(sides):
+        shift = (int(rng.integers(-3, 4)), int(rng.integers(-30, 31)), int(rng.integers(-30, 31)))
+        feats = []
+        a = time.perf_counter()
+        for k, sh in enumerate(((0, 0, 0), shift)):
+            nuc = nuclei_torch(7200 + i, (sz, side, side), int(2000 * side * side / 3000 ** 2), dev, shift=sh,
+                               noise_seed=7300 + 2 * i + k)
+            torch.cuda.synchronize()
+            a0 = time.perf_counter()
+            pl, mi = dapi_pipe(nuc)
+            feats.append(types.SimpleNamespace(Xh_plus=pl.clone(), Xh_minus=mi.clone()))
+            torch.cuda.synchronize()
+            a += time.perf_counter() - a0
+        b = time.perf_counter()
+        f, tp, tm, n_p, n_m = DualAligner(feats[0], th=4).get_best_translation_pointsV2(feats[1])
+        drift_ms = (time.perf_counter() - b) * 1e3
+        ok = bool(np.array_equal(np.round(f).astype(int), -np.array(shift)))
+        pairs.append({"side": side, "shift": list(shift), "tzxyf": [float(v) for v in f], "recovered": ok,
+                      "N_plus": int(n_p), "N_minus": int(n_m), "drift_ms": drift_ms})
+        log(f"drift {side}^2: shift {shift}, tzxyf {np.round(f, 3).tolist()}, recovered {ok}, {drift_ms:.1f} ms")
+    torch.cuda.empty_cache()
+    return {"batch": batch, "drift_pairs": pairs, "drift_all_recovered": all(p["recovered"] for p in pairs)}
+
+
+def stream_requests(dev, cfg, n, seed=41):
+    """Config-4 style requests: 3 x 40 x H x W uint16 (2 hyb channels + the DAPI channel, the reference's last
+    channel), H = W uniform in {2048, 2304, 2800}, single or multi-PSF mode 50/50, FOV blocks of two
+    acquisitions (the second DAPI shifted), host inputs in pinned memory.  Returns factories of FovRequests."""
+    import torch
+    from mermake_amd.synth import nuclei_torch, stack_torch
+    rng = np.random.default_rng(seed)
+    pool = {}
+    for side in (2048, 2304, 2800):
+        for k in range(2):
+            chans, flat = [], None
+            for c in range(2):
+                s_, flat = stack_torch(8000 + 10 * side + 2 * k + c, (cfg["sz"], side, side), cfg["density"], dev,
+                                       flat=True)
+                chans.append(s_.cpu().pin_memory())
+            sh = (0, 0, 0) if k == 0 else (1, 7, -11)
+            nuc = nuclei_torch(8100 + side, (cfg["sz"], side, side), int(2000 * side * side / 3000 ** 2), dev,
+                               shift=sh, noise_seed=8200 + 2 * side + k)
+            pool[(side, k)] = (chans, flat.cpu().pin_memory(), nuc.cpu().pin_memory())
+            torch.cuda.synchronize()
+    plan = []
+    for b in range(n // 2):
+        side = int(rng.choice([2048, 2304, 2800]))
+        for k in range(2):
+            plan.append((b, side, k, "single" if rng.random() < 0.5 else "grid"))
+    return pool, plan
+
+
+def stream_leg(dev, cfg, pipes, dapi_pipe, args, rates):
+    """The served stream: requests of ``stream_requests`` through one HybServer holding both PSF modes, the DAPI
+    pipeline and the drift of each FOV block, at Poisson rates ``rates`` and saturated; against the plain form
+    (one request at a time: the hyb channels one tile at a time with host syncs, the DAPI channel, the drift at
+    the end of each block) with the same host-to-device copies."""
+    import types
+    import torch
+    from mermake_amd.align import DualAligner
+    from mermake_amd.pipeline import HybPipeline
+    from mermake_amd.serve import FovRequest, HybServer, poisson_arrivals
+    n = args.serve_requests
+    pool, plan = stream_requests(dev, cfg, n)
+
+    def make(arr):
+        return [FovRequest(i, pool[(side, k)][0], pool[(side, k)][1], arrival=float(a), dapi=pool[(side, k)][2],
+                           psf=mode, fov=b) for i, ((b, side, k, mode), a) in enumerate(zip(plan, arr))]
+
+    srv = HybServer(pipes, batch_tiles={"single": 36, "grid": 50}, slots=2, dapi=dapi_pipe)
+    srv.run(make(np.zeros(4))[:4])   # warm the slots and the DAPI buffers
+    vox_per_req = [3 * cfg["sz"] * side * side for _, side, _, _ in plan]
+    out = {"workload": f"{n} requests of 3 x 40 x H x W uint16 (2 hyb channels + DAPI), H = W uniform in "
+                       "{2048, 2304, 2800}, single-PSF (tile 500) or 100-PSF keyed grid (tile 300) 50/50, FOV blocks "
+                       "of 2 acquisitions with drift, host pinned inputs (H2D inside the loop)",
+           "rates": {}}
+    for rate in list(rates) + [0]:
+        arr = poisson_arrivals(n, rate, seed=int(rate * 10) + 3) if rate > 0 else np.zeros(n)
+        reqs = make(arr)
+        srv.run(reqs)
+        st = dict(srv.stats)
+        st["voxels_per_s"] = sum(vox_per_req) / st["wall_s"]
+        st["drift_recovered"] = all(
+            r.drift is not None and (r.fov is None or np.array_equal(
+                np.round(r.drift[0]).astype(int), np.array((1, 7, -11)) * (1 if plan[r.rid][2] == 0 else 0)))
+            for r in reqs)
+        key = "saturated" if rate == 0 else f"poisson_{rate:g}"
+        out["rates"][key] = st
+        log(f"served stream {key}: {st['fovs_per_s']:.2f} FOV/s, p50 {st['p50_latency_s'] * 1e3:.0f} ms, "
+            f"p99 {st['p99_latency_s'] * 1e3:.0f} ms, drift ok {st['drift_recovered']}")
+    del srv
+    # plain form: one request at a time
+    plain = {k: HybPipeline(p.d, plain=True, blur_radius=p.blur_radius, threshold=p.threshold, delta=p.delta,
+                            delta_fit=p.delta_fit, sigmaZ=p.sigmaZ, sigmaXY=p.sigmaXY) for k, p in pipes.items()}
+    m = min(n, 8)
+    torch.cuda.synchronize()
+    a = time.perf_counter()
+    feats = {}
+    for b, side, k, mode in plan[:m]:
+        chans, flat, nuc = pool[(side, k)]
+        fl = flat.to(dev, non_blocking=True)
+        for c in chans:
+            plain[mode](c.to(dev, non_blocking=True), fl)
+        pl, mi = dapi_pipe(nuc.to(dev, non_blocking=True))
+        feats.setdefault(b, []).append(types.SimpleNamespace(Xh_plus=pl.clone(), Xh_minus=mi.clone()))
+        if len(feats[b]) == 2:
+            dual = DualAligner(feats[b][1], th=4)
+            for f in feats[b]:
+                dual.get_best_translation_pointsV2(f)
+    torch.cuda.synchronize()
+    plain_s = (time.perf_counter() - a) / m
+    out["plain_fovs_per_s"] = 1.0 / plain_s
+    out["speedup_vs_plain"] = out["rates"]["saturated"]["fovs_per_s"] * plain_s
+    log(f"served stream plain form: {1.0 / plain_s:.2f} FOV/s; saturated stream {out['speedup_vs_plain']:.2f}x")
+    return out
+
+
 def host_bytes(cfg):
     return cfg["channels"] * cfg["sz"] * cfg["sx"] * cfg["sy"] * 2
 
@@ -256,8 +463,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-tile GPU form")
     ap.add_argument("--no-serve", action="store_true", help="skip the served (request loop) leg")
-    ap.add_argument("--serve-requests", type=int, default=24)
-    ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate of the served leg (FOV/s)")
+    ap.add_argument("--serve-requests", type=int, default=48, help="requests of each served-stream run")
+    ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate of the served stream (FOV/s)")
+    ap.add_argument("--serve-sweep", action="store_true",
+                    help="run the served stream at Poisson rates 0.5, 1, 2 and 4 FOV/s (about 3 minutes)")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="skip the multi-PSF grid, uneven-batch + drift and served-stream legs")
     ap.add_argument("--no-feed", action="store_true", help="skip the host feed leg (zarr decode, pinned H2D)")
     ap.add_argument("--no-dapi", action="store_true", help="skip the DAPI stack leg")
     ap.add_argument("--calib-copy", action="store_true",
@@ -370,6 +581,30 @@ def main():
     if not args.no_dapi:
         dapi = dapi_leg(stacks[0], flat, psf, cfg)
 
+    legs = None
+    if not args.no_legs:
+        from mermake_amd.deconvolver import Deconvolver
+        from mermake_amd.pipeline import DapiPipeline
+        log("building the 100-PSF grid spectra (hyb and DAPI)")
+        gpsfs = grid_psfs(10, 300)
+        hyb_kw = dict(blur_radius=cfg["blur_radius"], threshold=cfg["threshold"], delta=cfg["delta"],
+                      delta_fit=cfg["delta_fit"], sigmaZ=cfg["sigmaZ"], sigmaXY=cfg["sigmaXY"])
+        dapi_kw = dict(blur_radius=50, threshold=3.0, delta=5, delta_fit=5, sigmaZ=1, sigmaXY=1.5)
+        grid = HybPipeline.from_settings(gpsfs, (cfg["sz"], 3000, 3000), tile_size=300, overlap=cfg["overlap"],
+                                         beta=cfg["beta"], psf_mode="keyed", **hyb_kw)
+        dapi_grid = DapiPipeline(Deconvolver(gpsfs, (cfg["sz"], 3000, 3000), tile_size=300, zpad=(cfg["sz"] - 1) // 2,
+                                             overlap=cfg["overlap"], beta=0.01, psf_mode="keyed"), **dapi_kw)
+        dapi_one = DapiPipeline(Deconvolver(psf, shape, tile_size=cfg["tile_size"], zpad=(cfg["sz"] - 1) // 2,
+                                            overlap=cfg["overlap"], beta=0.01), **dapi_kw)
+        legs = {"multi_psf_grid": multi_psf_leg(dev, cfg, grid, dapi_grid)}
+        del dapi_grid
+        torch.cuda.empty_cache()
+        legs["uneven_batch_drift"] = uneven_leg(dev, cfg, pipe, dapi_one)
+        rates = [0.5, 1.0, 2.0, 4.0] if args.serve_sweep else [args.serve_rate]
+        legs["served_stream"] = stream_leg(dev, cfg, {"single": pipe, "grid": grid}, dapi_one, args, rates)
+        del grid, dapi_one
+        torch.cuda.empty_cache()
+
     feed = None
     if not args.no_feed:
         feed = feed_leg(dev, cfg)
@@ -404,6 +639,7 @@ def main():
         "plain_gpu": plain,
         "served": served,
         "dapi": dapi,
+        "legs": legs,
         "feed": feed,
         "cpu_baseline": cpu,
     }
