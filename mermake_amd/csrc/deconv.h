@@ -113,6 +113,8 @@ struct mm_deconv {
   PfShape pfS;
   int rd, rdP;              // the prime rdP (113 or 239) runs by Rader's algorithm (tables rdF / rdI)
   int zfft;                 // z pass by length-80 DFTs (zpad = sz - 1, sz <= 40): operator planes, not w
+  int zdual;                // z-kernel plan without zfft: W holds w (Zp planes) then W itself (Zp planes), so
+                            // that batches with one PSF per tile run the z FFT form
   int blk;                  // spectra column-blocked by blk columns (0: natural layout)
   int Hs;                   // columns of one spectrum row in memory (H rounded up to blk)
   RdTab rdF, rdI;

@@ -173,8 +173,10 @@ __global__ __launch_bounds__(256) void k_tile_r2c(const mm_tile_dev* __restrict_
 }
 
 // ------------------------------------------------------------------ z pass
-// Columns c = kx*H + ky of the (sz, T*H) half spectrum.  Block = G adjacent
-// columns x a group of `tpb` tiles.  W layout: (n_psf, Zp, T*H).
+// Columns c = kx*H + ky of the (sz, T*H) half spectrum.  Work item = G adjacent
+// columns x a group of `tpb` tiles; the blocks are persistent over the items
+// (column groups fastest), so the twiddle tables are set up once per block.
+// W layout: (n_psf, planes, TH) with the Zp planes of W from w_off.
 __global__ __launch_bounds__(256) void k_tile_zwiener(float2* S, long long s_tstride, int TH, int sz, int zpad,
                                                       const int* __restrict__ zmap,
                                                       const float2* __restrict__ W, long long w_stride,
@@ -188,40 +190,44 @@ __global__ __launch_bounds__(256) void k_tile_zwiener(float2* S, long long s_tst
   float2* wl = (float2*)L.extra;
   int* zm = (int*)(wl + (size_t)G * Zp);
   const int ncb = (TH + G - 1) / G;
-  const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
-  const int cb = bid % ncb;
-  const int tg = bid / ncb;
-  const int c0 = cb * G;
-  const int gmax = min(G, TH - c0);
+  const int ngroups = (ntiles + tpb - 1) / tpb;
+  const long long nitems = (long long)ncb * ngroups;
   for (int i = threadIdx.x; i < Zp; i += blockDim.x) zm[i] = zmap[i];
-  int cached = -1;
-  const int t_end = min(ntiles, (tg + 1) * tpb);
-  for (int ti = tg * tpb; ti < t_end; ++ti) {
-    const int psf = tiles[ti].psf;
-    __syncthreads();
-    if (psf != cached) {
-      const float2* Wp = W + (long long)psf * w_stride + c0;
+  for (long long item = xcd_remap(blockIdx.x, gridDim.x); item < nitems; item += gridDim.x) {
+    const int cb = (int)(item % ncb);
+    const int tg = (int)(item / ncb);
+    const int c0 = cb * G;
+    const int gmax = min(G, TH - c0);
+    int cached = -1;
+    const int t_end = min(ntiles, (tg + 1) * tpb);
+    for (int ti = tg * tpb; ti < t_end; ++ti) {
+      const int psf = tiles[ti].psf;
+      __syncthreads();
+      if (psf != cached) {
+        const float2* Wp = W + (long long)psf * w_stride + c0;
+        for (int t = threadIdx.x; t < G * Zp; t += blockDim.x) {
+          const int g = t % G, p = t / G;
+          wl[g * Zp + p] = g < gmax ? Wp[(long long)p * TH + g] : make_float2(0.f, 0.f);
+        }
+        cached = psf;
+      }
+      float2* St = S + (long long)ti * s_tstride + c0;
       for (int t = threadIdx.x; t < G * Zp; t += blockDim.x) {
         const int g = t % G, p = t / G;
-        wl[g * Zp + p] = g < gmax ? Wp[(long long)p * TH + g] : make_float2(0.f, 0.f);
+        const int zs = zm[p];
+        a[g * Zp + p] = (g < gmax && zs >= 0) ? St[(long long)zs * TH + g] : make_float2(0.f, 0.f);
       }
-      cached = psf;
+      __syncthreads();
+      float2* r = fft_lds<false>(a, b, L.tw, P, G, L.coef);
+      for (int t = threadIdx.x; t < G * Zp; t += blockDim.x) r[t] = cmul(r[t], wl[t]);
+      __syncthreads();
+      float2* o = fft_lds<true>(r, r == a ? b : a, L.tw, P, G, L.coef);
+      for (int t = threadIdx.x; t < G * sz; t += blockDim.x) {
+        const int g = t % G, p = t / G;
+        if (g < gmax) St[(long long)p * TH + g] = o[g * Zp + p + zpad];
+      }
     }
-    float2* St = S + (long long)ti * s_tstride + c0;
-    for (int t = threadIdx.x; t < G * Zp; t += blockDim.x) {
-      const int g = t % G, p = t / G;
-      const int zs = zm[p];
-      a[g * Zp + p] = (g < gmax && zs >= 0) ? St[(long long)zs * TH + g] : make_float2(0.f, 0.f);
-    }
-    __syncthreads();
-    float2* r = fft_lds<false>(a, b, L.tw, P, G, L.coef);
-    for (int t = threadIdx.x; t < G * Zp; t += blockDim.x) r[t] = cmul(r[t], wl[t]);
-    __syncthreads();
-    float2* o = fft_lds<true>(r, r == a ? b : a, L.tw, P, G, L.coef);
-    for (int t = threadIdx.x; t < G * sz; t += blockDim.x) {
-      const int g = t % G, p = t / G;
-      if (g < gmax) St[(long long)p * TH + g] = o[g * Zp + p + zpad];
-    }
+    __syncthreads();  // the item's last reads of o before the next item's stores into a
   }
 }
 
@@ -1467,14 +1473,15 @@ static int launch_zconv(mm_deconv* d, float2* S, long long s_ts, const SpecLayou
     mm_set_error("z pass: %zu bytes of LDS for %d planes, Zp %d", lds, sz, Zp);
     return MM_EINVAL;
   }
+  const long long wstride = (long long)(d->zdual ? 2 : 1) * Zp * TH;  // planes of one PSF's operator
 #define MM_ZCX(M, E)                                                                                            \
   do {                                                                                                          \
     if (ch == ZC_CH)                                                                                            \
       hipLaunchKernelGGL((k_tile_zconv<M, E, ZC_CH>), dim3((unsigned)nblk), dim3(threads), lds, st, S, s_ts, TH, \
-                         sz, d->zpad, Zp, src, Wz, (long long)Zp * TH, td, ntiles, tpb, G);            \
+                         sz, d->zpad, Zp, src, Wz, wstride, td, ntiles, tpb, G);                       \
     else                                                                                                        \
       hipLaunchKernelGGL((k_tile_zconv<M, E, 1>), dim3((unsigned)nblk), dim3(threads), lds, st, S, s_ts, TH, sz, \
-                         d->zpad, Zp, src, Wz, (long long)Zp * TH, td, ntiles, tpb, G);                \
+                         d->zpad, Zp, src, Wz, wstride, td, ntiles, tpb, G);                           \
   } while (0)
 #define MM_ZC(M) MM_ZCX(M, false)
   if (sz == 40) MM_ZCX(40, true);
@@ -1598,6 +1605,7 @@ extern "C" int mm_deconv_create(int sz, int tile_size, int overlap, int zpad, mm
     }
   }
   d->zfft = d->zconv && zpad == sz - 1 && sz <= 40;
+  d->zdual = d->zconv && !d->zfft;
   // Rader passes with the z-kernel pass: spectra column-blocked by the x-pass group
   // width, so that every x transform group is one contiguous run
   d->blk = d->rd && d->zconv ? d->pfG : 0;
@@ -1669,6 +1677,17 @@ extern "C" int mm_deconv_build_wiener(mm_deconv* d, const float* psf_centered, d
   const double scale = 1.0 / ((double)Zp * T * T);
   hipLaunchKernelGGL(k_wiener64, dim3(4096), dim3(256), 0, st, Hs, Zp, T, H, beta, scale);
   MM_LAUNCH_CHECK();
+  if (d->zdual) {  // W itself into planes [Zp, 2 Zp), for the z FFT form of one-PSF-per-tile batches
+    const long long wplane = d->blk ? (long long)T * d->Hs : (long long)T * H;
+    float2* Wf = (float2*)W_out + (long long)Zp * wplane;
+    if (d->blk) {
+      const SpecLayout Ls{T, H, d->blk, (long long)T * d->Hs};
+      hipLaunchKernelGGL(k_d2f_blocked, dim3(4096), dim3(256), 0, st, Hs, Wf, Zp, Ls);
+    } else {
+      hipLaunchKernelGGL(k_d2f, dim3(4096), dim3(256), 0, st, Hs, Wf, (long long)Zp * T * H);
+    }
+    MM_LAUNCH_CHECK();
+  }
   if (d->zconv)  // z-domain kernels w = unnormalised inverse z-DFT of W (the 1/Zp is already in W)
     if ((rc = launch_c2c<true>(Hs, (long long)T * H, 1, T * H, 0, 1, d->pZ64, d->gZ64, 1.0, st))) return rc;
   if (d->zfft) {
@@ -1691,7 +1710,7 @@ extern "C" int mm_deconv_spec_layout(const mm_deconv* d, int* blk, int* Hs, int*
   if (!d) return MM_EINVAL;
   if (blk) *blk = d->blk;
   if (Hs) *Hs = d->Hs;
-  if (wplanes) *wplanes = d->zfft ? ZF_OPS : d->Zp;
+  if (wplanes) *wplanes = d->zfft ? ZF_OPS : (d->zdual ? 2 * d->Zp : d->Zp);
   return MM_OK;
 }
 
@@ -1758,16 +1777,23 @@ static int deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles,
   const int tpb = tiles_per_block < 1 ? 1 : (tiles_per_block > ntiles ? ntiles : tiles_per_block);
   if (d->zfft) {  // 3. z pass by length-80 DFTs of each column
     if ((rc = launch_zfft(d, S, s_ts, Ls, (const float2*)W, td, ntiles, tpb, st))) return rc;
-  } else if (d->zconv) {  // 3. z pass as the folded z-kernel matrix
+  } else if (d->zconv && !(d->zdual && tpb == 1)) {  // 3. z pass as the folded z-kernel matrix
     if ((rc = launch_zconv(d, S, s_ts, Ls, (const float2*)W, td, ntiles, tpb, st))) return rc;
   } else {  // 3. z pass with z FFTs and the Wiener multiply
+    // (with one PSF per tile, no tile shares the z-kernel matrix the folded pass builds: the FFT form
+    // reads W, Zp planes per tile, and costs a fraction of the matrix build)
     const int G = d->gZ;
-    const int TH = T * H;
+    const int TH = (int)Ls.plane;
     const int ngroups = (ntiles + tpb - 1) / tpb;
     const long long nblk = (long long)((TH + G - 1) / G) * ngroups;
     const size_t lds = fft_lds_bytes(d->pZ, G, (size_t)8 * Zp * G + sizeof(int) * Zp);
-    hipLaunchKernelGGL(k_tile_zwiener, dim3((unsigned)nblk), dim3(256), lds, st, S, s_ts, TH, sz, d->zpad,
-                       d->zmap_dev, (const float2*)W, (long long)Zp * TH, td, ntiles, tpb, d->pZ, G);
+    const float2* Wf = (const float2*)W + (d->zdual ? (long long)Zp * TH : 0);
+    // persistent blocks: a few rounds of the CUs' concurrent blocks
+    const long long bpc = lds ? (long long)(MM_LDS_MAX / lds) : 8;
+    const long long cap = 256 * (bpc < 1 ? 1 : (bpc > 8 ? 8 : bpc)) * 4;
+    const long long nlaunch = nblk < cap ? nblk : cap;
+    hipLaunchKernelGGL(k_tile_zwiener, dim3((unsigned)nlaunch), dim3(256), lds, st, S, s_ts, TH, sz, d->zpad,
+                       d->zmap_dev, Wf, (long long)(d->zdual ? 2 : 1) * Zp * TH, td, ntiles, tpb, d->pZ, G);
     MM_LAUNCH_CHECK();
   }
   MM_MARK(3);

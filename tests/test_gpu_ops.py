@@ -431,12 +431,13 @@ def test_wiener_closed_form(dev):
     psf[sz // 2, ts // 2, ts // 2] = 1
     d = Deconvolver(psf, (sz, 40, 40), tile_size=ts, zpad=zpad, overlap=ov, beta=beta)
     Zp, Tt = sz + 2 * zpad, ts + 2 * ov
-    W = d.psf_fft[0].cpu().numpy().astype(np.complex128)
-    W = np.fft.fft(W, axis=0) / Zp      # stacks of <= 48 planes keep w = unnormalised ifft_z(W)
+    ops = d.psf_fft[0].cpu().numpy().astype(np.complex128)
+    assert ops.shape[0] == 2 * Zp       # stacks of <= 48 planes: w = unnormalised ifft_z(W), then W itself
     L2 = odec.laplacian_power((Zp, Tt, Tt))[:, :, :Tt // 2 + 1]
     want = 1.0 / (1.0 + beta * L2) / (Zp * Tt * Tt)
-    assert np.abs(W - want).max() / want.max() < 1e-6
-    assert abs(W[0, 0, 0].real * Zp * Tt * Tt - 1 / (1 + beta)) < 1e-6
+    for W in (np.fft.fft(ops[:Zp], axis=0) / Zp, ops[Zp:]):
+        assert np.abs(W - want).max() / want.max() < 1e-6
+        assert abs(W[0, 0, 0].real * Zp * Tt * Tt - 1 / (1 + beta)) < 1e-6
 
 
 def test_fixture_golden_tiles(dev):
