@@ -36,6 +36,22 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+class mark:
+    """A named range on the profiler's marker trace (roctx through torch.cuda.nvtx; free when no profiler
+    listens), so that kernel traces can be split by benchmark phase."""
+
+    def __init__(self, name):
+        self.name = name
+
+    def __enter__(self):
+        import torch
+        torch.cuda.nvtx.range_push(self.name)
+
+    def __exit__(self, *exc):
+        import torch
+        torch.cuda.nvtx.range_pop()
+
+
 def m1_bytes_per_stack(geom, sz, T, Zp, flat=True):
     """Algorithmic-bytes model M1 per channel stack (one HBM round trip per stage)."""
     H = T // 2 + 1
@@ -146,7 +162,8 @@ def serve_leg(pipe, stacks, flat, cfg, args, dev):
                            ("saturated_u8", np.zeros(len(order)), fovs_u8)):
         srv.run([FovRequest(-1 - i, *src[side]) for i, side in enumerate(order[:2])])  # warm the slots
         reqs = [FovRequest(i, *src[side], arrival=float(a)) for i, (side, a) in enumerate(zip(order, arr))]
-        srv.run(reqs)
+        with mark(f"served_hyb_{name}"):
+            srv.run(reqs)
         st = dict(srv.stats)
         vox = sum(cfg["channels"] * cfg["sz"] * s_ * s_ for s_ in order)
         st["voxels_per_s"] = vox / st["wall_s"]
@@ -413,7 +430,8 @@ def stream_leg(dev, cfg, pipes, dapi_pipe, args, rates):
     for rate in list(rates) + [0]:
         arr = poisson_arrivals(n, rate, seed=int(rate * 10) + 3) if rate > 0 else np.zeros(n)
         reqs = make(arr)
-        srv.run(reqs)
+        with mark(f"stream_{'saturated' if rate == 0 else f'poisson_{rate:g}'}"):
+            srv.run(reqs)
         st = dict(srv.stats)
         st["voxels_per_s"] = sum(vox_per_req) / st["wall_s"]
         st["drift_recovered"] = all(
@@ -512,14 +530,15 @@ def main():
     torch.cuda.synchronize()
     log("timing", args.steps, "FOVs")
     lat = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        a = time.perf_counter()
-        out = fov()
+    with mark("fov_steps"):
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            a = time.perf_counter()
+            out = fov()
+            torch.cuda.synchronize()
+            lat.append(time.perf_counter() - a)
         torch.cuda.synchronize()
-        lat.append(time.perf_counter() - a)
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
+        el = time.perf_counter() - t0
     nspots = [int(o.shape[0]) for o in out]
     fov_vox = cfg["channels"] * cfg["sz"] * cfg["sx"] * cfg["sy"]
     value = fov_vox * args.steps / el

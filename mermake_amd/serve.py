@@ -21,7 +21,8 @@ belongs to.  Requests arrive over time.  The loop
 * runs each request's DAPI channel through a ``DapiPipeline`` (``__main__.py:213-229``) on a stream
   of its own as soon as its copy has landed, and the drift (``align.drift``, ``__main__.py:272-282``)
   of a FOV block once the block is closed (a later request of another FOV, or ``last``) and its
-  members are done.  The block's ``DualAligner`` keeps the reference raster spectra resident for
+  members are done.  Both run on a host thread of their own, so that their host synchronisations
+  do not hold up the batch loop.  The block's ``DualAligner`` keeps the reference raster spectra resident for
   all of its members;
 * reads results back asynchronously into pinned memory (counts first, then exactly the kept rows)
   and completes a request when its last tile and its DAPI features are back, recording the latency
@@ -33,6 +34,7 @@ Each channel's spots equal ``HybPipeline(stack, flat)`` of the request's PSF mod
 computed by the same kernels whatever batch it lands in, and rows are assembled in tile order.
 """
 import collections
+import concurrent.futures
 import ctypes
 import time
 import types
@@ -86,6 +88,7 @@ class FovRequest:
         self._dapi = None     # (device DAPI stack, device flat)
         self._evs = None      # copy-done event of each channel
         self._dapi_ev = None
+        self._dapi_fut = None
         self._dapi_done = True
         self._next = 0        # channels whose tiles are queued
         self._rows = None
@@ -350,15 +353,20 @@ class HybServer:
         for ti in range(len(recs)):
             fifo.append(_Job(req, ch, ti, recs[ti], req._mode))
 
-    def _run_dapi(self, req):
-        """The DAPI block of ``__main__.py:217-225`` for one request, on the DAPI stream."""
-        stack, flat = req._dapi
+    def _dapi_work(self, stack, flat, ev):
+        """The DAPI block of ``__main__.py:217-225`` for one request, on the DAPI stream (runs on the
+        server's DAPI thread: its host synchronisations do not hold up the batch loop)."""
         with torch.cuda.stream(self.dapi_stream):
-            self.dapi_stream.wait_event(req._dapi_ev)
+            self.dapi_stream.wait_event(ev)
             plus, minus = self.dapi(stack, flat)
-            req.Xh_plus = plus.cpu().numpy()
-            req.Xh_minus = minus.cpu().numpy()
+            return plus.cpu().numpy(), minus.cpu().numpy()
+
+    def _dapi_collect(self, req):
+        """The DAPI features once the DAPI thread has them (its exception fails the request)."""
+        plus, minus = req._dapi_fut.result()
+        req.Xh_plus, req.Xh_minus = plus, minus
         req._dapi = None
+        req._dapi_fut = None
         req._dapi_done = True
 
     # ------------------------------------------------------------------ completion
@@ -376,6 +384,7 @@ class HybServer:
         req._rows = None
         req._left = 0
         req._dapi_done = True
+        req._dapi_fut = None
         req.t_done = now
         self._release(req)
         done.append(req)
@@ -455,6 +464,13 @@ class HybServer:
         """Serve ``requests`` (arrival times relative to now); returns them completed, with ``spots``,
         ``Xh_plus`` / ``Xh_minus``, ``drift``, ``error``, ``t_admit`` and ``t_done`` set, and fills
         ``self.stats``."""
+        aux = concurrent.futures.ThreadPoolExecutor(max_workers=1, thread_name_prefix="mm-dapi")
+        try:
+            return self._run(requests, aux)
+        finally:
+            aux.shutdown(wait=True)
+
+    def _run(self, requests, aux):
         reqs = sorted(requests, key=lambda r: r.arrival)
         t0 = time.perf_counter()
         nxt = 0
@@ -471,7 +487,7 @@ class HybServer:
 
         def close_blocks(keep_open=None):
             for b in blocks:
-                if b[0] != keep_open:
+                if b[0] != keep_open and b[2] is False:
                     b[2] = True
 
         while True:
@@ -513,8 +529,11 @@ class HybServer:
                         self._enqueue_channel(r, r._next, fifos[r._mode])
                         r._next += 1
                         progressed = True
-                    if not r._dapi_done and r._dapi_ev.query():
-                        self._run_dapi(r)
+                    if not r._dapi_done and r._dapi_fut is None and r._dapi_ev.query():
+                        r._dapi_fut = aux.submit(self._dapi_work, r._dapi[0], r._dapi[1], r._dapi_ev)
+                        progressed = True
+                    if r._dapi_fut is not None and r._dapi_fut.done():
+                        self._dapi_collect(r)
                         progressed = True
                 except Exception as e:  # noqa: BLE001 - a request fails alone
                     self._fail(r, e, time.perf_counter() - t0, done)
@@ -567,8 +586,11 @@ class HybServer:
             if nxt == len(reqs) and not waiting:
                 close_blocks()
             for b in blocks:
-                if b[2] and all(r.t_done is not None for r in b[1]):
-                    self._run_drift(b[1])
+                if b[2] is True and all(r.t_done is not None for r in b[1]):
+                    b[2] = aux.submit(self._run_drift, b[1])   # closed and done: drift on the DAPI thread
+                    progressed = True
+                elif b[2] not in (True, False) and b[2].done():
+                    b[2].result()
                     b[2] = None          # drifted
                     progressed = True
             blocks = [b for b in blocks if b[2] is not None]
