@@ -149,6 +149,13 @@ int mm_deconv_stitch(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, co
 size_t mm_rms_ws_bytes(void);
 int mm_rms_normalize(float* a, long long n, void* ws, float* rms_out, void* stream);
 
+/* *out (device float) = the median of the n float32 values at a (device): the
+ * middle value, or for an even n the float32 mean of the two middle values,
+ * NaN when any value is NaN (the cp.median of load_flats, mermake/io.py:52).
+ * Exact, by radix selection.  ws: mm_median_ws_bytes() bytes. */
+size_t mm_median_ws_bytes(void);
+int mm_median(const float* a, long long n, float* out, void* ws, size_t ws_bytes, void* stream);
+
 /* In-place 3D C2C FFT of a contiguous complex64 (n0,n1,n2) volume over the axes
  * in axes_mask (bit i = axis i); inverse scales by 1/n per axis.  Replaces the
  * cupy.fft.fftn/ifftn calls of utils.fftconvolve (mermake/utils.py:10-36). */

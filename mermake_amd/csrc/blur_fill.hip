@@ -450,3 +450,115 @@ extern "C" int mm_rms_normalize(float* a, long long n, void* ws, float* rms_out,
   MM_LAUNCH_CHECK();
   return MM_OK;
 }
+
+// ------------------------------------------------------------------ median (load_flats)
+// Exact median of a float32 array by radix selection on the order-preserving
+// 32-bit keys of the values (sign-flipped for negatives, so -0 < +0): four
+// passes of 8-bit digit histograms, most significant digit first, each over the
+// elements whose higher digits equal the prefix found so far; after each pass one
+// thread walks the histogram to the digit holding the k-th element.  Any NaN
+// makes the result NaN (numpy.median).  Workspace: 256 counters, the state
+// (prefix, k) and a NaN count.
+struct KthState {
+  unsigned prefix, mask;
+  unsigned long long k;
+  unsigned nan;
+  unsigned pad;
+};
+
+__device__ __forceinline__ unsigned f32_key(float f) {
+  const unsigned u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float key_f32(unsigned k) {
+  return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+__global__ void k_kth_init(unsigned* hist, KthState* st, unsigned long long k) {
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) hist[i] = 0u;
+  if (threadIdx.x == 0) *st = KthState{0u, 0u, k, 0u, 0u};
+}
+
+__global__ __launch_bounds__(256) void k_kth_hist(const float* __restrict__ a, long long n, int shift,
+                                                  unsigned* __restrict__ hist, KthState* __restrict__ st) {
+  __shared__ unsigned h[256];
+  __shared__ unsigned nn;
+  h[threadIdx.x] = 0u;
+  if (threadIdx.x == 0) nn = 0u;
+  __syncthreads();
+  const unsigned prefix = st->prefix, mask = st->mask;
+  unsigned nan = 0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const float v = a[i];
+    if (v != v) {
+      ++nan;
+      continue;
+    }
+    const unsigned key = f32_key(v);
+    if ((key & mask) == prefix) atomicAdd(&h[(key >> shift) & 255u], 1u);
+  }
+  if (shift == 24 && nan) atomicAdd(&nn, nan);
+  __syncthreads();
+  if (h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+  if (shift == 24 && threadIdx.x == 0 && nn) atomicAdd(&st->nan, nn);
+}
+
+// Digit of the k-th key; the histogram is cleared for the next pass.  On the last
+// pass the selected value goes to out.
+__global__ void k_kth_select(unsigned* hist, KthState* st, int shift, float* out) {
+  if (threadIdx.x != 0) return;
+  unsigned long long k = st->k, run = 0;
+  unsigned d = 255;
+  for (unsigned i = 0; i < 256; ++i) {
+    const unsigned long long c = hist[i];
+    if (k < run + c) {
+      d = i;
+      break;
+    }
+    run += c;
+  }
+  st->k = k - run;
+  st->prefix |= d << shift;
+  st->mask |= 255u << shift;
+  for (int i = 0; i < 256; ++i) hist[i] = 0u;
+  if (shift == 0 && out) *out = st->nan ? __uint_as_float(0x7fc00000u) : key_f32(st->prefix);
+}
+
+__global__ void k_median_pair(const float* lo, const float* hi, float* out) {
+  if (threadIdx.x == 0) *out = (*lo + *hi) / 2.0f;
+}
+
+extern "C" size_t mm_median_ws_bytes(void) { return 256 * sizeof(unsigned) + 2 * sizeof(KthState) + 2 * sizeof(float) + 64; }
+
+static int kth_smallest(const float* a, long long n, unsigned long long k, unsigned* hist, KthState* st, float* out,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_kth_init, dim3(1), dim3(256), 0, s, hist, st, k);
+  MM_LAUNCH_CHECK();
+  long long blocks = (n + 255) / 256;
+  if (blocks > 2048) blocks = 2048;
+  for (int shift = 24; shift >= 0; shift -= 8) {
+    hipLaunchKernelGGL(k_kth_hist, dim3((unsigned)blocks), dim3(256), 0, s, a, n, shift, hist, st);
+    MM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_kth_select, dim3(1), dim3(64), 0, s, hist, st, shift, out);
+    MM_LAUNCH_CHECK();
+  }
+  return MM_OK;
+}
+
+extern "C" int mm_median(const float* a, long long n, float* out, void* ws, size_t ws_bytes, void* stream) {
+  if (!a || !out || !ws || n < 1 || ws_bytes < mm_median_ws_bytes()) {
+    mm_set_error("mm_median: bad argument");
+    return MM_EINVAL;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  unsigned* hist = (unsigned*)ws;
+  KthState* st = (KthState*)(hist + 256);
+  float* tmp = (float*)(st + 2);
+  int rc;
+  if (n % 2) return kth_smallest(a, n, (unsigned long long)(n / 2), hist, st, out, s);
+  if ((rc = kth_smallest(a, n, (unsigned long long)(n / 2 - 1), hist, st, tmp, s))) return rc;
+  if ((rc = kth_smallest(a, n, (unsigned long long)(n / 2), hist, st, tmp + 1, s))) return rc;
+  hipLaunchKernelGGL(k_median_pair, dim3(1), dim3(64), 0, s, tmp, tmp + 1, out);
+  MM_LAUNCH_CHECK();
+  return MM_OK;
+}

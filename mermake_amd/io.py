@@ -34,6 +34,7 @@ import numpy as np
 import torch
 
 from . import __version__
+from . import _lib as L
 from . import blur as _blur
 from ._dev import as_device
 
@@ -164,15 +165,13 @@ def center_crop(A, shape):
 
 
 def _median(v):
-    """``cupy.median`` of a float32 device tensor: the middle value, or for an even count the
-    float32 mean of the two middle values."""
-    v = v.reshape(-1)
-    n = v.numel()
-    if n % 2:
-        return torch.kthvalue(v, n // 2 + 1).values
-    lo = torch.kthvalue(v, n // 2).values
-    hi = torch.kthvalue(v, n // 2 + 1).values
-    return (lo + hi) / 2
+    """``cupy.median`` of a float32 device tensor (``mm_median``): the middle value, or for an even
+    count the float32 mean of the two middle values; a 0-d device tensor."""
+    v = v.reshape(-1).contiguous()
+    out = torch.empty((), dtype=torch.float32, device=v.device)
+    ws = torch.empty(L.lib().mm_median_ws_bytes(), dtype=torch.uint8, device=v.device)
+    L.call("mm_median", L.ptr(v), v.numel(), L.ptr(out), L.ptr(ws), ws.numel(), L.stream_ptr(v.device))
+    return out
 
 
 def load_flats(flat_field_tag, shape=None, device=None, **kwargs):

@@ -81,3 +81,22 @@ def test_fov_feeder_reports_unreadable_fov(dev, tmp_path):
             next(feed)
     finally:
         feed.close()
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 1000, 1001, 4096 * 1024 + 3])
+def test_median_matches_numpy(dev, n):
+    """``mm_median`` (radix selection) equals numpy.median in float32: odd and even counts, duplicates,
+    negative values and zeros of both signs."""
+    from mermake_amd.io import _median
+    rng = np.random.default_rng(n)
+    a = (rng.standard_normal(n) * 100).astype(np.float32)
+    a[: n // 3] = np.round(a[: n // 3])            # many duplicates
+    if n > 4:
+        a[:2] = [0.0, -0.0]
+    got = float(_median(torch.from_numpy(a).to(dev)).item())
+    want = np.median(a)
+    assert got == np.float32(want), (got, want)
+    if n > 2:
+        b = a.copy()
+        b[n // 2] = np.nan
+        assert np.isnan(float(_median(torch.from_numpy(b).to(dev)).item()))
