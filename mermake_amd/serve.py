@@ -356,7 +356,7 @@ class HybServer:
     def _dapi_work(self, stack, flat, ev):
         """The DAPI block of ``__main__.py:217-225`` for one request, on the DAPI stream (runs on the
         server's DAPI thread: its host synchronisations do not hold up the batch loop)."""
-        with torch.cuda.stream(self.dapi_stream):
+        with torch.cuda.device(self.device), torch.cuda.stream(self.dapi_stream):
             self.dapi_stream.wait_event(ev)
             plus, minus = self.dapi(stack, flat)
             return plus.cpu().numpy(), minus.cpu().numpy()
@@ -444,7 +444,7 @@ class HybServer:
             return
         ref = members[len(members) // 2]
         t0 = time.perf_counter()
-        with torch.cuda.stream(self.dapi_stream):
+        with torch.cuda.device(self.device), torch.cuda.stream(self.dapi_stream):
             def feats(r):
                 return types.SimpleNamespace(Xh_plus=torch.from_numpy(r.Xh_plus).to(self.device),
                                              Xh_minus=torch.from_numpy(r.Xh_minus).to(self.device))
