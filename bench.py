@@ -489,6 +489,9 @@ def main():
                     help="skip the multi-PSF grid, uneven-batch + drift and served-stream legs")
     ap.add_argument("--no-feed", action="store_true", help="skip the host feed leg (zarr decode, pinned H2D)")
     ap.add_argument("--no-dapi", action="store_true", help="skip the DAPI stack leg")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default schedule instead of timing every admitted one first")
+    ap.add_argument("--no-graph", action="store_true", help="skip the hipGraph replay leg")
     ap.add_argument("--calib-copy", action="store_true",
                     help="end with torch.neg over 1 GiB of float32 (reads 1 GiB, writes 1 GiB): the reference "
                          "kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -515,10 +518,19 @@ def main():
     psf = gaussian_psf(cfg["psf_shape"], *cfg["psf_sigma"])
     torch.cuda.synchronize()
     log("building Wiener spectrum")
+    hyb_kw = dict(blur_radius=cfg["blur_radius"], threshold=cfg["threshold"], delta=cfg["delta"],
+                  delta_fit=cfg["delta_fit"], sigmaZ=cfg["sigmaZ"], sigmaXY=cfg["sigmaXY"])
+    # the autotuner times every schedule the tile admits on this FOV's three channels (not cached:
+    # the table is part of the record) and keeps the fastest
+    tune = None if args.no_autotune else {"stacks": cfg["channels"], "reps": 2, "cache": False}
+    if tune:
+        log("autotuning the deconvolution schedule")
     pipe = HybPipeline.from_settings(psf, shape, tile_size=cfg["tile_size"], overlap=cfg["overlap"], beta=cfg["beta"],
-                                     blur_radius=cfg["blur_radius"], threshold=cfg["threshold"], delta=cfg["delta"],
-                                     delta_fit=cfg["delta_fit"], sigmaZ=cfg["sigmaZ"], sigmaXY=cfg["sigmaXY"])
+                                     autotune=tune, **hyb_kw)
     d = pipe.d
+    if d.tuning:
+        log("schedule", d.tuning["xy"], d.tuning["z"], "tiles per launch", d.tuning["batch_tiles"] or "all",
+            {k: round(v, 1) for k, v in d.tuning["ms_forms"].items()})
     T, Zp = d._plan.T, d._plan.Zp
     geom = tile_geometry(shape, cfg["tile_size"], cfg["overlap"], 1)
 
@@ -565,10 +577,26 @@ def main():
     m1 = m1_bytes_per_stack(geom, cfg["sz"], T, Zp)
     m1_frac = (m1 * cfg["channels"] / (ms * 1e-3)) / HBM_PEAK
 
+    graph = None
+    if not args.no_graph:
+        # the same batched FOV step with each batch's kernel chain replayed as a hipGraph
+        gp = HybPipeline(d, graph=True, **hyb_kw)
+        gout = gp.run_many(stacks, [flat] * len(stacks))        # eager first batch, then the capture
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        with mark("fov_steps_graph"):
+            for _ in range(args.steps):
+                gout = gp.run_many(stacks, [flat] * len(stacks))
+            torch.cuda.synchronize()
+        gms = (time.perf_counter() - a) / args.steps * 1e3
+        same = all(torch.equal(x, y) for x, y in zip(gout, out))
+        graph = {"ms_per_fov": gms, "eager_ms_per_fov": ms, "graphs": len(gp._graphs), "equal_to_eager": same}
+        del gp, gout
+        torch.cuda.empty_cache()
+
     plain = None
     if not args.no_plain:
-        pp = HybPipeline(d, plain=True, blur_radius=cfg["blur_radius"], threshold=cfg["threshold"],
-                         delta=cfg["delta"], delta_fit=cfg["delta_fit"], sigmaZ=cfg["sigmaZ"], sigmaXY=cfg["sigmaXY"])
+        pp = HybPipeline(d, plain=True, **hyb_kw)
         [pp(s, flat) for s in stacks]
         torch.cuda.synchronize()
         a = time.perf_counter()
@@ -606,8 +634,6 @@ def main():
         from mermake_amd.pipeline import DapiPipeline
         log("building the 100-PSF grid spectra (hyb and DAPI)")
         gpsfs = grid_psfs(10, 300)
-        hyb_kw = dict(blur_radius=cfg["blur_radius"], threshold=cfg["threshold"], delta=cfg["delta"],
-                      delta_fit=cfg["delta_fit"], sigmaZ=cfg["sigmaZ"], sigmaXY=cfg["sigmaXY"])
         dapi_kw = dict(blur_radius=50, threshold=3.0, delta=5, delta_fit=5, sigmaZ=1, sigmaXY=1.5)
         grid = HybPipeline.from_settings(gpsfs, (cfg["sz"], 3000, 3000), tile_size=300, overlap=cfg["overlap"],
                                          beta=cfg["beta"], psf_mode="keyed", **hyb_kw)
@@ -654,6 +680,9 @@ def main():
                      "kernel_ms": pt[dom], "kernel_algorithmic_bytes": pb[dom]},
         "m1_roofline_frac": m1_frac,
         "pass_ms": pt, "stack_ms": stack_ms,
+        "schedule": {k: d.tuning[k] for k in ("xy", "z", "batch_tiles", "ms_forms", "ms_batch_tiles", "tune_s")}
+        if d.tuning else d.schedule(),
+        "graph": graph,
         "latency_ms": {"p50": 1e3 * lat_sorted[len(lat) // 2], "p99": 1e3 * lat_sorted[-1]},
         "plain_gpu": plain,
         "served": served,

@@ -96,6 +96,30 @@ typedef struct mm_tile_desc {
 int mm_deconv_create(int sz, int tile_size, int overlap, int zpad, mm_deconv** out);
 int mm_deconv_destroy(mm_deconv* d);
 int mm_deconv_dims(const mm_deconv* d, int* T, int* Zp, int* H);
+/* Schedule of mm_deconv_tiles: the form of the T-length passes along y and x
+ * and of the z pass.  No counterpart in the reference, whose FFTs are cuFFT's
+ * plans (deconvolver.py:130-158); the project's autotuner times the forms a
+ * geometry admits and selects one.
+ *   MM_XY_STOCKHAM     LDS-staged Stockham radix passes (any T)
+ *   MM_XY_PRIME_FIRST  T = p * M: a matrix-product stage of the prime p, then radix M
+ *   MM_XY_RADER        T = 113 * M or 239 * 2: the prime by Rader's algorithm in
+ *                      Good-Thomas form, spectra column-blocked
+ *   MM_Z_FFT           z FFTs of length Zp with the Wiener multiply
+ *   MM_Z_MATRIX        the folded z-kernel matrix (sz <= 48)
+ *   MM_Z_DFT80         length-80 DFTs (zpad = sz - 1, sz <= 40)
+ * mm_deconv_schedule reports the forms in use and the masks (1 << form) of
+ * those the geometry admits; a plan starts with the fastest admitted form of
+ * each.  mm_deconv_set_schedule selects forms (MM_EINVAL if not admitted);
+ * the spectrum layout, mm_deconv_spec_bytes and the operator planes follow
+ * it, so W is built again with mm_deconv_build_wiener after a change. */
+#define MM_XY_STOCKHAM 0
+#define MM_XY_PRIME_FIRST 1
+#define MM_XY_RADER 2
+#define MM_Z_FFT 0
+#define MM_Z_MATRIX 1
+#define MM_Z_DFT80 2
+int mm_deconv_schedule(const mm_deconv* d, int* xy, int* z, int* xy_mask, int* z_mask);
+int mm_deconv_set_schedule(mm_deconv* d, int xy, int z);
 /* In-plane layout of the half spectra (T rows x, H columns ky) in W and in the
  * spectrum workspace: blk = 0 is natural (x * H + ky); blk > 0 is
  * column-blocked, (ky / blk) * T * blk + x * blk + ky % blk with Hs = H rounded

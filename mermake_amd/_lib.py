@@ -37,6 +37,9 @@ _SIGS = {
     "mm_deconv_destroy": (_i, [_vp]),
     "mm_deconv_dims": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "mm_deconv_spec_layout": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "mm_deconv_schedule": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i),
+                                ctypes.POINTER(_i)]),
+    "mm_deconv_set_schedule": (_i, [_vp, _i, _i]),
     "mm_deconv_wiener_ws_bytes": (_sz, [_vp]),
     "mm_deconv_build_wiener": (_i, [_vp, _vp, _d, _vp, _vp, _sz, _vp]),
     "mm_deconv_spec_bytes": (_sz, [_vp]),

@@ -115,6 +115,9 @@ struct mm_deconv {
   int zfft;                 // z pass by length-80 DFTs (zpad = sz - 1, sz <= 40): operator planes, not w
   int zdual;                // z-kernel plan without zfft: W holds w (Zp planes) then W itself (Zp planes), so
                             // that batches with one PSF per tile run the z FFT form
+  int xy_ok, z_ok;          // masks (1 << form) of the MM_XY_* / MM_Z_* forms the geometry admits
+  int xy_form, z_form;      // the schedule in use (mm_deconv_set_schedule)
+  int pfM0, pfG0, rdG;      // prime-first radix and group width; Rader group width
   int blk;                  // spectra column-blocked by blk columns (0: natural layout)
   int Hs;                   // columns of one spectrum row in memory (H rounded up to blk)
   RdTab rdF, rdI;

@@ -1515,6 +1515,20 @@ static void host_zmap(int sz, int zpad, std::vector<int>& m) {
   }
 }
 
+// Pass forms of the schedule (xy_form, z_form) -> the flags the launchers read.
+// The Rader passes keep the spectra column-blocked by their x-pass group width,
+// so that every x transform group is one contiguous run.
+static void apply_schedule(mm_deconv* d) {
+  d->rd = d->xy_form == MM_XY_RADER;
+  d->pfM = d->xy_form == MM_XY_STOCKHAM ? 0 : d->pfM0;
+  d->pfG = d->rd ? d->rdG : d->pfG0;
+  d->zconv = d->z_form != MM_Z_FFT;
+  d->zfft = d->z_form == MM_Z_DFT80;
+  d->zdual = d->z_form == MM_Z_MATRIX;
+  d->blk = d->rd ? d->pfG : 0;
+  d->Hs = d->blk ? (d->H + d->blk - 1) / d->blk * d->blk : d->H;
+}
+
 extern "C" int mm_deconv_create(int sz, int tile_size, int overlap, int zpad, mm_deconv** out) {
   if (!out || sz < 1 || tile_size < 1 || overlap < 0 || zpad < 0) {
     mm_set_error("mm_deconv_create: need sz, tile_size >= 1 and overlap, zpad >= 0 (got %d %d %d %d)", sz,
@@ -1564,6 +1578,8 @@ extern "C" int mm_deconv_create(int sz, int tile_size, int overlap, int zpad, mm
       if (G > 0) { d->pfM = M; d->pfG = G; d->pfS = S; }
     }
   }
+  d->pfM0 = d->pfM;
+  d->pfG0 = d->pfG;
   // Rader form of the prime 113 (the full-FOV tile T = 678 = 113 * 6)
   d->rd = 0;
   d->rdP = 0;
@@ -1576,7 +1592,7 @@ extern "C" int mm_deconv_create(int sz, int tile_size, int overlap, int zpad, mm
     if (G > 0 && (rc = mm_get_rader(p, &d->rdF, &d->rdI)) == MM_OK) {
       d->rd = 1;
       d->rdP = p;
-      d->pfG = G;
+      d->rdG = G;
     } else if (rc) {
       delete d;
       return rc;
@@ -1605,12 +1621,35 @@ extern "C" int mm_deconv_create(int sz, int tile_size, int overlap, int zpad, mm
     }
   }
   d->zfft = d->zconv && zpad == sz - 1 && sz <= 40;
-  d->zdual = d->zconv && !d->zfft;
-  // Rader passes with the z-kernel pass: spectra column-blocked by the x-pass group
-  // width, so that every x transform group is one contiguous run
-  d->blk = d->rd && d->zconv ? d->pfG : 0;
-  d->Hs = d->blk ? (d->H + d->blk - 1) / d->blk * d->blk : d->H;
+  d->xy_ok = (1 << MM_XY_STOCKHAM) | (d->pfM0 ? 1 << MM_XY_PRIME_FIRST : 0) | (d->rd ? 1 << MM_XY_RADER : 0);
+  d->z_ok = (1 << MM_Z_FFT) | (d->zconv ? 1 << MM_Z_MATRIX : 0) | (d->zfft ? 1 << MM_Z_DFT80 : 0);
+  // default schedule: the fastest form of each pass the geometry admits
+  d->xy_form = d->rd ? MM_XY_RADER : (d->pfM0 ? MM_XY_PRIME_FIRST : MM_XY_STOCKHAM);
+  d->z_form = d->zfft ? MM_Z_DFT80 : (d->zconv ? MM_Z_MATRIX : MM_Z_FFT);
+  apply_schedule(d);
   *out = d;
+  return MM_OK;
+}
+
+extern "C" int mm_deconv_schedule(const mm_deconv* d, int* xy, int* z, int* xy_mask, int* z_mask) {
+  if (!d) { mm_set_error("mm_deconv_schedule: null plan"); return MM_EINVAL; }
+  if (xy) *xy = d->xy_form;
+  if (z) *z = d->z_form;
+  if (xy_mask) *xy_mask = d->xy_ok;
+  if (z_mask) *z_mask = d->z_ok;
+  return MM_OK;
+}
+
+extern "C" int mm_deconv_set_schedule(mm_deconv* d, int xy, int z) {
+  if (!d) { mm_set_error("mm_deconv_set_schedule: null plan"); return MM_EINVAL; }
+  if (xy < 0 || xy > 2 || !((d->xy_ok >> xy) & 1) || z < 0 || z > 2 || !((d->z_ok >> z) & 1)) {
+    mm_set_error("mm_deconv_set_schedule: forms (%d, %d) not admitted by the tile %dx%dx%d (masks %d, %d)", xy, z,
+                 d->Zp, d->T, d->T, d->xy_ok, d->z_ok);
+    return MM_EINVAL;
+  }
+  d->xy_form = xy;
+  d->z_form = z;
+  apply_schedule(d);
   return MM_OK;
 }
 

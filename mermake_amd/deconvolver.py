@@ -55,6 +55,11 @@ def batch_laplacian_fft(batch_size, shape):
     return fftn_c2c(laplacian_3d(shape))[None]
 
 
+# forms of the T-length passes and of the z pass (mm_deconv_set_schedule, in the order of MM_XY_* / MM_Z_*)
+XY_FORMS = ("stockham", "prime_first", "rader")
+Z_FORMS = ("fft", "matrix", "dft80")
+
+
 class _Plan:
     def __init__(self, sz, tile_size, overlap, zpad):
         h = ctypes.c_void_p()
@@ -63,10 +68,27 @@ class _Plan:
         T, Zp, H = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         L.call("mm_deconv_dims", h, ctypes.byref(T), ctypes.byref(Zp), ctypes.byref(H))
         self.T, self.Zp, self.H = T.value, Zp.value, H.value
+        self.refresh()
+
+    def refresh(self):
+        """Read the spectrum layout of the current schedule."""
+        h = self.h
         blk, Hs, wp = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         L.call("mm_deconv_spec_layout", h, ctypes.byref(blk), ctypes.byref(Hs), ctypes.byref(wp))
         self.blk, self.Hs, self.wplanes = blk.value, Hs.value, wp.value
         self.spec_bytes = L.lib().mm_deconv_spec_bytes(h)
+
+    def schedule(self):
+        """``(xy form, z form, admitted xy forms, admitted z forms)`` as names (``XY_FORMS``, ``Z_FORMS``)."""
+        v = [ctypes.c_int() for _ in range(4)]
+        L.call("mm_deconv_schedule", self.h, *[ctypes.byref(x) for x in v])
+        xy, z, xm, zm = (x.value for x in v)
+        return (XY_FORMS[xy], Z_FORMS[z], [f for i, f in enumerate(XY_FORMS) if xm >> i & 1],
+                [f for i, f in enumerate(Z_FORMS) if zm >> i & 1])
+
+    def set_schedule(self, xy, z):
+        L.call("mm_deconv_set_schedule", self.h, XY_FORMS.index(xy), Z_FORMS.index(z))
+        self.refresh()
 
     def spectrum_shape(self, n):
         """Shape of ``n`` Wiener operators in the plan's layout: (n, planes, T, H) natural, or
@@ -206,7 +228,7 @@ class Deconvolver:
     batch_bytes = int(os.environ.get("MERMAKE_BATCH_BYTES", str(24 << 30)))
 
     def __init__(self, psfs, channel_shape, tile_size=300, zpad=0, overlap=89, beta=0.001, xp=None, device=None,
-                 psf_mode="zip", **kwargs):
+                 psf_mode="zip", autotune=False, **kwargs):
         self.tile_size = tile_size
         self.psf_mode = psf_mode  # "zip" (the reference's pairing) or "keyed" (nearest PSF key per tile)
         self.tile_height = channel_shape[0]
@@ -214,27 +236,61 @@ class Deconvolver:
         self.zpad = zpad
         self.beta = beta
         self.device = torch.device(device) if device is not None else default_device()
-        if isinstance(psfs, dict):
-            self.psf_keys = list(psfs.keys())
-            centred = [self.center_psf(np.asarray(p)) for p in psfs.values()]
-        else:
-            self.psf_keys = None
-            centred = [self.center_psf(np.asarray(psfs))]
+        self.psf_keys = list(psfs.keys()) if isinstance(psfs, dict) else None
         if psf_mode not in ("zip", "keyed"):
             raise ValueError(f"psf_mode must be 'zip' or 'keyed', not {psf_mode!r}")
+        self._psfs = psfs
+        self.batch_tiles = None   # tiles per launch chosen by the autotuner (None: as many as memory allows)
+        self.tuning = None
         self._plan = _Plan(self.tile_height, tile_size, overlap, zpad)
-        self.psf_fft = torch.empty(self._plan.spectrum_shape(len(centred)), dtype=torch.complex64,
-                                   device=self.device)
+        self.psf_fft = None
+        self.generation = 0       # counts the builds of psf_fft (captured kernel chains hold its address)
+        self._ws = {}
+        self._build_spectra()
+        if autotune:
+            from .autotune import tune
+            tune(self, channel_shape, **(autotune if isinstance(autotune, dict) else {}))
+
+    def _build_spectra(self):
+        """The Wiener operators of every PSF in the layout of the plan's schedule."""
+        psfs = list(self._psfs.values()) if isinstance(self._psfs, dict) else [self._psfs]
+        self.psf_fft = None   # the operators of the previous schedule go before the new ones come
+        self.psf_fft = torch.empty(self._plan.spectrum_shape(len(psfs)), dtype=torch.complex64, device=self.device)
         ws_bytes = L.lib().mm_deconv_wiener_ws_bytes(self._plan.h)
         ws = torch.empty(ws_bytes, dtype=torch.uint8, device=self.device)
         st = L.stream_ptr(self.device)
-        for i, p in enumerate(centred):
-            pd = torch.from_numpy(np.ascontiguousarray(p, dtype=np.float32)).to(self.device)
-            L.call("mm_deconv_build_wiener", self._plan.h, L.ptr(pd), float(beta), L.ptr(self.psf_fft[i]),
+        for i, p in enumerate(psfs):
+            c = self.center_psf(np.asarray(p))
+            pd = torch.from_numpy(np.ascontiguousarray(c, dtype=np.float32)).to(self.device)
+            L.call("mm_deconv_build_wiener", self._plan.h, L.ptr(pd), float(self.beta), L.ptr(self.psf_fft[i]),
                    L.ptr(ws), ws_bytes, st)
         torch.cuda.current_stream(self.device).synchronize()
         del ws
-        self._ws = {}
+        self.generation += 1
+
+    def schedule(self):
+        """The deconvolution schedule: ``{"xy": form, "z": form, "batch_tiles": n or None}`` and the
+        forms the tile geometry admits (``"xy_forms"``, ``"z_forms"``); see ``set_schedule``."""
+        xy, z, xs, zs = self._plan.schedule()
+        return {"xy": xy, "z": z, "batch_tiles": self.batch_tiles, "xy_forms": xs, "z_forms": zs}
+
+    def set_schedule(self, xy=None, z=None, batch_tiles=None):
+        """Select the form of the T-length y/x FFT passes (``XY_FORMS``: Stockham radix passes,
+        prime-first matrix stage + radix M, Rader prime stage) and of the z pass (``Z_FORMS``: z FFTs,
+        folded z-kernel matrix, length-80 DFTs), and the tiles per launch.  Forms the geometry does
+        not admit raise ``MermakeLibraryError``.  The Wiener operators are rebuilt in the new layout,
+        and buffers sized by the old one (``HybServer`` slots) must be made again."""
+        cur = self._plan.schedule()
+        xy = cur[0] if xy is None else xy
+        z = cur[1] if z is None else z
+        if batch_tiles is not None:
+            self.batch_tiles = max(1, int(batch_tiles))
+        if (xy, z) != cur[:2]:
+            if xy not in XY_FORMS or z not in Z_FORMS:
+                raise ValueError(f"unknown forms ({xy!r}, {z!r}); xy in {XY_FORMS}, z in {Z_FORMS}")
+            self._plan.set_schedule(xy, z)
+            self._ws = {}
+            self._build_spectra()
 
     def geometry(self, shape):
         """Tiles of a (z, x, y) channel shape with their PSF indices (``psf_mode``)."""
@@ -294,7 +350,10 @@ class Deconvolver:
     def batch_size(self, ntiles, extra_per_tile=0):
         T = self._plan.T
         per = self._plan.spec_bytes + 4 * self.tile_height * T * T + extra_per_tile
-        return max(1, min(ntiles, self.batch_bytes // per))
+        n = min(ntiles, self.batch_bytes // per)
+        if self.batch_tiles:
+            n = min(n, self.batch_tiles)
+        return max(1, n)
 
     @staticmethod
     def fused_blur_maxima(blur_radius):

@@ -31,7 +31,7 @@ from .maxima import find_local_maxima_signed
 
 class HybPipeline:
     def __init__(self, deconvolver, blur_radius=30, threshold=3600, delta=1, delta_fit=3, sigmaZ=1, sigmaXY=1.5,
-                 cap=16384, plain=False, window=True, **kwargs):
+                 cap=16384, plain=False, window=True, graph=False, **kwargs):
         if delta > 5:
             raise TypeError("Delta must be an less than or equal to 5 due to MAX_KERNEL_POINTS")
         self.d = deconvolver
@@ -44,19 +44,39 @@ class HybPipeline:
         self.cap = int(cap)
         self.plain = plain
         self.window = window
+        self.graph = bool(graph) and not plain
         self._tables = {}
         self._ws = {}
+        self._graphs = {}
 
     @classmethod
     def from_settings(cls, psfs, channel_shape, tile_size=500, overlap=89, beta=1e-4, zpad=None, psf_mode="zip",
-                      **hybs):
+                      autotune=False, **hybs):
         """Deconvolver with ``zpad = sz - 1`` as in ``__main__.py:184,195``; ``psf_mode`` pairs
-        several PSFs with the tiles (``Deconvolver``)."""
+        several PSFs with the tiles (``Deconvolver``).  ``autotune`` (True or the keyword arguments of
+        ``autotune.tune``) times the deconvolution schedules on this channel shape, with the
+        deconvolution window of the pipeline, and keeps the fastest."""
         if zpad is None:
             zpad = channel_shape[0] - 1
         d = Deconvolver(psfs, channel_shape, tile_size=tile_size, zpad=zpad, overlap=overlap, beta=beta,
                         psf_mode=psf_mode)
-        return cls(d, **hybs)
+        pipe = cls(d, **hybs)
+        if autotune:
+            from .autotune import tune
+            opts = dict(autotune) if isinstance(autotune, dict) else {}
+            opts.setdefault("window", pipe.deconv_window())
+            tune(d, channel_shape, **opts)
+        return pipe
+
+    def deconv_window(self):
+        """Rows and columns [lo, hi) of every tile that the deconvolution computes for this pipeline
+        (the blur window widened by the box half-width), or None for whole tiles."""
+        win = self.windows()
+        if win is None:
+            return None
+        T = self.d._plan.T
+        delta = 0 if self.blur_radius is None else int(self.blur_radius) // 2
+        return max(0, win[0][0] - delta), min(T, win[0][1] + delta)
 
     def _buf(self, name, shape, dtype):
         t = self._ws.get(name)
@@ -126,15 +146,58 @@ class HybPipeline:
                self.delta_fit, self.sigmaZ, self.sigmaXY, L.ptr(fit), st)
         return counts, fit
 
+    def _keep_launch(self, table, n, counts, fit, cap, xh, stats):
+        d = self.d
+        L.call("mm_tiles_keep", L.ptr(table), n, d.overlap, L.ptr(fit), L.ptr(counts), cap, L.ptr(stats[2:]),
+               L.ptr(xh), n * cap, L.ptr(stats[1:2]), L.ptr(stats[0:1]), L.stream_ptr(d.device))
+
     def _keep(self, table, n, counts, fit, cap):
         """Keep-filter of a batch: the kept rows (device) and, on the host, ``[largest candidate
         count, rows kept, kept per tile...]`` (one synchronisation)."""
-        d = self.d
         stats = self._buf("stats", (n + 2,), torch.int32)
-        xh = torch.empty((n * cap, 8), dtype=torch.float32, device=d.device)
-        L.call("mm_tiles_keep", L.ptr(table), n, d.overlap, L.ptr(fit), L.ptr(counts), cap, L.ptr(stats[2:]),
-               L.ptr(xh), n * cap, L.ptr(stats[1:2]), L.ptr(stats[0:1]), L.stream_ptr(d.device))
+        xh = torch.empty((n * cap, 8), dtype=torch.float32, device=self.d.device)
+        self._keep_launch(table, n, counts, fit, cap, xh, stats)
         return xh, stats.cpu()
+
+    def _graph_batch(self, table, n, single, cap):
+        """``_spots_batch`` + ``_keep`` as the replay of a hipGraph of the batch's kernel chain, one graph
+        per (tiles, cap).  A graph owns its buffers and a table of ``n`` descriptors that each batch is
+        copied into; the first batch of a shape runs eagerly (allocating those buffers and the library's
+        lazy tables), and the graph is captured after it.  Returns the kept rows (a copy, since the
+        graph's buffers are reused) and the host statistics of ``_keep``."""
+        d = self.d
+        key = (n, cap, single, d.generation)
+        ent = self._graphs.get(key)
+        if ent is not None:
+            g, tbuf, xh, stats, _ = ent
+            tbuf.copy_(table)
+            g.replay()
+            st = stats.cpu()
+            return xh[:int(st[1])].clone(), st
+        saved = (self._ws, d._ws)
+        bufs = ({}, {})
+        self._ws, d._ws = bufs
+        try:
+            tbuf = torch.empty_like(table)
+            tbuf.copy_(table)
+            stats = self._buf("stats", (n + 2,), torch.int32)
+            xh = self._buf("xh", (n * cap, 8), torch.float32)
+            counts, fit = self._spots_batch(tbuf, n, single, cap)
+            self._keep_launch(tbuf, n, counts, fit, cap, xh, stats)
+            st = stats.cpu()
+            rows = xh[:int(st[1])].clone()
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g, stream=torch.cuda.Stream(d.device), capture_error_mode="thread_local"):
+                counts, fit = self._spots_batch(tbuf, n, single, cap)
+                self._keep_launch(tbuf, n, counts, fit, cap, xh, stats)
+        finally:
+            self._ws, d._ws = saved
+        for k in [k for k in self._graphs if k[3] != d.generation]:
+            del self._graphs[k]     # chains of an older schedule hold the address of its operators
+        while len(self._graphs) >= 4:
+            self._graphs.pop(next(iter(self._graphs)))
+        self._graphs[key] = (g, tbuf, xh, stats, bufs)
+        return rows, st
 
     def tile_rows(self, table, n):
         """Kept spot rows of each of ``n`` tile descriptors (a device table), as host arrays;
@@ -152,6 +215,19 @@ class HybPipeline:
             rows = xh[:int(k.sum())].cpu().numpy()
             offs = np.concatenate([[0], np.cumsum(k)])
             return [rows[offs[i]:offs[i + 1]].copy() for i in range(n)]
+
+    @staticmethod
+    def _split(parts, owner, b0, nb, kept_h, xh):
+        """Rows of a batch to their stacks: tiles of one stack are contiguous."""
+        off = 0
+        for i in range(nb):
+            si = owner[b0 + i]
+            if kept_h[i]:
+                if parts[si] and parts[si][-1][2] is xh and parts[si][-1][1] == off:
+                    parts[si][-1][1] = off + kept_h[i]
+                else:
+                    parts[si].append([off, off + kept_h[i], xh])
+            off += kept_h[i]
 
     def __call__(self, stack, flat_field=None):
         """Spots of one channel stack: ``(N, 8)`` float32 rows in tile order, then raster order."""
@@ -186,6 +262,15 @@ class HybPipeline:
             table = table_all[b0 * rec:(b0 + nb) * rec]
             cap = self.cap
             while True:
+                if self.graph:
+                    xh, stats = self._graph_batch(table, nb, single, cap)
+                    mx = int(stats[0])
+                    if mx > cap:
+                        cap = mx
+                        continue
+                    kept_h = stats[2:].tolist()
+                    self._split(parts, owner, b0, nb, kept_h, xh)
+                    break
                 counts, fit = self._spots_batch(table, nb, single, cap)
                 if self.plain:
                     mx = int(counts.cpu().max())        # sync after the maxima, as maxima.py:46-50
@@ -200,15 +285,7 @@ class HybPipeline:
                     cap = mx
                     continue
                 kept_h = stats[2:].tolist()
-                off = 0
-                for i in range(nb):                    # tiles of one stack are contiguous: split the rows by owner
-                    si = owner[b0 + i]
-                    if kept_h[i]:
-                        if parts[si] and parts[si][-1][2] is xh and parts[si][-1][1] == off:
-                            parts[si][-1][1] = off + kept_h[i]
-                        else:
-                            parts[si].append([off, off + kept_h[i], xh])
-                    off += kept_h[i]
+                self._split(parts, owner, b0, nb, kept_h, xh)
                 break
         del alive   # every batch has synchronised on its keep-filter statistics
         out = []

@@ -35,7 +35,6 @@ computed by the same kernels whatever batch it lands in, and rows are assembled 
 """
 import collections
 import concurrent.futures
-import ctypes
 import time
 import types
 
@@ -121,9 +120,12 @@ def _fused_blur(pipe):
 class _Slot:
     """Device and pinned buffers of one in-flight batch, sized for the largest batch of any mode."""
 
-    def __init__(self, pipes, B_of, cap, device):
+    def __init__(self, pipes, B_of, cap, device, graph=False):
         self.cap = cap
         self.B = max(B_of.values())
+        self.device = device
+        self.graphs = {} if graph else None
+        self.replays = 0
         self.stream = torch.cuda.Stream(device)
         spec = max(B_of[k] * p.d._plan.spec_bytes for k, p in pipes.items())
         res = max(B_of[k] * p.d.tile_height * p.d._plan.T ** 2 for k, p in pipes.items())
@@ -152,16 +154,13 @@ class _Slot:
         self.n = 0
         self.total = 0
 
-    def launch(self, pipe, jobs):
+    def _chain(self, pipe, n):
+        """Enqueue deconvolution -> blur -> maxima -> fit -> keep of the ``n`` descriptors in the slot's table
+        on the current stream, into the slot's buffers."""
         d = pipe.d
-        n = len(jobs)
         T, sz = d._plan.T, d.tile_height
         single = d.psf_fft.shape[0] == 1
-        recs = np.empty(n, dtype=L.TILE_DTYPE)
-        for i, j in enumerate(jobs):
-            recs[i] = j.rec
-        self.table_h[:n * _REC].copy_(torch.from_numpy(recs.view(np.uint8)))
-        st = ctypes.c_void_p(self.stream.cuda_stream)
+        st = L.stream_ptr(self.device)
         counts, kept, total = self.stats[:n], self.stats[self.B:self.B + n], self.stats[2 * self.B:]
         res = self.res[:n * sz * T * T]
         tmp = None if self.tmp is None else self.tmp[:n * sz * T * T]
@@ -169,43 +168,69 @@ class _Slot:
         # candidates only where the kept spots can come from
         win = pipe.windows()
         delta = None if pipe.blur_radius is None else int(pipe.blur_radius) // 2
+        tab = L.ptr(self.table)
+        dargs = (d._plan.h, tab, n, L.ptr(d.psf_fft), d.psf_fft.shape[0], n if single else 1, L.ptr(self.spec),
+                 L.ptr(res))
+        if win is None or delta is None:
+            L.call("mm_deconv_tiles", *dargs, st)
+        else:
+            L.call("mm_deconv_tiles_window", *dargs, max(0, win[0][0] - delta), min(T, win[0][1] + delta), st)
+        out = res
+        fused = win is not None and d.fused_blur_maxima(pipe.blur_radius)
+        if delta is not None:
+            # blur subtraction into the spectrum buffer, which is free again (deconvolver.py:160-163)
+            out = self.spec[:n * sz * T * T * 4].view(torch.float32)
+            if win is None:
+                L.call("mm_blur_sub_xy", L.ptr(res), L.ptr(out), L.ptr(tmp), n * sz, T, delta, st)
+            elif fused:  # blur + maxima threshold bits, then the maxima of the candidates
+                L.call("mm_tiles_blur_maxima_window", d._plan.h, tab, n, L.ptr(res), L.ptr(out), delta,
+                       win[0][0], win[0][1], pipe.threshold, pipe.delta, win[1][0], win[1][1],
+                       L.ptr(self.coords), self.cap, L.ptr(counts), L.ptr(self.mws), self.mws_bytes, st)
+            else:
+                L.call("mm_blur_sub_xy_window", L.ptr(res), L.ptr(out), L.ptr(tmp), n * sz, T, delta,
+                       win[0][0], win[0][1], st)
+        if win is None:
+            L.call("mm_tiles_maxima", d._plan.h, tab, n, L.ptr(out), pipe.threshold, pipe.delta,
+                   L.ptr(self.coords), self.cap, L.ptr(counts), L.ptr(self.mws), self.mws_bytes, st)
+        elif not fused:
+            L.call("mm_tiles_maxima_window", d._plan.h, tab, n, L.ptr(out), pipe.threshold, pipe.delta,
+                   win[1][0], win[1][1], L.ptr(self.coords), self.cap, L.ptr(counts), L.ptr(self.mws),
+                   self.mws_bytes, st)
+        L.call("mm_tiles_fit", d._plan.h, tab, n, L.ptr(out), L.ptr(self.coords), L.ptr(counts), self.cap,
+               pipe.delta_fit, pipe.sigmaZ, pipe.sigmaXY, L.ptr(self.fit), st)
+        L.call("mm_tiles_keep", tab, n, d.overlap, L.ptr(self.fit), L.ptr(counts), self.cap, L.ptr(kept),
+               L.ptr(self.xh), n * self.cap, L.ptr(total), None, st)
+
+    def launch(self, pipe, jobs):
+        d = pipe.d
+        n = len(jobs)
+        T, sz = d._plan.T, d.tile_height
+        if n > self.B or n * d._plan.spec_bytes > self.spec.numel() or n * sz * T * T > self.res.numel():
+            raise RuntimeError("the slot's buffers do not fit this batch (was the schedule changed after the "
+                               "server was made?)")
+        recs = np.empty(n, dtype=L.TILE_DTYPE)
+        for i, j in enumerate(jobs):
+            recs[i] = j.rec
+        self.table_h[:n * _REC].copy_(torch.from_numpy(recs.view(np.uint8)))
         self.jobs, self.n, self.pipe, self.state = jobs, n, pipe, "stats"
+        # one hipGraph per (pipeline, schedule, tiles): the first batch of a shape runs eagerly, the graph
+        # is captured after it and replayed by the next ones (the table is the graph's input)
+        key = (id(pipe), d.generation, n)
+        g = None if self.graphs is None else self.graphs.get(key)
         with torch.cuda.stream(self.stream):
             self.table[:n * _REC].copy_(self.table_h[:n * _REC], non_blocking=True)
-            tab = L.ptr(self.table)
-            dargs = (d._plan.h, tab, n, L.ptr(d.psf_fft), d.psf_fft.shape[0], n if single else 1, L.ptr(self.spec),
-                     L.ptr(res))
-            if win is None or delta is None:
-                L.call("mm_deconv_tiles", *dargs, st)
+            if g is not None:
+                g.replay()
+                self.replays += 1
             else:
-                L.call("mm_deconv_tiles_window", *dargs, max(0, win[0][0] - delta), min(T, win[0][1] + delta), st)
-            out = res
-            fused = win is not None and d.fused_blur_maxima(pipe.blur_radius)
-            if delta is not None:
-                # blur subtraction into the spectrum buffer, which is free again (deconvolver.py:160-163)
-                out = self.spec[:n * sz * T * T * 4].view(torch.float32)
-                if win is None:
-                    L.call("mm_blur_sub_xy", L.ptr(res), L.ptr(out), L.ptr(tmp), n * sz, T, delta, st)
-                elif fused:  # blur + maxima threshold bits, then the maxima of the candidates
-                    L.call("mm_tiles_blur_maxima_window", d._plan.h, tab, n, L.ptr(res), L.ptr(out), delta,
-                           win[0][0], win[0][1], pipe.threshold, pipe.delta, win[1][0], win[1][1],
-                           L.ptr(self.coords), self.cap, L.ptr(counts), L.ptr(self.mws), self.mws_bytes, st)
-                else:
-                    L.call("mm_blur_sub_xy_window", L.ptr(res), L.ptr(out), L.ptr(tmp), n * sz, T, delta,
-                           win[0][0], win[0][1], st)
-            if win is None:
-                L.call("mm_tiles_maxima", d._plan.h, tab, n, L.ptr(out), pipe.threshold, pipe.delta,
-                       L.ptr(self.coords), self.cap, L.ptr(counts), L.ptr(self.mws), self.mws_bytes, st)
-            elif not fused:
-                L.call("mm_tiles_maxima_window", d._plan.h, tab, n, L.ptr(out), pipe.threshold, pipe.delta,
-                       win[1][0], win[1][1], L.ptr(self.coords), self.cap, L.ptr(counts), L.ptr(self.mws),
-                       self.mws_bytes, st)
-            L.call("mm_tiles_fit", d._plan.h, tab, n, L.ptr(out), L.ptr(self.coords), L.ptr(counts), self.cap,
-                   pipe.delta_fit, pipe.sigmaZ, pipe.sigmaXY, L.ptr(self.fit), st)
-            L.call("mm_tiles_keep", tab, n, d.overlap, L.ptr(self.fit), L.ptr(counts), self.cap, L.ptr(kept),
-                   L.ptr(self.xh), n * self.cap, L.ptr(total), None, st)
+                self._chain(pipe, n)
             self.stats_h.copy_(self.stats, non_blocking=True)
             self.ev_stats.record(self.stream)
+        if self.graphs is not None and g is None:
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g, capture_error_mode="thread_local"):
+                self._chain(pipe, n)
+            self.graphs[key] = g
 
     def fetch_rows(self):
         """After the counts are back: copy exactly the kept rows (or report an overflowing tile)."""
@@ -232,10 +257,11 @@ class HybServer:
 
     ``pipe``: a ``HybPipeline``, or a dict {mode key: HybPipeline} (requests choose with ``psf=key``);
     ``batch_tiles``: tiles per launch, an int or a dict per mode; ``dapi``: a ``DapiPipeline`` for the
-    requests' DAPI channels; ``drift_th``: the threshold of ``align.drift`` (``align.py:169``)."""
+    requests' DAPI channels; ``drift_th``: the threshold of ``align.drift`` (``align.py:169``); ``graph``:
+    each slot replays its batches' kernel chain as a hipGraph, one per (mode, tiles in the batch)."""
 
     def __init__(self, pipe, batch_tiles=36, slots=2, cap=None, hbm_reserve=4 << 30, hbm_budget=None, dapi=None,
-                 drift_th=4):
+                 drift_th=4, graph=False):
         self.pipes = dict(pipe) if isinstance(pipe, dict) else {None: pipe}
         if not self.pipes:
             raise ValueError("HybServer needs at least one pipeline")
@@ -248,7 +274,7 @@ class HybServer:
         self.B_of = {k: int(batch_tiles[k] if isinstance(batch_tiles, dict) else batch_tiles) for k in self.pipes}
         self.B = max(self.B_of.values())
         self.cap = int(cap or max(p.cap for p in self.pipes.values()))
-        self.slots = [_Slot(self.pipes, self.B_of, self.cap, self.device) for _ in range(slots)]
+        self.slots = [_Slot(self.pipes, self.B_of, self.cap, self.device, graph) for _ in range(slots)]
         self.copy_stream = torch.cuda.Stream(self.device)
         self.dapi = dapi
         self.dapi_stream = torch.cuda.Stream(self.device) if dapi is not None else None
@@ -614,6 +640,7 @@ class HybServer:
             "dapi_stacks": sum(1 for r in ok if r.Xh_plus is not None),
             "drift_blocks": len(self._drift_s), "drift_s_total": float(sum(self._drift_s)),
             "hbm_budget_bytes": self.budget,
+            "graph_replays": sum(s.replays for s in self.slots),
         }
         return reqs
 

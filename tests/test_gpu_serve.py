@@ -75,6 +75,59 @@ def test_run_many_host_stacks_equal_alone(fovs, pipe, alone, dev):
         assert np.array_equal(g.cpu().numpy(), alone[i][c]), (i, c)
 
 
+def test_graph_replays_equal_eager(fovs, pipe, alone, dev):
+    """``HybPipeline(graph=True)`` runs the first batch of a shape eagerly, captures the batch's kernel
+    chain as a hipGraph and replays it for the next batches of that shape: every replay gives the
+    eager spots, with other eager work and synchronisations between the replays; batches of another
+    shape (uneven FOVs in one ``run_many``) get their own graph."""
+    from mermake_amd.pipeline import HybPipeline
+    data, _ = fovs
+    gp = HybPipeline(pipe.d, graph=True, **HYB)
+    chans, flat = data[0]
+    fl = torch.from_numpy(flat).to(dev)
+    for rep in range(3):
+        for c, ch in enumerate(chans):
+            got = gp(torch.from_numpy(ch).to(dev), fl)
+            assert np.array_equal(got.cpu().numpy(), alone[0][c]), (rep, c)
+            x = torch.randn(1 << 20, device=dev)
+            assert float((x * x).sum()) > 0
+            torch.cuda.synchronize()
+    assert len(gp._graphs) == 1
+    stacks, flats, where = [], [], []
+    for i, (chs, f) in enumerate(data):
+        fd = torch.from_numpy(f).to(dev)
+        for c, ch in enumerate(chs):
+            stacks.append(torch.from_numpy(ch).to(dev))
+            flats.append(fd)
+            where.append((i, c))
+    for rep in range(2):
+        got = gp.run_many(stacks, flats)
+        for (i, c), g in zip(where, got):
+            assert np.array_equal(g.cpu().numpy(), alone[i][c]), (rep, i, c)
+    assert len(gp._graphs) == 2
+
+
+def test_server_graph_replays_equal_alone(fovs, pipe, alone, dev):
+    """Slots that replay their batches' kernel chains as hipGraphs (``HybServer(graph=True)``): two
+    runs of the same requests (the second replays the graphs the first captured) give every
+    channel's spots alone."""
+    from mermake_amd.serve import FovRequest, HybServer
+    data, _ = fovs
+    srv = HybServer(pipe, batch_tiles=5, slots=2, graph=True)
+    for rep in range(2):
+        reqs = [FovRequest(i, [torch.from_numpy(c).pin_memory() for c in chans], torch.from_numpy(flat).pin_memory(),
+                           arrival=0.003 * i) for i, (chans, flat) in enumerate(data)]
+        done = srv.run(reqs)
+        assert len(done) == len(reqs)
+        for r in done:
+            assert r.error is None
+            for c, s in enumerate(r.spots):
+                assert np.array_equal(s, alone[r.rid][c]), (rep, r.rid, c)
+    graphs = sum(len(s.graphs) for s in srv.slots)
+    assert 1 <= graphs <= 2 * 5
+    assert srv.stats["graph_replays"] > 0
+
+
 @pytest.mark.parametrize("host_inputs,batch_tiles,budget_fovs", [(False, 5, None), (True, 7, 1.5), (True, 36, None)])
 def test_server_mix_equals_alone(fovs, pipe, alone, dev, host_inputs, batch_tiles, budget_fovs):
     """Requests arriving while others run are admitted as memory frees up and batched together;
