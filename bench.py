@@ -161,6 +161,14 @@ def serve_leg(pipe, stacks, flat, cfg, args, dev):
     for name, arr, src in (("saturated", np.zeros(len(order)), fovs),
                            ("saturated_u8", np.zeros(len(order)), fovs_u8)):
         srv.run([FovRequest(-1 - i, *src[side]) for i, side in enumerate(order[:2])])  # warm the slots
+        # warm the host-to-device path of this process: the first few GB copied on a fresh GPU run at
+        # about half the DMA rate
+        with torch.cuda.stream(srv.copy_stream):
+            for side in sides:
+                for t in src[side][0]:
+                    for _ in range(2):
+                        torch.empty(t.shape, dtype=t.dtype, device=dev).copy_(t, non_blocking=True)
+        torch.cuda.synchronize()
         reqs = [FovRequest(i, *src[side], arrival=float(a)) for i, (side, a) in enumerate(zip(order, arr))]
         with mark(f"served_hyb_{name}"):
             srv.run(reqs)

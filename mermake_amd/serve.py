@@ -5,9 +5,10 @@ A request is one FOV as the microscope delivers it (the per-image body of
 device) with their flat fields, optionally the DAPI channel, a PSF mode and the FOV block it
 belongs to.  Requests arrive over time.  The loop
 
-* admits an arrived request when its inputs fit the HBM budget (device memory free after the
-  server's own buffers are allocated, minus a reserve, minus the inputs of the requests still in
-  flight) and copies host inputs to the device on a copy stream of its own.  A request that cannot
+* admits an arrived request when its host inputs fit the free room of the server's input arena
+  (device memory allocated once at start, ``inflight_cap`` bytes within the HBM budget: free memory
+  after the server's own buffers, minus a reserve), and copies them into it on a copy stream of
+  its own.  A request that cannot
   be served (wrong rank or depth, flat of the wrong shape, unknown PSF mode, over the budget, a
   failure while its tiles are prepared) fails alone: its ``error`` is set, its bytes are released
   and the loop goes on, as the reference's image queue skips a file it cannot read
@@ -115,6 +116,63 @@ class _Job:
 
 def _fused_blur(pipe):
     return pipe.blur_radius is not None and int(pipe.blur_radius) // 2 in (15, 25)
+
+
+class _Arena:
+    """Device memory for the inputs of the requests in flight, allocated (and so first touched) once
+    when the server starts: each admitted request takes one block of it, in admission order, as a
+    ring.  Blocks are freed in any order; the ring's tail passes a block once it and every block
+    before it are free."""
+
+    ALIGN = 256
+
+    def __init__(self, nbytes, device):
+        self.size = max(0, int(nbytes)) // self.ALIGN * self.ALIGN
+        self.buf = torch.empty(self.size, dtype=torch.uint8, device=device) if self.size else None
+        if self.buf is not None:
+            self.buf.zero_()
+        self.blocks = collections.deque()   # [start, end, free]
+        self.head = 0
+
+    @classmethod
+    def rounded(cls, n):
+        return -(-int(n) // cls.ALIGN) * cls.ALIGN
+
+    def alloc(self, n):
+        """A block of ``n`` bytes (a multiple of ALIGN) as [start, end, free], or None if it does not fit now."""
+        if n > self.size:
+            return None
+        if not self.blocks:
+            self.head = 0
+            start = 0
+        else:
+            tail = self.blocks[0][0]
+            if self.head > tail:                    # live blocks in [tail, head): room at the end, then the start
+                if self.size - self.head >= n:
+                    start = self.head
+                elif n <= tail:
+                    start = 0
+                else:
+                    return None
+            elif tail - self.head >= n:             # wrapped: the room is [head, tail)
+                start = self.head
+            else:
+                return None
+        blk = [start, start + n, False]
+        self.blocks.append(blk)
+        self.head = start + n
+        return blk
+
+    def free(self, blk):
+        blk[2] = True
+        while self.blocks and self.blocks[0][2]:
+            self.blocks.popleft()
+
+    def view(self, blk, off, t):
+        """Device tensor of ``t``'s dtype and shape at byte ``off`` of the block (keeps ``t``'s tensor type)."""
+        n = t.numel() * t.element_size()
+        v = self.buf[blk[0] + off:blk[0] + off + n].view(t.dtype).view(t.shape)
+        return v if type(t) is torch.Tensor else v.as_subclass(type(t))
 
 
 class _Slot:
@@ -258,10 +316,15 @@ class HybServer:
     ``pipe``: a ``HybPipeline``, or a dict {mode key: HybPipeline} (requests choose with ``psf=key``);
     ``batch_tiles``: tiles per launch, an int or a dict per mode; ``dapi``: a ``DapiPipeline`` for the
     requests' DAPI channels; ``drift_th``: the threshold of ``align.drift`` (``align.py:169``); ``graph``:
-    each slot replays its batches' kernel chain as a hipGraph, one per (mode, tiles in the batch)."""
+    each slot replays its batches' kernel chain as a hipGraph, one per (mode, tiles in the batch);
+    ``inflight_cap``: bytes of the device arena that holds the host inputs of the requests in flight
+    (allocated once, here, within the HBM budget).  Inputs admitted far ahead of the GPU only wait; and memory allocated fresh
+    while the loop runs is cleared by the driver on the DMA engines that also copy the inputs, which
+    halves their rate on a fresh GPU.  A request larger than the arena gets memory of its own within the
+    HBM budget and runs when no other such request is in flight."""
 
     def __init__(self, pipe, batch_tiles=36, slots=2, cap=None, hbm_reserve=4 << 30, hbm_budget=None, dapi=None,
-                 drift_th=4, graph=False):
+                 drift_th=4, graph=False, inflight_cap=16 << 30):
         self.pipes = dict(pipe) if isinstance(pipe, dict) else {None: pipe}
         if not self.pipes:
             raise ValueError("HybServer needs at least one pipeline")
@@ -282,7 +345,8 @@ class HybServer:
         torch.cuda.synchronize(self.device)
         free, _ = torch.cuda.mem_get_info(self.device)
         self.budget = int(hbm_budget) if hbm_budget is not None else max(0, free - int(hbm_reserve))
-        self.inflight_bytes = 0
+        self.arena = _Arena(min(int(inflight_cap), self.budget), self.device)
+        self.inflight_bytes = 0      # bytes of a request in flight outside the arena (larger than it)
         self.stats = {}
 
     # ------------------------------------------------------------------ admission
@@ -294,10 +358,18 @@ class HybServer:
             return 0 if t.is_cuda else t.numel() * t.element_size()
         return np.asarray(t).nbytes
 
+    def _host_inputs(self, req):
+        """The request's host-resident inputs, each once (a flat shared by several channels is copied once)."""
+        seen, out = set(), []
+        for t in list(req.channels) + list(req.channel_flats()) + [req.dapi, req.dapi_flat]:
+            if t is not None and id(t) not in seen and self._host_bytes(t):
+                seen.add(id(t))
+                out.append(t)
+        return out
+
     def _need(self, req):
-        flats = {id(f): f for f in (req.channel_flats() if isinstance(req.flat, (list, tuple)) else [req.flat])}
-        n = sum(self._host_bytes(c) for c in req.channels) + sum(self._host_bytes(f) for f in flats.values())
-        return n + self._host_bytes(req.dapi) + self._host_bytes(req.dapi_flat)
+        """Device bytes of the request's host inputs (each rounded to the arena's alignment)."""
+        return sum(_Arena.rounded(self._host_bytes(t)) for t in self._host_inputs(req))
 
     @staticmethod
     def _shape(t):
@@ -328,39 +400,69 @@ class HybServer:
         if need > self.budget:
             raise ValueError(f"request {req.rid} needs {need} bytes of HBM, over the budget {self.budget}")
 
-    def _to_device(self, t):
+    def _to_device(self, t, req=None):
+        """``t`` on the device: device tensors as they are; host inputs copied (on the current stream) into
+        the request's arena block, or into memory of their own for a request larger than the arena."""
         if t is None:
             return None
         if isinstance(t, torch.Tensor) and t.is_cuda:
             return t
         h = t if isinstance(t, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(t))
-        return h.to(self.device, non_blocking=True)
+        if req is None or req._blk is None:
+            return h.to(self.device, non_blocking=True)
+        if not h.is_contiguous():
+            h = h.contiguous()
+        off = req._placed.get(id(t))
+        if off is None:
+            off = req._fill
+            req._placed[id(t)] = off
+            req._fill += _Arena.rounded(h.numel() * h.element_size())
+        d = self.arena.view(req._blk, off, h)
+        d.copy_(h, non_blocking=True)
+        return d
+
+    def _reserve(self, req):
+        """Device memory for the request's host inputs: a block of the arena, or, for a request larger
+        than the arena (and within the HBM budget, which ``_validate`` checks), memory of its own while no
+        other request is in flight.  False when it has to wait."""
+        need = self._need(req)
+        req._blk, req._bytes = None, 0
+        if need == 0:
+            return True
+        blk = self.arena.alloc(need)
+        if blk is not None:
+            req._blk = blk
+            return True
+        if need <= self.arena.size or self.inflight_bytes or self.arena.blocks:
+            return False                 # room comes back as requests in flight complete
+        req._bytes = need
+        self.inflight_bytes += need
+        return True
 
     def _admit(self, req, now):
         """Start the request's host-to-device copies on the copy stream, each channel's flat before the
         channel, with an event after each channel: a channel's tiles are queued as soon as its own copy is
         done."""
-        req._bytes = self._need(req)
-        self.inflight_bytes += req._bytes
         req.t_admit = now
         req._rows = [None] * len(req.channels)
         req._left = 0
         req._next = 0
         req._dapi_done = req.dapi is None
         req._mode = req.psf if req.psf in self.pipes else self.default_mode
+        req._placed, req._fill = {}, 0
         copied = {}
         chans, flats, evs = [], [], []
         with torch.cuda.stream(self.copy_stream):
             for c, f in zip(req.channels, req.channel_flats()):
                 if f is not None and id(f) not in copied:
-                    copied[id(f)] = self._to_device(f)
+                    copied[id(f)] = self._to_device(f, req)
                 flats.append(None if f is None else copied[id(f)])
-                chans.append(self._to_device(c))
+                chans.append(self._to_device(c, req))
                 ev = torch.cuda.Event()
                 ev.record(self.copy_stream)
                 evs.append(ev)
             if req.dapi is not None:
-                req._dapi = (self._to_device(req.dapi), self._to_device(req.dapi_flat))
+                req._dapi = (self._to_device(req.dapi, req), self._to_device(req.dapi_flat, req))
                 req._dapi_ev = torch.cuda.Event()
                 req._dapi_ev.record(self.copy_stream)
         req._chan, req._flats, req._evs = chans, flats, evs
@@ -397,7 +499,10 @@ class HybServer:
 
     # ------------------------------------------------------------------ completion
     def _release(self, req):
-        if req._bytes:
+        if getattr(req, "_blk", None) is not None:
+            self.arena.free(req._blk)
+            req._blk = None
+        if getattr(req, "_bytes", 0):
             self.inflight_bytes -= req._bytes
             req._bytes = 0
         req._chan = req._flats = req._dapi = req._evs = req._dapi_ev = None
@@ -537,7 +642,7 @@ class HybServer:
                     progressed = True
                     continue
                 waiting.append(r)
-            while waiting and self.inflight_bytes + self._need(waiting[0]) <= self.budget:
+            while waiting and self._reserve(waiting[0]):
                 r = waiting.popleft()
                 try:
                     self._admit(r, now)
@@ -639,7 +744,7 @@ class HybServer:
             "overflow_batches": overflows, "batch_tiles": self.B, "slots": len(self.slots),
             "dapi_stacks": sum(1 for r in ok if r.Xh_plus is not None),
             "drift_blocks": len(self._drift_s), "drift_s_total": float(sum(self._drift_s)),
-            "hbm_budget_bytes": self.budget,
+            "hbm_budget_bytes": self.budget, "arena_bytes": self.arena.size,
             "graph_replays": sum(s.replays for s in self.slots),
         }
         return reqs

@@ -104,7 +104,8 @@ def test_autotune_selects_the_fastest_and_caches_it(dev, tmp_path, monkeypatch):
     assert all(v > 0 for v in t["ms_forms"].values())
     assert min(t["ms_forms"], key=t["ms_forms"].get) == f"{t['xy']}+{t['z']}" == f"{s['xy']}+{s['z']}"
     assert set(t["ms_batch_tiles"]) == {"4", "2", "1"}
-    B = int(min(t["ms_batch_tiles"], key=t["ms_batch_tiles"].get))
+    fastest = min(t["ms_batch_tiles"].values())
+    B = max(int(b) for b, ms in t["ms_batch_tiles"].items() if ms <= 1.02 * fastest)
     assert d.batch_tiles == (None if B == 4 else B) == s["batch_tiles"]
     assert (tmp_path / "schedule.json").exists()
     d2 = Deconvolver(psf, shape, autotune=True, **kw)

@@ -128,10 +128,11 @@ def test_server_graph_replays_equal_alone(fovs, pipe, alone, dev):
     assert srv.stats["graph_replays"] > 0
 
 
-@pytest.mark.parametrize("host_inputs,batch_tiles,budget_fovs", [(False, 5, None), (True, 7, 1.5), (True, 36, None)])
-def test_server_mix_equals_alone(fovs, pipe, alone, dev, host_inputs, batch_tiles, budget_fovs):
-    """Requests arriving while others run are admitted as memory frees up and batched together;
-    every request's spots equal its channels run alone."""
+@pytest.mark.parametrize("host_inputs,batch_tiles,budget_fovs,cap_fovs",
+                         [(False, 5, None, None), (True, 7, 1.5, None), (True, 36, None, None), (True, 7, None, 1.5)])
+def test_server_mix_equals_alone(fovs, pipe, alone, dev, host_inputs, batch_tiles, budget_fovs, cap_fovs):
+    """Requests arriving while others run are admitted as memory frees up (within the HBM budget and the
+    in-flight cap) and batched together; every request's spots equal its channels run alone."""
     from mermake_amd.serve import FovRequest, HybServer
     data, _ = fovs
     reqs = []
@@ -144,11 +145,10 @@ def test_server_mix_equals_alone(fovs, pipe, alone, dev, host_inputs, batch_tile
             ch = [torch.from_numpy(c).to(dev) for c in chans]
             fl = torch.from_numpy(flat).to(dev)
         reqs.append(FovRequest(i, ch, fl, arrival=arrivals[i]))
-    budget = None
-    if budget_fovs is not None:
-        per = max(sum(c.nbytes for c in chans) + flat.nbytes for chans, flat in data)
-        budget = int(budget_fovs * per)
-    srv = HybServer(pipe, batch_tiles=batch_tiles, slots=2, hbm_budget=budget)
+    per = max(sum(c.nbytes for c in chans) + flat.nbytes for chans, flat in data)
+    budget = None if budget_fovs is None else int(budget_fovs * per)
+    kw = {} if cap_fovs is None else {"inflight_cap": int(cap_fovs * per)}
+    srv = HybServer(pipe, batch_tiles=batch_tiles, slots=2, hbm_budget=budget, **kw)
     done = srv.run(reqs)
     assert len(done) == len(reqs)
     for r in done:
@@ -159,7 +159,7 @@ def test_server_mix_equals_alone(fovs, pipe, alone, dev, host_inputs, batch_tile
     assert st["requests"] == 4 and st["p99_latency_s"] >= st["p50_latency_s"] > 0
     ntiles = sum(2 * -(-sh[1] // 100) * -(-sh[2] // 100) for sh in SHAPES)
     assert st["tiles"] == ntiles and st["launches"] >= -(-ntiles // batch_tiles)
-    if budget_fovs is not None:
+    if budget_fovs is not None or cap_fovs is not None:
         # with room for one request at a time, admissions follow completions
         order = sorted(done, key=lambda r: r.t_admit)
         for a, b in zip(order, order[1:]):

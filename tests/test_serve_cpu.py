@@ -29,3 +29,49 @@ def test_request_channel_flats():
     assert FovRequest(0, chans, None).channel_flats() == [None, None]
     with pytest.raises(ValueError):
         FovRequest(0, chans, [a]).channel_flats()
+
+
+def test_arena_ring_blocks_never_overlap():
+    """The input arena's ring: random admissions and out-of-order completions never hand out
+    overlapping or out-of-range blocks, a block is refused only when it does not fit the free room, and
+    an emptied ring starts again at 0."""
+    import random
+    from mermake_amd.serve import _Arena
+    A = _Arena(64 * 256, "cpu")
+    assert A.size == 64 * 256 and A.alloc(A.size + 256) is None
+    rng = random.Random(5)
+    live, refused = [], 0
+    for _ in range(20000):
+        if live and rng.random() < 0.45:
+            A.free(live.pop(rng.randrange(len(live))))
+        else:
+            n = 256 * rng.choice([1, 2, 4, 6, 8, 13])
+            b = A.alloc(n)
+            if b is None:
+                refused += 1
+                assert live        # an empty ring takes any block up to its size
+                continue
+            assert 0 <= b[0] < b[1] <= A.size and b[1] - b[0] == n and b[0] % 256 == 0
+            assert all(b[1] <= o[0] or o[1] <= b[0] for o in live)
+            live.append(b)
+    assert refused > 0
+    for b in live:
+        A.free(b)
+    assert not A.blocks and A.alloc(A.size) == [0, A.size, False]
+
+
+def test_arena_views_keep_dtype_shape_and_tensor_type():
+    import torch
+    from mermake_amd.deconvolver import SquaredU8
+    from mermake_amd.serve import _Arena
+    A = _Arena(1 << 16, "cpu")
+    blk = A.alloc(_Arena.rounded(2 * 3 * 4 * 2) + _Arena.rounded(5 * 7))
+    h = torch.arange(24, dtype=torch.int32).to(torch.uint16).view(2, 3, 4)
+    u = SquaredU8.of(np.arange(35, dtype=np.uint8).reshape(1, 5, 7))
+    v = A.view(blk, 0, h)
+    v.copy_(h)
+    w = A.view(blk, _Arena.rounded(h.numel() * 2), u)
+    w.copy_(u)
+    assert v.dtype == torch.uint16 and tuple(v.shape) == (2, 3, 4) and type(v) is torch.Tensor
+    assert type(w) is SquaredU8 and tuple(w.shape) == (1, 5, 7)
+    assert torch.equal(v.to(torch.int32), h.to(torch.int32)) and torch.equal(w.as_subclass(torch.Tensor), u)
