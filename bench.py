@@ -129,6 +129,17 @@ def cpu_baseline(stack_np, flat_np, psf, cfg, ntiles_per_stack):
                       f"{ntiles_per_stack} tiles x {cfg['channels']} channels"}
 
 
+def warm_copies(srv, tensors, dev, times=2):
+    """Warm this process's host-to-device path on the server's copy stream with the stream's own pinned
+    inputs: on a fresh GPU the first several GB copied move at about half the DMA rate."""
+    import torch
+    with torch.cuda.stream(srv.copy_stream):
+        for t in tensors:
+            for _ in range(times):
+                torch.empty(t.shape, dtype=t.dtype, device=dev).copy_(t, non_blocking=True)
+    torch.cuda.synchronize()
+
+
 def serve_leg(pipe, stacks, flat, cfg, args, dev):
     """The request loop saturated with hyb FOVs of mixed sizes (40 x {2048, 2304, 2800}^2, 3 channels each, host
     inputs in pinned memory, all requests at t = 0): uint16 stacks, then the uint8 frames whose squares are the
@@ -161,14 +172,7 @@ def serve_leg(pipe, stacks, flat, cfg, args, dev):
     for name, arr, src in (("saturated", np.zeros(len(order)), fovs),
                            ("saturated_u8", np.zeros(len(order)), fovs_u8)):
         srv.run([FovRequest(-1 - i, *src[side]) for i, side in enumerate(order[:2])])  # warm the slots
-        # warm the host-to-device path of this process: the first few GB copied on a fresh GPU run at
-        # about half the DMA rate
-        with torch.cuda.stream(srv.copy_stream):
-            for side in sides:
-                for t in src[side][0]:
-                    for _ in range(2):
-                        torch.empty(t.shape, dtype=t.dtype, device=dev).copy_(t, non_blocking=True)
-        torch.cuda.synchronize()
+        warm_copies(srv, [t for side in sides for t in src[side][0]], dev)
         reqs = [FovRequest(i, *src[side], arrival=float(a)) for i, (side, a) in enumerate(zip(order, arr))]
         with mark(f"served_hyb_{name}"):
             srv.run(reqs)
@@ -327,6 +331,10 @@ def multi_psf_leg(dev, cfg, grid_pipe, dapi_grid, reps=3):
             "spots_plus": int(pl.shape[0]), "spots_minus": int(mi.shape[0])}
     log(f"DAPI nuclei (100-PSF grid): {ms:.1f} ms per stack ({vox / ms / 1e6:.1f} G voxels/s, M1 "
         f"{dapi['m1_roofline_frac']:.3f}), {pl.shape[0]} + {mi.shape[0]} spots")
+    for rec, dd_ in ((hyb, d), (dapi, dd)):
+        t = dd_.tuning
+        rec["schedule"] = ({k: t[k] for k in ("xy", "z", "batch_tiles", "ms_forms", "ms_batch_tiles", "tune_s")}
+                           if t else dd_.schedule())
     del nuc
     torch.cuda.empty_cache()
     return {"hyb": hyb, "dapi": dapi}
@@ -430,6 +438,7 @@ def stream_leg(dev, cfg, pipes, dapi_pipe, args, rates):
 
     srv = HybServer(pipes, batch_tiles={"single": 36, "grid": 50}, slots=2, dapi=dapi_pipe)
     srv.run(make(np.zeros(4))[:4])   # warm the slots and the DAPI buffers
+    warm_copies(srv, list({id(t): t for chans, fl, nuc in pool.values() for t in list(chans) + [nuc]}.values()), dev)
     vox_per_req = [3 * cfg["sz"] * side * side for _, side, _, _ in plan]
     out = {"workload": f"{n} requests of 3 x 40 x H x W uint16 (2 hyb channels + DAPI), H = W uniform in "
                        "{2048, 2304, 2800}, single-PSF (tile 500) or 100-PSF keyed grid (tile 300) 50/50, FOV blocks "
@@ -643,10 +652,12 @@ def main():
         log("building the 100-PSF grid spectra (hyb and DAPI)")
         gpsfs = grid_psfs(10, 300)
         dapi_kw = dict(blur_radius=50, threshold=3.0, delta=5, delta_fit=5, sigmaZ=1, sigmaXY=1.5)
+        gtune = None if args.no_autotune else {"reps": 2, "cache": False}
         grid = HybPipeline.from_settings(gpsfs, (cfg["sz"], 3000, 3000), tile_size=300, overlap=cfg["overlap"],
-                                         beta=cfg["beta"], psf_mode="keyed", **hyb_kw)
+                                         beta=cfg["beta"], psf_mode="keyed", autotune=gtune, **hyb_kw)
         dapi_grid = DapiPipeline(Deconvolver(gpsfs, (cfg["sz"], 3000, 3000), tile_size=300, zpad=(cfg["sz"] - 1) // 2,
-                                             overlap=cfg["overlap"], beta=0.01, psf_mode="keyed"), **dapi_kw)
+                                             overlap=cfg["overlap"], beta=0.01, psf_mode="keyed",
+                                             autotune=gtune or False), **dapi_kw)
         dapi_one = DapiPipeline(Deconvolver(psf, shape, tile_size=cfg["tile_size"], zpad=(cfg["sz"] - 1) // 2,
                                             overlap=cfg["overlap"], beta=0.01), **dapi_kw)
         legs = {"multi_psf_grid": multi_psf_leg(dev, cfg, grid, dapi_grid)}
