@@ -129,10 +129,12 @@ def test_server_graph_replays_equal_alone(fovs, pipe, alone, dev):
 
 
 @pytest.mark.parametrize("host_inputs,batch_tiles,budget_fovs,cap_fovs",
-                         [(False, 5, None, None), (True, 7, 1.5, None), (True, 36, None, None), (True, 7, None, 1.5)])
+                         [(False, 5, None, None), (True, 7, 1.5, None), (True, 36, None, None), (True, 7, None, 1.5),
+                          (True, 7, None, 0.5)])
 def test_server_mix_equals_alone(fovs, pipe, alone, dev, host_inputs, batch_tiles, budget_fovs, cap_fovs):
     """Requests arriving while others run are admitted as memory frees up (within the HBM budget and the
-    in-flight cap) and batched together; every request's spots equal its channels run alone."""
+    input arena) and batched together; every request's spots equal its channels run alone.  An arena
+    smaller than a request (cap 0.5) gives each request memory of its own, one request at a time."""
     from mermake_amd.serve import FovRequest, HybServer
     data, _ = fovs
     reqs = []
