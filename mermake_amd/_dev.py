@@ -49,3 +49,18 @@ def shares_memory(a, b):
     a0, a1 = span(a)
     b0, b1 = span(b)
     return not (a1 < b0 or b1 < a0)
+
+
+def capture_graph(fn, stream):
+    """hipGraph of the device work that ``fn`` enqueues on the current stream, captured on ``stream`` (a
+    side stream with no work pending).  Unlike ``torch.cuda.graph`` this neither synchronises the device
+    nor empties torch's memory cache, so a serving loop can capture while other streams run; the capture
+    mode is thread-local, so other host threads may keep allocating and synchronising."""
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(stream):
+        g.capture_begin(capture_error_mode="thread_local")
+        try:
+            fn()
+        finally:
+            g.capture_end()
+    return g

@@ -17,7 +17,7 @@ own kernels, in several forms per pass (``Deconvolver.set_schedule``):
 the caller's channel shape (a seeded synthetic uint16 stack), keeps the fastest, then
 times the tiles per launch with it.  Every form computes the same deconvolution
 (relative L2 within 1e-5 of the float64 result); they differ in speed.  Results are
-cached per device, library version and geometry in a JSON file
+cached per device, library build and geometry in a JSON file
 (``MERMAKE_AMD_TUNE_CACHE``, default ``~/.cache/mermake_amd/schedule.json``), so a
 second process with the same geometry reads the schedule instead of timing it.
 """
@@ -38,7 +38,11 @@ def cache_path():
 
 def _key(d, channel_shape, ntiles, window):
     name = torch.cuda.get_device_name(d.device)
-    return "|".join([name, f"lib{L.lib().mm_version()}", f"sz{d.tile_height}", f"T{d._plan.T}", f"Zp{d._plan.Zp}",
+    try:
+        stamp = int(os.path.getmtime(L.LIBPATH))
+    except OSError:
+        stamp = 0
+    return "|".join([name, f"lib{L.lib().mm_version()}.{stamp}", f"sz{d.tile_height}", f"T{d._plan.T}", f"Zp{d._plan.Zp}",
                      f"npsf{'1' if d.psf_fft.shape[0] == 1 else 'n'}", f"tiles{ntiles}",
                      f"win{window[0]}-{window[1]}" if window else "win-"])
 

@@ -25,6 +25,7 @@ import numpy as np
 import torch
 
 from . import _lib as L
+from ._dev import capture_graph
 from .deconvolver import Deconvolver, prepare_flat, prepare_stack, tile_table
 from .maxima import find_local_maxima_signed
 
@@ -48,6 +49,7 @@ class HybPipeline:
         self._tables = {}
         self._ws = {}
         self._graphs = {}
+        self._cap_stream = None
 
     @classmethod
     def from_settings(cls, psfs, channel_shape, tile_size=500, overlap=89, beta=1e-4, zpad=None, psf_mode="zip",
@@ -186,10 +188,13 @@ class HybPipeline:
             self._keep_launch(tbuf, n, counts, fit, cap, xh, stats)
             st = stats.cpu()
             rows = xh[:int(st[1])].clone()
-            g = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(g, stream=torch.cuda.Stream(d.device), capture_error_mode="thread_local"):
+            if self._cap_stream is None:
+                self._cap_stream = torch.cuda.Stream(d.device)
+
+            def chain():
                 counts, fit = self._spots_batch(tbuf, n, single, cap)
                 self._keep_launch(tbuf, n, counts, fit, cap, xh, stats)
+            g = capture_graph(chain, self._cap_stream)
         finally:
             self._ws, d._ws = saved
         for k in [k for k in self._graphs if k[3] != d.generation]:

@@ -43,6 +43,7 @@ import numpy as np
 import torch
 
 from . import _lib as L
+from ._dev import capture_graph
 from .deconvolver import prepare_flat, prepare_stack, tile_records
 
 _REC = L.TILE_DTYPE.itemsize
@@ -184,6 +185,7 @@ class _Slot:
         self.device = device
         self.graphs = {} if graph else None
         self.replays = 0
+        self.cap_stream = torch.cuda.Stream(device) if graph else None
         self.stream = torch.cuda.Stream(device)
         spec = max(B_of[k] * p.d._plan.spec_bytes for k, p in pipes.items())
         res = max(B_of[k] * p.d.tile_height * p.d._plan.T ** 2 for k, p in pipes.items())
@@ -259,7 +261,7 @@ class _Slot:
         L.call("mm_tiles_keep", tab, n, d.overlap, L.ptr(self.fit), L.ptr(counts), self.cap, L.ptr(kept),
                L.ptr(self.xh), n * self.cap, L.ptr(total), None, st)
 
-    def launch(self, pipe, jobs):
+    def launch(self, pipe, jobs, capture=True):
         d = pipe.d
         n = len(jobs)
         T, sz = d._plan.T, d.tile_height
@@ -272,7 +274,8 @@ class _Slot:
         self.table_h[:n * _REC].copy_(torch.from_numpy(recs.view(np.uint8)))
         self.jobs, self.n, self.pipe, self.state = jobs, n, pipe, "stats"
         # one hipGraph per (pipeline, schedule, tiles): the first batch of a shape runs eagerly, the graph
-        # is captured after it and replayed by the next ones (the table is the graph's input)
+        # is captured after it and replayed by the next ones (the table is the graph's input); the server
+        # captures full batches only, so that partial batches of every size do not each add a graph
         key = (id(pipe), d.generation, n)
         g = None if self.graphs is None else self.graphs.get(key)
         with torch.cuda.stream(self.stream):
@@ -284,11 +287,8 @@ class _Slot:
                 self._chain(pipe, n)
             self.stats_h.copy_(self.stats, non_blocking=True)
             self.ev_stats.record(self.stream)
-        if self.graphs is not None and g is None:
-            g = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(g, capture_error_mode="thread_local"):
-                self._chain(pipe, n)
-            self.graphs[key] = g
+        if self.graphs is not None and g is None and capture:
+            self.graphs[key] = capture_graph(lambda: self._chain(pipe, n), self.cap_stream)
 
     def fetch_rows(self):
         """After the counts are back: copy exactly the kept rows (or report an overflowing tile)."""
@@ -316,7 +316,7 @@ class HybServer:
     ``pipe``: a ``HybPipeline``, or a dict {mode key: HybPipeline} (requests choose with ``psf=key``);
     ``batch_tiles``: tiles per launch, an int or a dict per mode; ``dapi``: a ``DapiPipeline`` for the
     requests' DAPI channels; ``drift_th``: the threshold of ``align.drift`` (``align.py:169``); ``graph``:
-    each slot replays its batches' kernel chain as a hipGraph, one per (mode, tiles in the batch);
+    each slot replays its full batches' kernel chain as a hipGraph, one per mode;
     ``inflight_cap``: bytes of the device arena that holds the host inputs of the requests in flight
     (allocated once, here, within the HBM budget).  Inputs admitted far ahead of the GPU only wait; and memory allocated fresh
     while the loop runs is cleared by the driver on the DMA engines that also copy the inputs, which
@@ -693,7 +693,7 @@ class HybServer:
                     if not batch:
                         continue
                     try:
-                        slot.launch(self.pipes[mode], batch)
+                        slot.launch(self.pipes[mode], batch, capture=len(batch) == self.B_of[mode])
                     except Exception:  # noqa: BLE001 - retried tile by tile; failing tiles fail their request
                         torch.cuda.synchronize(self.device)
                         slot.state = "free"
