@@ -400,12 +400,49 @@ extern "C" int mm_reflect(float* arr, int ndim, const long long* shape, const lo
 // ------------------------------------------------------------ RMS normalise
 // DAPI path of mermake/__main__.py:221-222: buffer /= ||buffer||_2 / sqrt(N),
 // with the norm accumulated in float64 and the divisor rounded to float32.
-__global__ __launch_bounds__(256) void k_sumsq(const float* __restrict__ a, long long n, double* __restrict__ part) {
+#define RMS_BLOCKS 2048
+// Element forms (any alignment).
+__global__ __launch_bounds__(256) void k_sumsq1(const float* __restrict__ a, long long n, double* __restrict__ part) {
   double s = 0.0;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     const double v = a[i];
     s += v * v;
   }
+  __shared__ double sh[256];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(256) void k_div_by1(float* __restrict__ a, long long n, const float* __restrict__ d) {
+  const float q = *d;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    a[i] = a[i] / q;
+}
+
+// Partial sums of squares (float64) per block: float4 loads when the array is
+// 16-byte aligned (n4 of them), then the ntail elements after them.
+__global__ __launch_bounds__(256) void k_sumsq(const float* __restrict__ a, long long n4, int ntail,
+                                               double* __restrict__ part) {
+  double s = 0.0;
+  const float4* a4 = (const float4*)a;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n4; i += (long long)gridDim.x * blockDim.x) {
+    const float4 v = a4[i];
+    const double x = v.x, y = v.y, z = v.z, w = v.w;
+    s += x * x;
+    s += y * y;
+    s += z * z;
+    s += w * w;
+  }
+  if (blockIdx.x == 0)
+    for (int i = threadIdx.x; i < ntail; i += blockDim.x) {
+      const double v = a[4 * n4 + i];
+      s += v * v;
+    }
   __shared__ double sh[256];
   sh[threadIdx.x] = s;
   __syncthreads();
@@ -430,23 +467,48 @@ __global__ __launch_bounds__(256) void k_rms_final(const double* __restrict__ pa
   if (threadIdx.x == 0) *rms = (float)(sqrt(sh[0]) / sqrt((double)n));
 }
 
-__global__ __launch_bounds__(256) void k_div_by(float* __restrict__ a, long long n, const float* __restrict__ d) {
+// a[i] / rms for the n4 float4 of the aligned part, then the ntail elements after them.
+__global__ __launch_bounds__(256) void k_div_by(float* __restrict__ a, long long n4, int ntail,
+                                                const float* __restrict__ d) {
   const float q = *d;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-    a[i] = a[i] / q;
+  float4* a4 = (float4*)a;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n4; i += (long long)gridDim.x * blockDim.x) {
+    float4 v = a4[i];
+    v.x = v.x / q;
+    v.y = v.y / q;
+    v.z = v.z / q;
+    v.w = v.w / q;
+    a4[i] = v;
+  }
+  if (blockIdx.x == 0)
+    for (int i = threadIdx.x; i < ntail; i += blockDim.x) a[4 * n4 + i] = a[4 * n4 + i] / q;
 }
 
-extern "C" size_t mm_rms_ws_bytes(void) { return 1024 * sizeof(double) + 256; }
+extern "C" size_t mm_rms_ws_bytes(void) { return RMS_BLOCKS * sizeof(double) + 256; }
 
 extern "C" int mm_rms_normalize(float* a, long long n, void* ws, float* rms_out, void* stream) {
   if (!a || !ws || !rms_out || n < 1) { mm_set_error("mm_rms_normalize: bad argument"); return MM_EINVAL; }
   hipStream_t st = (hipStream_t)stream;
   double* part = (double*)ws;
-  hipLaunchKernelGGL(k_sumsq, dim3(1024), dim3(256), 0, st, a, n, part);
+  // float4 loads for a 16-byte aligned array (the last n % 4 elements by block 0); an unaligned
+  // array of more than a few elements takes the element kernels
+  const bool al = ((uintptr_t)a & 15) == 0;
+  const long long n4 = al ? n / 4 : 0;
+  const long long rest = n - 4 * n4;
+  if (rest > 1024) {
+    hipLaunchKernelGGL(k_sumsq1, dim3(RMS_BLOCKS), dim3(256), 0, st, a, n, part);
+    MM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_rms_final, dim3(1), dim3(256), 0, st, part, RMS_BLOCKS, n, rms_out);
+    MM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_div_by1, dim3(4096), dim3(256), 0, st, a, n, rms_out);
+    MM_LAUNCH_CHECK();
+    return MM_OK;
+  }
+  hipLaunchKernelGGL(k_sumsq, dim3(RMS_BLOCKS), dim3(256), 0, st, a, n4, (int)rest, part);
   MM_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_rms_final, dim3(1), dim3(256), 0, st, part, 1024, n, rms_out);
+  hipLaunchKernelGGL(k_rms_final, dim3(1), dim3(256), 0, st, part, RMS_BLOCKS, n, rms_out);
   MM_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_div_by, dim3(4096), dim3(256), 0, st, a, n, rms_out);
+  hipLaunchKernelGGL(k_div_by, dim3(4096), dim3(256), 0, st, a, n4, (int)rest, rms_out);
   MM_LAUNCH_CHECK();
   return MM_OK;
 }

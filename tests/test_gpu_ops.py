@@ -456,3 +456,25 @@ def test_fixture_golden_tiles(dev):
         assert _rel_l2(mine, t[:, OV:OV + TS, OV:OV + TS]) < 1e-5, (x, y)
         exact, worst = golden_accept(mine[list(gz)], wins[(x, y)][2])
         assert exact > 0.995 and worst <= 1.0, (x, y, exact, worst)
+
+
+@pytest.mark.parametrize("n,offset", [(1, 0), (3, 0), (4096 * 37 + 3, 0), (4096 * 37 + 3, 1), (5000, 1),
+                                      (40 * 300 * 301, 0)])
+def test_rms_normalize_matches_float64_norm(dev, n, offset):
+    """mm_rms_normalize (__main__.py:221-222): the divisor is the float64 norm over sqrt(N) rounded to float32,
+    and every element is divided by it in float32, for 16-byte aligned arrays (float4 loads and a tail) and
+    unaligned ones (element kernels)."""
+    from mermake_amd import _lib as L
+    rng = np.random.default_rng(n + offset)
+    a = (rng.standard_normal(n) * 100).astype(np.float32)
+    buf = torch.zeros(n + offset, dtype=torch.float32, device=dev)
+    view = buf[offset:]
+    view.copy_(torch.from_numpy(a))
+    ws = torch.empty(L.lib().mm_rms_ws_bytes(), dtype=torch.uint8, device=dev)
+    rms = torch.empty(1, dtype=torch.float32, device=dev)
+    L.call("mm_rms_normalize", L.ptr(view), n, L.ptr(ws), L.ptr(rms), L.stream_ptr(dev))
+    want_rms = np.float32(np.sqrt(np.sum(a.astype(np.float64) ** 2)) / np.sqrt(np.float64(n)))
+    assert rms.item() == want_rms
+    assert np.array_equal(view.cpu().numpy(), a / want_rms)
+    if offset:
+        assert buf[0].item() == 0.0
