@@ -504,6 +504,8 @@ def main():
                     help="run the served stream at Poisson rates 0.5, 1, 2 and 4 FOV/s (about 3 minutes)")
     ap.add_argument("--no-legs", action="store_true",
                     help="skip the multi-PSF grid, uneven-batch + drift and served-stream legs")
+    ap.add_argument("--legs", default="multi_psf_grid,uneven_batch_drift,served_stream",
+                    help="the legs to run (comma list of multi_psf_grid, uneven_batch_drift, served_stream)")
     ap.add_argument("--no-feed", action="store_true", help="skip the host feed leg (zarr decode, pinned H2D)")
     ap.add_argument("--no-dapi", action="store_true", help="skip the DAPI stack leg")
     ap.add_argument("--no-autotune", action="store_true",
@@ -660,12 +662,17 @@ def main():
                                              autotune=gtune or False), **dapi_kw)
         dapi_one = DapiPipeline(Deconvolver(psf, shape, tile_size=cfg["tile_size"], zpad=(cfg["sz"] - 1) // 2,
                                             overlap=cfg["overlap"], beta=0.01), **dapi_kw)
-        legs = {"multi_psf_grid": multi_psf_leg(dev, cfg, grid, dapi_grid)}
+        which = set(args.legs.split(","))
+        legs = {}
+        if "multi_psf_grid" in which:
+            legs["multi_psf_grid"] = multi_psf_leg(dev, cfg, grid, dapi_grid)
         del dapi_grid
         torch.cuda.empty_cache()
-        legs["uneven_batch_drift"] = uneven_leg(dev, cfg, pipe, dapi_one)
+        if "uneven_batch_drift" in which:
+            legs["uneven_batch_drift"] = uneven_leg(dev, cfg, pipe, dapi_one)
         rates = [0.5, 1.0, 2.0, 4.0] if args.serve_sweep else [args.serve_rate]
-        legs["served_stream"] = stream_leg(dev, cfg, {"single": pipe, "grid": grid}, dapi_one, args, rates)
+        if "served_stream" in which:
+            legs["served_stream"] = stream_leg(dev, cfg, {"single": pipe, "grid": grid}, dapi_one, args, rates)
         del grid, dapi_one
         torch.cuda.empty_cache()
 

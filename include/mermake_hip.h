@@ -131,8 +131,10 @@ size_t mm_deconv_wiener_ws_bytes(const mm_deconv* d);
 /* W = conj(H)/(H conj(H) + beta |L|^2) / (Zp*T*T) as Zp planes of complex64
  * half spectra in the mm_deconv_spec_layout layout, from one centred PSF
  * (sz, tile_size, tile_size) float32 (deconvolver.py:60-74; the PSF FFT runs
- * in float64).  For stacks of up to 48 planes it holds the z-domain kernels
- * w = ifft_z(W) instead, and with zpad = sz - 1 (sz <= 40) the 240 planes of
+ * in float64).  What the planes hold follows the plan's z form
+ * (mm_deconv_set_schedule): MM_Z_FFT, W itself (Zp planes); MM_Z_MATRIX, the
+ * z-domain kernels w = ifft_z(W) and then W (2 Zp planes; W for batches with
+ * one PSF per tile); MM_Z_DFT80 (zpad = sz - 1, sz <= 40), the 240 planes of
  * the length-80 z-pass operator built from w (A = DFT80 of w on |m| < sz, G =
  * DFT80 of w[n] + w[n - 2(sz - 1)] on n < 2 sz - 1, both / 80; w[t] and
  * w[t - sz + 1] for t < 40). */

@@ -24,6 +24,7 @@
 // The 1/(Zp*T*T) normalisation of ifftn is folded into W.
 #include "deconv.h"
 #include "zfft80.h"
+#include "zfft78.h"
 #include "fft_pf.h"
 #include "fft_plan.h"
 #include "mermake_hip.h"
@@ -228,6 +229,75 @@ __global__ __launch_bounds__(256) void k_tile_zwiener(float2* S, long long s_tst
       }
     }
     __syncthreads();  // the item's last reads of o before the next item's stores into a
+  }
+}
+
+// Padded plane p -> data plane of the z map (host_zmap), or -1 for a zero plane:
+// data at [zpad, zpad + sz), the lower pad an 'in' reflection about zpad (a
+// reflection into the still-empty upper pad reads zero), the upper pad an 'out'
+// reflection about zpad + sz - 1 of what is there.
+__host__ __device__ constexpr int zmap_src(int sz, int zpad, int p) {
+  const int Zp = sz + 2 * zpad, top = zpad + sz - 1;
+  if (p >= zpad && p <= top) return p - zpad;
+  if (p < zpad) {
+    const int s = 2 * zpad - p;
+    return s >= Zp || s > top ? -1 : s - zpad;
+  }
+  const int s = 2 * top - p;
+  return s < 0 ? -1 : zmap_src(sz, zpad, s);
+}
+
+// z pass of the z FFT form for a tile of SZ planes padded by ZPAD to 78 (the DAPI
+// setting sz 40, zpad 19), with the length-78 transforms in registers: one thread
+// per column of one tile.  The padding planes are copies of data planes (the z map,
+// at compile time); W is read per (tile, column) in natural plane order, and the
+// tiles of one block of columns follow each other on one XCD, so a W that they
+// share comes from L2.
+template <int SZ, int ZPAD>
+__global__ __launch_bounds__(256, 2) void k_tile_zfft78(float2* S, long long s_tstride, int TH,
+                                                       const float2* __restrict__ W, long long w_stride,
+                                                       const mm_tile_dev* __restrict__ tiles, int ntiles) {
+  static_assert(SZ + 2 * ZPAD == 78, "length-78 z pass");
+  const int ncb = (TH + 255) / 256;
+  const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int j = (int)(bid % (unsigned)ntiles), cb = (int)(bid / (unsigned)ntiles);
+  const int c = cb * 256 + threadIdx.x;
+  if (cb >= ncb || c >= TH) return;
+  float2* col = S + (long long)j * s_tstride + c;
+  float2 x[SZ];
+  {
+    const float2* p = col;
+#pragma unroll
+    for (int s = 0; s < SZ; ++s) {
+      x[s] = *p;
+      p += TH;
+      asm volatile("" : "+v"(p));
+    }
+  }
+  float2 a[78];
+#pragma unroll
+  for (int pos = 0; pos < 78; ++pos) {
+    const int src = zmap_src(SZ, ZPAD, zf78::n_of(pos));
+    a[pos] = src >= 0 ? x[src] : make_float2(0.f, 0.f);
+  }
+  zf78::fft78<false>(a);
+  const float2* Wp = W + (long long)tiles[j].psf * w_stride + c;
+  // W 13 values at a time (a row of i3), so that its loads do not all wait in registers at once
+#pragma unroll
+  for (int r = 0; r < 6; ++r) {
+    float2 w[13];
+#pragma unroll
+    for (int i = 0; i < 13; ++i) w[i] = Wp[(long long)zf78::k_of(13 * r + i) * TH];
+#pragma unroll
+    for (int i = 0; i < 13; ++i) a[13 * r + i] = cmul(a[13 * r + i], w[i]);
+    asm volatile("" : "+v"(Wp));
+  }
+  zf78::fft78<true>(a);
+#pragma unroll
+  for (int t = 0; t < SZ; ++t) {
+    *col = a[zf78::pos_of_n(ZPAD + t)];
+    col += TH;
+    asm volatile("" : "+v"(col));
   }
 }
 
@@ -1623,9 +1693,10 @@ extern "C" int mm_deconv_create(int sz, int tile_size, int overlap, int zpad, mm
   d->zfft = d->zconv && zpad == sz - 1 && sz <= 40;
   d->xy_ok = (1 << MM_XY_STOCKHAM) | (d->pfM0 ? 1 << MM_XY_PRIME_FIRST : 0) | (d->rd ? 1 << MM_XY_RADER : 0);
   d->z_ok = (1 << MM_Z_FFT) | (d->zconv ? 1 << MM_Z_MATRIX : 0) | (d->zfft ? 1 << MM_Z_DFT80 : 0);
-  // default schedule: the fastest form of each pass the geometry admits
+  // default schedule: the fastest form of each pass the geometry admits (the z FFT form of the
+  // DAPI tile, sz 40 / zpad 19, runs its length-78 transforms in registers and beats the matrix)
   d->xy_form = d->rd ? MM_XY_RADER : (d->pfM0 ? MM_XY_PRIME_FIRST : MM_XY_STOCKHAM);
-  d->z_form = d->zfft ? MM_Z_DFT80 : (d->zconv ? MM_Z_MATRIX : MM_Z_FFT);
+  d->z_form = d->zfft ? MM_Z_DFT80 : (sz == 40 && zpad == 19) ? MM_Z_FFT : (d->zconv ? MM_Z_MATRIX : MM_Z_FFT);
   apply_schedule(d);
   *out = d;
   return MM_OK;
@@ -1827,12 +1898,20 @@ static int deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles,
     const long long nblk = (long long)((TH + G - 1) / G) * ngroups;
     const size_t lds = fft_lds_bytes(d->pZ, G, (size_t)8 * Zp * G + sizeof(int) * Zp);
     const float2* Wf = (const float2*)W + (d->zdual ? (long long)Zp * TH : 0);
-    // persistent blocks: a few rounds of the CUs' concurrent blocks
-    const long long bpc = lds ? (long long)(MM_LDS_MAX / lds) : 8;
-    const long long cap = 256 * (bpc < 1 ? 1 : (bpc > 8 ? 8 : bpc)) * 4;
-    const long long nlaunch = nblk < cap ? nblk : cap;
-    hipLaunchKernelGGL(k_tile_zwiener, dim3((unsigned)nlaunch), dim3(256), lds, st, S, s_ts, TH, sz, d->zpad,
-                       d->zmap_dev, Wf, (long long)(d->zdual ? 2 : 1) * Zp * TH, td, ntiles, tpb, d->pZ, G);
+    const long long wstride = (long long)(d->zdual ? 2 : 1) * Zp * TH;  // planes of one PSF's W
+    if (sz == 40 && d->zpad == 19) {  // the DAPI tile: the length-78 transforms in registers
+      const long long nb = (long long)((TH + 255) / 256) * ntiles;
+      if (nb > 0x7fffffffLL) { mm_set_error("z pass: too many blocks"); return MM_EINVAL; }
+      hipLaunchKernelGGL((k_tile_zfft78<40, 19>), dim3((unsigned)nb), dim3(256), 0, st, S, s_ts, TH, Wf, wstride, td,
+                         ntiles);
+    } else {
+      // persistent blocks: a few rounds of the CUs' concurrent blocks
+      const long long bpc = lds ? (long long)(MM_LDS_MAX / lds) : 8;
+      const long long cap = 256 * (bpc < 1 ? 1 : (bpc > 8 ? 8 : bpc)) * 4;
+      const long long nlaunch = nblk < cap ? nblk : cap;
+      hipLaunchKernelGGL(k_tile_zwiener, dim3((unsigned)nlaunch), dim3(256), lds, st, S, s_ts, TH, sz, d->zpad,
+                         d->zmap_dev, Wf, wstride, td, ntiles, tpb, d->pZ, G);
+    }
     MM_LAUNCH_CHECK();
   }
   MM_MARK(3);

@@ -7,10 +7,12 @@
 * ``psf_fft`` holds the half spectra ``(n_psf, Zp, T, T//2+1)`` (complex64),
   already divided by ``Zp*T*T`` (the ``ifftn`` normalisation), instead of the
   full ``(n_psf, Zp, T, T)`` spectra.  The PSF FFT behind it runs in float64.
-  Stacks of up to 48 planes hold the z-domain kernels ``ifft_z`` of those
-  spectra (with ``zpad = sz - 1``, sz <= 40: the 240 planes of the length-80
-  z-pass operator), and tiles T = 113*M hold them column-blocked,
-  ``(n_psf, planes, Hs/blk, T, blk)`` (``mm_deconv_spec_layout``).
+  By default stacks of up to 48 planes hold the z-domain kernels ``ifft_z`` of
+  those spectra followed by the spectra (with ``zpad = sz - 1``, sz <= 40: the
+  240 planes of the length-80 z-pass operator; 40 planes with zpad 19 keep the
+  spectra alone, for the length-78 z transforms), and tiles T = 113*M hold
+  them column-blocked, ``(n_psf, planes, Hs/blk, T, blk)``
+  (``mm_deconv_spec_layout``; ``set_schedule`` picks other forms).
 * ``tile_wise`` computes tiles in batches; the yielded views stay valid until
   the generator computes the next batch (the reference overwrote them at every
   step).  ``tile_pad`` / ``tile_res`` are shape-only (``meta``) tensors.

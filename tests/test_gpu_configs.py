@@ -258,13 +258,14 @@ def test_multi_psf_grid_dapi_deconvolution(dev, dapi_stack):
 
 
 def test_multi_psf_grid_dapi_own_psf_deconvolution(dev, dapi_stack):
-    """The DAPI pass with the 3x3 PSF grid (one PSF per tile, zpad 19, Zp 78): the batches of own-PSF tiles take
-    the z FFT form of the z pass (frequency-domain W), against the float64 oracle."""
+    """The DAPI pass with the 3x3 PSF grid (one PSF per tile, zpad 19, Zp 78): the z FFT form of the z pass
+    (frequency-domain W, 78 planes; its length-78 transforms in registers), against the float64 oracle."""
     from mermake_amd.deconvolver import Deconvolver
     raw, _ = dapi_stack
     psfs = _grid_psfs()
     d = Deconvolver(psfs, raw.shape, tile_size=300, zpad=19, overlap=89, beta=0.01)
-    assert (d._plan.T, d._plan.Zp, d._plan.wplanes, d.psf_fft.shape[0]) == (478, 78, 156, 9)
+    assert (d._plan.T, d._plan.Zp, d._plan.wplanes, d.psf_fft.shape[0]) == (478, 78, 78, 9)
+    assert d.schedule()["z"] == "fft"
     W = odec.wiener(psfs, 40, 300, 19, 89, 0.01, mode="fp64")
     only = {(0, 0), (300, 300), (600, 0), (600, 600)}
     assert_deconv_parity(d, T(raw, dev), None, raw, None, W, 300, 89, 19, only, blur_radius=50,

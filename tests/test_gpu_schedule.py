@@ -37,6 +37,8 @@ GEOMS = [
     ((4, 70, 70), 50, 8, 2, 2, True, {"stockham", "prime_first"}, {"fft", "matrix"}),
     ((40, 120, 120), 80, 10, 39, 1, False, {"stockham"}, {"fft", "matrix", "dft80"}),
     ((50, 140, 140), 100, 20, 3, 1, False, {"stockham"}, {"fft"}),
+    ((40, 260, 260), 200, 13, 19, 1, True, {"stockham", "prime_first", "rader"}, {"fft", "matrix"}),
+    ((40, 260, 260), 200, 13, 19, 2, False, {"stockham", "prime_first", "rader"}, {"fft", "matrix"}),
 ]
 
 
@@ -44,7 +46,9 @@ GEOMS = [
 def test_every_schedule_vs_oracle(dev, shape, ts, ov, zpad, npsf, flat, xy_forms, z_forms):
     """T = 226 = 113 * 2 and T = 678 = 113 * 6 (Rader, prime-first, Stockham), T = 478 = 239 * 2,
     T = 66 = 11 * 6, T = 100 and T = 140 (Stockham only); 40 planes with zpad = sz - 1 (all three z
-    forms), 3 and 4 planes (no length-80 pass), 50 planes (above the z-kernel matrix's 48)."""
+    forms), 3 and 4 planes (no length-80 pass), 50 planes (above the z-kernel matrix's 48); 40 planes
+    with zpad 19 (Zp 78, the DAPI setting: the z FFT form runs its length-78 transforms in registers),
+    one PSF and one per tile."""
     from mermake_amd.deconvolver import Deconvolver
     img, psfs, fl = _case(shape, npsf, flat)
     d = Deconvolver(psfs, shape, tile_size=ts, zpad=zpad, overlap=ov, beta=1e-3)
