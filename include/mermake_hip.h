@@ -199,6 +199,12 @@ int mm_fft3d_c2c(void* data, int n0, int n1, int n2, int axes_mask, int inverse,
 size_t mm_maxima_ws_bytes(int D, int H, int W, int nvol);
 int mm_local_maxima(const float* img, int D, int H, int W, long long sZ, long long sX, float sign, float threshold,
                     int delta, int* coords, int cap, int* count, void* ws, size_t ws_bytes, void* stream);
+/* mm_local_maxima of sign +1 and of sign -1 in one pass over the volume: the
+ * two find_local_maxima calls of the DAPI block on +buffer and -buffer
+ * (mermake/__main__.py:224-225).  coords: (2, cap, 3) int32, the + maxima then
+ * the - maxima; counts: 2 device ints.  ws: mm_maxima_ws_bytes(D, H, W, 2). */
+int mm_local_maxima_pm(const float* img, int D, int H, int W, long long sZ, long long sX, float threshold, int delta,
+                       int* coords, int cap, int* counts, void* ws, size_t ws_bytes, void* stream);
 /* mm_local_maxima without a workspace argument and without any allocation: the
  * row masks and counts live in a library-owned device array of
  * MM_MAXIMA_SCRATCH_BYTES bytes (one per GPU, held until the process ends).

@@ -56,6 +56,7 @@ _SIGS = {
     "mm_rms_normalize": (_i, [_vp, _ll, _vp, _vp, _vp]),
     "mm_maxima_ws_bytes": (_sz, [_i, _i, _i, _i]),
     "mm_local_maxima": (_i, [_vp, _i, _i, _i, _ll, _ll, _f, _f, _i, _vp, _i, _vp, _vp, _sz, _vp]),
+    "mm_local_maxima_pm": (_i, [_vp, _i, _i, _i, _ll, _ll, _f, _i, _vp, _i, _vp, _vp, _sz, _vp]),
     "mm_local_maxima_noalloc": (_i, [_vp, _i, _i, _i, _ll, _ll, _f, _f, _i, _vp, _i, _vp, _vp]),
     "mm_delta_fit": (_i, [_vp, _i, _i, _i, _ll, _ll, _f, _vp, _i, _ll, _ll, _vp, _vp, _i, _i, _f, _f, _vp, _vp]),
     "mm_tiles_maxima": (_i, [_vp, _vp, _i, _vp, _f, _i, _vp, _i, _vp, _vp, _sz, _vp]),

@@ -228,6 +228,73 @@ __global__ __launch_bounds__(256) void k_max_mark(VS vs, float sign, float thr, 
   if (lane == 0) rcnt[ridx] = cnt;
 }
 
+// Both signs of k_max_mark in one pass over the volume (the DAPI maxima of
+// +buffer and -buffer, __main__.py:224-225): the centre values are read once,
+// candidates of +v go to the workspace's volume 0 and those of -v to volume 1.
+template <class VS>
+__global__ __launch_bounds__(256) void k_max_mark_pm(VS vs, float thr, int delta,
+                                                     unsigned long long* __restrict__ masks,
+                                                     int* __restrict__ rcnt, int nrow_max, int hl, int ww) {
+  const VolDesc V = vs(0);
+  const int lane = threadIdx.x & 63;
+  __shared__ int soff[MX_SPH_N];
+  const bool sph = delta >= 2 && delta <= MX_SPH_DMAX;
+  const int nsph = sph ? kSph.n[delta] : 0;
+  for (int i = threadIdx.x; i < nsph; i += blockDim.x)
+    soff[i] = (int)(kSph.d[i][0] * V.sZ + kSph.d[i][1] * V.sX + kSph.d[i][2]);
+  if (sph) __syncthreads();
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= nrow_max) return;
+  int cntp = 0, cntm = 0;
+  const int z = row / hl, x = row - (row / hl) * hl;
+  if (z < V.D && x < V.H) {
+    const float* rp = V.p + z * V.sZ + x * V.sX;
+    const bool zx_in = z >= delta && z < V.D - delta && x >= delta && x < V.H - delta;  // uniform
+    unsigned long long* mwp = masks + (long long)row * ww;
+    unsigned long long* mwm = masks + ((long long)nrow_max + row) * ww;
+    for (int k0 = 0; k0 < ww; k0 += MX_CHUNK) {
+      float c[MX_CHUNK];
+#pragma unroll
+      for (int k = 0; k < MX_CHUNK; ++k) {
+        const int y = (k0 + k) * 64 + lane;
+        c[k] = y < V.W ? rp[y] : 0.0f;
+      }
+#pragma unroll
+      for (int k = 0; k < MX_CHUNK; ++k) {
+        if (k0 + k >= ww) break;
+        const int y = (k0 + k) * 64 + lane;
+        const bool in = y < V.W;
+        bool mp = in && !(c[k] <= thr);   // maxima.cu:19, for +v and for -v
+        bool mm = in && !(-c[k] <= thr);
+        const bool interior = zx_in && y >= delta && y < V.W - delta;
+        if (mp) {
+          if (delta == 1) mp = is_max_d1(V, z, x, y, c[k], 1.0f);
+          else if (sph) mp = interior ? is_max_sph_in(rp + y, soff, nsph, c[k], 1.0f)
+                                      : is_max_sph_border(V, z, x, y, c[k], 1.0f, nsph);
+          else mp = is_max_at(V, z, x, y, c[k], 1.0f, delta);
+        }
+        if (mm) {
+          if (delta == 1) mm = is_max_d1(V, z, x, y, -c[k], -1.0f);
+          else if (sph) mm = interior ? is_max_sph_in(rp + y, soff, nsph, -c[k], -1.0f)
+                                      : is_max_sph_border(V, z, x, y, -c[k], -1.0f, nsph);
+          else mm = is_max_at(V, z, x, y, -c[k], -1.0f, delta);
+        }
+        const unsigned long long bp = __ballot(mp), bm = __ballot(mm);
+        if (lane == 0) {
+          mwp[k0 + k] = bp;
+          mwm[k0 + k] = bm;
+        }
+        cntp += __popcll(bp);
+        cntm += __popcll(bm);
+      }
+    }
+  }
+  if (lane == 0) {
+    rcnt[row] = cntp;
+    rcnt[nrow_max + row] = cntm;
+  }
+}
+
 // The same marks from threshold bits already in the masks (written by the blur
 // that produced the volume: bit y of row r set iff !(sign v <= thr)): one thread
 // per row turns every non-zero word into its maxima word in place, so only the
@@ -671,6 +738,37 @@ extern "C" int mm_local_maxima(const float* img, int D, int H, int W, long long 
   OneVol vs{VolDesc{img, D, H, W, sZ, sX, 0, 0x7fffffff}};
   return run_maxima(vs, 1, D, H, W, sign, threshold, delta, coords, cap, count, ws, ws_bytes,
                     (hipStream_t)stream);
+}
+
+extern "C" int mm_local_maxima_pm(const float* img, int D, int H, int W, long long sZ, long long sX,
+                                  float threshold, int delta, int* coords, int cap, int* counts, void* ws,
+                                  size_t ws_bytes, void* stream) {
+  if (!img || !coords || !counts || !ws || D < 0 || H < 0 || W < 0 || cap < 0) {
+    mm_set_error("mm_local_maxima_pm: bad argument");
+    return MM_EINVAL;
+  }
+  if (delta < 0 || delta > 31) { mm_set_error("delta %d outside [0, 31]", delta); return MM_EINVAL; }
+  if (ws_bytes < mm_maxima_ws_bytes(D, H, W, 2)) { mm_set_error("maxima workspace too small"); return MM_EINVAL; }
+  hipStream_t st = (hipStream_t)stream;
+  OneVol vs{VolDesc{img, D, H, W, sZ, sX, 0, 0x7fffffff}};
+  const int nrow = D * H;
+  const int ww = W > 0 ? (W + 63) / 64 : 0;
+  unsigned long long* masks = (unsigned long long*)ws;
+  int* rcnt = (int*)(masks + (size_t)2 * nrow * ww);
+  if (nrow == 0 || ww == 0) {  // empty volume: zero counts without a memset node
+    hipLaunchKernelGGL(k_max_scan, dim3(2), dim3(1024), 0, st, rcnt, 0, counts);
+    MM_LAUNCH_CHECK();
+    return MM_OK;
+  }
+  hipLaunchKernelGGL((k_max_mark_pm<OneVol>), dim3((unsigned)((nrow + 3) / 4)), dim3(256), 0, st, vs, threshold,
+                     delta, masks, rcnt, nrow, H, ww);
+  MM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_max_scan, dim3(2), dim3(1024), 0, st, rcnt, nrow, counts);
+  MM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_max_compact, dim3((nrow + 255) / 256, 2), dim3(256), 0, st, masks, rcnt, counts, nrow, H, ww,
+                     coords, cap);
+  MM_LAUNCH_CHECK();
+  return MM_OK;
 }
 
 extern "C" int mm_delta_fit(const float* img, int D, int H, int W, long long sZ, long long sX, float sign,

@@ -110,3 +110,36 @@ def find_local_maxima_signed(image, raw, threshold, delta, delta_fit, sigmaZ, si
            raw_type(raw), raw.stride(0), raw.stride(1), L.ptr(coords), L.ptr(count), n,
            int(delta_fit), float(sigmaZ), float(sigmaXY), L.ptr(out), L.stream_ptr(image.device))
     return out
+
+
+def find_local_maxima_pm(image, raw, threshold, delta, delta_fit, sigmaZ, sigmaXY):
+    """``(find_local_maxima_signed(..., sign=+1), find_local_maxima_signed(..., sign=-1))`` with the
+    maxima of both signs from one pass over the volume (``mm_local_maxima_pm``) and one host
+    synchronisation for both counts."""
+    raw = _raw_view(raw, image.shape)
+    D, H, W = image.shape
+    dev = image.device
+    st = L.stream_ptr(dev)
+    ws_bytes = L.lib().mm_maxima_ws_bytes(D, H, W, 2)
+    ws = _workspace(dev, ws_bytes)
+    counts = torch.zeros(2, dtype=torch.int32, device=dev)
+    cap = max(1, int(min(D * H * W, 1 << 16)))
+    while True:
+        coords = torch.empty((2, cap, 3), dtype=torch.int32, device=dev)
+        L.call("mm_local_maxima_pm", L.ptr(image), D, H, W, image.stride(0), image.stride(1), float(threshold),
+               int(delta), L.ptr(coords), cap, L.ptr(counts), L.ptr(ws), ws.numel(), st)
+        n = counts.tolist()
+        if max(n) <= cap:
+            break
+        cap = max(n)
+    out = []
+    for v, sign in ((0, 1.0), (1, -1.0)):
+        if n[v] == 0:
+            out.append(torch.zeros((0, 8), dtype=torch.float32, device=dev))
+            continue
+        rows = torch.zeros((n[v], 8), dtype=torch.float32, device=dev)
+        L.call("mm_delta_fit", L.ptr(image), D, H, W, image.stride(0), image.stride(1), sign, L.ptr(raw),
+               raw_type(raw), raw.stride(0), raw.stride(1), L.ptr(coords[v]), L.ptr(counts[v:v + 1]), n[v],
+               int(delta_fit), float(sigmaZ), float(sigmaXY), L.ptr(rows), st)
+        out.append(rows)
+    return out[0], out[1]

@@ -18,8 +18,8 @@ sampled voxels instead of materialising the padded tile.
 one tile at a time, host syncs after the maxima and after the fit.
 
 ``DapiPipeline`` is ``__main__.py:217-225``: ``apply`` with the DAPI settings,
-RMS normalisation, maxima of +buffer and of -buffer (the negation is folded
-into the kernels' sign).
+RMS normalisation, maxima of +buffer and of -buffer (both signs in one pass
+over the volume; the negation is folded into the kernel).
 """
 import numpy as np
 import torch
@@ -27,7 +27,7 @@ import torch
 from . import _lib as L
 from ._dev import capture_graph
 from .deconvolver import Deconvolver, prepare_flat, prepare_stack, tile_table
-from .maxima import find_local_maxima_signed
+from .maxima import find_local_maxima_pm
 
 
 class HybPipeline:
@@ -324,7 +324,5 @@ class DapiPipeline:
         ws = torch.empty(L.lib().mm_rms_ws_bytes(), dtype=torch.uint8, device=d.device)
         rms = torch.empty(1, dtype=torch.float32, device=d.device)
         L.call("mm_rms_normalize", L.ptr(buf), buf.numel(), L.ptr(ws), L.ptr(rms), L.stream_ptr(d.device))
-        raw = img
-        plus = find_local_maxima_signed(buf, raw, sign=1.0, **self.kw)
-        minus = find_local_maxima_signed(buf, raw, sign=-1.0, **self.kw)
-        return plus, minus
+        # the maxima of +buffer and -buffer (__main__.py:224-225) from one pass over the volume
+        return find_local_maxima_pm(buf, img, **self.kw)

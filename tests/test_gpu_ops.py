@@ -478,3 +478,25 @@ def test_rms_normalize_matches_float64_norm(dev, n, offset):
     assert np.array_equal(view.cpu().numpy(), a / want_rms)
     if offset:
         assert buf[0].item() == 0.0
+
+
+@pytest.mark.parametrize("shape,delta,thr", [((12, 40, 70), 5, 3.0), ((10, 33, 130), 1, 2.0), ((9, 20, 64), 2, -0.5),
+                                             ((6, 6, 6), 5, 0.0), ((1, 1, 1), 1, -1.0)])
+def test_maxima_both_signs_in_one_pass(dev, shape, delta, thr):
+    """find_local_maxima_pm (mm_local_maxima_pm) gives the + and - maxima and fits of the two signed calls
+    bit for bit, with the candidate sets of the oracle; a negative threshold makes voxels candidates of
+    both signs."""
+    from mermake_amd.maxima import find_local_maxima_pm, find_local_maxima_signed
+    rng = np.random.default_rng(shape[2] + delta)
+    img = (rng.standard_normal(shape) * 2).astype(np.float32)
+    raw = rng.integers(0, 4000, shape).astype(np.uint16)
+    kw = dict(threshold=thr, delta=delta, delta_fit=3, sigmaZ=1.0, sigmaXY=1.5)
+    ti, tr = T(img, dev), T(raw, dev)
+    plus, minus = find_local_maxima_pm(ti, tr, **kw)
+    p1 = find_local_maxima_signed(ti, tr, sign=1.0, **kw)
+    m1 = find_local_maxima_signed(ti, tr, sign=-1.0, **kw)
+    for a, b in ((plus, p1), (minus, m1)):   # (the correlations of a flat neighbourhood are NaN)
+        assert np.array_equal(a.cpu().numpy(), b.cpu().numpy(), equal_nan=True)
+    for sgn, got in ((1.0, plus), (-1.0, minus)):
+        z, x, y = omax.local_maxima(sgn * img, thr, delta)
+        assert got.shape[0] == len(z)
