@@ -79,14 +79,12 @@ def pass_bytes(geom, sz, T, wplanes, window=(None, None), single_psf=True, flat=
 
 
 def deconv_window(pipe, T):
-    """Rows and columns [lo, hi) of the deconvolved tiles the hyb pipeline computes (the blur window grown by the
-    blur half width, in whole C2R row pairs), or (None, None)."""
-    win = pipe.windows()
+    """Rows and columns [lo, hi) of the deconvolved tiles the hyb pipeline computes (the blur window, grown by the
+    blur half width unless the blur rides on the operator, in whole C2R row pairs), or (None, None)."""
+    win = pipe.deconv_window()
     if win is None:
         return None, None
-    d = pipe.blur_radius // 2
-    lo, hi = max(0, win[0][0] - d) & ~1, min(T, (win[0][1] + d + 1) & ~1)
-    return lo, hi
+    return win[0] & ~1, min(T, (win[1] + 1) & ~1)
 
 
 PASS_KERNEL = {"r2c": "k_rd_r2c", "x_fwd": "k_rd_c2c<false", "z_wiener": "k_tile_zfft", "x_inv": "k_rd_c2c<true",

@@ -140,6 +140,17 @@ size_t mm_deconv_wiener_ws_bytes(const mm_deconv* d);
  * w[t - sz + 1] for t < 40). */
 int mm_deconv_build_wiener(mm_deconv* d, const float* psf_centered, double beta, void* W_out, void* ws,
                            size_t ws_bytes, void* stream);
+/* mm_deconv_build_wiener with the blur subtraction of deconvolver.py:160-163
+ * (blur.py box_1d of half-width box_half along x, then along y, subtracted)
+ * folded into W: W(kz, kx, ky) * (1 - b(kx) b(ky)), b(k) = the transfer
+ * function of the (2 box_half + 1)-tap mean on the T-periodic tile, in
+ * float64.  A tile deconvolved with it holds res - box_y(box_x(res)) at
+ * every voxel whose box stays inside the tile (rows and columns [box_half,
+ * T - box_half)): exactly there in real arithmetic, and within float32
+ * rounding of the reference's sequential sums.  box_half = 0 is
+ * mm_deconv_build_wiener.  0 <= 2 box_half + 1 <= T. */
+int mm_deconv_build_wiener_box(mm_deconv* d, const float* psf_centered, double beta, int box_half, void* W_out,
+                               void* ws, size_t ws_bytes, void* stream);
 /* bytes of the per-tile half-spectrum workspace (sz planes of T*Hs complex64) */
 size_t mm_deconv_spec_bytes(const mm_deconv* d);
 /* Replaces the FFT part of Deconvolver.tile_wise (deconvolver.py:116-157) for

@@ -42,6 +42,7 @@ _SIGS = {
     "mm_deconv_set_schedule": (_i, [_vp, _i, _i]),
     "mm_deconv_wiener_ws_bytes": (_sz, [_vp]),
     "mm_deconv_build_wiener": (_i, [_vp, _vp, _d, _vp, _vp, _sz, _vp]),
+    "mm_deconv_build_wiener_box": (_i, [_vp, _vp, _d, _i, _vp, _vp, _sz, _vp]),
     "mm_deconv_spec_bytes": (_sz, [_vp]),
     "mm_deconv_tiles": (_i, [_vp, _vp, _i, _vp, _i, _i, _vp, _vp, _vp]),
     "mm_deconv_tiles_window": (_i, [_vp, _vp, _i, _vp, _i, _i, _vp, _vp, _i, _i, _vp]),

@@ -1169,7 +1169,21 @@ __global__ void k_psf_place(const float* __restrict__ psf, int sz, int ts, int z
 // Laplacian of deconvolver.py:24-36 (6 at the centre, -1 at z+-1, x+-1 and only
 // y-1 on the last axis): |L|^2 = (6 - 2cos a - 2cos b - cos c)^2 + sin^2 c.
 // Computed in float64, in place over the PSF spectrum.
-__global__ void k_wiener64(double2* __restrict__ Hs, int Zp, int T, int H, double beta, double scale) {
+//
+// box_half = D > 0 also multiplies W by 1 - b(kx) b(ky), the transfer function
+// of "subtract the separable box of width 2D + 1" (blur.py box_1d along x,
+// then y, deconvolver.py:160-163) on the T-periodic tile:
+//   b(k) = (1 / (2D + 1)) sum_{m=-D..D} cos(2 pi k m / T)
+//        = sin(pi (2D + 1) k / T) / ((2D + 1) sin(pi k / T)),  b(0) = 1.
+// The circular box equals the reference's clipped box at every voxel whose
+// box stays inside the tile, so the inverse transform gives res - blur there.
+__device__ __forceinline__ double box_tf(int k, int T, int D) {
+  if (k == 0) return 1.0;
+  const double n = 2.0 * D + 1.0;
+  return sinpi(n * k / T) / (n * sinpi((double)k / T));
+}
+
+__global__ void k_wiener64(double2* __restrict__ Hs, int Zp, int T, int H, double beta, double scale, int box_half) {
   const long long total = (long long)Zp * T * H;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
        i += (long long)gridDim.x * blockDim.x) {
@@ -1180,7 +1194,8 @@ __global__ void k_wiener64(double2* __restrict__ Hs, int Zp, int T, int H, doubl
     const double re = 6.0 - 2.0 * cospi(2.0 * kz / Zp) - 2.0 * cospi(2.0 * kx / T) - cospi(2.0 * ky / T);
     const double im = sinpi(2.0 * ky / T);
     const double den = h.x * h.x + h.y * h.y + beta * (re * re + im * im);
-    Hs[i] = make_double2(h.x / den * scale, -h.y / den * scale);
+    const double sc = box_half > 0 ? scale * (1.0 - box_tf(kx, T, box_half) * box_tf(ky, T, box_half)) : scale;
+    Hs[i] = make_double2(h.x / den * sc, -h.y / den * sc);
   }
 }
 
@@ -1762,9 +1777,10 @@ static int launch_c2c(C* data, long long estride, long long cstride, int ncols, 
   return MM_OK;
 }
 
-extern "C" int mm_deconv_build_wiener(mm_deconv* d, const float* psf_centered, double beta, void* W_out, void* ws,
-                                      size_t ws_bytes, void* stream) {
+static int build_wiener(mm_deconv* d, const float* psf_centered, double beta, int box_half, void* W_out, void* ws,
+                        size_t ws_bytes, void* stream) {
   if (!d || !psf_centered || !W_out || !ws) { mm_set_error("mm_deconv_build_wiener: null argument"); return MM_EINVAL; }
+  if (box_half < 0 || 2 * box_half + 1 > d->T) { mm_set_error("mm_deconv_build_wiener_box: box half-width %d", box_half); return MM_EINVAL; }
   if (ws_bytes < mm_deconv_wiener_ws_bytes(d)) { mm_set_error("mm_deconv_build_wiener: workspace too small"); return MM_EINVAL; }
   hipStream_t st = (hipStream_t)stream;
   const int T = d->T, Zp = d->Zp, H = d->H;
@@ -1785,7 +1801,7 @@ extern "C" int mm_deconv_build_wiener(mm_deconv* d, const float* psf_centered, d
   if ((rc = launch_c2c<false>(Hs, H, 1, H, (long long)T * H, Zp, d->pT64, d->gT64_x, 1.0, st))) return rc;
   if ((rc = launch_c2c<false>(Hs, (long long)T * H, 1, T * H, 0, 1, d->pZ64, d->gZ64, 1.0, st))) return rc;
   const double scale = 1.0 / ((double)Zp * T * T);
-  hipLaunchKernelGGL(k_wiener64, dim3(4096), dim3(256), 0, st, Hs, Zp, T, H, beta, scale);
+  hipLaunchKernelGGL(k_wiener64, dim3(4096), dim3(256), 0, st, Hs, Zp, T, H, beta, scale, box_half);
   MM_LAUNCH_CHECK();
   if (d->zdual) {  // W itself into planes [Zp, 2 Zp), for the z FFT form of one-PSF-per-tile batches
     const long long wplane = d->blk ? (long long)T * d->Hs : (long long)T * H;
@@ -1814,6 +1830,16 @@ extern "C" int mm_deconv_build_wiener(mm_deconv* d, const float* psf_centered, d
   }
   MM_LAUNCH_CHECK();
   return MM_OK;
+}
+
+extern "C" int mm_deconv_build_wiener(mm_deconv* d, const float* psf_centered, double beta, void* W_out, void* ws,
+                                      size_t ws_bytes, void* stream) {
+  return build_wiener(d, psf_centered, beta, 0, W_out, ws, ws_bytes, stream);
+}
+
+extern "C" int mm_deconv_build_wiener_box(mm_deconv* d, const float* psf_centered, double beta, int box_half,
+                                          void* W_out, void* ws, size_t ws_bytes, void* stream) {
+  return build_wiener(d, psf_centered, beta, box_half, W_out, ws, ws_bytes, stream);
 }
 
 extern "C" int mm_deconv_spec_layout(const mm_deconv* d, int* blk, int* Hs, int* wplanes) {

@@ -253,22 +253,49 @@ class Deconvolver:
             from .autotune import tune
             tune(self, channel_shape, **(autotune if isinstance(autotune, dict) else {}))
 
-    def _build_spectra(self):
-        """The Wiener operators of every PSF in the layout of the plan's schedule."""
+    def _operators(self, box_half=0):
+        """The Wiener operators of every PSF in the layout of the plan's schedule; ``box_half > 0``
+        folds the subtraction of the box blur of that half-width into them
+        (``mm_deconv_build_wiener_box``)."""
         psfs = list(self._psfs.values()) if isinstance(self._psfs, dict) else [self._psfs]
-        self.psf_fft = None   # the operators of the previous schedule go before the new ones come
-        self.psf_fft = torch.empty(self._plan.spectrum_shape(len(psfs)), dtype=torch.complex64, device=self.device)
+        out = torch.empty(self._plan.spectrum_shape(len(psfs)), dtype=torch.complex64, device=self.device)
         ws_bytes = L.lib().mm_deconv_wiener_ws_bytes(self._plan.h)
         ws = torch.empty(ws_bytes, dtype=torch.uint8, device=self.device)
         st = L.stream_ptr(self.device)
         for i, p in enumerate(psfs):
             c = self.center_psf(np.asarray(p))
             pd = torch.from_numpy(np.ascontiguousarray(c, dtype=np.float32)).to(self.device)
-            L.call("mm_deconv_build_wiener", self._plan.h, L.ptr(pd), float(self.beta), L.ptr(self.psf_fft[i]),
-                   L.ptr(ws), ws_bytes, st)
+            L.call("mm_deconv_build_wiener_box", self._plan.h, L.ptr(pd), float(self.beta), int(box_half),
+                   L.ptr(out[i]), L.ptr(ws), ws_bytes, st)
         torch.cuda.current_stream(self.device).synchronize()
         del ws
+        return out
+
+    def _build_spectra(self):
+        """The Wiener operators of every PSF in the layout of the plan's schedule."""
+        self.psf_fft = None   # the operators of the previous schedule go before the new ones come
+        self._box_ops = {}
+        self.psf_fft = self._operators()
         self.generation += 1
+
+    def box_spectra(self, blur_radius):
+        """``psf_fft`` with the blur subtraction of ``blur_radius`` (box half-width ``blur_radius // 2``)
+        folded in, built on first use and kept with the operators of the current schedule."""
+        half = int(blur_radius) // 2
+        ops = self._box_ops.get(half)
+        if ops is None:
+            ops = self._box_ops[half] = self._operators(half)
+        return ops
+
+    def spectral_blur(self, blur_radius, window):
+        """Whether the blur subtraction of ``blur_radius`` can ride on the Wiener operator for a
+        result read only at rows and columns ``window = (lo, hi)``: every box around those voxels
+        stays inside the tile (``lo >= half`` and ``hi + half <= T``), so the box on the periodic
+        tile equals the reference's clipped box there (``blur.cu:16-45``)."""
+        if blur_radius is None or window is None:
+            return False
+        half = int(blur_radius) // 2
+        return half > 0 and window[0] >= half and window[1] + half <= self._plan.T
 
     def schedule(self):
         """The deconvolution schedule: ``{"xy": form, "z": form, "batch_tiles": n or None}`` and the
@@ -363,8 +390,13 @@ class Deconvolver:
         (the fused blur widths, box half-widths 15 and 25)."""
         return blur_radius is not None and int(blur_radius) // 2 in (15, 25)
 
-    def run_tiles(self, table, ntiles, tiles_per_block, blur_radius, res_out=None, window=None, maxima=None):
+    def run_tiles(self, table, ntiles, tiles_per_block, blur_radius, res_out=None, window=None, maxima=None,
+                  spectral=False):
         """Deconvolution (+ blur subtraction) of ``ntiles`` descriptors: ``(ntiles, sz, T, T)``.
+
+        ``spectral=True`` with a ``window`` that ``spectral_blur`` admits subtracts the blur through
+        the operator (``box_spectra``): the deconvolution computes res - blur at rows and columns
+        [lo, hi) directly, into ``res_out``, with no blur pass (``maxima`` must then be None).
 
         With a blur the result lives in the spectrum workspace (valid until the next call).
         ``window=(lo, hi)`` computes the blur subtraction only at rows and columns [lo, hi)
@@ -374,16 +406,21 @@ class Deconvolver:
         and a fused blur width also finds the maxima of the result at rows and columns [lo, hi)
         in the same entry (``mm_tiles_blur_maxima_window``, the outputs of
         ``mm_tiles_maxima_window``)."""
-        if maxima is not None and (window is None or not self.fused_blur_maxima(blur_radius)):
-            raise ValueError("maxima in run_tiles need a window and a blur half-width of 15 or 25")
+        spectral = bool(spectral) and self.spectral_blur(blur_radius, window)
+        if maxima is not None and (window is None or not self.fused_blur_maxima(blur_radius) or spectral):
+            raise ValueError("maxima in run_tiles need a window, a blur half-width of 15 or 25 and the blur pass")
         T, sz = self._plan.T, self.tile_height
         spec = self._workspace("spec", ntiles * self._plan.spec_bytes)
         res = res_out if res_out is not None else torch.empty((ntiles, sz, T, T), dtype=torch.float32,
                                                                device=self.device)
         st = L.stream_ptr(self.device)
         delta = None if blur_radius is None else int(blur_radius) // 2
-        args = (self._plan.h, L.ptr(table), ntiles, L.ptr(self.psf_fft), self.psf_fft.shape[0],
+        ops = self.box_spectra(blur_radius) if spectral else self.psf_fft
+        args = (self._plan.h, L.ptr(table), ntiles, L.ptr(ops), ops.shape[0],
                 int(tiles_per_block), L.ptr(spec), L.ptr(res))
+        if spectral:
+            L.call("mm_deconv_tiles_window", *args, int(window[0]), int(window[1]), st)
+            return res
         if window is None:
             L.call("mm_deconv_tiles", *args, st)
         else:
@@ -408,7 +445,7 @@ class Deconvolver:
                    int(window[0]), int(window[1]), st)
         return out
 
-    def _batches(self, image, flat_field, blur_radius, want_raw, window=None):
+    def _batches(self, image, flat_field, blur_radius, want_raw, window=None, spectral=False):
         self._check(image)
         img = prepare_stack(image)
         flat = prepare_flat(flat_field, img)
@@ -419,7 +456,8 @@ class Deconvolver:
         for b0 in range(0, len(geom), B):
             g = geom[b0:b0 + B]
             table = tile_table(g, img, flat)
-            res = self.run_tiles(table, len(g), len(g) if single else 1, blur_radius, window=window)
+            res = self.run_tiles(table, len(g), len(g) if single else 1, blur_radius, window=window,
+                                 spectral=spectral)
             raw = None
             if want_raw:
                 raw = torch.empty((len(g), sz, T, T), dtype=torch.float32, device=self.device)
@@ -433,8 +471,9 @@ class Deconvolver:
             for i, t in enumerate(g):
                 yield t["x"], t["y"], res[i, :, :t["xe"], :t["ye"]], raw[i, :, :t["xe"], :t["ye"]]
 
-    def apply(self, image, flat_field=None, blur_radius=None, output=None, **kwargs):
-        """``deconvolver.py:172-184``: stitched float32 volume."""
+    def apply(self, image, flat_field=None, blur_radius=None, output=None, spectral=False, **kwargs):
+        """``deconvolver.py:172-184``: stitched float32 volume.  ``spectral=True`` subtracts the blur
+        through the operator where ``spectral_blur`` admits the stitched window (the DAPI caller)."""
         if output is None:
             output = torch.empty(tuple(image.shape), dtype=torch.float32, device=self.device)
         if output.dtype != torch.float32 or output.stride(-1) != 1:
@@ -442,7 +481,8 @@ class Deconvolver:
         sz, sx, sy = image.shape
         # the stitch reads rows and columns [ov, T - ov) of every tile: only those are computed
         T, ov = self._plan.T, self.overlap
-        for g, table, res, _, _ in self._batches(image, flat_field, blur_radius, False, window=(ov, T - ov)):
+        for g, table, res, _, _ in self._batches(image, flat_field, blur_radius, False, window=(ov, T - ov),
+                                                 spectral=spectral):
             L.call("mm_deconv_stitch", self._plan.h, L.ptr(table), len(g), L.ptr(res), L.ptr(output),
                    output.stride(0), output.stride(1), sx, sy, L.stream_ptr(self.device))
         return output

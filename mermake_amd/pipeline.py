@@ -32,7 +32,7 @@ from .maxima import find_local_maxima_pm
 
 class HybPipeline:
     def __init__(self, deconvolver, blur_radius=30, threshold=3600, delta=1, delta_fit=3, sigmaZ=1, sigmaXY=1.5,
-                 cap=16384, plain=False, window=True, graph=False, **kwargs):
+                 cap=16384, plain=False, window=True, graph=False, spectral=True, **kwargs):
         if delta > 5:
             raise TypeError("Delta must be an less than or equal to 5 due to MAX_KERNEL_POINTS")
         self.d = deconvolver
@@ -46,6 +46,10 @@ class HybPipeline:
         self.plain = plain
         self.window = window
         self.graph = bool(graph) and not plain
+        # the blur subtraction rides on the Wiener operator when every box the windows read stays
+        # inside the tile (Deconvolver.spectral_blur); otherwise the blur pass runs
+        self.spectral = bool(spectral)
+        self.prepare()
         self._tables = {}
         self._ws = {}
         self._graphs = {}
@@ -68,7 +72,14 @@ class HybPipeline:
             opts = dict(autotune) if isinstance(autotune, dict) else {}
             opts.setdefault("window", pipe.deconv_window())
             tune(d, channel_shape, **opts)
+            pipe.prepare()
         return pipe
+
+    def prepare(self):
+        """Build the operators this pipeline reads (the box-folded ones of a spectral blur) now
+        rather than inside the first batch."""
+        if self.spectral_blur():
+            self.d.box_spectra(self.blur_radius)
 
     def deconv_window(self):
         """Rows and columns [lo, hi) of every tile that the deconvolution computes for this pipeline
@@ -76,9 +87,16 @@ class HybPipeline:
         win = self.windows()
         if win is None:
             return None
+        if self.spectral_blur():
+            return win[0]
         T = self.d._plan.T
         delta = 0 if self.blur_radius is None else int(self.blur_radius) // 2
         return max(0, win[0][0] - delta), min(T, win[0][1] + delta)
+
+    def spectral_blur(self):
+        """Whether the blur subtraction is folded into the Wiener operator for this pipeline."""
+        win = self.windows()
+        return self.spectral and win is not None and self.d.spectral_blur(self.blur_radius, win[0])
 
     def _buf(self, name, shape, dtype):
         t = self._ws.get(name)
@@ -132,12 +150,14 @@ class HybPipeline:
         coords = self._buf("coords", (ntiles, cap, 3), torch.int32)
         counts = self._buf("counts", (ntiles,), torch.int32)
         fit = self._buf("fit", (ntiles, cap, 8), torch.float32)
-        # the threshold test of the maxima rides on the blur (one entry) where the blur width allows
-        fused = win is not None and d.fused_blur_maxima(self.blur_radius)
+        # the blur subtraction rides on the operator, or else the threshold test of the maxima rides
+        # on the blur pass (one entry) where the blur width allows
+        spectral = self.spectral_blur()
+        fused = not spectral and win is not None and d.fused_blur_maxima(self.blur_radius)
         mx = (self.threshold, self.delta, win[1], coords, cap, counts, ws, ws_bytes) if fused else None
         res = self._buf("res", (ntiles, sz, T, T), torch.float32)
         res = d.run_tiles(table, ntiles, ntiles if single else 1, self.blur_radius, res_out=res,
-                          window=None if win is None else win[0], maxima=mx)
+                          window=None if win is None else win[0], maxima=mx, spectral=spectral)
         if win is None:
             L.call("mm_tiles_maxima", d._plan.h, L.ptr(table), ntiles, L.ptr(res), self.threshold, self.delta,
                    L.ptr(coords), cap, L.ptr(counts), L.ptr(ws), ws_bytes, st)
@@ -147,6 +167,39 @@ class HybPipeline:
         L.call("mm_tiles_fit", d._plan.h, L.ptr(table), ntiles, L.ptr(res), L.ptr(coords), L.ptr(counts), cap,
                self.delta_fit, self.sigmaZ, self.sigmaXY, L.ptr(fit), st)
         return counts, fit
+
+    def tile_volumes(self, stack, flat_field=None, fill=-1e30):
+        """Host copies ``(x, y, volume, raw)`` of what the maxima and the fit of each tile read, over the
+        tile's view ``(sz, xend, yend)`` (``deconvolver.py:170``): the deconvolved tile minus its blur as
+        this pipeline computes it (through the operator or by the blur pass), with ``fill`` outside the
+        blur window, which nothing reads, and the raw view.  The oracle's maxima, fit and keep filter on
+        these volumes give this pipeline's kept rows; tests use it for that."""
+        d = self.d
+        d._check(stack)
+        img = prepare_stack(stack)
+        flat = prepare_flat(flat_field, img)
+        geom, table = self._table(img, flat)
+        T, sz = d._plan.T, d.tile_height
+        n = len(geom)
+        single = d.psf_fft.shape[0] == 1
+        win = self.windows()
+        res = self._buf("res", (n, sz, T, T), torch.float32)
+        vol = d.run_tiles(table, n, n if single else 1, self.blur_radius, res_out=res,
+                          window=None if win is None else win[0], spectral=self.spectral_blur())
+        vol = vol.cpu().numpy()
+        raw = torch.empty((n, sz, T, T), dtype=torch.float32, device=d.device)
+        L.call("mm_deconv_tile_raw", d._plan.h, L.ptr(table), n, L.ptr(raw), L.stream_ptr(d.device))
+        raw = raw.cpu().numpy()
+        out = []
+        for i, t in enumerate(geom):
+            v = vol[i, :, :t["xe"], :t["ye"]].copy()
+            if win is not None:
+                lo, hi = win[0]
+                m = np.full(v.shape[1:], True)
+                m[lo:hi, lo:hi] = False
+                v[:, m] = fill
+            out.append((t["x"], t["y"], v, raw[i, :, :t["xe"], :t["ye"]].copy()))
+        return out
 
     def _keep_launch(self, table, n, counts, fit, cap, xh, stats):
         d = self.d
@@ -305,7 +358,7 @@ class HybPipeline:
 
 class DapiPipeline:
     def __init__(self, deconvolver, blur_radius=50, threshold=3.0, delta=5, delta_fit=5, sigmaZ=1, sigmaXY=1.5,
-                 **kwargs):
+                 spectral=True, **kwargs):
         if delta > 5:
             raise TypeError("Delta must be an less than or equal to 5 due to MAX_KERNEL_POINTS")
         self.d = deconvolver
@@ -313,6 +366,19 @@ class DapiPipeline:
         self.kw = dict(threshold=float(torch.tensor(float(threshold), dtype=torch.float32)), delta=int(delta),
                        delta_fit=int(delta_fit), sigmaZ=float(sigmaZ), sigmaXY=float(sigmaXY))
         self._buffer = None
+        # the blur subtraction rides on the Wiener operator where every box of the stitched window
+        # stays inside the tile (Deconvolver.spectral_blur)
+        self.spectral = bool(spectral)
+        self.prepare()
+
+    def _spectral(self):
+        T, ov = self.d._plan.T, self.d.overlap
+        return self.spectral and self.d.spectral_blur(self.blur_radius, (ov, T - ov))
+
+    def prepare(self):
+        """Build the box-folded operators of a spectral blur now rather than inside the first stack."""
+        if self._spectral():
+            self.d.box_spectra(self.blur_radius)
 
     def __call__(self, chan, flat_field=None):
         """``(Xh_plus, Xh_minus)`` of one DAPI stack."""
@@ -320,7 +386,8 @@ class DapiPipeline:
         img = prepare_stack(chan)
         if self._buffer is None or tuple(self._buffer.shape) != tuple(img.shape):
             self._buffer = torch.empty(tuple(img.shape), dtype=torch.float32, device=d.device)
-        buf = d.apply(img, flat_field=flat_field, blur_radius=self.blur_radius, output=self._buffer)
+        buf = d.apply(img, flat_field=flat_field, blur_radius=self.blur_radius, output=self._buffer,
+                      spectral=self.spectral)
         ws = torch.empty(L.lib().mm_rms_ws_bytes(), dtype=torch.uint8, device=d.device)
         rms = torch.empty(1, dtype=torch.float32, device=d.device)
         L.call("mm_rms_normalize", L.ptr(buf), buf.numel(), L.ptr(ws), L.ptr(rms), L.stream_ptr(d.device))

@@ -192,7 +192,7 @@ class _Slot:
         self.spec = torch.empty(spec, dtype=torch.uint8, device=device)
         self.res = torch.empty(res, dtype=torch.float32, device=device)
         self.tmp = None
-        if any(p.blur_radius is not None and not _fused_blur(p) for p in pipes.values()):
+        if any(p.blur_radius is not None and not p.spectral_blur() and not _fused_blur(p) for p in pipes.values()):
             self.tmp = torch.empty(res, dtype=torch.float32, device=device)
         self.mws_bytes = max(L.lib().mm_maxima_ws_bytes(p.d.tile_height, p.d._plan.T, p.d._plan.T, B_of[k])
                              for k, p in pipes.items())
@@ -229,15 +229,19 @@ class _Slot:
         win = pipe.windows()
         delta = None if pipe.blur_radius is None else int(pipe.blur_radius) // 2
         tab = L.ptr(self.table)
-        dargs = (d._plan.h, tab, n, L.ptr(d.psf_fft), d.psf_fft.shape[0], n if single else 1, L.ptr(self.spec),
-                 L.ptr(res))
+        # the blur subtraction rides on the Wiener operator where the windows allow (HybPipeline)
+        spectral = pipe.spectral_blur()
+        ops = d.box_spectra(pipe.blur_radius) if spectral else d.psf_fft
+        dargs = (d._plan.h, tab, n, L.ptr(ops), ops.shape[0], n if single else 1, L.ptr(self.spec), L.ptr(res))
         if win is None or delta is None:
             L.call("mm_deconv_tiles", *dargs, st)
+        elif spectral:
+            L.call("mm_deconv_tiles_window", *dargs, win[0][0], win[0][1], st)
         else:
             L.call("mm_deconv_tiles_window", *dargs, max(0, win[0][0] - delta), min(T, win[0][1] + delta), st)
         out = res
-        fused = win is not None and d.fused_blur_maxima(pipe.blur_radius)
-        if delta is not None:
+        fused = not spectral and win is not None and d.fused_blur_maxima(pipe.blur_radius)
+        if delta is not None and not spectral:
             # blur subtraction into the spectrum buffer, which is free again (deconvolver.py:160-163)
             out = self.spec[:n * sz * T * T * 4].view(torch.float32)
             if win is None:
@@ -328,6 +332,10 @@ class HybServer:
         self.pipes = dict(pipe) if isinstance(pipe, dict) else {None: pipe}
         if not self.pipes:
             raise ValueError("HybServer needs at least one pipeline")
+        for p in self.pipes.values():
+            p.prepare()     # box-folded operators before the first request, not inside its batch
+        if dapi is not None:
+            dapi.prepare()
         self.default_mode = next(iter(self.pipes))
         self.pipe = self.pipes[self.default_mode]
         devs = {p.d.device for p in self.pipes.values()}

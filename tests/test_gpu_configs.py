@@ -4,8 +4,10 @@ Every test runs the HIP path through the C ABI at a workload's own geometry (sta
 overlap, zpad, PSF set, flat) and checks it against the float64 oracle:
 
 * deconvolution, per tile: relative L2 <= 1e-5 against the oracle's float64 ``tile_wise``;
-* spots, given the GPU's own deconvolved tiles: the oracle's maxima, fit and keep filter
-  (``maxima.py:17-98``, ``__main__.py:254-265``) run on those tiles must give exactly the kept rows of
+* the volume the maxima read (deconvolved tile minus its blur, the blur folded into the Wiener operator):
+  relative L2 <= 1e-5 against the oracle's float64 ``tile_wise`` with the blur, over the blur window;
+* spots, given the GPU's own volumes (``HybPipeline.tile_volumes``): the oracle's maxima, fit and keep
+  filter (``maxima.py:17-98``, ``__main__.py:254-265``) run on those volumes must give exactly the kept rows of
   ``HybPipeline`` (same rows in the same order; fitted coordinates within 1e-3 px; background, habs and h
   equal; correlations within 1e-4).  A row whose float64 fitted x or y lies within 1e-3 px of a keep
   boundary may fall on either side;
@@ -36,7 +38,6 @@ pytestmark = pytest.mark.gpu
 
 HYB = dict(blur_radius=30, threshold=3600, delta=1, delta_fit=3, sigmaZ=1, sigmaXY=1.5)
 DAPI = dict(blur_radius=50, threshold=3.0, delta=5, delta_fit=5, sigmaZ=1, sigmaXY=1.5)
-BOUNDARY_TOL = 1e-3
 
 
 def T(a, dev):
@@ -48,40 +49,7 @@ def log(*a):
 
 
 # ------------------------------------------------------------------ comparison helpers
-def _close(g, w, tol=1e-3):
-    return (np.abs(g[:3].astype(np.float64) - w[:3]).max() <= tol and g[3] == w[3] and g[5] == w[5]
-            and g[7] == w[7] and np.abs(g[[4, 6]].astype(np.float64) - w[[4, 6]]).max() <= 1e-4)
-
-
-def oracle_tile_rows(tiles, overlap, hyb):
-    """The hyb loop body of ``__main__.py:254-265`` in the oracle, over tiles (x, y, tile, raw) given as
-    host arrays: every fitted candidate as (row offset to stack coordinates, kept, near a keep boundary)."""
-    out = []
-    for x, y, tile, raw in tiles:
-        Xh = omax.find_local_maxima(tile, raw=raw, dtype=np.float64, **hyb)
-        hx, hy = tile.shape[1] - overlap, tile.shape[2] - overlap
-        for r in Xh:
-            keep = overlap <= r[1] < hx and overlap <= r[2] < hy
-            amb = min(abs(r[1] - overlap), abs(r[1] - hx), abs(r[2] - overlap), abs(r[2] - hy)) <= BOUNDARY_TOL
-            row = r.copy()
-            row[1] += x - overlap
-            row[2] += y - overlap
-            out.append((row, keep, amb))
-    return out
-
-
-def assert_kept_rows(got, cand, what):
-    """``got`` (the HIP kept rows, in order) against the oracle's candidates: every kept row not near a
-    boundary matches the next HIP row, a row near a boundary may or may not, the others are absent."""
-    i = 0
-    for row, keep, amb in cand:
-        if i < len(got) and (keep or amb) and _close(got[i], row):
-            i += 1
-        elif keep and not amb:
-            raise AssertionError(f"{what}: oracle row {row} missing at HIP row {i} "
-                                 f"({got[i] if i < len(got) else None})")
-    assert i == len(got), f"{what}: {len(got) - i} HIP rows beyond the oracle's kept rows"
-    assert sum(k for _, k, _ in cand) > 0, what
+from spot_rows import BOUNDARY_TOL, _close, assert_kept_rows, oracle_tile_rows  # noqa: E402,F401
 
 
 def gpu_tiles(d, stack, flat, blur_radius, keep=None):
@@ -101,6 +69,23 @@ def assert_deconv_parity(d, stack_d, flat_d, stack, flat, W, ts, ov, zpad, only,
         assert g.shape == t.shape, (what, x, y, g.shape, t.shape)
         e = float(np.linalg.norm(g - t) / np.linalg.norm(t))
         log(f"{what} tile ({x}, {y}) {g.shape}: deconvolution relative L2 {e:.2e}")
+        assert e <= 1e-5, (what, x, y, e)
+        n += 1
+    assert n == len(only)
+
+
+def assert_volume_parity(pipe, stack_d, flat_d, stack, flat, W, ts, ov, zpad, only, what=""):
+    """The volume ``pipe``'s maxima and fit read (with the blur folded into the Wiener operator) against the
+    float64 oracle's ``tile_wise`` with the blur (``deconvolver.py:155-163``), over the blur window."""
+    assert pipe.spectral_blur(), what
+    (lo, hi), _ = pipe.windows()
+    mine = {(x, y): v for x, y, v, _ in pipe.tile_volumes(stack_d, flat_d)}
+    n = 0
+    for x, y, t, _ in odec.tile_wise(stack, W, flat, pipe.blur_radius, ts, ov, zpad, only=only):
+        g, w = mine[(x, y)][:, lo:hi, lo:hi], t[:, lo:hi, lo:hi]
+        assert g.shape == w.shape, (what, x, y, g.shape, w.shape)
+        e = float(np.linalg.norm(g - w) / np.linalg.norm(w))
+        log(f"{what} tile ({x}, {y}) window [{lo}, {hi}): deconvolution minus blur, relative L2 {e:.2e}")
         assert e <= 1e-5, (what, x, y, e)
         n += 1
     assert n == len(only)
@@ -144,9 +129,11 @@ def test_plain_small_spots(dev, plain_small):
     raw, psf, W = plain_small
     pipe = HybPipeline.from_settings(psf, raw.shape, tile_size=500, overlap=89, beta=1e-4, **HYB)
     got = pipe(T(raw, dev)).cpu().numpy()
-    tiles = gpu_tiles(pipe.d, T(raw, dev), None, HYB["blur_radius"])
+    tiles = pipe.tile_volumes(T(raw, dev))
     assert len(tiles) == 4
     assert_kept_rows(got, oracle_tile_rows(tiles, 89, HYB), "plain 20x512^2")
+    assert_volume_parity(pipe, T(raw, dev), None, raw, None, W, 500, 89, 19, {(0, 0), (500, 500)},
+                         what="plain 20x512^2")
     want = odec.hyb_spots(raw, W, None, 500, 89, 19, **HYB)
     assert_end_to_end(got, want, "plain 20x512^2 end to end")
 
@@ -176,9 +163,11 @@ def test_hyb_fov_tile_class_spots(dev, hyb_fov):
     pipe = HybPipeline.from_settings(psf, raw.shape, tile_size=500, overlap=89, beta=1e-4, **HYB)
     assert pipe.windows() is not None
     got = pipe(T(raw, dev), T(flat, dev)).cpu().numpy()
-    tiles = gpu_tiles(pipe.d, T(raw, dev), T(flat, dev), HYB["blur_radius"])
+    tiles = pipe.tile_volumes(T(raw, dev), T(flat, dev))
     assert len(tiles) == 9
     assert_kept_rows(got, oracle_tile_rows(tiles, 89, HYB), "hyb 40x1200^2")
+    assert_volume_parity(pipe, T(raw, dev), T(flat, dev), raw, flat, W, 500, 89, 39, {(0, 500), (1000, 1000)},
+                         what="hyb 40x1200^2")
     # end to end on the corner, edge, interior and far-corner tiles' own spots
     only = {(0, 0), (0, 500), (500, 500), (1000, 1000)}
     want = odec.hyb_spots(raw, W, flat, 500, 89, 39, tile_filter=only, **HYB)
@@ -232,9 +221,11 @@ def test_multi_psf_grid_spots(dev, multi_psf):
     raw, flat, psfs, W = multi_psf
     pipe = HybPipeline.from_settings(psfs, raw.shape, tile_size=300, overlap=89, beta=1e-4, **HYB)
     got = pipe(T(raw, dev), T(flat, dev)).cpu().numpy()
-    tiles = gpu_tiles(pipe.d, T(raw, dev), T(flat, dev), HYB["blur_radius"])
+    tiles = pipe.tile_volumes(T(raw, dev), T(flat, dev))
     assert len(tiles) == 9
     assert_kept_rows(got, oracle_tile_rows(tiles, 89, HYB), "multi-PSF 40x900^2")
+    assert_volume_parity(pipe, T(raw, dev), T(flat, dev), raw, flat, W, 300, 89, 39, {(300, 300), (600, 0)},
+                         what="multi-PSF 40x900^2")
 
 
 @pytest.fixture(scope="module")
@@ -270,6 +261,25 @@ def test_multi_psf_grid_dapi_own_psf_deconvolution(dev, dapi_stack):
     only = {(0, 0), (300, 300), (600, 0), (600, 600)}
     assert_deconv_parity(d, T(raw, dev), None, raw, None, W, 300, 89, 19, only, blur_radius=50,
                          what="DAPI 40x900^2 3x3 PSF grid")
+
+
+def test_dapi_spectral_apply_vs_oracle(dev, dapi_stack):
+    """DapiPipeline's volume before normalisation: ``apply`` with the blur of radius 50 folded into the
+    Wiener operator (every box of the stitched window stays inside its tile) against the float64 oracle's
+    ``apply`` with the blur pass, and against the HIP blur pass itself."""
+    from mermake_amd.deconvolver import Deconvolver
+    raw, psf = dapi_stack
+    d = Deconvolver(psf, raw.shape, tile_size=300, zpad=19, overlap=89, beta=0.01)
+    T_, ov = d._plan.T, d.overlap
+    assert d.spectral_blur(50, (ov, T_ - ov))
+    W = odec.wiener(psf, 40, 300, 19, 89, 0.01, mode="fp64")
+    got = d.apply(T(raw, dev), blur_radius=50, spectral=True).cpu().numpy()
+    exact = d.apply(T(raw, dev), blur_radius=50).cpu().numpy()
+    want = odec.apply(raw, W, None, 50, 300, 89, 19)
+    e = float(np.linalg.norm(got - want) / np.linalg.norm(want))
+    e2 = float(np.linalg.norm(exact - want) / np.linalg.norm(want))
+    log(f"DAPI 40x900^2 apply, blur on the operator: relative L2 {e:.2e} (blur pass {e2:.2e})")
+    assert e <= 1e-5 and e2 <= 1e-5, (e, e2)
 
 
 def test_multi_psf_grid_dapi_spots_given_volume(dev, dapi_stack):

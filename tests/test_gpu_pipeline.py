@@ -88,8 +88,8 @@ def test_hyb_window_equals_whole_tiles(dev, small_hyb, delta, delta_fit, ts, ov)
     from mermake_amd.pipeline import HybPipeline
     raw, flat, psf = small_hyb
     hyb = dict(HYB, delta=delta, delta_fit=delta_fit, threshold=2000)
-    win = HybPipeline.from_settings(psf, raw.shape, tile_size=ts, overlap=ov, beta=1e-4, **hyb)
-    assert win.windows() is not None
+    win = HybPipeline.from_settings(psf, raw.shape, tile_size=ts, overlap=ov, beta=1e-4, spectral=False, **hyb)
+    assert win.windows() is not None and not win.spectral_blur()
     full = HybPipeline(win.d, window=False, **hyb)
     a = win(T(raw, dev), T(flat, dev)).cpu().numpy()
     b = full(T(raw, dev), T(flat, dev)).cpu().numpy()
@@ -116,6 +116,35 @@ def test_hyb_window_equals_whole_tiles(dev, small_hyb, delta, delta_fit, ts, ov)
         f = cf[t, :nf[t]]
         keep = (f[:, 1] >= c0) & (f[:, 1] < c1) & (f[:, 2] >= c0) & (f[:, 2] < c1)
         assert np.array_equal(cw[t, :nw[t]], f[keep]), t
+
+
+@pytest.mark.parametrize("delta,delta_fit,ts,ov", [(1, 3, 100, 30), (2, 1, 100, 30),
+                                                   (1, 3, 150, 38),   # T = 226: Rader 113, blocked spectra
+                                                   (1, 3, 300, 89)])  # T = 478: Rader 239
+def test_hyb_spectral_blur_kept_rows_given_volume(dev, small_hyb, delta, delta_fit, ts, ov):
+    """With the blur subtraction folded into the Wiener operator, the oracle's maxima, fit and keep filter on
+    the pipeline's own volumes give exactly its kept rows, and those volumes equal the blur pass's over the
+    blur window within float32 rounding (relative L2 <= 1e-5); the blur-pass pipeline's spots match."""
+    from spot_rows import assert_kept_rows, oracle_tile_rows
+    from mermake_amd.pipeline import HybPipeline
+    raw, flat, psf = small_hyb
+    hyb = dict(HYB, delta=delta, delta_fit=delta_fit, threshold=2000)
+    sp = HybPipeline.from_settings(psf, raw.shape, tile_size=ts, overlap=ov, beta=1e-4, **hyb)
+    assert sp.spectral_blur()
+    ex = HybPipeline(sp.d, spectral=False, **hyb)
+    a = sp(T(raw, dev), T(flat, dev)).cpu().numpy()
+    b = ex(T(raw, dev), T(flat, dev)).cpu().numpy()
+    va = sp.tile_volumes(T(raw, dev), T(flat, dev))
+    vb = ex.tile_volumes(T(raw, dev), T(flat, dev))
+    print("spectral blur", delta, delta_fit, sp.d._plan.T, len(a), len(b), flush=True)
+    assert_kept_rows(a, oracle_tile_rows(va, ov, hyb), f"spectral blur T {sp.d._plan.T}")
+    (lo, hi), _ = sp.windows()
+    for (x, y, u, _), (_, _, v, _) in zip(va, vb):
+        u, v = u[:, lo:hi, lo:hi], v[:, lo:hi, lo:hi]
+        assert np.linalg.norm(u - v) <= 1e-5 * np.linalg.norm(v), (x, y)
+    assert len(a) > 50 and abs(len(a) - len(b)) <= max(2, 0.01 * len(b)), (len(a), len(b))
+    err, rate = _match(a, b, 0.5)
+    assert rate > 0.99 and err < 1e-3, (err, rate)
 
 
 @pytest.mark.parametrize("delta,blur_radius,thr,ts,ov,same_window", [
