@@ -262,6 +262,21 @@ int mm_tiles_maxima_window(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntil
 int mm_tiles_blur_maxima_window(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const float* res, float* out,
                                 int blur_delta, int blo, int bhi, float threshold, int delta, int lo, int hi,
                                 int* coords, int cap, int* counts, void* ws, size_t ws_bytes, void* stream);
+/* mm_deconv_tiles_window(res, rows and columns [dlo, dhi)) followed by
+ * mm_tiles_maxima_window(res, [lo, hi) inside [dlo, dhi)), with the same
+ * outputs as those two calls.  With W from mm_deconv_build_wiener_box, res is
+ * the deconvolved tile minus its blur, so this is the hyb caller's
+ * tile_wise + find_local_maxima (deconvolver.py:116-170, maxima.py:17-98,
+ * __main__.py:254) with no blur pass.  When the plan's y passes are the Rader
+ * form, the inverse y transform records the threshold test of every voxel it
+ * stores as bits in the maxima workspace, so the maxima pass reads only
+ * candidates and their neighbours; other forms run the whole maxima pass.
+ * The neighbour tests read out at [lo - delta, hi + delta), which must lie
+ * inside [dlo, dhi) except where dlo = 0 or dhi = T (MM_EINVAL otherwise). */
+int mm_tiles_deconv_maxima_window(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const void* W, int n_psf,
+                                  int tiles_per_block, void* spec_ws, float* res, int dlo, int dhi, float threshold,
+                                  int delta, int lo, int hi, int* coords, int cap, int* counts, void* ws,
+                                  size_t ws_bytes, void* stream);
 int mm_tiles_fit(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const float* res, const int* coords,
                  const int* counts, int cap, int delta_fit, float sigmaZ, float sigmaXY, float* out8, void* stream);
 /* keep-filter on the fitted x,y (ov <= . < view extent - ov), offset to stack

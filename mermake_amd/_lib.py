@@ -62,6 +62,8 @@ _SIGS = {
     "mm_delta_fit": (_i, [_vp, _i, _i, _i, _ll, _ll, _f, _vp, _i, _ll, _ll, _vp, _vp, _i, _i, _f, _f, _vp, _vp]),
     "mm_tiles_maxima": (_i, [_vp, _vp, _i, _vp, _f, _i, _vp, _i, _vp, _vp, _sz, _vp]),
     "mm_tiles_maxima_window": (_i, [_vp, _vp, _i, _vp, _f, _i, _i, _i, _vp, _i, _vp, _vp, _sz, _vp]),
+    "mm_tiles_deconv_maxima_window": (_i, [_vp, _vp, _i, _vp, _i, _i, _vp, _vp, _i, _i, _f, _i, _i, _i, _vp, _i,
+                                           _vp, _vp, _sz, _vp]),
     "mm_tiles_blur_maxima_window": (_i, [_vp, _vp, _i, _vp, _vp, _i, _i, _i, _f, _i, _i, _i, _vp, _i, _vp, _vp, _sz,
                                          _vp]),
     "mm_tiles_fit": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _i, _i, _f, _f, _vp, _vp]),

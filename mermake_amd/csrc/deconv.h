@@ -124,3 +124,12 @@ struct mm_deconv {
   int prof;                 // record events around the five passes of mm_deconv_tiles
   hipEvent_t ev[6];
 };
+
+// mm_deconv_tiles_window that also writes the maxima threshold bits (!(v <= thr))
+// of every computed voxel into `marks`, the masks of the maxima workspace (rows
+// (tile * sz + z) * T + x, (T + 63) / 64 words each), when the plan's C2R is
+// the Rader form (*marked = 1); other forms leave `marks` alone (*marked = 0).
+int deconv_tiles_marked(mm_deconv* d, const struct mm_tile_desc* tiles_dev, int ntiles, const void* W, int n_psf,
+                        int tiles_per_block, void* spec_ws, float* res, int lo, int hi, unsigned long long* marks,
+                        float thr, int* marked, void* stream);
+

@@ -966,15 +966,25 @@ __global__ __launch_bounds__(256, 2) void k_rd_r2c(const mm_tile_dev* __restrict
 
 // Rows (z, x) with x in [xlo, xlo + nx) only (nrows = sz * nx of them, taken
 // in that order), output columns in [clo, chi) only.
-template <int PR, int M>
+// MARK: also the maxima threshold test (!(v <= thr), maxima.cu:19) of every
+// stored output, as ww 64-bit words per row (bit c & 63 of word c >> 6 for
+// column c; zero for the columns not stored) at marks[(tile * N + z) * N + x]
+// (the row layout of the maxima workspace), for every computed row.
+template <int PR, int M, bool MARK = false>
 __global__ __launch_bounds__(256, 2) void k_rd_c2r(const float2* Sin, long long s_tstride, SpecLayout Ls, float* res,
                                                 long long r_tstride, int ntiles, int nrows, int xlo, int nx, int clo,
-                                                int chi, RdTab tab, const float2* __restrict__ tw) {
+                                                int chi, RdTab tab, const float2* __restrict__ tw,
+                                                unsigned long long* __restrict__ marks = nullptr, float thr = 0.0f,
+                                                int ww = 0) {
   constexpr int G = rd_row_g<PR>(M, true), NC = rd_row_nc<PR>(M), NR = 2 * G;
   static_assert(G * M * RdSpec<PR>::B <= 256, "in-place Rader stages: one item per thread");
   extern __shared__ __align__(16) unsigned char smem[];
   const int N = PR * M, H = N / 2 + 1;
   const RdLds L = rd_carve<PR>(smem, N, M, G, tw, tab);
+  // MARK: the threshold words of the group's NR rows (after the Rader buffer)
+  unsigned long long* wl = (unsigned long long*)L.extra;
+  if (MARK)
+    for (int i = threadIdx.x; i < NR * ww; i += blockDim.x) wl[i] = 0ull;
   __syncthreads();
   const RdCols<NC> C = rd_cols<PR, M, NC>(L.slot_in);
   int kk[NC];  // in-row offset of the half-spectrum bin of column n (bin 0 for the absent columns n >= N)
@@ -1050,12 +1060,29 @@ __global__ __launch_bounds__(256, 2) void k_rd_c2r(const float2* Sin, long long 
       float* ob = out + ((long long)zb * N + xb) * N + j;
 #pragma unroll
       for (int r = 0; r < M; ++r) {
-        const bool cin = j + r * PR >= clo && j + r * PR < chi;
+        const int c = j + r * PR;
+        const bool cin = c >= clo && c < chi;
         if (cin && q < nrows) oa[r * PR] = v[r].x;
         if (cin && q + 1 < nrows) ob[r * PR] = v[r].y;
+        if (MARK && cin) {  // rare: LDS OR of the set bits
+          const unsigned long long bit = 1ull << (c & 63);
+          if (!(v[r].x <= thr)) atomicOr(wl + (2 * g) * ww + (c >> 6), bit);
+          if (!(v[r].y <= thr)) atomicOr(wl + (2 * g + 1) * ww + (c >> 6), bit);
+        }
       }
     });
     __syncthreads();  // the final stage's reads before the next group's stores
+    if (MARK) {  // the group's words out; the thread that stores word i clears it for the next group
+      for (int i = threadIdx.x; i < NR * ww; i += blockDim.x) {
+        const int r = i / ww, q = q0 + r;
+        if (q < nrows) {
+          int z, x;
+          rowzx(q0, z0, r, z, x);
+          marks[(((long long)tile * (nrows / nx) + z) * N + x) * ww + (i - r * ww)] = wl[i];
+        }
+        wl[i] = 0ull;
+      }
+    }
   }
 }
 
@@ -1135,13 +1162,19 @@ static void launch_rd_r2c(const mm_tile_dev* td, int ntiles, int T, int nrows, f
 
 template <int PR, int M>
 static void launch_rd_c2r(const float2* S, long long s_ts, const SpecLayout& Ls, float* res, long long r_ts,
-                          int ntiles, int sz, int lo, int hi, const RdTab& tab, const float2* tw, hipStream_t st) {
+                          int ntiles, int sz, int lo, int hi, const RdTab& tab, const float2* tw, hipStream_t st,
+                          unsigned long long* marks = nullptr, float thr = 0.0f) {
   constexpr int G = rd_row_g<PR>(M, true);
-  const size_t lds = rd_lds_bytes<PR>(PR * M, M, G, 0);
+  const int ww = (PR * M + 63) / 64;
+  const size_t lds = rd_lds_bytes<PR>(PR * M, M, G, marks ? (size_t)8 * 2 * G * ww : 0);
   const int nrows = sz * (hi - lo);
   const long long ngrp = (long long)((nrows + 2 * G - 1) / (2 * G)) * ntiles;
-  hipLaunchKernelGGL((k_rd_c2r<PR, M>), dim3(pf_grid(ngrp, lds)), dim3(256), lds, st, S, s_ts, Ls, res, r_ts, ntiles,
-                     nrows, lo, hi - lo, lo, hi, tab, tw);
+  if (marks)
+    hipLaunchKernelGGL((k_rd_c2r<PR, M, true>), dim3(pf_grid(ngrp, lds)), dim3(256), lds, st, S, s_ts, Ls, res, r_ts,
+                       ntiles, nrows, lo, hi - lo, lo, hi, tab, tw, marks, thr, ww);
+  else
+    hipLaunchKernelGGL((k_rd_c2r<PR, M>), dim3(pf_grid(ngrp, lds)), dim3(256), lds, st, S, s_ts, Ls, res, r_ts, ntiles,
+                       nrows, lo, hi - lo, lo, hi, tab, tw, nullptr, 0.0f, 0);
 }
 
 // ------------------------------------------------------------ Wiener spectra
@@ -1857,7 +1890,9 @@ extern "C" size_t mm_deconv_spec_bytes(const mm_deconv* d) {
 // res rows and columns [lo, hi) of every plane (the Rader C2R computes only
 // those; the other forms compute whole planes).
 static int deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const void* W, int n_psf,
-                        int tiles_per_block, void* spec_ws, float* res, int lo, int hi, void* stream) {
+                        int tiles_per_block, void* spec_ws, float* res, int lo, int hi, void* stream,
+                        unsigned long long* marks = nullptr, float thr = 0.0f, int* marked = nullptr) {
+  if (marked) *marked = 0;
   if (!d || !tiles_dev || !W || !spec_ws || !res || ntiles < 0 || n_psf < 1 || lo < 0 || hi > d->T || lo >= hi) {
     mm_set_error("mm_deconv_tiles: bad argument");
     return MM_EINVAL;
@@ -1949,8 +1984,10 @@ static int deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles,
   MM_MARK(4);
   if (d->rd) {  // 5. C2R along y into res (prime-first form, Rader prime stage)
     RD_SWITCH(d->rdP, d->pfM,
-              (launch_rd_c2r<PP, MM>(S, s_ts, Ls, res, (long long)sz * T * T, ntiles, sz, lo, hi, d->rdI, d->pT.tw, st)));
+              (launch_rd_c2r<PP, MM>(S, s_ts, Ls, res, (long long)sz * T * T, ntiles, sz, lo, hi, d->rdI, d->pT.tw, st,
+                                     marks, thr)));
     MM_LAUNCH_CHECK();
+    if (marked && marks) *marked = 1;
   } else if (d->pfM) {  // 5. C2R along y into res (prime-first form)
     const int G = d->pfG;
     const size_t lds = pf_lds_bytes(d->pfS, d->pfM, G, 0);
@@ -1979,6 +2016,13 @@ extern "C" int mm_deconv_tiles_window(mm_deconv* d, const mm_tile_desc* tiles_de
                                       int n_psf, int tiles_per_block, void* spec_ws, float* res, int lo, int hi,
                                       void* stream) {
   return deconv_tiles(d, tiles_dev, ntiles, W, n_psf, tiles_per_block, spec_ws, res, lo, hi, stream);
+}
+
+int deconv_tiles_marked(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const void* W, int n_psf,
+                        int tiles_per_block, void* spec_ws, float* res, int lo, int hi, unsigned long long* marks,
+                        float thr, int* marked, void* stream) {
+  return deconv_tiles(d, tiles_dev, ntiles, W, n_psf, tiles_per_block, spec_ws, res, lo, hi, stream, marks, thr,
+                      marked);
 }
 
 extern "C" int mm_deconv_set_profiling(mm_deconv* d, int on) {

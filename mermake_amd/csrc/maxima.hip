@@ -816,6 +816,36 @@ extern "C" int mm_tiles_maxima_window(mm_deconv* d, const mm_tile_desc* tiles_de
                     (hipStream_t)stream);
 }
 
+extern "C" int mm_tiles_deconv_maxima_window(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const void* W,
+                                             int n_psf, int tiles_per_block, void* spec_ws, float* res, int dlo,
+                                             int dhi, float threshold, int delta, int lo, int hi, int* coords, int cap,
+                                             int* counts, void* ws, size_t ws_bytes, void* stream) {
+  if (!d || !tiles_dev || !W || !spec_ws || !res || dlo < 0 || dhi > d->T || dlo >= dhi || lo < dlo || hi > dhi ||
+      hi <= lo) {
+    mm_set_error("mm_tiles_deconv_maxima_window: bad argument (deconvolution [%d, %d) maxima [%d, %d) T %d)", dlo, dhi,
+                 lo, hi, d ? d->T : 0);
+    return MM_EINVAL;
+  }
+  if (delta < 0 || delta > 31) { mm_set_error("delta %d outside [0, 31]", delta); return MM_EINVAL; }
+  // the neighbour tests read out at [lo - delta, hi + delta): inside the computed window, or up to the
+  // tile edge where the window reaches it (the tests reflect there)
+  if ((dlo > 0 && lo - delta < dlo) || (dhi < d->T && hi + delta > dhi)) {
+    mm_set_error("mm_tiles_deconv_maxima_window: maxima [%d, %d) with delta %d reads outside the window [%d, %d)",
+                 lo, hi, delta, dlo, dhi);
+    return MM_EINVAL;
+  }
+  if (ntiles == 0) return MM_OK;
+  const int T = d->T, sz = d->sz;
+  if (ws_bytes < mm_maxima_ws_bytes(sz, T, T, ntiles)) { mm_set_error("maxima workspace too small"); return MM_EINVAL; }
+  hipStream_t st = (hipStream_t)stream;
+  int marked = 0;
+  int rc = deconv_tiles_marked(d, tiles_dev, ntiles, W, n_psf, tiles_per_block, spec_ws, res, dlo, dhi,
+                               (unsigned long long*)ws, threshold, &marked, stream);
+  if (rc) return rc;
+  TileVols vs{(const mm_tile_dev*)tiles_dev, res, sz, T, lo, hi};
+  return run_maxima(vs, ntiles, sz, T, T, 1.0f, threshold, delta, coords, cap, counts, ws, ws_bytes, st, marked != 0);
+}
+
 extern "C" int mm_tiles_blur_maxima_window(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles,
                                            const float* res, float* out, int blur_delta, int blo, int bhi,
                                            float threshold, int delta, int lo, int hi, int* coords, int cap,

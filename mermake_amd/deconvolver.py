@@ -396,7 +396,8 @@ class Deconvolver:
 
         ``spectral=True`` with a ``window`` that ``spectral_blur`` admits subtracts the blur through
         the operator (``box_spectra``): the deconvolution computes res - blur at rows and columns
-        [lo, hi) directly, into ``res_out``, with no blur pass (``maxima`` must then be None).
+        [lo, hi) directly, into ``res_out``, with no blur pass; ``maxima`` then rides on the
+        deconvolution (``mm_tiles_deconv_maxima_window``: threshold bits from the inverse y transform).
 
         With a blur the result lives in the spectrum workspace (valid until the next call).
         ``window=(lo, hi)`` computes the blur subtraction only at rows and columns [lo, hi)
@@ -407,8 +408,9 @@ class Deconvolver:
         in the same entry (``mm_tiles_blur_maxima_window``, the outputs of
         ``mm_tiles_maxima_window``)."""
         spectral = bool(spectral) and self.spectral_blur(blur_radius, window)
-        if maxima is not None and (window is None or not self.fused_blur_maxima(blur_radius) or spectral):
-            raise ValueError("maxima in run_tiles need a window, a blur half-width of 15 or 25 and the blur pass")
+        if maxima is not None and (window is None or not (spectral or self.fused_blur_maxima(blur_radius))):
+            raise ValueError("maxima in run_tiles need a window and the blur on the operator or of half-width "
+                             "15 or 25")
         T, sz = self._plan.T, self.tile_height
         spec = self._workspace("spec", ntiles * self._plan.spec_bytes)
         res = res_out if res_out is not None else torch.empty((ntiles, sz, T, T), dtype=torch.float32,
@@ -418,6 +420,11 @@ class Deconvolver:
         ops = self.box_spectra(blur_radius) if spectral else self.psf_fft
         args = (self._plan.h, L.ptr(table), ntiles, L.ptr(ops), ops.shape[0],
                 int(tiles_per_block), L.ptr(spec), L.ptr(res))
+        if spectral and maxima is not None:
+            thr, mdelta, (lo, hi), coords, cap, counts, ws, ws_bytes = maxima
+            L.call("mm_tiles_deconv_maxima_window", *args, int(window[0]), int(window[1]), float(thr), int(mdelta),
+                   int(lo), int(hi), L.ptr(coords), int(cap), L.ptr(counts), L.ptr(ws), int(ws_bytes), st)
+            return res
         if spectral:
             L.call("mm_deconv_tiles_window", *args, int(window[0]), int(window[1]), st)
             return res

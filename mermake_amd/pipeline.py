@@ -153,7 +153,7 @@ class HybPipeline:
         # the blur subtraction rides on the operator, or else the threshold test of the maxima rides
         # on the blur pass (one entry) where the blur width allows
         spectral = self.spectral_blur()
-        fused = not spectral and win is not None and d.fused_blur_maxima(self.blur_radius)
+        fused = win is not None and (spectral or d.fused_blur_maxima(self.blur_radius))
         mx = (self.threshold, self.delta, win[1], coords, cap, counts, ws, ws_bytes) if fused else None
         res = self._buf("res", (ntiles, sz, T, T), torch.float32)
         res = d.run_tiles(table, ntiles, ntiles if single else 1, self.blur_radius, res_out=res,

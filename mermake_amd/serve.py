@@ -235,12 +235,14 @@ class _Slot:
         dargs = (d._plan.h, tab, n, L.ptr(ops), ops.shape[0], n if single else 1, L.ptr(self.spec), L.ptr(res))
         if win is None or delta is None:
             L.call("mm_deconv_tiles", *dargs, st)
-        elif spectral:
-            L.call("mm_deconv_tiles_window", *dargs, win[0][0], win[0][1], st)
+        elif spectral:  # deconvolution minus blur, the threshold bits from its inverse y transform, maxima
+            L.call("mm_tiles_deconv_maxima_window", *dargs, win[0][0], win[0][1], pipe.threshold, pipe.delta,
+                   win[1][0], win[1][1], L.ptr(self.coords), self.cap, L.ptr(counts), L.ptr(self.mws),
+                   self.mws_bytes, st)
         else:
             L.call("mm_deconv_tiles_window", *dargs, max(0, win[0][0] - delta), min(T, win[0][1] + delta), st)
         out = res
-        fused = not spectral and win is not None and d.fused_blur_maxima(pipe.blur_radius)
+        fused = win is not None and (spectral or d.fused_blur_maxima(pipe.blur_radius))
         if delta is not None and not spectral:
             # blur subtraction into the spectrum buffer, which is free again (deconvolver.py:160-163)
             out = self.spec[:n * sz * T * T * 4].view(torch.float32)

@@ -96,7 +96,8 @@ def test_hyb_window_equals_whole_tiles(dev, small_hyb, delta, delta_fit, ts, ov)
     assert len(a) > 50 and np.array_equal(a.view(np.uint32), b.view(np.uint32))  # delta_fit 0 fits NaN columns
     # candidates: whole-tile maxima restricted to the candidate window
     d = win.d
-    geom, table = full._table(*_prepared(d, raw, flat, dev))
+    inputs = _prepared(d, raw, flat, dev)   # the table points into these: keep them alive
+    geom, table = full._table(*inputs)
     n = len(geom)
     (b0, b1), (c0, c1) = win.windows()
     Tt, sz = d._plan.T, d.tile_height
@@ -163,7 +164,8 @@ def test_blur_maxima_entry_equals_separate_calls(dev, small_hyb, delta, blur_rad
     hyb = dict(HYB, delta=delta, delta_fit=3, threshold=thr, blur_radius=blur_radius)
     win = HybPipeline.from_settings(psf, raw.shape, tile_size=ts, overlap=ov, beta=1e-4, **hyb)
     d = win.d
-    geom, table = win._table(*_prepared(d, raw, flat, dev))
+    inputs = _prepared(d, raw, flat, dev)   # the table points into these: keep them alive
+    geom, table = win._table(*inputs)
     n = len(geom)
     (b0, b1), (c0, c1) = win.windows()
     if same_window:
@@ -196,6 +198,86 @@ def test_blur_maxima_entry_equals_separate_calls(dev, small_hyb, delta, blur_rad
     for t in range(n):
         m = min(int(n1[t]), cap)
         assert np.array_equal(k1[t, :m], k2[t, :m]), t
+
+
+@pytest.mark.parametrize("delta,thr,ts,ov,xy", [
+    (1, 2000.0, 150, 38, "rader"),    # T = 226: Rader 113 C2R writes the threshold bits
+    (1, -50.0, 150, 38, "rader"),     # almost every voxel a candidate
+    (5, 1000.0, 300, 89, "rader"),    # T = 478: Rader 239; the largest sphere
+    (2, float("nan"), 300, 89, "rader"),  # NaN threshold rejects nothing
+    (1, 2000.0, 100, 30, None),       # T = 160: prime-first C2R, no bits (whole maxima pass)
+    (1, 2000.0, 150, 38, "stockham")])
+def test_deconv_maxima_entry_equals_separate_calls(dev, small_hyb, delta, thr, ts, ov, xy):
+    """``mm_tiles_deconv_maxima_window`` (threshold bits written by the inverse y transform, maxima from the
+    bits) gives the deconvolved window and the candidate lists of ``mm_deconv_tiles_window`` +
+    ``mm_tiles_maxima_window``, bit for bit, edge tiles included, over a stale workspace."""
+    from mermake_amd import _lib as L
+    from mermake_amd.pipeline import HybPipeline
+    raw, flat, psf = small_hyb
+    hyb = dict(HYB, delta=delta, delta_fit=3, threshold=thr)
+    win = HybPipeline.from_settings(psf, raw.shape, tile_size=ts, overlap=ov, beta=1e-4, **hyb)
+    d = win.d
+    if xy is not None:
+        d.set_schedule(xy=xy)
+    assert win.spectral_blur()
+    inputs = _prepared(d, raw, flat, dev)   # the table points into these: keep them alive
+    geom, table = win._table(*inputs)
+    n = len(geom)
+    (b0, b1), (c0, c1) = win.windows()
+    Tt, sz = d._plan.T, d.tile_height
+    ops = d.box_spectra(30)
+    print("deconv+maxima entry", delta, thr, Tt, d.schedule()["xy"], n, (b0, b1), (c0, c1), flush=True)
+    ws_bytes = L.lib().mm_maxima_ws_bytes(sz, Tt, Tt, n)
+    spec = torch.empty(n * d._plan.spec_bytes, dtype=torch.uint8, device=dev)
+    cap = 1 << 15
+    st = L.stream_ptr(dev)
+    got = []
+    for fused in (False, True):
+        ws = torch.full((ws_bytes,), 0xA5, dtype=torch.uint8, device=dev)  # stale workspace contents
+        res = torch.full((n, sz, Tt, Tt), 7.0, dtype=torch.float32, device=dev)
+        coords = torch.zeros((n, cap, 3), dtype=torch.int32, device=dev)
+        counts = torch.zeros(n, dtype=torch.int32, device=dev)
+        dargs = (d._plan.h, L.ptr(table), n, L.ptr(ops), ops.shape[0], n, L.ptr(spec), L.ptr(res), b0, b1)
+        if fused:
+            L.call("mm_tiles_deconv_maxima_window", *dargs, thr, delta, c0, c1, L.ptr(coords), cap, L.ptr(counts),
+                   L.ptr(ws), ws_bytes, st)
+        else:
+            L.call("mm_deconv_tiles_window", *dargs, st)
+            L.call("mm_tiles_maxima_window", d._plan.h, L.ptr(table), n, L.ptr(res), thr, delta, c0, c1,
+                   L.ptr(coords), cap, L.ptr(counts), L.ptr(ws), ws_bytes, st)
+        torch.cuda.synchronize(dev)
+        got.append((res[:, :, b0:b1, b0:b1].cpu().numpy(), coords.cpu().numpy(), counts.cpu().numpy()))
+    (o1, k1, n1), (o2, k2, n2) = got
+    assert np.array_equal(o1.view(np.uint32), o2.view(np.uint32))
+    assert np.array_equal(n1, n2) and n1.sum() > 0
+    for t in range(n):
+        m = min(int(n1[t]), cap)
+        assert np.array_equal(k1[t, :m], k2[t, :m]), t
+
+
+def test_deconv_maxima_entry_rejects_reads_outside_window(dev, small_hyb):
+    """Maxima whose neighbour tests would read outside the computed window are refused."""
+    from mermake_amd import _lib as L
+    from mermake_amd.pipeline import HybPipeline
+    raw, flat, psf = small_hyb
+    win = HybPipeline.from_settings(psf, raw.shape, tile_size=150, overlap=38, beta=1e-4, **HYB)
+    d = win.d
+    inputs = _prepared(d, raw, flat, dev)   # the table points into these: keep them alive
+    geom, table = win._table(*inputs)
+    n = len(geom)
+    (b0, b1), _ = win.windows()
+    Tt, sz = d._plan.T, d.tile_height
+    ops = d.box_spectra(30)
+    ws_bytes = L.lib().mm_maxima_ws_bytes(sz, Tt, Tt, n)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+    spec = torch.empty(n * d._plan.spec_bytes, dtype=torch.uint8, device=dev)
+    res = torch.empty((n, sz, Tt, Tt), dtype=torch.float32, device=dev)
+    coords = torch.zeros((n, 16, 3), dtype=torch.int32, device=dev)
+    counts = torch.zeros(n, dtype=torch.int32, device=dev)
+    with pytest.raises(L.MermakeLibraryError, match="reads outside"):
+        L.call("mm_tiles_deconv_maxima_window", d._plan.h, L.ptr(table), n, L.ptr(ops), 1, n, L.ptr(spec),
+               L.ptr(res), b0, b1, 2000.0, 2, b0 + 1, b1 - 2, L.ptr(coords), 16, L.ptr(counts), L.ptr(ws), ws_bytes,
+               L.stream_ptr(dev))
 
 
 @pytest.mark.parametrize("ts,ov", [(100, 30), (200, 13), (300, 89)])  # prime-first, Rader 113, Rader 239 loaders
