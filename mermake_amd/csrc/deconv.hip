@@ -799,7 +799,7 @@ __device__ __forceinline__ RdCols<NC> rd_cols(const int* __restrict__ slot_in) {
 }
 
 // Raw samples of the R2C group: pixel bits and flat values of row i (2G rows of
-// the padded tile from q0), column k; row i exists and maps to data <=> bit i of
+// the padded tile from x0), column k; row i exists and maps to data <=> bit i of
 // rowok (uniform), column k maps to data <=> bit k of colok.
 template <int NR, int NC>
 struct RdR2cRaw {
@@ -809,56 +809,79 @@ struct RdR2cRaw {
   static_assert(NR <= 32 && NC <= 32, "validity masks are 32-bit");
 };
 
+// What a work item (tile, band of NR rows) fixes for all its planes: the data
+// column of each of the thread's columns (absent columns read data column 0),
+// and each row's offset in plane 0 of the stack (absent rows read the stack's
+// first row).
+template <int NR, int NC>
+struct RdR2cItem {
+  unsigned c[NC];
+  long long ro[NR];
+};
+
 __device__ __forceinline__ long long rd_readlane64(long long v, int lane) {
   const unsigned lo = __builtin_amdgcn_readlane((unsigned)(unsigned long long)v, lane);
   const unsigned hi = __builtin_amdgcn_readlane((unsigned)((unsigned long long)v >> 32), lane);
   return (long long)(((unsigned long long)hi << 32) | lo);
 }
 
-// Rows x0 .. x0 + NR - 1 of plane z (those < T exist).
-template <int PR, int M, int NR, int NC, int PIX, bool FLAT>
-__device__ __forceinline__ void rd_r2c_load(RdR2cRaw<NR, NC>& R, const mm_tile_dev& td, const RdCols<NC>& C, int T,
-                                            int z, int x0) {
-  unsigned c[NC];  // column offsets in elements (absent columns read data column 0 of the row)
+// Set up rows x0 .. x0 + NR - 1 of tile td (those < T exist): the column and row
+// maps, and the flat values of every sample (the flat is the same in every plane).
+template <int PR, int M, int NR, int NC, bool FLAT>
+__device__ __forceinline__ void rd_r2c_item(RdR2cItem<NR, NC>& It, RdR2cRaw<NR, NC>& R, const mm_tile_dev& td,
+                                            const RdCols<NC>& C, int T, int x0) {
   R.colok = 0;
 #pragma unroll
   for (int k = 0; k < NC; ++k) {
     const int cc = C.n[k] < PR * M ? pad_map(C.n[k], td.ys, td.yd, td.yfirst, T) : -1;
     R.colok |= (cc >= 0 ? 1u : 0u) << k;
-    c[k] = cc < 0 ? 0u : (unsigned)cc;
+    It.c[k] = cc < 0 ? 0u : (unsigned)cc;
   }
   // Row i of the group is worked out by lane i (vector work in parallel instead
   // of NR rounds of scalar 64-bit arithmetic), then broadcast with readlane.
   const int lane = threadIdx.x & 63;
-  long long ro_l = 0, fo_l = 0;  // absent rows read the stack's first row
+  long long ro_l = 0, fo_l = 0;
   bool ok_l = false;
   if (lane < NR) {
     const int x = x0 + lane;
     const int rr = x < T ? pad_map(x, td.xs, td.xd, td.xfirst, T) : -1;
     ok_l = rr >= 0;
     if (ok_l) {
-      ro_l = (long long)z * td.img_sz + (long long)(td.X0 + rr) * td.img_sx + td.Y0;
+      ro_l = (long long)(td.X0 + rr) * td.img_sx + td.Y0;
       fo_l = (long long)(td.X0 + rr) * td.flat_sx + td.Y0;
     }
   }
   R.rowok = (unsigned)__ballot(ok_l);  // lanes 0..NR-1 (NR <= 32)
 #pragma unroll
   for (int i = 0; i < NR; ++i) {
-    const long long ro = rd_readlane64(ro_l, i), fo = rd_readlane64(fo_l, i);
+    It.ro[i] = rd_readlane64(ro_l, i);
+    if (FLAT) {
+      const gptr<const char> fb = (gptr<const char>)td.flat + rd_readlane64(fo_l, i) * 4;
+#pragma unroll
+      for (int k = 0; k < NC; ++k) R.F[i][k] = *(gptr<const float>)(fb + It.c[k] * 4u);
+    }
+  }
+}
+
+// The pixels of the item's rows in plane z.
+template <int NR, int NC, int PIX>
+__device__ __forceinline__ void rd_r2c_pix(RdR2cRaw<NR, NC>& R, const RdR2cItem<NR, NC>& It, const mm_tile_dev& td,
+                                           int z) {
+  const long long zo = (long long)z * td.img_sz;
+#pragma unroll
+  for (int i = 0; i < NR; ++i) {
     // uniform row pointers in the global address space: global loads with a
     // scalar base and a 32-bit lane offset
     constexpr int ES = PIX == MM_PIX_F32 ? 4 : PIX == MM_PIX_U16 ? 2 : 1;  // element bytes
-    const gptr<const char> rb = (gptr<const char>)td.img + ro * ES;
-    const gptr<const char> fb = (gptr<const char>)td.flat + fo * 4;
+    const gptr<const char> rb = (gptr<const char>)td.img + (zo + It.ro[i]) * ES;
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
-      if (PIX == MM_PIX_F32) R.P[i][k] = *(gptr<const unsigned>)(rb + c[k] * 4u);
-      else if (PIX == MM_PIX_U16) R.P[i][k] = (unsigned)*(gptr<const uint16_t>)(rb + c[k] * 2u);
+      if (PIX == MM_PIX_F32) R.P[i][k] = *(gptr<const unsigned>)(rb + It.c[k] * 4u);
+      else if (PIX == MM_PIX_U16) R.P[i][k] = (unsigned)*(gptr<const uint16_t>)(rb + It.c[k] * 2u);
       else {
-        const unsigned u = *(gptr<const uint8_t>)(rb + c[k]);
+        const unsigned u = *(gptr<const uint8_t>)(rb + It.c[k]);
         R.P[i][k] = u * u;
       }
-      if (FLAT) R.F[i][k] = *(gptr<const float>)(fb + c[k] * 4u);
     }
   }
 }
@@ -878,10 +901,15 @@ __device__ __forceinline__ void rd_r2c_store(float2* __restrict__ bufA, const Rd
     }
 }
 
+// Work items (tile, band of NR rows, chunk of zc planes); each block runs the
+// planes of its item in turn, so the column and row maps and the flat values are
+// read once per item (the flat is the same in every plane), and prefetches the
+// next plane's pixels (the next item's first plane) during the current plane's
+// stages.  zc = sz unless the batch has too few (tile, band) pairs to fill the GPU.
 template <int PR, int M>
 __global__ __launch_bounds__(256, 2) void k_rd_r2c(const mm_tile_dev* __restrict__ tiles, int ntiles, int T, int nrows,
                                                 float2* Sout, long long s_tstride, SpecLayout Ls, RdTab tab,
-                                                const float2* __restrict__ tw) {
+                                                const float2* __restrict__ tw, int zc) {
   constexpr int G = rd_row_g<PR>(M, false), NC = rd_row_nc<PR>(M), NR = 2 * G;
   static_assert(G * M * RdSpec<PR>::B <= 256, "in-place Rader stages: one item per thread");
   extern __shared__ __align__(16) unsigned char smem[];
@@ -889,78 +917,84 @@ __global__ __launch_bounds__(256, 2) void k_rd_r2c(const mm_tile_dev* __restrict
   const RdLds L = rd_carve<PR>(smem, N, M, G, tw, tab);
   __syncthreads();  // slot_in
   const RdCols<NC> C = rd_cols<PR, M, NC>(L.slot_in);
-  // Group (tile, band of NR rows, plane) with the plane fastest: consecutive
-  // groups read the same flat-field rows, which then stay in L2.
-  const int sz = nrows / T, nband = (T + NR - 1) / NR;
-  const long long ngrp = (long long)sz * nband * ntiles;
-  auto where = [&](long long gi, int& tile, int& z, int& x0) {
-    tile = (int)(gi / ((long long)sz * nband));
-    const int rem = (int)(gi - (long long)tile * sz * nband);
-    const int band = rem / sz;
-    z = rem - band * sz;
-    x0 = band * NR;
-  };
+  const int sz = nrows / T, nband = (T + NR - 1) / NR, nch = (sz + zc - 1) / zc;
+  const long long nitem = (long long)nband * ntiles * nch;
   RdR2cRaw<NR, NC> R;
-  auto fetch = [&](long long gi) {
-    int tile, z, x0;
-    where(gi, tile, z, x0);
+  RdR2cItem<NR, NC> It;
+  int tile = 0, x0 = 0, z0 = 0, z1 = 0, pix = MM_PIX_U16;
+  bool flat = false;
+  // the item's maps and flat values
+  auto setup = [&](long long item) {
+    const long long tb = item / nch;
+    const int ch = (int)(item - tb * nch);
+    tile = (int)(tb / nband);
+    x0 = (int)(tb - (long long)tile * nband) * NR;
+    z0 = ch * zc;
+    z1 = min(sz, z0 + zc);
     const mm_tile_dev& td = tiles[tile];
-    if (td.img_type == MM_PIX_F32) {
-      if (td.flat) rd_r2c_load<PR, M, NR, NC, MM_PIX_F32, true>(R, td, C, T, z, x0);
-      else rd_r2c_load<PR, M, NR, NC, MM_PIX_F32, false>(R, td, C, T, z, x0);
-    } else if (td.img_type == MM_PIX_U8SQ) {
-      if (td.flat) rd_r2c_load<PR, M, NR, NC, MM_PIX_U8SQ, true>(R, td, C, T, z, x0);
-      else rd_r2c_load<PR, M, NR, NC, MM_PIX_U8SQ, false>(R, td, C, T, z, x0);
-    } else {
-      if (td.flat) rd_r2c_load<PR, M, NR, NC, MM_PIX_U16, true>(R, td, C, T, z, x0);
-      else rd_r2c_load<PR, M, NR, NC, MM_PIX_U16, false>(R, td, C, T, z, x0);
-    }
+    pix = td.img_type;
+    flat = td.flat != nullptr;
+    if (flat) rd_r2c_item<PR, M, NR, NC, true>(It, R, td, C, T, x0);
+    else rd_r2c_item<PR, M, NR, NC, false>(It, R, td, C, T, x0);
   };
-  auto tile_of = [&](long long gi) { return (int)(gi / ((long long)sz * nband)); };
-  auto put = [&](long long gi) {
-    const mm_tile_dev& td = tiles[tile_of(gi)];
-    if (td.img_type == MM_PIX_F32) {
-      if (td.flat) rd_r2c_store<PR, M, NR, NC, true, true>(L.bufA, R, C);
+  auto fetch = [&](int z) {
+    const mm_tile_dev& td = tiles[tile];
+    if (pix == MM_PIX_F32) rd_r2c_pix<NR, NC, MM_PIX_F32>(R, It, td, z);
+    else if (pix == MM_PIX_U8SQ) rd_r2c_pix<NR, NC, MM_PIX_U8SQ>(R, It, td, z);
+    else rd_r2c_pix<NR, NC, MM_PIX_U16>(R, It, td, z);
+  };
+  auto put = [&]() {
+    if (pix == MM_PIX_F32) {
+      if (flat) rd_r2c_store<PR, M, NR, NC, true, true>(L.bufA, R, C);
       else rd_r2c_store<PR, M, NR, NC, true, false>(L.bufA, R, C);
     } else {
-      if (td.flat) rd_r2c_store<PR, M, NR, NC, false, true>(L.bufA, R, C);
+      if (flat) rd_r2c_store<PR, M, NR, NC, false, true>(L.bufA, R, C);
       else rd_r2c_store<PR, M, NR, NC, false, false>(L.bufA, R, C);
     }
   };
-  const int H = N / 2 + 1;
-  long long grp = xcd_remap(blockIdx.x, gridDim.x);
-  if (grp < ngrp) fetch(grp);
-  for (; grp < ngrp; grp += gridDim.x) {
-    put(grp);
-    __syncthreads();
-    if (grp + gridDim.x < ngrp) fetch(grp + gridDim.x);
-    rd_stages<PR, M, false, false>(L, G, RdMid{});
-    int tile, z, x0;
-    where(grp, tile, z, x0);
-    float2* out = Sout + (long long)tile * s_tstride + z * Ls.plane;
-    auto rows = [&](int g, float2*& pa, float2*& pb) {
-      const int xa = x0 + 2 * g;
-      pa = xa < T ? out + Ls.row(xa) : nullptr;
-      pb = xa + 1 < T ? out + Ls.row(xa + 1) : nullptr;
-    };
-    if (Ls.blk == 6) {  // the M = 6 plan: constant block width, the division is a multiply-shift
-      const int bstr = T * 6;
-      pf_final_r2c<M>(PfShape(N, PR), (const float2*)L.bufB, L.tw, G, rows, [&](int n) {
-        const int b = (int)(__umul24((unsigned)n, 43691u) >> 18);  // n / 6 for n < 2^15
-        return __mul24(b, bstr) + (n - 6 * b);
-      });
-    } else if (Ls.blk) {
-      const FastDiv fb(Ls.blk);
-      const int bstr = T * Ls.blk;
-      pf_final_r2c<M>(PfShape(N, PR), (const float2*)L.bufB, L.tw, G, rows, [&](int n) {
-        int b, r;
-        fb.divmod(n, b, r);
-        return b * bstr + r;
-      });
-    } else {
-      pf_final_r2c<M>(PfShape(N, PR), (const float2*)L.bufB, L.tw, G, rows);
+  long long item = xcd_remap(blockIdx.x, gridDim.x);
+  if (item >= nitem) return;  // uniform over the block
+  setup(item);
+  fetch(z0);
+  for (;;) {
+    const int ctile = tile, cx0 = x0, cz1 = z1;
+    for (int z = z0; z < cz1; ++z) {
+      put();
+      __syncthreads();
+      if (z + 1 < cz1) {
+        fetch(z + 1);
+      } else if (item + gridDim.x < nitem) {
+        setup(item + gridDim.x);
+        fetch(z0);
+      }
+      rd_stages<PR, M, false, false>(L, G, RdMid{});
+      float2* out = Sout + (long long)ctile * s_tstride + z * Ls.plane;
+      auto rows = [&](int g, float2*& pa, float2*& pb) {
+        const int xa = cx0 + 2 * g;
+        pa = xa < T ? out + Ls.row(xa) : nullptr;
+        pb = xa + 1 < T ? out + Ls.row(xa + 1) : nullptr;
+      };
+      if (Ls.blk == 6) {  // the M = 6 plan: constant block width, the division is a multiply-shift
+        const int bstr = T * 6;
+        pf_final_r2c<M>(PfShape(N, PR), (const float2*)L.bufB, L.tw, G, rows, [&](int n) {
+          const int b = (int)(__umul24((unsigned)n, 43691u) >> 18);  // n / 6 for n < 2^15
+          return __mul24(b, bstr) + (n - 6 * b);
+        });
+      } else if (Ls.blk) {
+        const FastDiv fb(Ls.blk);
+        const int bstr = T * Ls.blk;
+        pf_final_r2c<M>(PfShape(N, PR), (const float2*)L.bufB, L.tw, G, rows, [&](int n) {
+          int b, r;
+          fb.divmod(n, b, r);
+          return b * bstr + r;
+        });
+      } else {
+        pf_final_r2c<M>(PfShape(N, PR), (const float2*)L.bufB, L.tw, G, rows);
+      }
+      __syncthreads();  // the final stage's reads before the next group's stores
     }
-    __syncthreads();  // the final stage's reads before the next group's stores
+    item += gridDim.x;
+    if (item >= nitem) break;
   }
 }
 
@@ -1155,9 +1189,15 @@ static void launch_rd_r2c(const mm_tile_dev* td, int ntiles, int T, int nrows, f
                           const SpecLayout& Ls, const RdTab& tab, const float2* tw, hipStream_t st) {
   constexpr int G = rd_row_g<PR>(M, false);
   const size_t lds = rd_lds_bytes<PR>(T, M, G, 0);
-  const long long ngrp = (long long)(nrows / T) * ((T + 2 * G - 1) / (2 * G)) * ntiles;
-  hipLaunchKernelGGL((k_rd_r2c<PR, M>), dim3(pf_grid(ngrp, lds)), dim3(256), lds, st, td, ntiles, T, nrows, S, s_ts,
-                     Ls, tab, tw);
+  // items of (tile, band) run all planes, unless they are too few to fill the GPU: then chunks of planes
+  const int sz = nrows / T;
+  const long long nb = (long long)((T + 2 * G - 1) / (2 * G)) * ntiles;
+  const int want = (int)((2048 + nb - 1) / nb);
+  const int nch = want < 1 ? 1 : (want > sz ? sz : want);
+  const int zc = (sz + nch - 1) / nch;
+  const long long nitem = nb * ((sz + zc - 1) / zc);
+  hipLaunchKernelGGL((k_rd_r2c<PR, M>), dim3(pf_grid(nitem, lds)), dim3(256), lds, st, td, ntiles, T, nrows, S, s_ts,
+                     Ls, tab, tw, zc);
 }
 
 template <int PR, int M>
