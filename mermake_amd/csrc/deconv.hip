@@ -778,7 +778,7 @@ constexpr int rd_row_g(int M, bool c2r) {
   return PR == 239 ? (c2r ? 3 : 4) : (M == 2 ? 16 : (M == 6 ? (c2r ? 2 : 3) : 9));
 }
 template <int PR>
-constexpr int rd_row_nc(int M) { return (PR * M + 255) / 256; }
+constexpr int rd_row_nc(int M, int bs = 256) { return (PR * M + bs - 1) / bs; }
 
 template <int NC>
 struct RdCols {
@@ -786,12 +786,12 @@ struct RdCols {
   int cs[NC];  // LDS slot of the column within one transform's M * S block, -1 for n >= N
 };
 
-template <int PR, int M, int NC>
+template <int PR, int M, int NC, int BS = 256>
 __device__ __forceinline__ RdCols<NC> rd_cols(const int* __restrict__ slot_in) {
   RdCols<NC> c;
 #pragma unroll
   for (int k = 0; k < NC; ++k) {
-    const int n = threadIdx.x + k * 256;
+    const int n = threadIdx.x + k * BS;
     c.n[k] = n;
     c.cs[k] = n < PR * M ? (n % M) * RdSpec<PR>::S + slot_in[n / M] : -1;
   }
@@ -906,17 +906,17 @@ __device__ __forceinline__ void rd_r2c_store(float2* __restrict__ bufA, const Rd
 // read once per item (the flat is the same in every plane), and prefetches the
 // next plane's pixels (the next item's first plane) during the current plane's
 // stages.  zc = sz unless the batch has too few (tile, band) pairs to fill the GPU.
-template <int PR, int M>
-__global__ __launch_bounds__(256, 2) void k_rd_r2c(const mm_tile_dev* __restrict__ tiles, int ntiles, int T, int nrows,
-                                                float2* Sout, long long s_tstride, SpecLayout Ls, RdTab tab,
-                                                const float2* __restrict__ tw, int zc) {
-  constexpr int G = rd_row_g<PR>(M, false), NC = rd_row_nc<PR>(M), NR = 2 * G;
-  static_assert(G * M * RdSpec<PR>::B <= 256, "in-place Rader stages: one item per thread");
+template <int PR, int M, int BS = 256>
+__global__ __launch_bounds__(BS, 2) void k_rd_r2c(const mm_tile_dev* __restrict__ tiles, int ntiles, int T,
+                                                       int nrows, float2* Sout, long long s_tstride, SpecLayout Ls,
+                                                       RdTab tab, const float2* __restrict__ tw, int zc) {
+  constexpr int G = rd_row_g<PR>(M, false), NC = rd_row_nc<PR>(M, BS), NR = 2 * G;
+  static_assert(G * M * RdSpec<PR>::B <= BS, "in-place Rader stages: one item per thread");
   extern __shared__ __align__(16) unsigned char smem[];
   const int N = PR * M;
   const RdLds L = rd_carve<PR>(smem, N, M, G, tw, tab);
   __syncthreads();  // slot_in
-  const RdCols<NC> C = rd_cols<PR, M, NC>(L.slot_in);
+  const RdCols<NC> C = rd_cols<PR, M, NC, BS>(L.slot_in);
   const int sz = nrows / T, nband = (T + NR - 1) / NR, nch = (sz + zc - 1) / zc;
   const long long nitem = (long long)nband * ntiles * nch;
   RdR2cRaw<NR, NC> R;
@@ -1004,14 +1004,15 @@ __global__ __launch_bounds__(256, 2) void k_rd_r2c(const mm_tile_dev* __restrict
 // stored output, as ww 64-bit words per row (bit c & 63 of word c >> 6 for
 // column c; zero for the columns not stored) at marks[(tile * N + z) * N + x]
 // (the row layout of the maxima workspace), for every computed row.
-template <int PR, int M, bool MARK = false>
-__global__ __launch_bounds__(256, 2) void k_rd_c2r(const float2* Sin, long long s_tstride, SpecLayout Ls, float* res,
-                                                long long r_tstride, int ntiles, int nrows, int xlo, int nx, int clo,
-                                                int chi, RdTab tab, const float2* __restrict__ tw,
-                                                unsigned long long* __restrict__ marks = nullptr, float thr = 0.0f,
-                                                int ww = 0) {
-  constexpr int G = rd_row_g<PR>(M, true), NC = rd_row_nc<PR>(M), NR = 2 * G;
-  static_assert(G * M * RdSpec<PR>::B <= 256, "in-place Rader stages: one item per thread");
+template <int PR, int M, bool MARK = false, int BS = 256>
+__global__ __launch_bounds__(BS, BS == 128 ? 4 : 2) void k_rd_c2r(const float2* Sin, long long s_tstride, SpecLayout Ls,
+                                                       float* res, long long r_tstride, int ntiles, int nrows, int xlo,
+                                                       int nx, int clo, int chi, RdTab tab,
+                                                       const float2* __restrict__ tw,
+                                                       unsigned long long* __restrict__ marks = nullptr,
+                                                       float thr = 0.0f, int ww = 0) {
+  constexpr int G = rd_row_g<PR>(M, true), NC = rd_row_nc<PR>(M, BS), NR = 2 * G;
+  static_assert(G * M * RdSpec<PR>::B <= BS, "in-place Rader stages: one item per thread");
   extern __shared__ __align__(16) unsigned char smem[];
   const int N = PR * M, H = N / 2 + 1;
   const RdLds L = rd_carve<PR>(smem, N, M, G, tw, tab);
@@ -1020,7 +1021,7 @@ __global__ __launch_bounds__(256, 2) void k_rd_c2r(const float2* Sin, long long 
   if (MARK)
     for (int i = threadIdx.x; i < NR * ww; i += blockDim.x) wl[i] = 0ull;
   __syncthreads();
-  const RdCols<NC> C = rd_cols<PR, M, NC>(L.slot_in);
+  const RdCols<NC> C = rd_cols<PR, M, NC, BS>(L.slot_in);
   int kk[NC];  // in-row offset of the half-spectrum bin of column n (bin 0 for the absent columns n >= N)
 #pragma unroll
   for (int k = 0; k < NC; ++k) {
@@ -1133,10 +1134,10 @@ static size_t pf_r2c_extra(int T, int G) { return 2 * (mm_align16((size_t)T * 4)
 
 // Grid of a persistent prime-first pass: as many blocks as fit on the chip at
 // once (LDS bound), at most one per group.
-static unsigned pf_grid(long long ngrp, size_t lds) {
+static unsigned pf_grid(long long ngrp, size_t lds, int max_per_cu = 4) {
   long long per_cu = (long long)(MM_LDS_MAX / (lds ? lds : 1));
   if (per_cu < 1) per_cu = 1;
-  if (per_cu > 4) per_cu = 4;
+  if (per_cu > max_per_cu) per_cu = max_per_cu;
   const long long g = per_cu * 256;
   return (unsigned)(ngrp < g ? ngrp : g);
 }
@@ -1189,6 +1190,9 @@ static void launch_rd_r2c(const mm_tile_dev* td, int ntiles, int T, int nrows, f
                           const SpecLayout& Ls, const RdTab& tab, const float2* tw, hipStream_t st) {
   constexpr int G = rd_row_g<PR>(M, false);
   const size_t lds = rd_lds_bytes<PR>(T, M, G, 0);
+  // 256 threads (4 waves per SIMD at 115 VGPRs); blocks of 128 (161 VGPRs, 3 waves per SIMD) measured
+  // 1.45 -> 1.52 ms per 36-tile stack
+  constexpr int BS = 256;
   // items of (tile, band) run all planes, unless they are too few to fill the GPU: then chunks of planes
   const int sz = nrows / T;
   const long long nb = (long long)((T + 2 * G - 1) / (2 * G)) * ntiles;
@@ -1196,8 +1200,8 @@ static void launch_rd_r2c(const mm_tile_dev* td, int ntiles, int T, int nrows, f
   const int nch = want < 1 ? 1 : (want > sz ? sz : want);
   const int zc = (sz + nch - 1) / nch;
   const long long nitem = nb * ((sz + zc - 1) / zc);
-  hipLaunchKernelGGL((k_rd_r2c<PR, M>), dim3(pf_grid(nitem, lds)), dim3(256), lds, st, td, ntiles, T, nrows, S, s_ts,
-                     Ls, tab, tw, zc);
+  hipLaunchKernelGGL((k_rd_r2c<PR, M, BS>), dim3(pf_grid(nitem, lds, 512 / BS * 2)), dim3(BS), lds, st, td, ntiles, T,
+                     nrows, S, s_ts, Ls, tab, tw, zc);
 }
 
 template <int PR, int M>
@@ -1205,15 +1209,18 @@ static void launch_rd_c2r(const float2* S, long long s_ts, const SpecLayout& Ls,
                           int ntiles, int sz, int lo, int hi, const RdTab& tab, const float2* tw, hipStream_t st,
                           unsigned long long* marks = nullptr, float thr = 0.0f) {
   constexpr int G = rd_row_g<PR>(M, true);
+  // blocks of 128 threads when the group's in-place stages fit them (fewer idle lanes in stages 1 and 3)
+  constexpr int BS = G * M * RdSpec<PR>::B <= 128 ? 128 : 256;
   const int ww = (PR * M + 63) / 64;
   const size_t lds = rd_lds_bytes<PR>(PR * M, M, G, marks ? (size_t)8 * 2 * G * ww : 0);
   const int nrows = sz * (hi - lo);
   const long long ngrp = (long long)((nrows + 2 * G - 1) / (2 * G)) * ntiles;
+  const unsigned grid = pf_grid(ngrp, lds, 512 / BS * 2);
   if (marks)
-    hipLaunchKernelGGL((k_rd_c2r<PR, M, true>), dim3(pf_grid(ngrp, lds)), dim3(256), lds, st, S, s_ts, Ls, res, r_ts,
-                       ntiles, nrows, lo, hi - lo, lo, hi, tab, tw, marks, thr, ww);
+    hipLaunchKernelGGL((k_rd_c2r<PR, M, true, BS>), dim3(grid), dim3(BS), lds, st, S, s_ts, Ls, res, r_ts, ntiles,
+                       nrows, lo, hi - lo, lo, hi, tab, tw, marks, thr, ww);
   else
-    hipLaunchKernelGGL((k_rd_c2r<PR, M>), dim3(pf_grid(ngrp, lds)), dim3(256), lds, st, S, s_ts, Ls, res, r_ts, ntiles,
+    hipLaunchKernelGGL((k_rd_c2r<PR, M, false, BS>), dim3(grid), dim3(BS), lds, st, S, s_ts, Ls, res, r_ts, ntiles,
                        nrows, lo, hi - lo, lo, hi, tab, tw, nullptr, 0.0f, 0);
 }
 
