@@ -1015,7 +1015,7 @@ __global__ __launch_bounds__(BS, 2) void k_rd_r2c(const mm_tile_dev* __restrict_
 // column c; zero for the columns not stored) at marks[(tile * N + z) * N + x]
 // (the row layout of the maxima workspace), for every computed row.
 template <int PR, int M, bool MARK = false, int BS = 256>
-__global__ __launch_bounds__(BS, BS == 128 ? 4 : 2) void k_rd_c2r(const float2* Sin, long long s_tstride, SpecLayout Ls,
+__global__ __launch_bounds__(BS, BS == 128 && PR == 113 ? 4 : 2) void k_rd_c2r(const float2* Sin, long long s_tstride, SpecLayout Ls,
                                                        float* res, long long r_tstride, int ntiles, int nrows, int xlo,
                                                        int nx, int clo, int chi, RdTab tab,
                                                        const float2* __restrict__ tw,

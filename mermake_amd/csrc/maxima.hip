@@ -484,7 +484,11 @@ __device__ __forceinline__ Sample get_sample(const VolDesc& V, float sign, const
   return s;
 }
 
-template <class VS, class RS>
+// NCH > 0: the candidate's samples (at most NCH chunks of 64) are gathered
+// once and kept in registers for the four passes; NCH = 0 gathers them again
+// in every pass (spheres of more than 9 chunks).  The same arithmetic in the
+// same order either way.
+template <class VS, class RS, int NCH>
 __global__ __launch_bounds__(256) void k_fit(VS vs, float sign, RS raw, const int* __restrict__ coords,
                                              const int* __restrict__ counts, int cap, const int* __restrict__ offs,
                                              int P, float sigmaZ, float sigmaXY, float* __restrict__ out) {
@@ -496,15 +500,28 @@ __global__ __launch_bounds__(256) void k_fit(VS vs, float sign, RS raw, const in
   // reference expression dz*dz / (2*sigmaZ*sigmaZ) as a multiply by the reciprocal
   const float isz = 1.0f / (2.0f * sigmaZ * sigmaZ);
   const float isxy = 1.0f / (2.0f * sigmaXY * sigmaXY);
-  const int nch = (P + 63) / 64;
+  const int nch = NCH ? NCH : (P + 63) / 64;
   for (int i = blockIdx.x * 4 + wave; i < n; i += gridDim.x * 4) {
     const int* c = coords + ((long long)v * cap + i) * 3;
     const int z0 = c[0], x0 = c[1], y0 = c[2];
+    Sample S[NCH ? NCH : 1];
+    int run = 0;
+    auto sample = [&](int ch) -> Sample {
+      if (NCH) return S[ch];
+      return get_sample(V, sign, raw, rc, offs, P, ch * 64 + lane, z0, x0, y0, isz, isxy, run);
+    };
+    if (NCH) {
+#pragma unroll
+      for (int ch = 0; ch < (NCH ? NCH : 1); ++ch)
+        S[ch] = get_sample(V, sign, raw, rc, offs, P, ch * 64 + lane, z0, x0, y0, isz, isxy, run);
+    }
     // pass 1: count, sum G, min
-    int cnt = 0, run = 0;
+    int cnt = 0;
     float sg = 0.f, mn = 3.0e38f;
+    run = 0;
+#pragma unroll
     for (int ch = 0; ch < nch; ++ch) {
-      const Sample s = get_sample(V, sign, raw, rc, offs, P, ch * 64 + lane, z0, x0, y0, isz, isxy, run);
+      const Sample s = sample(ch);
       if (s.ok) { cnt += 1; sg += s.g; mn = fminf(mn, s.v); }
     }
     cnt = wsumi(cnt);
@@ -520,8 +537,9 @@ __global__ __launch_bounds__(256) void k_fit(VS vs, float sign, RS raw, const in
     // pass 2: var G, sums of s and raw, centroid moments
     float vg = 0.f, ss = 0.f, sr = 0.f, mz = 0.f, mx = 0.f, my = 0.f;
     run = 0;
+#pragma unroll
     for (int ch = 0; ch < nch; ++ch) {
-      const Sample s = get_sample(V, sign, raw, rc, offs, P, ch * 64 + lane, z0, x0, y0, isz, isxy, run);
+      const Sample s = sample(ch);
       if (s.ok) {
         const float dg = s.g - mg;
         vg += dg * dg;
@@ -539,8 +557,9 @@ __global__ __launch_bounds__(256) void k_fit(VS vs, float sign, RS raw, const in
     // pass 3: variances of s and raw
     float vs2 = 0.f, vr2 = 0.f;
     run = 0;
+#pragma unroll
     for (int ch = 0; ch < nch; ++ch) {
-      const Sample s = get_sample(V, sign, raw, rc, offs, P, ch * 64 + lane, z0, x0, y0, isz, isxy, run);
+      const Sample s = sample(ch);
       if (s.ok) {
         const float a = (s.v - bg) - ms, b = s.r - mr;
         vs2 += a * a;
@@ -554,8 +573,9 @@ __global__ __launch_bounds__(256) void k_fit(VS vs, float sign, RS raw, const in
     // pass 4: correlations
     float hn = 0.f, aa = 0.f;
     run = 0;
+#pragma unroll
     for (int ch = 0; ch < nch; ++ch) {
-      const Sample s = get_sample(V, sign, raw, rc, offs, P, ch * 64 + lane, z0, x0, y0, isz, isxy, run);
+      const Sample s = sample(ch);
       if (s.ok) {
         const float ng = (s.g - mg) / stdg;
         hn += ((s.v - bg) - ms) / ds * ng;
@@ -581,6 +601,23 @@ __global__ __launch_bounds__(256) void k_fit(VS vs, float sign, RS raw, const in
       o[7] = sign * V.p[ci];
     }
   }
+}
+
+// k_fit with the samples of spheres of up to 2 (delta_fit <= 3) or 9 (delta_fit
+// <= 5) chunks of 64 held in registers.
+template <class VS, class RS>
+static void launch_fit(const VS& vs, float sign, const RS& rs, const int* coords, const int* counts, int cap,
+                       const int* offs, int P, float sigmaZ, float sigmaXY, float* out8, dim3 grid, hipStream_t st) {
+  const int nch = (P + 63) / 64;
+  if (nch <= 2)
+    hipLaunchKernelGGL((k_fit<VS, RS, 2>), grid, dim3(256), 0, st, vs, sign, rs, coords, counts, cap, offs, P, sigmaZ,
+                       sigmaXY, out8);
+  else if (nch <= 9)
+    hipLaunchKernelGGL((k_fit<VS, RS, 9>), grid, dim3(256), 0, st, vs, sign, rs, coords, counts, cap, offs, P, sigmaZ,
+                       sigmaXY, out8);
+  else
+    hipLaunchKernelGGL((k_fit<VS, RS, 0>), grid, dim3(256), 0, st, vs, sign, rs, coords, counts, cap, offs, P, sigmaZ,
+                       sigmaXY, out8);
 }
 
 // ------------------------------------------------------------- keep filter
@@ -789,8 +826,7 @@ extern "C" int mm_delta_fit(const float* img, int D, int H, int W, long long sZ,
   int blocks = (cap + 3) / 4;
   if (blocks > 2048) blocks = 2048;
   if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL((k_fit<OneVol, RawPlain>), dim3(blocks, 1), dim3(256), 0, (hipStream_t)stream, vs, sign, rs,
-                     coords, count, cap, offs, P, sigmaZ, sigmaXY, out8);
+  launch_fit(vs, sign, rs, coords, count, cap, offs, P, sigmaZ, sigmaXY, out8, dim3(blocks, 1), (hipStream_t)stream);
   MM_LAUNCH_CHECK();
   return MM_OK;
 }
@@ -889,8 +925,8 @@ extern "C" int mm_tiles_fit(mm_deconv* d, const mm_tile_desc* tiles_dev, int nti
   RawTile rs{(const mm_tile_dev*)tiles_dev, T};
   int blocks = (cap + 3) / 4;
   if (blocks > 256) blocks = 256;
-  hipLaunchKernelGGL((k_fit<TileVols, RawTile>), dim3(blocks, ntiles), dim3(256), 0, (hipStream_t)stream, vs, 1.0f,
-                     rs, coords, counts, cap, offs, P, sigmaZ, sigmaXY, out8);
+  launch_fit(vs, 1.0f, rs, coords, counts, cap, offs, P, sigmaZ, sigmaXY, out8, dim3(blocks, ntiles),
+             (hipStream_t)stream);
   MM_LAUNCH_CHECK();
   return MM_OK;
 }
