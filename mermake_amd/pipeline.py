@@ -291,6 +291,65 @@ class HybPipeline:
         """Spots of one channel stack: ``(N, 8)`` float32 rows in tile order, then raster order."""
         return self.run_many([stack], [flat_field])[0]
 
+    def submit(self, stacks, flat_fields=None):
+        """``run_many`` without waiting: the batch's kernel chain is enqueued on one of two side streams, each
+        with its own workspaces, and a ``PendingSpots`` is returned; its ``result()`` waits and gives the
+        ``run_many`` output.  A caller keeps up to two batches in flight, so the host work between batches
+        and the last small kernels of one batch overlap the next batch's passes.  A third ``submit`` waits for
+        the oldest batch first.  Stacks whose tiles do not fit one batch, graph pipelines and plain pipelines
+        run ``run_many`` at once (the returned result is then already complete)."""
+        d = self.d
+        if flat_fields is None:
+            flat_fields = [None] * len(stacks)
+        if self.plain or self.graph:
+            return PendingSpots(done=self.run_many(stacks, flat_fields))
+        tables, owner, alive = [], [], []
+        for si, (stack, flat_field) in enumerate(zip(stacks, flat_fields)):
+            d._check(stack)
+            img = prepare_stack(stack)
+            flat = prepare_flat(flat_field, img)
+            alive.append((img, flat))
+            geom, tab = self._table(img, flat)
+            tables.append(tab)
+            owner += [si] * len(geom)
+        n = len(owner)
+        if d.batch_size(n, self.cap * 48 + d.tile_height * d._plan.T ** 2 // 8) < n:
+            return PendingSpots(done=self.run_many(stacks, flat_fields))
+        if not hasattr(self, "_slots"):
+            self._slots = [{"ws": {}, "dws": {}, "stream": torch.cuda.Stream(d.device), "pending": None}
+                           for _ in range(2)]
+            self._next = 0
+        slot = self._slots[self._next]
+        self._next ^= 1
+        if slot["pending"] is not None:
+            slot["pending"].result()      # its workspaces are about to be reused
+        single = d.psf_fft.shape[0] == 1
+        cap = self.cap
+        cur = torch.cuda.current_stream(d.device)
+        st = slot["stream"]
+        st.wait_stream(cur)               # the inputs were made on the caller's stream
+        saved = (self._ws, d._ws)
+        self._ws, d._ws = slot["ws"], slot["dws"]
+        try:
+            with torch.cuda.stream(st):
+                table = tables[0] if len(tables) == 1 else torch.cat(tables)
+                counts, fit = self._spots_batch(table, n, single, cap)
+                stats = self._buf("stats", (n + 2,), torch.int32)
+                xh = torch.empty((n * cap, 8), dtype=torch.float32, device=d.device)
+                self._keep_launch(table, n, counts, fit, cap, xh, stats)
+                stats_h = slot.get("stats_h")
+                if stats_h is None or stats_h.numel() < n + 2:
+                    stats_h = slot["stats_h"] = torch.empty(n + 2, dtype=torch.int32, pin_memory=True)
+                stats_h[:n + 2].copy_(stats, non_blocking=True)
+                ev = torch.cuda.Event()
+                ev.record(st)
+        finally:
+            self._ws, d._ws = saved
+        p = PendingSpots(pipe=self, stacks=stacks, flats=flat_fields, alive=(alive, table), owner=owner, n=n,
+                         cap=cap, xh=xh, stats_h=stats_h, event=ev, stream=st)
+        slot["pending"] = p
+        return p
+
     def run_many(self, stacks, flat_fields=None):
         """Spots of several channel stacks batched together: tiles of different stacks (of any x, y size,
         the same depth) share every launch.  One ``(N, 8)`` tensor per stack, equal to ``self(stack, flat)``."""
@@ -353,6 +412,47 @@ class HybPipeline:
                 out.append(torch.zeros((0, 8), dtype=torch.float32, device=d.device))
             else:
                 out.append(views[0] if len(views) == 1 else torch.cat(views))
+        return out
+
+
+class PendingSpots:
+    """Spots of a ``HybPipeline.submit`` batch; ``result()`` waits for them (once) and returns one ``(N, 8)``
+    tensor per stack, equal to ``run_many`` of the same stacks."""
+
+    def __init__(self, done=None, pipe=None, stacks=None, flats=None, alive=None, owner=None, n=0, cap=0, xh=None,
+                 stats_h=None, event=None, stream=None):
+        self._out = done
+        self._pipe, self._stacks, self._flats, self._alive = pipe, stacks, flats, alive
+        self._owner, self._n, self._cap, self._xh = owner, n, cap, xh
+        self._stats_h, self._event, self._stream = stats_h, event, stream
+
+    def result(self):
+        if self._out is not None:
+            return self._out
+        self._event.synchronize()
+        st = self._stats_h[:self._n + 2].clone()
+        pipe = self._pipe
+        if int(st[0]) > self._cap:        # a tile overflowed the candidate cap: run the batch again, larger
+            out = pipe.run_many(self._stacks, self._flats)
+        else:
+            parts = [[] for _ in self._stacks]
+            HybPipeline._split(parts, self._owner, 0, self._n, st[2:].tolist(), self._xh)
+            out = []
+            dev = pipe.d.device
+            for si in range(len(self._stacks)):
+                views = [x[a:b] for a, b, x in parts[si]]
+                if not views:
+                    out.append(torch.zeros((0, 8), dtype=torch.float32, device=dev))
+                else:
+                    out.append(views[0] if len(views) == 1 else torch.cat(views))
+        # the rows were written on the side stream: later work on the caller's stream may use them, and their
+        # memory goes back to the side stream's pool only after that work
+        cur = torch.cuda.current_stream(pipe.d.device)
+        cur.wait_stream(self._stream)
+        if self._xh is not None:
+            self._xh.record_stream(cur)
+        self._out = out
+        self._alive = self._stacks = self._flats = self._xh = None
         return out
 
 

@@ -119,6 +119,24 @@ def test_hyb_window_equals_whole_tiles(dev, small_hyb, delta, delta_fit, ts, ov)
         assert np.array_equal(cw[t, :nw[t]], f[keep]), t
 
 
+@pytest.mark.parametrize("cap", [16384, 8])   # 8: every tile overflows the candidate cap, result() runs again
+def test_submit_in_flight_equals_run_many(dev, small_hyb, cap):
+    """``submit`` (two batches in flight on side streams, a third waiting for the first) gives ``run_many``'s
+    rows bit for bit, for uneven stacks, host stacks and a cap every tile overflows."""
+    from mermake_amd.pipeline import HybPipeline
+    raw, flat, psf = small_hyb
+    pipe = HybPipeline.from_settings(psf, raw.shape, tile_size=100, overlap=30, beta=1e-4, cap=cap, **HYB)
+    stacks = [T(raw, dev), T(raw[:, :250, :280], dev), raw[:, 20:, 10:]]   # the last one from the host
+    flats = [T(flat, dev), T(flat[:250, :280], dev), flat[20:, 10:]]
+    want = [w.cpu().numpy() for w in pipe.run_many(stacks, flats)]
+    assert all(len(w) > 20 for w in want)
+    pend = [pipe.submit(stacks, flats) for _ in range(3)]
+    for p in reversed(pend):
+        got = [g.cpu().numpy() for g in p.result()]
+        assert all(np.array_equal(g.view(np.uint32), w.view(np.uint32)) for g, w in zip(got, want))
+    assert pend[0].result() is pend[0].result()
+
+
 @pytest.mark.parametrize("delta,delta_fit,ts,ov", [(1, 3, 100, 30), (2, 1, 100, 30),
                                                    (1, 3, 150, 38),   # T = 226: Rader 113, blocked spectra
                                                    (1, 3, 300, 89)])  # T = 478: Rader 239
