@@ -167,6 +167,34 @@ def test_laplacian_closed_form_and_wiener_delta():
     assert np.allclose(W, 1 / (1 + beta * odec.laplacian_power(W.shape)))
 
 
+def test_blur_fold_identity():
+    """Deconvolving with W * (1 - b(kx) b(ky)), b the (2D+1)-tap mean's transfer function on the T-periodic tile
+    (summed term by term), gives the reference's deconvolution minus its clipped box blur (deconvolver.py:160-163)
+    wherever the boxes stay inside the tile, to float64 rounding; near the tile edges, where the reference clips the
+    boxes, the two differ.  The closed form the HIP build uses (Dirichlet kernel) equals the term-by-term sum."""
+    from mermake_amd.synth import gaussian_psf, stack_np
+    sz, ts, ov, zpad, D = 8, 40, 14, 7, 5
+    raw, _ = stack_np(5, (sz, 68, 68), 30)
+    psf = gaussian_psf((sz, 15, 15), 1.5, 1.2)
+    W = odec.wiener(psf, sz, ts, zpad, ov, 1e-3)[0]
+    T = W.shape[1]
+    k = np.arange(T)
+    m = np.arange(-D, D + 1)
+    b = np.cos(2 * np.pi * np.outer(k, m) / T).sum(1) / (2 * D + 1)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        closed = np.sin(np.pi * (2 * D + 1) * k / T) / ((2 * D + 1) * np.sin(np.pi * k / T))
+    closed[0] = 1.0
+    assert np.abs(closed - b).max() < 1e-12
+    Wf = W * (1.0 - b[None, :, None] * b[None, None, :])
+    blur = list(odec.tile_wise(raw, W, None, 2 * D, ts, ov, zpad))
+    fold = list(odec.tile_wise(raw, Wf, None, None, ts, ov, zpad))
+    for (x, y, a, _), (_, _, f, _) in zip(blur, fold):
+        win = (slice(None), slice(D, min(a.shape[1], T - D)), slice(D, min(a.shape[2], T - D)))
+        scale = np.abs(a[win]).max()
+        assert np.abs(a[win] - f[win]).max() <= 1e-12 * scale, (x, y)
+        assert np.abs(a - f).max() > 1e-6 * scale, (x, y)   # the clipped edge boxes are not periodic
+
+
 def test_center_psf_and_tiled():
     """tests/test_deconvolver.py:158-218 of the reference."""
     for shp in [(20, 50, 50), (10, 25, 25), (30, 75, 75)]:
