@@ -1493,7 +1493,10 @@ __global__ __launch_bounds__(256, SZ ? 2 : 1) void k_tile_zfft(float2* S, long l
 // The same z pass when every tile has its own PSF (tiles_per_block = 1: a grid
 // of PSFs): nothing to share, so no LDS staging; a block takes 256 columns of
 // one tile and every thread reads its column's operator straight from memory
-// (coalesced across the block), so all 256 threads have work.
+// (coalesced across the block), so all 256 threads have work.  The operator
+// reads bound this pass, so for the 40-plane tile the end-plane corrections are
+// applied in the frequency domain (zf_middle_ends) and only the A and G planes,
+// two thirds of the operator, are read.
 template <int SZ>
 __global__ __launch_bounds__(256, SZ ? 2 : 1) void k_tile_zfft_own(float2* S, long long s_tstride, int TH, int sz_rt,
                                                        const float2* __restrict__ Ops, long long ops_stride,
@@ -1528,18 +1531,28 @@ __global__ __launch_bounds__(256, SZ ? 2 : 1) void k_tile_zfft_own(float2* S, lo
     const float2 a = Op[(long long)k * TH], b = Op[(long long)(80 + k) * TH];
     return make_float4(a.x, a.y, b.x, b.y);
   };
-  zf_middle<0>(U, AGk);
-  zf_middle<1>(U, AGk);
-  zf_middle<2>(U, AGk);
+  if (SZ) {  // the end-plane corrections in the frequency domain: the operator's 160 (A, G) planes only
+    zf_middle_ends<0, SZ - 1>(U, AGk, xf, xl);
+    zf_middle_ends<1, SZ - 1>(U, AGk, xf, xl);
+    zf_middle_ends<2, SZ - 1>(U, AGk, xf, xl);
+  } else {
+    zf_middle<0>(U, AGk);
+    zf_middle<1>(U, AGk);
+    zf_middle<2>(U, AGk);
+  }
   float2 y[40];
   zf_inv_stage3(U, y);
   asm volatile("" : "+v"(col));
 #pragma unroll
   for (int t = 0; t < 40; ++t) {
     if (t < sz) {
-      const float2 a = Op[(long long)(160 + t) * TH], b = Op[(long long)(200 + t) * TH];  // c0[t], c1[t]
-      *col = make_float2(y[t].x - (a.x * xf.x - a.y * xf.y) - (b.x * xl.x - b.y * xl.y),
-                         y[t].y - (a.x * xf.y + a.y * xf.x) - (b.x * xl.y + b.y * xl.x));
+      if (SZ) {
+        *col = y[t];
+      } else {
+        const float2 a = Op[(long long)(160 + t) * TH], b = Op[(long long)(200 + t) * TH];  // c0[t], c1[t]
+        *col = make_float2(y[t].x - (a.x * xf.x - a.y * xf.y) - (b.x * xl.x - b.y * xl.y),
+                           y[t].y - (a.x * xf.y + a.y * xf.x) - (b.x * xl.y + b.y * xl.x));
+      }
     }
     col += TH;
     asm volatile("" : "+v"(col));

@@ -168,3 +168,45 @@ __device__ __forceinline__ void zf_middle(float2 (&U)[5][16], AGf AG) {
   bfly16<true>(U[a]);
   if (b != a) bfly16<true>(U[b]);
 }
+
+// a * w80^{-m} for any m < 80 (kW80 holds m < 64; w80^{-m} = -w80^{-(m - 40)})
+__device__ __forceinline__ float2 zf_rot(float2 a, int m) {
+  if (m < 40) return zf_tw<false>(a, m);
+  const float2 r = zf_tw<false>(a, m - 40);
+  return make_float2(-r.x, -r.y);
+}
+
+// zf_middle with the end-plane corrections folded into the frequency domain: the
+// Toeplitz term acts on x without its planes 0 and SH = sz - 1,
+//   Y[k] = A[k] (X[k] - x0 - xl w80^{-SH k}) + G[k] X[-k],
+// so that IDFT80(Y)[t] is the whole operator and c0, c1 are not read (they are
+// the taps A already holds: c0[t] = w[t], c1[t] = w[t - SH]).
+template <int a, int SH, class AGf>
+__device__ __forceinline__ void zf_middle_ends(float2 (&U)[5][16], AGf AG, float2 x0, float2 xl) {
+  constexpr int b = (5 - a) % 5;
+  bfly16<false>(U[a]);
+  if (b != a) bfly16<false>(U[b]);
+#pragma unroll
+  for (int k2 = 0; k2 < 16; ++k2) {
+    constexpr int adj = a ? 1 : 0;
+    const int m2 = (32 - k2 - adj) % 16;  // X[-k] = U[b][m2]
+    if (b == a && m2 < k2) continue;      // pair already done
+    const int k = a + 5 * k2, kn = b + 5 * m2;
+    const float2 Xk = U[a][k2], Xn = U[b][m2];
+    const float2 ek = zf_rot(xl, (SH * k) % 80), en = zf_rot(xl, (SH * kn) % 80);
+    const float2 Tk = make_float2(Xk.x - x0.x - ek.x, Xk.y - x0.y - ek.y);
+    const float2 Tn = make_float2(Xn.x - x0.x - en.x, Xn.y - x0.y - en.y);
+    const float4 p = AG(k);
+    const float2 yk = make_float2(p.x * Tk.x - p.y * Tk.y + p.z * Xn.x - p.w * Xn.y,
+                                  p.x * Tk.y + p.y * Tk.x + p.z * Xn.y + p.w * Xn.x);
+    if (kn != k) {
+      const float4 r = AG(kn);
+      U[b][m2] = make_float2(r.x * Tn.x - r.y * Tn.y + r.z * Xk.x - r.w * Xk.y,
+                             r.x * Tn.y + r.y * Tn.x + r.z * Xk.y + r.w * Xk.x);
+    }
+    U[a][k2] = yk;
+  }
+  bfly16<true>(U[a]);
+  if (b != a) bfly16<true>(U[b]);
+}
+

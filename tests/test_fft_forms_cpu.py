@@ -149,3 +149,32 @@ def test_zpass_toeplitz_hankel_fft80(sz):
     got = _zpass_fft80(w, x, sz)
     assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
 
+
+def _zpass_fft80_ends(w, x, sz):
+    """The same operator with the end-plane corrections in the frequency domain (k_tile_zfft_own): the Toeplitz
+    term acts on x without its planes 0 and sz - 1, Y = A (X - x0 - x[sz-1] e^{-2 pi i (sz-1) k / 80}) + G X[-k]."""
+    Zp = 3 * sz - 2
+    a = np.zeros(80, complex)
+    for m in range(-(sz - 1), sz):
+        a[m % 80] = w[m % Zp]
+    g = np.zeros(80, complex)
+    for m in range(2 * sz - 1):
+        g[m] = w[m] + w[(m - 2 * (sz - 1)) % Zp]
+    A, G = np.fft.fft(a) / 80, np.fft.fft(g) / 80
+    X = np.fft.fft(np.concatenate([x, np.zeros(80 - sz)]))
+    k = np.arange(80)
+    Xt = X - x[0] - x[sz - 1] * np.exp(-2j * np.pi * ((sz - 1) * k % 80) / 80)
+    y = np.fft.ifft(A * Xt + G * X[(-k) % 80]) * 80
+    return y[:sz]
+
+
+@pytest.mark.parametrize("sz", [1, 2, 3, 7, 17, 39, 40])
+def test_zpass_fft80_end_planes_in_frequency_domain(sz):
+    rng = np.random.default_rng(100 + sz)
+    Zp = 3 * sz - 2
+    w = rng.standard_normal(Zp) + 1j * rng.standard_normal(Zp)
+    x = rng.standard_normal(sz) + 1j * rng.standard_normal(sz)
+    want = _zpass_matrix(w, sz, sz - 1) @ x
+    got = _zpass_fft80_ends(w, x, sz)
+    assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
+
