@@ -180,6 +180,15 @@ int mm_deconv_tile_raw(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, 
 /* Deconvolver.apply's stitch (deconvolver.py:179-183) into out (sz, SX, SY). */
 int mm_deconv_stitch(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const float* res, float* out,
                      long long o_sz, long long o_sx, int SX, int SY, void* stream);
+/* mm_deconv_tiles followed by mm_deconv_stitch without the result tiles: the
+ * inverse y transform stores the owned rows and columns [ov, xe - ov) x
+ * [ov, ye - ov) of every tile straight into out (sz, SX, SY) (Deconvolver.apply,
+ * deconvolver.py:116-183), bit for bit what the two calls store.  Only for
+ * plans whose y passes are the Rader form (MM_EINVAL otherwise, then use the
+ * two calls). */
+int mm_deconv_tiles_stitch(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const void* W, int n_psf,
+                           int tiles_per_block, void* spec_ws, float* out, long long o_sz, long long o_sx, int SX,
+                           int SY, void* stream);
 
 /* DAPI normalisation (mermake/__main__.py:221-222): a /= ||a||_2/sqrt(n) in place;
  * *rms_out (device float) receives the divisor.  ws: mm_rms_ws_bytes(). */

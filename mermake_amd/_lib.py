@@ -49,6 +49,7 @@ _SIGS = {
     "mm_deconv_tile_raw": (_i, [_vp, _vp, _i, _vp, _vp]),
     "mm_deconv_set_profiling": (_i, [_vp, _i]),
     "mm_deconv_pass_times": (_i, [_vp, ctypes.POINTER(_f)]),
+    "mm_deconv_tiles_stitch": (_i, [_vp, _vp, _i, _vp, _i, _i, _vp, _vp, _ll, _ll, _i, _i, _vp]),
     "mm_deconv_stitch": (_i, [_vp, _vp, _i, _vp, _vp, _ll, _ll, _i, _i, _vp]),
     "mm_fft3d_c2c": (_i, [_vp, _i, _i, _i, _i, _i, _vp]),
     "mm_median_ws_bytes": (_sz, []),

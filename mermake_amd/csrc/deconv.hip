@@ -1008,19 +1008,31 @@ __global__ __launch_bounds__(BS, 2) void k_rd_r2c(const mm_tile_dev* __restrict_
   }
 }
 
+// Where the C2R stores the owned part of each tile straight into the stitched
+// volume (deconvolver.py:179-183, as k_stitch): rows and columns [ov, xe - ov) x
+// [ov, ye - ov) of tile t go to out[z o_sz + (ox + x - ov) o_sx + oy + y - ov]
+// inside (SX, SY).
+struct C2rStitch {
+  const mm_tile_dev* tiles;
+  float* out;
+  long long o_sz, o_sx;
+  int SX, SY, ov;
+};
+
 // Rows (z, x) with x in [xlo, xlo + nx) only (nrows = sz * nx of them, taken
 // in that order), output columns in [clo, chi) only.
 // MARK: also the maxima threshold test (!(v <= thr), maxima.cu:19) of every
 // stored output, as ww 64-bit words per row (bit c & 63 of word c >> 6 for
 // column c; zero for the columns not stored) at marks[(tile * N + z) * N + x]
 // (the row layout of the maxima workspace), for every computed row.
-template <int PR, int M, bool MARK = false, int BS = 256>
+// STITCH: the stores go to the stitched volume (C2rStitch) instead of res.
+template <int PR, int M, bool MARK = false, int BS = 256, bool STITCH = false>
 __global__ __launch_bounds__(BS, BS == 128 && PR == 113 ? 4 : 2) void k_rd_c2r(const float2* Sin, long long s_tstride, SpecLayout Ls,
                                                        float* res, long long r_tstride, int ntiles, int nrows, int xlo,
                                                        int nx, int clo, int chi, RdTab tab,
                                                        const float2* __restrict__ tw,
                                                        unsigned long long* __restrict__ marks = nullptr,
-                                                       float thr = 0.0f, int ww = 0) {
+                                                       float thr = 0.0f, int ww = 0, C2rStitch sti = C2rStitch{}) {
   constexpr int G = rd_row_g<PR>(M, true), NC = rd_row_nc<PR>(M, BS), NR = 2 * G;
   static_assert(G * M * RdSpec<PR>::B <= BS, "in-place Rader stages: one item per thread");
   extern __shared__ __align__(16) unsigned char smem[];
@@ -1096,11 +1108,33 @@ __global__ __launch_bounds__(BS, BS == 128 && PR == 113 ? 4 : 2) void k_rd_c2r(c
     rd_stages<PR, M, false, false>(L, G, RdMid{});
     float* out = res + (long long)tile * r_tstride;
     const int z0 = q0 / nx;
+    // STITCH: the owned rows and columns of this tile, and where they go
+    int s_x1 = 0, s_y1 = 0, s_dx = 0, s_dy = 0;
+    if (STITCH) {
+      const mm_tile_dev& td = sti.tiles[tile];
+      s_dx = td.ox - sti.ov;
+      s_dy = td.oy - sti.ov;
+      s_x1 = min(td.xe - sti.ov, sti.SX - s_dx);  // rows x in [ov, s_x1)
+      s_y1 = min(td.ye - sti.ov, sti.SY - s_dy);  // columns c in [ov, s_y1)
+    }
     pf_final<M, true>(PfShape(N, PR), (const float2*)L.bufB, L.tw, G, [&](int g, int j, const float2* v) {
       const int q = q0 + 2 * g;
       int za, xa, zb, xb;
       rowzx(q0, z0, 2 * g, za, xa);
       rowzx(q0, z0, 2 * g + 1, zb, xb);
+      if (STITCH) {
+        const bool oka = q < nrows && xa >= sti.ov && xa < s_x1, okb = q + 1 < nrows && xb >= sti.ov && xb < s_x1;
+        float* sa = sti.out + za * sti.o_sz + (long long)(s_dx + xa) * sti.o_sx + s_dy + j;
+        float* sb = sti.out + zb * sti.o_sz + (long long)(s_dx + xb) * sti.o_sx + s_dy + j;
+#pragma unroll
+        for (int r = 0; r < M; ++r) {
+          const int c = j + r * PR;
+          const bool cin = c >= sti.ov && c < s_y1;
+          if (cin && oka) sa[r * PR] = v[r].x;
+          if (cin && okb) sb[r * PR] = v[r].y;
+        }
+        return;
+      }
       float* oa = out + ((long long)za * N + xa) * N + j;
       float* ob = out + ((long long)zb * N + xb) * N + j;
 #pragma unroll
@@ -1217,7 +1251,7 @@ static void launch_rd_r2c(const mm_tile_dev* td, int ntiles, int T, int nrows, f
 template <int PR, int M>
 static void launch_rd_c2r(const float2* S, long long s_ts, const SpecLayout& Ls, float* res, long long r_ts,
                           int ntiles, int sz, int lo, int hi, const RdTab& tab, const float2* tw, hipStream_t st,
-                          unsigned long long* marks = nullptr, float thr = 0.0f) {
+                          unsigned long long* marks = nullptr, float thr = 0.0f, const C2rStitch* sti = nullptr) {
   constexpr int G = rd_row_g<PR>(M, true);
   // blocks of 128 threads when the group's in-place stages fit them (fewer idle lanes in stages 1 and 3)
   constexpr int BS = G * M * RdSpec<PR>::B <= 128 ? 128 : 256;
@@ -1226,7 +1260,10 @@ static void launch_rd_c2r(const float2* S, long long s_ts, const SpecLayout& Ls,
   const int nrows = sz * (hi - lo);
   const long long ngrp = (long long)((nrows + 2 * G - 1) / (2 * G)) * ntiles;
   const unsigned grid = pf_grid(ngrp, lds, 512 / BS * 2);
-  if (marks)
+  if (sti)
+    hipLaunchKernelGGL((k_rd_c2r<PR, M, false, BS, true>), dim3(grid), dim3(BS), lds, st, S, s_ts, Ls, res, r_ts,
+                       ntiles, nrows, lo, hi - lo, lo, hi, tab, tw, nullptr, 0.0f, 0, *sti);
+  else if (marks)
     hipLaunchKernelGGL((k_rd_c2r<PR, M, true, BS>), dim3(grid), dim3(BS), lds, st, S, s_ts, Ls, res, r_ts, ntiles,
                        nrows, lo, hi - lo, lo, hi, tab, tw, marks, thr, ww);
   else
@@ -1961,8 +1998,10 @@ extern "C" size_t mm_deconv_spec_bytes(const mm_deconv* d) {
 // those; the other forms compute whole planes).
 static int deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const void* W, int n_psf,
                         int tiles_per_block, void* spec_ws, float* res, int lo, int hi, void* stream,
-                        unsigned long long* marks = nullptr, float thr = 0.0f, int* marked = nullptr) {
+                        unsigned long long* marks = nullptr, float thr = 0.0f, int* marked = nullptr,
+                        const C2rStitch* sti = nullptr) {
   if (marked) *marked = 0;
+  if (sti && !d->rd) { mm_set_error("mm_deconv_tiles_stitch: the plan's y passes are not the Rader form"); return MM_EINVAL; }
   if (!d || !tiles_dev || !W || !spec_ws || !res || ntiles < 0 || n_psf < 1 || lo < 0 || hi > d->T || lo >= hi) {
     mm_set_error("mm_deconv_tiles: bad argument");
     return MM_EINVAL;
@@ -2055,7 +2094,7 @@ static int deconv_tiles(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles,
   if (d->rd) {  // 5. C2R along y into res (prime-first form, Rader prime stage)
     RD_SWITCH(d->rdP, d->pfM,
               (launch_rd_c2r<PP, MM>(S, s_ts, Ls, res, (long long)sz * T * T, ntiles, sz, lo, hi, d->rdI, d->pT.tw, st,
-                                     marks, thr)));
+                                     marks, thr, sti)));
     MM_LAUNCH_CHECK();
     if (marked && marks) *marked = 1;
   } else if (d->pfM) {  // 5. C2R along y into res (prime-first form)
@@ -2153,6 +2192,16 @@ __global__ void k_stitch(const mm_tile_dev* __restrict__ tiles, int T, int sz, i
     const int X = td.ox + x, Y = td.oy + y;
     if (X < SX && Y < SY) out[z * o_sz + (long long)X * o_sx + Y] = r[((long long)z * T + ov + x) * T + ov + y];
   }
+}
+
+extern "C" int mm_deconv_tiles_stitch(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const void* W,
+                                      int n_psf, int tiles_per_block, void* spec_ws, float* out, long long o_sz,
+                                      long long o_sx, int SX, int SY, void* stream) {
+  if (!d || !out) { mm_set_error("mm_deconv_tiles_stitch: null argument"); return MM_EINVAL; }
+  const C2rStitch sti{(const mm_tile_dev*)tiles_dev, out, o_sz, o_sx, SX, SY, d->ov};
+  // the stitch reads rows and columns [ov, T - ov) of every tile; res is not written
+  return deconv_tiles(d, tiles_dev, ntiles, W, n_psf, tiles_per_block, spec_ws, out, d->ov, d->T - d->ov, stream,
+                      nullptr, 0.0f, nullptr, &sti);
 }
 
 extern "C" int mm_deconv_stitch(mm_deconv* d, const mm_tile_desc* tiles_dev, int ntiles, const float* res, float* out,

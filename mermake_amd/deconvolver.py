@@ -488,6 +488,24 @@ class Deconvolver:
         sz, sx, sy = image.shape
         # the stitch reads rows and columns [ov, T - ov) of every tile: only those are computed
         T, ov = self._plan.T, self.overlap
+        spectral = bool(spectral) and self.spectral_blur(blur_radius, (ov, T - ov))
+        if self._plan.schedule()[0] == "rader" and (blur_radius is None or spectral):
+            # the inverse y transform stores the owned parts straight into the output
+            self._check(image)
+            img = prepare_stack(image)
+            flat = prepare_flat(flat_field, img)
+            geom = self.geometry(img.shape)
+            ops = self.box_spectra(blur_radius) if spectral else self.psf_fft
+            single = ops.shape[0] == 1
+            B = self.batch_size(len(geom))
+            for b0 in range(0, len(geom), B):
+                g = geom[b0:b0 + B]
+                table = tile_table(g, img, flat)
+                spec = self._workspace("spec", len(g) * self._plan.spec_bytes)
+                L.call("mm_deconv_tiles_stitch", self._plan.h, L.ptr(table), len(g), L.ptr(ops), ops.shape[0],
+                       len(g) if single else 1, L.ptr(spec), L.ptr(output), output.stride(0), output.stride(1), sx,
+                       sy, L.stream_ptr(self.device))
+            return output
         for g, table, res, _, _ in self._batches(image, flat_field, blur_radius, False, window=(ov, T - ov),
                                                  spectral=spectral):
             L.call("mm_deconv_stitch", self._plan.h, L.ptr(table), len(g), L.ptr(res), L.ptr(output),

@@ -119,6 +119,52 @@ def test_hyb_window_equals_whole_tiles(dev, small_hyb, delta, delta_fit, ts, ov)
         assert np.array_equal(cw[t, :nw[t]], f[keep]), t
 
 
+@pytest.mark.parametrize("ts,ov,zpad,shape", [(150, 38, 9, (20, 300, 280)),    # T 226: Rader 113
+                                               (300, 89, 9, (20, 700, 640)),    # T 478: Rader 239
+                                               (500, 89, 19, (40, 1100, 600))])  # T 678 (the DAPI tile class)
+def test_deconv_tiles_stitch_equals_two_calls(dev, ts, ov, zpad, shape):
+    """``mm_deconv_tiles_stitch`` (the inverse y transform storing each tile's owned part into the stitched volume)
+    writes exactly what ``mm_deconv_tiles_window`` + ``mm_deconv_stitch`` write, into a strided output larger than
+    the stack (the rest untouched), edge tiles included; ``apply`` uses it."""
+    from mermake_amd import _lib as L
+    from mermake_amd.deconvolver import Deconvolver, prepare_flat, prepare_stack, tile_table
+    from mermake_amd.synth import gaussian_psf, stack_np
+    raw, flat = stack_np(21, shape, 200, flat=True)
+    d = Deconvolver(gaussian_psf((shape[0], 31, 31), 1.5, 1.2), shape, tile_size=ts, overlap=ov, zpad=zpad,
+                    beta=1e-2)
+    assert d.schedule()["xy"] == "rader"
+    img = prepare_stack(T(raw, dev))
+    fl = prepare_flat(T(flat, dev), img)
+    geom = d.geometry(img.shape)
+    table = tile_table(geom, img, fl)
+    n, Tt, sz = len(geom), d._plan.T, shape[0]
+    st = L.stream_ptr(dev)
+    ops = d.box_spectra(50)
+    spec = torch.empty(n * d._plan.spec_bytes, dtype=torch.uint8, device=dev)
+    res = torch.empty((n, sz, Tt, Tt), dtype=torch.float32, device=dev)
+    outs = []
+    for fused in (False, True):
+        big = torch.full((sz, shape[1] + 3, shape[2] + 5), float("nan"), dtype=torch.float32, device=dev)
+        out = big[:, 1:1 + shape[1], 2:2 + shape[2]]
+        if fused:
+            L.call("mm_deconv_tiles_stitch", d._plan.h, L.ptr(table), n, L.ptr(ops), 1, n, L.ptr(spec), L.ptr(out),
+                   out.stride(0), out.stride(1), shape[1], shape[2], st)
+        else:
+            L.call("mm_deconv_tiles_window", d._plan.h, L.ptr(table), n, L.ptr(ops), 1, n, L.ptr(spec), L.ptr(res),
+                   ov, Tt - ov, st)
+            L.call("mm_deconv_stitch", d._plan.h, L.ptr(table), n, L.ptr(res), L.ptr(out), out.stride(0),
+                   out.stride(1), shape[1], shape[2], st)
+        torch.cuda.synchronize(dev)
+        outs.append(big.cpu().numpy())
+    print("tiles_stitch", shape, ts, ov, Tt, n, flush=True)
+    a, b = outs
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert np.isfinite(a[:, 1:1 + shape[1], 2:2 + shape[2]]).all() and np.isnan(a[:, 0]).all()
+    # apply (fused) = the two calls
+    got = d.apply(T(raw, dev), flat_field=T(flat, dev), blur_radius=50, spectral=True).cpu().numpy()
+    assert np.array_equal(got.view(np.uint32), a[:, 1:1 + shape[1], 2:2 + shape[2]].view(np.uint32))
+
+
 @pytest.mark.parametrize("cap", [16384, 8])   # 8: every tile overflows the candidate cap, result() runs again
 def test_submit_in_flight_equals_run_many(dev, small_hyb, cap):
     """``submit`` (two batches in flight on side streams, a third waiting for the first) gives ``run_many``'s
