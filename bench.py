@@ -502,7 +502,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=48, help="requests of each served-stream run")
     ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate of the served stream (FOV/s)")
     ap.add_argument("--serve-sweep", action="store_true",
-                    help="run the served stream at Poisson rates 0.5, 1, 2 and 4 FOV/s (about 3 minutes)")
+                    help="run the served stream at Poisson rates 0.5, 1, 2, 3 and 4 FOV/s (about 4 minutes)")
     ap.add_argument("--no-legs", action="store_true",
                     help="skip the multi-PSF grid, uneven-batch + drift and served-stream legs")
     ap.add_argument("--legs", default="multi_psf_grid,uneven_batch_drift,served_stream",
@@ -700,7 +700,7 @@ def main():
         torch.cuda.empty_cache()
         if "uneven_batch_drift" in which:
             legs["uneven_batch_drift"] = uneven_leg(dev, cfg, pipe, dapi_one)
-        rates = [0.5, 1.0, 2.0, 4.0] if args.serve_sweep else [args.serve_rate]
+        rates = [0.5, 1.0, 2.0, 3.0, 4.0] if args.serve_sweep else [args.serve_rate]
         if "served_stream" in which:
             legs["served_stream"] = stream_leg(dev, cfg, {"single": pipe, "grid": grid}, dapi_one, args, rates)
         del grid, dapi_one
