@@ -248,6 +248,7 @@ class Deconvolver:
         self.psf_fft = None
         self.generation = 0       # counts the builds of psf_fft (captured kernel chains hold its address)
         self._ws = {}
+        self._tables = {}         # tile tables of the stacks seen last, by address and layout
         self._build_spectra()
         if autotune:
             from .autotune import tune
@@ -367,6 +368,19 @@ class Deconvolver:
         if image.shape[0] != self.tile_height:
             raise ValueError(f"stack depth {image.shape[0]} does not match the Deconvolver's {self.tile_height}")
 
+    def _tile_table(self, geom, img, flat):
+        """``tile_table`` of a whole stack, cached by the stack's and flat's addresses and layout (a pageable
+        host-to-device copy waits for the stream, so it is made once per stack, not per call).  The key holds
+        the stream too: a table is only used on the stream it was allocated on, so dropping it is stream-ordered."""
+        key = (img.data_ptr(), tuple(img.shape), tuple(img.stride()), img.dtype, type(img),
+               None if flat is None else (flat.data_ptr(), tuple(flat.stride())), L.stream_ptr(self.device).value)
+        t = self._tables.get(key)
+        if t is None:
+            if len(self._tables) >= 8:
+                self._tables.clear()
+            t = self._tables[key] = tile_table(geom, img, flat)
+        return t
+
     def _workspace(self, name, nbytes):
         t = self._ws.get(name)
         if t is None or t.numel() < nbytes:
@@ -460,9 +474,11 @@ class Deconvolver:
         T, sz = self._plan.T, self.tile_height
         single = self.psf_fft.shape[0] == 1
         B = self.batch_size(len(geom), 4 * sz * T * T if want_raw else 0)
+        table_all = self._tile_table(geom, img, flat)
+        rec = L.TILE_DTYPE.itemsize
         for b0 in range(0, len(geom), B):
             g = geom[b0:b0 + B]
-            table = tile_table(g, img, flat)
+            table = table_all[b0 * rec:(b0 + len(g)) * rec]
             res = self.run_tiles(table, len(g), len(g) if single else 1, blur_radius, window=window,
                                  spectral=spectral)
             raw = None
@@ -498,9 +514,11 @@ class Deconvolver:
             ops = self.box_spectra(blur_radius) if spectral else self.psf_fft
             single = ops.shape[0] == 1
             B = self.batch_size(len(geom))
+            table_all = self._tile_table(geom, img, flat)
+            rec = L.TILE_DTYPE.itemsize
             for b0 in range(0, len(geom), B):
                 g = geom[b0:b0 + B]
-                table = tile_table(g, img, flat)
+                table = table_all[b0 * rec:(b0 + len(g)) * rec]
                 spec = self._workspace("spec", len(g) * self._plan.spec_bytes)
                 L.call("mm_deconv_tiles_stitch", self._plan.h, L.ptr(table), len(g), L.ptr(ops), ops.shape[0],
                        len(g) if single else 1, L.ptr(spec), L.ptr(output), output.stride(0), output.stride(1), sx,
