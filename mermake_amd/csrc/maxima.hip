@@ -131,6 +131,39 @@ __device__ __forceinline__ bool is_max_sph_in(const float* __restrict__ c0, cons
   return true;
 }
 
+// The rest of the sphere, neighbours [first, n), for the interior candidates of one wavefront that survived
+// the first ones (bit l of pend: the voxel at row[l], centre value cs = sign * v in lane l): the whole wavefront
+// tests one candidate at a time, each lane loading up to 8 of its neighbours at once, so a candidate costs one
+// round of loads instead of (n - first) / 4 rounds of one lane's walk that the other lanes wait for.  Returns
+// the mask of the candidates that are maxima.
+#define MX_COOP_FIRST 8
+static_assert(MX_SPH_N - MX_COOP_FIRST <= 8 * 64, "the rest of the sphere fits 8 loads per lane");
+__device__ __forceinline__ unsigned long long is_max_sph_coop(const float* __restrict__ row, unsigned long long pend,
+                                                              float cs, float sign, const int* __restrict__ soff,
+                                                              int n) {
+  const int lane = threadIdx.x & 63;
+  const int nj = (n - MX_COOP_FIRST + 63) >> 6;  // uniform
+  unsigned long long keep = 0;
+  while (pend) {
+    const int l = __ffsll((long long)pend) - 1;
+    pend &= pend - 1;
+    const float c = __shfl(cs, l);
+    const float* c0 = row + l;
+    float v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int i = MX_COOP_FIRST + lane + 64 * j;
+      if (j < nj) v[j] = c0[i < n ? soff[i] : 0];  // past the sphere: the centre, never larger than itself
+    }
+    bool big = false;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (j < nj) big = big || c < sign * v[j];
+    if (__ballot(big) == 0) keep |= 1ull << l;
+  }
+  return keep;
+}
+
 // Voxel near a border: the same sphere, nearest first, with the single reflection of
 // maxima.cu:44-52 (neighbours still outside are skipped, as in is_max_at).
 __device__ __forceinline__ bool is_max_sph_border(const VolDesc& V, int z, int x, int y, float c, float sign, int n) {
@@ -209,14 +242,22 @@ __global__ __launch_bounds__(256) void k_max_mark(VS vs, float sign, float thr, 
         const int y = (k0 + k) * 64 + lane;
         // maxima.cu:19: v <= threshold is rejected (a NaN threshold rejects nothing)
         bool m = y >= y0 && y < y1 && !(c[k] <= thr);
+        const bool interior = zx_in && y >= delta && y < V.W - delta;
         if (m) {
           if (delta == 1) {
             m = is_max_d1(V, z, x, y, c[k], sign);
           } else if (sph) {
-            if (zx_in && y >= delta && y < V.W - delta) m = is_max_sph_in(rp + y, soff, nsph, c[k], sign);
+            if (interior) m = is_max_sph_in(rp + y, soff, min(nsph, MX_COOP_FIRST), c[k], sign);
             else m = is_max_sph_border(V, z, x, y, c[k], sign, nsph);
           } else {
             m = is_max_at(V, z, x, y, c[k], sign, delta);
+          }
+        }
+        if (sph && nsph > MX_COOP_FIRST) {  // interior survivors: the rest of the sphere, one at a time
+          const unsigned long long pend = __ballot(m && interior);
+          if (pend) {
+            const unsigned long long ok = is_max_sph_coop(rp + (k0 + k) * 64, pend, c[k], sign, soff, nsph);
+            if (interior) m = m && ((ok >> lane) & 1ull);
           }
         }
         const unsigned long long b = __ballot(m);
@@ -267,17 +308,30 @@ __global__ __launch_bounds__(256) void k_max_mark_pm(VS vs, float thr, int delta
         bool mp = in && !(c[k] <= thr);   // maxima.cu:19, for +v and for -v
         bool mm = in && !(-c[k] <= thr);
         const bool interior = zx_in && y >= delta && y < V.W - delta;
+        const int n1 = min(nsph, MX_COOP_FIRST);
         if (mp) {
           if (delta == 1) mp = is_max_d1(V, z, x, y, c[k], 1.0f);
-          else if (sph) mp = interior ? is_max_sph_in(rp + y, soff, nsph, c[k], 1.0f)
+          else if (sph) mp = interior ? is_max_sph_in(rp + y, soff, n1, c[k], 1.0f)
                                       : is_max_sph_border(V, z, x, y, c[k], 1.0f, nsph);
           else mp = is_max_at(V, z, x, y, c[k], 1.0f, delta);
         }
         if (mm) {
           if (delta == 1) mm = is_max_d1(V, z, x, y, -c[k], -1.0f);
-          else if (sph) mm = interior ? is_max_sph_in(rp + y, soff, nsph, -c[k], -1.0f)
+          else if (sph) mm = interior ? is_max_sph_in(rp + y, soff, n1, -c[k], -1.0f)
                                       : is_max_sph_border(V, z, x, y, -c[k], -1.0f, nsph);
           else mm = is_max_at(V, z, x, y, -c[k], -1.0f, delta);
+        }
+        if (sph && nsph > MX_COOP_FIRST) {  // interior survivors: the rest of the sphere, one at a time
+          const float* wrow = rp + (k0 + k) * 64;
+          const unsigned long long pp = __ballot(mp && interior), pm = __ballot(mm && interior);
+          if (pp) {
+            const unsigned long long ok = is_max_sph_coop(wrow, pp, c[k], 1.0f, soff, nsph);
+            if (interior) mp = mp && ((ok >> lane) & 1ull);
+          }
+          if (pm) {
+            const unsigned long long ok = is_max_sph_coop(wrow, pm, -c[k], -1.0f, soff, nsph);
+            if (interior) mm = mm && ((ok >> lane) & 1ull);
+          }
         }
         const unsigned long long bp = __ballot(mp), bm = __ballot(mm);
         if (lane == 0) {
