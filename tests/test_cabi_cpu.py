@@ -34,6 +34,48 @@ def test_library_exports_every_declared_symbol():
     assert set(_lib.exported_symbols()) == set(names)
 
 
+def _declarations():
+    """{name: (return type, [parameter types])} of the header's function declarations."""
+    src = open(os.path.join(ROOT, "include", "mermake_hip.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    src = re.sub(r"//[^\n]*", "", src)
+    out = {}
+    pat = r"^[ \t]*((?:const[ \t]+)?[A-Za-z_]\w*(?:[ \t]*\*)?)[ \t]*\b(mm_[a-z0-9_]+)\s*\(([^)]*)\)\s*;"
+    for ret, name, params in re.findall(pat, src, flags=re.M):
+        ps = [p.strip() for p in params.split(",") if p.strip() and p.strip() != "void"]
+        # drop the parameter name: the type is everything before the last identifier
+        types = [re.sub(r"\s*\b\w+$", "", p).replace("const ", "").strip() for p in ps]
+        out[name] = (ret.replace("const ", "").strip(), types)
+    return out
+
+
+def _ctype_ok(c_type, ct):
+    scalar = {"int": ctypes.c_int, "long long": ctypes.c_longlong, "float": ctypes.c_float,
+              "double": ctypes.c_double, "size_t": ctypes.c_size_t}
+    c_type = " ".join(c_type.split())
+    if c_type in scalar:
+        return ct is scalar[c_type]
+    if c_type == "char*":
+        return ct is ctypes.c_char_p
+    # pointers (hipStream_t is one): an address or a typed pointer
+    assert c_type.endswith("*") or c_type == "hipStream_t", c_type
+    return ct is ctypes.c_void_p or (isinstance(ct, type) and issubclass(ct, ctypes._Pointer))
+
+
+def test_bindings_match_the_header_declarations():
+    """Every ctypes signature of mermake_amd._lib has the header's parameter count and scalar types (a binding
+    that drifts from the header passes wrong sizes or strides without any error)."""
+    from mermake_amd import _lib
+    decl = _declarations()
+    assert set(decl) == set(_lib._SIGS)
+    for name, (ret, params) in decl.items():
+        res, args = _lib._SIGS[name]
+        assert len(args) == len(params), (name, params)
+        assert _ctype_ok(ret, res), (name, ret)
+        for i, (c_type, ct) in enumerate(zip(params, args)):
+            assert _ctype_ok(c_type, ct), (name, i, c_type, ct)
+
+
 def test_version_and_error_string():
     L = ctypes.CDLL(_lib_path())
     L.mm_version.restype = ctypes.c_int
