@@ -164,6 +164,43 @@ __device__ __forceinline__ unsigned long long is_max_sph_coop(const float* __res
   return keep;
 }
 
+// The same for candidates near a border (bit l of pend: the voxel at (z, x, ybase + l)): each lane reflects its
+// neighbours' coordinates as is_max_sph_border does and skips those still outside.
+__device__ __forceinline__ unsigned long long is_max_sph_coop_border(const VolDesc& V, int z, int x, int ybase,
+                                                                     unsigned long long pend, float cs, float sign,
+                                                                     int n) {
+  const int lane = threadIdx.x & 63;
+  const int nj = (n - MX_COOP_FIRST + 63) >> 6;  // uniform
+  unsigned long long keep = 0;
+  while (pend) {
+    const int l = __ffsll((long long)pend) - 1;
+    pend &= pend - 1;
+    const float c = __shfl(cs, l);
+    const int y = ybase + l;
+    float v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int i = MX_COOP_FIRST + lane + 64 * j;
+      if (j < nj) {
+        long long o = -1;
+        if (i < n) {
+          const int nz = refl1(z + kSph.d[i][0], V.D), nx = refl1(x + kSph.d[i][1], V.H),
+                    ny = refl1(y + kSph.d[i][2], V.W);
+          if (nz >= 0 && nz < V.D && nx >= 0 && nx < V.H && ny >= 0 && ny < V.W)
+            o = nz * V.sZ + nx * V.sX + ny;
+        }
+        v[j] = o >= 0 ? V.p[o] : -sign * INFINITY;  // outside: never larger than the centre
+      }
+    }
+    bool big = false;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (j < nj) big = big || c < sign * v[j];
+    if (__ballot(big) == 0) keep |= 1ull << l;
+  }
+  return keep;
+}
+
 // Voxel near a border: the same sphere, nearest first, with the single reflection of
 // maxima.cu:44-52 (neighbours still outside are skipped, as in is_max_at).
 __device__ __forceinline__ bool is_max_sph_border(const VolDesc& V, int z, int x, int y, float c, float sign, int n) {
@@ -248,17 +285,17 @@ __global__ __launch_bounds__(256) void k_max_mark(VS vs, float sign, float thr, 
             m = is_max_d1(V, z, x, y, c[k], sign);
           } else if (sph) {
             if (interior) m = is_max_sph_in(rp + y, soff, min(nsph, MX_COOP_FIRST), c[k], sign);
-            else m = is_max_sph_border(V, z, x, y, c[k], sign, nsph);
+            else m = is_max_sph_border(V, z, x, y, c[k], sign, min(nsph, MX_COOP_FIRST));
           } else {
             m = is_max_at(V, z, x, y, c[k], sign, delta);
           }
         }
-        if (sph && nsph > MX_COOP_FIRST) {  // interior survivors: the rest of the sphere, one at a time
-          const unsigned long long pend = __ballot(m && interior);
-          if (pend) {
-            const unsigned long long ok = is_max_sph_coop(rp + (k0 + k) * 64, pend, c[k], sign, soff, nsph);
-            if (interior) m = m && ((ok >> lane) & 1ull);
-          }
+        if (sph && nsph > MX_COOP_FIRST) {  // survivors: the rest of the sphere, one at a time
+          const unsigned long long pend = __ballot(m && interior), pendb = __ballot(m && !interior);
+          unsigned long long ok = ~0ull;
+          if (pend) ok &= is_max_sph_coop(rp + (k0 + k) * 64, pend, c[k], sign, soff, nsph) | ~pend;
+          if (pendb) ok &= is_max_sph_coop_border(V, z, x, (k0 + k) * 64, pendb, c[k], sign, nsph) | ~pendb;
+          m = m && ((ok >> lane) & 1ull);
         }
         const unsigned long long b = __ballot(m);
         if (lane == 0) mw[k0 + k] = b;
@@ -312,26 +349,27 @@ __global__ __launch_bounds__(256) void k_max_mark_pm(VS vs, float thr, int delta
         if (mp) {
           if (delta == 1) mp = is_max_d1(V, z, x, y, c[k], 1.0f);
           else if (sph) mp = interior ? is_max_sph_in(rp + y, soff, n1, c[k], 1.0f)
-                                      : is_max_sph_border(V, z, x, y, c[k], 1.0f, nsph);
+                                      : is_max_sph_border(V, z, x, y, c[k], 1.0f, n1);
           else mp = is_max_at(V, z, x, y, c[k], 1.0f, delta);
         }
         if (mm) {
           if (delta == 1) mm = is_max_d1(V, z, x, y, -c[k], -1.0f);
           else if (sph) mm = interior ? is_max_sph_in(rp + y, soff, n1, -c[k], -1.0f)
-                                      : is_max_sph_border(V, z, x, y, -c[k], -1.0f, nsph);
+                                      : is_max_sph_border(V, z, x, y, -c[k], -1.0f, n1);
           else mm = is_max_at(V, z, x, y, -c[k], -1.0f, delta);
         }
-        if (sph && nsph > MX_COOP_FIRST) {  // interior survivors: the rest of the sphere, one at a time
+        if (sph && nsph > MX_COOP_FIRST) {  // survivors: the rest of the sphere, one at a time
           const float* wrow = rp + (k0 + k) * 64;
+          const int yb = (k0 + k) * 64;
           const unsigned long long pp = __ballot(mp && interior), pm = __ballot(mm && interior);
-          if (pp) {
-            const unsigned long long ok = is_max_sph_coop(wrow, pp, c[k], 1.0f, soff, nsph);
-            if (interior) mp = mp && ((ok >> lane) & 1ull);
-          }
-          if (pm) {
-            const unsigned long long ok = is_max_sph_coop(wrow, pm, -c[k], -1.0f, soff, nsph);
-            if (interior) mm = mm && ((ok >> lane) & 1ull);
-          }
+          const unsigned long long ppb = __ballot(mp && !interior), pmb = __ballot(mm && !interior);
+          unsigned long long okp = ~0ull, okm = ~0ull;
+          if (pp) okp &= is_max_sph_coop(wrow, pp, c[k], 1.0f, soff, nsph) | ~pp;
+          if (ppb) okp &= is_max_sph_coop_border(V, z, x, yb, ppb, c[k], 1.0f, nsph) | ~ppb;
+          if (pm) okm &= is_max_sph_coop(wrow, pm, -c[k], -1.0f, soff, nsph) | ~pm;
+          if (pmb) okm &= is_max_sph_coop_border(V, z, x, yb, pmb, -c[k], -1.0f, nsph) | ~pmb;
+          mp = mp && ((okp >> lane) & 1ull);
+          mm = mm && ((okm >> lane) & 1ull);
         }
         const unsigned long long bp = __ballot(mp), bm = __ballot(mm);
         if (lane == 0) {
