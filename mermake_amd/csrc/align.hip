@@ -153,11 +153,17 @@ __global__ __launch_bounds__(256) void k_argmax_final(const float* __restrict__ 
 }
 
 // nearest reference point of X[i] - t (align.py:67-68); float64 distances
+// 16 lanes per query point: lane s of a query scans the reference points j = s (mod 16) of each
+// 256-point LDS tile, and the 16 partial minima are merged by (distance, then lowest index), so the
+// result is that of one ascending scan with a strict comparison, while a launch has n / 16 blocks of
+// work instead of n / 256 (a few thousand queries would otherwise leave most CUs idle).
+#define NN_LANES 16
 __global__ __launch_bounds__(256) void k_nn(const float* __restrict__ X, int n, int ldx, const float* __restrict__ R,
                                             int m, int ldr, const double* __restrict__ t, double* __restrict__ ds,
                                             int* __restrict__ idx) {
   __shared__ double sr[256][3];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = threadIdx.x % NN_LANES;
+  const int i = blockIdx.x * (blockDim.x / NN_LANES) + threadIdx.x / NN_LANES;
   double q0 = 0, q1 = 0, q2 = 0;
   if (i < n) {
     q0 = (double)X[(long long)i * ldx + 0] - t[0];
@@ -165,7 +171,7 @@ __global__ __launch_bounds__(256) void k_nn(const float* __restrict__ X, int n, 
     q2 = (double)X[(long long)i * ldx + 2] - t[2];
   }
   double best = 1.0e300;
-  int bi = 0;
+  int bi = 0x7fffffff;
   for (int j0 = 0; j0 < m; j0 += 256) {
     const int j = j0 + threadIdx.x;
     if (j < m) {
@@ -175,14 +181,21 @@ __global__ __launch_bounds__(256) void k_nn(const float* __restrict__ X, int n, 
     }
     __syncthreads();
     const int jm = min(256, m - j0);
-    for (int k = 0; k < jm; ++k) {
+    for (int k = s; k < jm; k += NN_LANES) {
       const double a = q0 - sr[k][0], b = q1 - sr[k][1], c = q2 - sr[k][2];
       const double d = a * a + b * b + c * c;
       if (d < best) { best = d; bi = j0 + k; }
     }
     __syncthreads();
   }
-  if (i < n) { ds[i] = sqrt(best); idx[i] = bi; }
+  // merge the 16 partial minima of the query: the smaller distance, the lower index on equal distances
+#pragma unroll
+  for (int off = NN_LANES / 2; off > 0; off >>= 1) {
+    const double ob = __shfl_xor(best, off, NN_LANES);
+    const int oi = __shfl_xor(bi, off, NN_LANES);
+    if (ob < best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+  }
+  if (i < n && s == 0) { ds[i] = sqrt(best); idx[i] = bi == 0x7fffffff ? 0 : bi; }
 }
 
 // One refinement round of align.py:66-76: keep pairs with ds < th; if any,
@@ -287,8 +300,8 @@ extern "C" int mm_nn_round(const float* X, int n, int ldx, const float* R, int m
     MM_CHECK(hipMemsetAsync(npts, 0, sizeof(int), (hipStream_t)stream));
     return MM_OK;
   }
-  hipLaunchKernelGGL(k_nn, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, X, n, ldx, R, m, ldr, t, ds_ws,
-                     idx_ws);
+  hipLaunchKernelGGL(k_nn, dim3((n + 256 / NN_LANES - 1) / (256 / NN_LANES)), dim3(256), 0, (hipStream_t)stream, X, n,
+                     ldx, R, m, ldr, t, ds_ws, idx_ws);
   MM_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_nn_mean, dim3(1), dim3(256), 0, (hipStream_t)stream, X, n, ldx, R, ldr, ds_ws, idx_ws, th, t,
                      npts);

@@ -453,6 +453,33 @@ def test_drift_recovers_integer_shift(dev, shift):
         mov[:, :3]))
 
 
+@pytest.mark.parametrize("n,m", [(1, 1), (5, 3), (37, 15), (300, 257), (1000, 700)])
+def test_nn_round_nearest_and_lowest_index_on_ties(dev, n, m):
+    """mm_nn_round's nearest reference point of every point, against a float64 brute force: the same distance
+    and, among equally near points (duplicated reference points, lattice points at equal distances), the
+    lowest index, as scipy's cKDTree query of the reference (align.py:61-77) returns for a unique nearest."""
+    from mermake_amd import _lib as L
+    rng = np.random.default_rng(n + m)
+    R = rng.integers(0, 6, (m, 3)).astype(np.float32)      # a small lattice: many equal distances
+    if m > 2:
+        R[m // 2] = R[1]                                      # a duplicate point
+    X = (rng.integers(0, 6, (n, 3)) + rng.choice([0.0, 0.5], (n, 3))).astype(np.float32)
+    t = np.array([0.25, -0.5, 0.0])
+    tx, tr = T(X, dev), T(R, dev)
+    td = torch.tensor(t, dtype=torch.float64, device=dev)
+    ds = torch.empty(n, dtype=torch.float64, device=dev)
+    ix = torch.empty(n, dtype=torch.int32, device=dev)
+    npts = torch.zeros(1, dtype=torch.int32, device=dev)
+    L.call("mm_nn_round", L.ptr(tx), n, 3, L.ptr(tr), m, 3, L.ptr(td), 1e9, L.ptr(ds), L.ptr(ix), L.ptr(npts),
+           L.stream_ptr(dev))
+    d2 = (((X.astype(np.float64) - t)[:, None, :] - R.astype(np.float64)[None, :, :]) ** 2).sum(-1)
+    want_i = d2.argmin(1)                                      # first index of the minimum
+    got_i = ix.cpu().numpy()
+    assert np.array_equal(got_i, want_i)
+    assert np.allclose(ds.cpu().numpy(), np.sqrt(d2[np.arange(n), want_i]), rtol=0, atol=1e-12)
+    assert int(npts.item()) == n
+
+
 def test_drift_fixture_known_answer(dev):
     """Reference drift pickle: N+ = 621, N- = 261, zero shift (y > 4 reference-threshold quirk)."""
     from fixtures import dapi_features, drift_known_answer
