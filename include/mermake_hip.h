@@ -207,6 +207,13 @@ int mm_median(const float* a, long long n, float* out, void* ws, size_t ws_bytes
  * cupy.fft.fftn/ifftn calls of utils.fftconvolve (mermake/utils.py:10-36). */
 int mm_fft3d_c2c(void* data, int n0, int n1, int n2, int axes_mask, int inverse, void* stream);
 
+/* Forward mm_fft3d_c2c over all three axes of a grid whose values are zero outside
+ * [0,m0) x [0,m1) x [0,n2): the lines that are zero before their pass are skipped.
+ * For the zero-padded rasters of the drift correlation, whose cupy.fft.fftn in
+ * utils.fftconvolve (mermake/utils.py:10-36, called at mermake/align.py:104-110)
+ * transforms the whole padded grid; the result is the same. */
+int mm_fft3d_c2c_box(void* data, int n0, int n1, int n2, int m0, int m1, void* stream);
+
 /* ------------------------------------------------------------------ maxima
  * Replaces local_maxima_count + local_maxima (mermake/maxima.cu:1-145) as
  * driven by find_local_maxima (mermake/maxima.py:30-77): voxels with

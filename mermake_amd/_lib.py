@@ -52,6 +52,7 @@ _SIGS = {
     "mm_deconv_tiles_stitch": (_i, [_vp, _vp, _i, _vp, _i, _i, _vp, _vp, _ll, _ll, _i, _i, _vp]),
     "mm_deconv_stitch": (_i, [_vp, _vp, _i, _vp, _vp, _ll, _ll, _i, _i, _vp]),
     "mm_fft3d_c2c": (_i, [_vp, _i, _i, _i, _i, _i, _vp]),
+    "mm_fft3d_c2c_box": (_i, [_vp, _i, _i, _i, _i, _i, _vp]),
     "mm_median_ws_bytes": (_sz, []),
     "mm_median": (_i, [_vp, _ll, _vp, _vp, _sz, _vp]),
     "mm_rms_ws_bytes": (_sz, []),

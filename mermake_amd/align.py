@@ -111,7 +111,7 @@ class Aligner:
             grid = tuple(smooth_length(max(f, 2 * b + 2)) for f, b in zip(full, Rb))
             fb = torch.zeros(grid, dtype=torch.complex64, device=self.X_ref.device)
             self._ref.write(fb, flip=True)
-            L.call("mm_fft3d_c2c", L.ptr(fb), grid[0], grid[1], grid[2], 7, 0, L.stream_ptr(fb.device))
+            L.call("mm_fft3d_c2c_box", L.ptr(fb), grid[0], grid[1], grid[2], Rb[0], Rb[1], L.stream_ptr(fb.device))
             self._grid, self._fb = grid, fb
         return self._grid, self._fb
 
@@ -125,7 +125,7 @@ class Aligner:
         st = L.stream_ptr(dev)
         fa = torch.zeros(grid, dtype=torch.complex64, device=dev)
         r.write(fa)
-        L.call("mm_fft3d_c2c", L.ptr(fa), grid[0], grid[1], grid[2], 7, 0, st)
+        L.call("mm_fft3d_c2c_box", L.ptr(fa), grid[0], grid[1], grid[2], Ra[0], Ra[1], st)   # the raster is in [0, Ra)
         L.call("mm_cmul", L.ptr(fa), L.ptr(fb), fa.numel(), st)
         L.call("mm_fft3d_c2c", L.ptr(fa), grid[0], grid[1], grid[2], 7, 1, st)
         ws = torch.empty(L.lib().mm_argmax_ws_bytes(), dtype=torch.uint8, device=dev)

@@ -2217,6 +2217,34 @@ extern "C" int mm_deconv_stitch(mm_deconv* d, const mm_tile_desc* tiles_dev, int
 // ------------------------------------------------- generic 3D FFT (C2C)
 // In-place C2C over a contiguous (n0, n1, n2) complex64 volume along the axes
 // selected by `axes_mask` (bit i = axis i); inverse transforms scale by 1/n.
+// Forward transform of a grid whose values outside [0, m0) x [0, m1) (any index along axis 2) are zero: the
+// axis-2 lines outside that box and the axis-1 lines of planes m0 .. n0 - 1 are zero before their pass and stay
+// zero, so they are not transformed (a quarter and a half of the lines for a raster zero-padded to twice its size).
+extern "C" int mm_fft3d_c2c_box(void* data, int n0, int n1, int n2, int m0, int m1, void* stream) {
+  if (!data || n0 < 1 || n1 < 1 || n2 < 1 || m0 < 0 || m0 > n0 || m1 < 0 || m1 > n1) {
+    mm_set_error("mm_fft3d_c2c_box: bad argument");
+    return MM_EINVAL;
+  }
+  if (m0 == 0 || m1 == 0) return MM_OK;  // an all-zero grid transforms to zeros
+  hipStream_t st = (hipStream_t)stream;
+  float2* p = (float2*)data;
+  const int n[3] = {n0, n1, n2};
+  for (int ax = 2; ax >= 0; --ax) {
+    FftPlanDev<float2> P;
+    int rc = mm_get_plan_f32(n[ax], &P);
+    if (rc) return rc;
+    const int G = mm_fft_pick_g(P, 98304, 0, 16);
+    if (ax == 2)
+      rc = launch_c2c<false>(p, 1, n2, m1, (long long)n1 * n2, m0, P, G, 1.0, st);
+    else if (ax == 1)
+      rc = launch_c2c<false>(p, n2, 1, n2, (long long)n1 * n2, m0, P, G, 1.0, st);
+    else
+      rc = launch_c2c<false>(p, (long long)n1 * n2, 1, n1 * n2, 0, 1, P, G, 1.0, st);
+    if (rc) return rc;
+  }
+  return MM_OK;
+}
+
 extern "C" int mm_fft3d_c2c(void* data, int n0, int n1, int n2, int axes_mask, int inverse, void* stream) {
   if (!data || n0 < 1 || n1 < 1 || n2 < 1) { mm_set_error("mm_fft3d_c2c: bad argument"); return MM_EINVAL; }
   hipStream_t st = (hipStream_t)stream;
