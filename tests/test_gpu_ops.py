@@ -501,7 +501,8 @@ def test_rms_normalize_matches_float64_norm(dev, n, offset):
 
 
 @pytest.mark.parametrize("shape,delta,thr", [((12, 40, 70), 5, 3.0), ((10, 33, 130), 1, 2.0), ((9, 20, 64), 2, -0.5),
-                                             ((6, 6, 6), 5, 0.0), ((1, 1, 1), 1, -1.0)])
+                                             ((6, 6, 6), 5, 0.0), ((1, 1, 1), 1, -1.0), ((5, 30, 70), 3, 1.0),
+                                             ((4, 9, 130), 4, 0.5), ((11, 11, 200), 5, 1.5)])
 def test_maxima_both_signs_in_one_pass(dev, shape, delta, thr):
     """find_local_maxima_pm (mm_local_maxima_pm) gives the + and - maxima and fits of the two signed calls
     bit for bit, with the candidate sets of the oracle; a negative threshold makes voxels candidates of
