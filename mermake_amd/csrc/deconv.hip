@@ -781,10 +781,13 @@ __global__ __launch_bounds__(256, 2) void k_rd_c2c(float2* data, long long estri
 // validity bits fit one 32-bit mask).  For M = 6 small groups (6 rows in the
 // R2C, 4 in the C2R: 25 KB and 19 KB of LDS) let 4 blocks share a CU; they
 // measured 5 to 10% faster than groups of 12 rows.
-// For p = 239 (M = 2) G M B <= 256 allows up to 9 transforms.
+// For p = 239 (M = 2) groups of 4 transforms (8 rows) in both passes, on 128-thread
+// blocks (G M B = 112 items in the in-place stages): the C2R with 3 transforms on
+// 128 threads took 1.93 ms per 100-tile T 478 stack, with 4 1.49 ms (5: 1.54, on
+// 256 threads; 6: 1.88); R2C groups of 6 on 256 threads were slower (2.52 ms).
 template <int PR>
 constexpr int rd_row_g(int M, bool c2r) {
-  return PR == 239 ? (c2r ? 3 : 4) : (M == 2 ? 16 : (M == 6 ? (c2r ? 2 : 3) : 9));
+  return PR == 239 ? 4 : (M == 2 ? 16 : (M == 6 ? (c2r ? 2 : 3) : 9));
 }
 template <int PR>
 constexpr int rd_row_nc(int M, int bs = 256) { return (PR * M + bs - 1) / bs; }
@@ -1234,9 +1237,10 @@ static void launch_rd_r2c(const mm_tile_dev* td, int ntiles, int T, int nrows, f
                           const SpecLayout& Ls, const RdTab& tab, const float2* tw, hipStream_t st) {
   constexpr int G = rd_row_g<PR>(M, false);
   const size_t lds = rd_lds_bytes<PR>(T, M, G, 0);
-  // 256 threads (4 waves per SIMD at 115 VGPRs); blocks of 128 (161 VGPRs, 3 waves per SIMD) measured
-  // 1.45 -> 1.52 ms per 36-tile stack
-  constexpr int BS = 256;
+  // p = 113: 256 threads (4 waves per SIMD at 115 VGPRs); blocks of 128 (161 VGPRs, 3 waves per SIMD) measured
+  // 1.45 -> 1.52 ms per 36-tile stack.  p = 239: the 4 transforms' in-place stages take 112 items, so blocks of
+  // 128 leave fewer lanes idle (2.09 -> 1.92-1.98 ms per 100-tile T 478 stack)
+  constexpr int BS = PR == 239 ? 128 : 256;
   // items of (tile, band) run all planes, unless they are too few to fill the GPU: then chunks of planes
   const int sz = nrows / T;
   const long long nb = (long long)((T + 2 * G - 1) / (2 * G)) * ntiles;
